@@ -1,0 +1,266 @@
+/*
+ * tetmag_hip.h -- C ABI of the MI355X (gfx950) effective-field + LLG-RHS path.
+ *
+ * This is the boundary between a tetmag-style host (TheLLG, DemagField,
+ * TheSimulation) and the HIP kernels.  Every entry point:
+ *   - takes plain pointers and sizes (no C++ or torch types),
+ *   - returns an int status (TM_OK or one of TM_ERR_*), never throws,
+ *   - names the reference interface it replaces (file:line in tetmag v1.7.7).
+ *
+ * Layout conventions (same as the reference):
+ *   - a "3n" field or magnetisation is the column-major n x 3 matrix
+ *     (x-block, y-block, z-block), i.e. the CVODE state vector layout
+ *     (TheLLG.cpp:156, typedefs.h:39-41);
+ *   - sparse operators are CSR with int32 indices and FP64 values, the arrays
+ *     of an Eigen RowMajor SparseMatrix (typedefs.h:35);
+ *   - the boundary-element matrix is dense bnx x bnx, column-major, rows and
+ *     columns in the order of the ascending boundary-node list
+ *     (Boundary.cpp:200-211, BEMprocessing.cpp:375-411);
+ *   - mesh geometry is xyz (n x 3 row-major, mesh units) and fel (ntet x 4
+ *     row-major).
+ *
+ * Pointer arguments carry a "where" flag: TM_HOST for host memory (the call
+ * copies and synchronises), TM_DEVICE for device memory on the context's GPU
+ * (the call is stream-ordered on the context stream and returns without
+ * synchronising, except where a scalar result is returned).
+ *
+ * Thread safety: one context per host thread.  Calls on one context are not
+ * reentrant (the reference GPU classes are not thread-safe either,
+ * EffFieldGPU.h:54-127, LLGWrapper.cu:43-44).
+ */
+#ifndef TETMAG_HIP_H_
+#define TETMAG_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TM_OK 0
+#define TM_ERR_NO_DEVICE 1   /* no usable GPU (SetDevice.cu:43-49 falls back to CPU) */
+#define TM_ERR_BAD_ARG 2     /* invalid size, pointer, layout or operator */
+#define TM_ERR_NONFINITE 3   /* NaN/Inf met in a solve (GPUSolver.cu:95-97 checks isfinite) */
+#define TM_ERR_MAXITER 4     /* solver did not reach the tolerance within maxit */
+#define TM_ERR_HIP 5         /* a HIP runtime call failed */
+#define TM_ERR_NOT_READY 6   /* a required operator/vector was not set */
+#define TM_ERR_UNSUPPORTED 7 /* matrix form the solver cannot handle (e.g. non-symmetric free block) */
+
+#define TM_HOST 0
+#define TM_DEVICE 1
+
+/* ---- sparse operator ids for tm_ctx_set_op ----------------------------- */
+#define TM_OP_EXCHANGE 0 /* XC_field_OP = -2 diag(A/V) K  (TheLLG.cpp:88-93 -> EffFieldGPU::setExchangeMatOnDev) */
+#define TM_OP_CURL_X 1   /* -tGradX as passed to EffFieldGPU::setGradientMatsOnDev (TheLLG.cpp:94-97) */
+#define TM_OP_CURL_Y 2
+#define TM_OP_CURL_Z 3
+#define TM_OP_TGRAD_X 4  /* tGrad as passed to GpuPotential::setGradientMatrices (DemagField.cpp:248) */
+#define TM_OP_TGRAD_Y 5
+#define TM_OP_TGRAD_Z 6
+#define TM_OP_GRAD_X 7   /* grad as passed to GpuPotential::setGradientMatrices */
+#define TM_OP_GRAD_Y 8
+#define TM_OP_GRAD_Z 9
+#define TM_OP_STT_GRAD_X 10 /* Grad ops of EffFieldGPU::setSTTDataOnDevice (EffFieldGPU.cu:253-287) */
+#define TM_OP_STT_GRAD_Y 11
+#define TM_OP_STT_GRAD_Z 12
+#define TM_NUM_OPS 13
+
+/* ---- LLG variants (TheLLG.cpp:129-141, selectLLGType) ------------------ */
+#define TM_LLG_NOPREC 0  /* dm/dtau = -alpha m x (m x H)           (EffFieldGPU.cu:341-362) */
+#define TM_LLG_CLASSIC 1 /* dm/dtau = -m x H - alpha m x (m x H)   (EffFieldGPU.cu:316-339) */
+
+/* ---- solid-angle form of the GPU boundary-element assembly -------------- */
+#define TM_BEM_REFERENCE 0 /* 2 acos(.) of Lindholm.cpp:41-60, same operation order (bitwise-faithful) */
+#define TM_BEM_ACCURATE 1  /* Van Oosterom-Strackee 2 atan2(.): full precision for distant triangles */
+
+/* ---- preconditioners for tm_solver_create ------------------------------ */
+#define TM_PRECOND_JACOBI 0
+
+/* ---- device selection: SetDevice (SetDevice.h:36-44) ------------------- */
+
+/* Number of visible GPUs; TM_ERR_NO_DEVICE when none (SetDevice.cu:41-50). */
+int tm_device_count(int* ndev);
+/* SetDevice::set (SetDevice.cu:41-59): selects `dev`, falling back to device
+ * 0 when dev is out of range; *chosen receives the device actually used. */
+int tm_set_device(int dev, int* chosen);
+
+/* ---- generic sparse matrix: SpMatCUDA (SpMatCUDA.h:49-79) -------------- */
+typedef struct tm_spmat tm_spmat;
+/* SpMatCUDA(const SpMat&) (SpMatCUDA.cu:59-75): deep copy of a CSR matrix to
+ * the device.  rows == cols is not required. */
+int tm_spmat_create(int rows, int cols, const int* rowptr, const int* colidx, const double* vals, tm_spmat** out);
+int tm_spmat_destroy(tm_spmat* A);
+/* SpMatCUDA::mvp / setX / mvpResEig / mvpResDev (SpMatCUDA.cu:131-169): y = A x. */
+int tm_spmat_mvp(tm_spmat* A, const double* x, int xwhere, double* y, int ywhere);
+
+/* ---- linear solver: GPUSolver (GPUSolver.h:65-87) ----------------------- */
+typedef struct tm_solver tm_solver;
+/* SolverFactory::setMatrix + GPUSolver::compute/setup (SolverFactory.cpp:45-48,
+ * GPUSolver.cu:57-79).  Accepts the matrices tetmag actually hands to its
+ * solvers: symmetric stiffness matrices some of whose rows were replaced by
+ * identity rows (Neumann: DemagField.cpp:204-220, Dirichlet:
+ * FEMprocessing.cpp:329-369).  Identity rows are treated as fixed values and
+ * the remaining free block, which must be symmetric, is solved by
+ * preconditioned CG.  Returns TM_ERR_UNSUPPORTED if the free block is not
+ * symmetric. */
+int tm_solver_create(int n, const int* rowptr, const int* colidx, const double* vals, int precond, tm_solver** out);
+int tm_solver_destroy(tm_solver* s);
+/* GPUSolver::solve (GPUSolver.cu:82-86): solves A x = b to
+ * ||b - A x|| <= tol ||b||.  x is the warm start on entry when warm != 0
+ * (the reference keeps x_d between calls); otherwise the previous solution
+ * held by the solver is used when warm == 2, and zero when warm == 0.
+ * Returns TM_ERR_MAXITER when maxit is reached first (the reference caps at
+ * 100 GMRES iterations silently, GPUSolver.cu:47).  iters/relres may be NULL. */
+int tm_solver_solve(tm_solver* s, const double* b, int bwhere, double* x, int xwhere, double tol, int maxit,
+                    int warm, int* iters, double* relres);
+
+/* ---- field context: EffFieldGPU + GpuPotential + DemagField state ------- */
+typedef struct tm_ctx tm_ctx;
+
+typedef struct tm_eval_stats {
+    int iters_neumann;      /* CG iterations of the u1 solve */
+    int iters_dirichlet;    /* CG iterations of the u2 solve */
+    double relres_neumann;  /* final relative residuals */
+    double relres_dirichlet;
+    double max_torque;      /* max_i |m_i x Heff_i| in A/m (TheLLG.cpp:426-439 multiplies by mu0) */
+} tm_eval_stats;
+
+/* EffFieldGPU(int) + init (EffFieldGPU.cu:50-53, 216-251) on the current device. */
+int tm_ctx_create(int n, tm_ctx** out);
+int tm_ctx_destroy(tm_ctx* c);
+/* Run the context's work on a caller-provided hipStream_t (NULL = the
+ * context's own stream). */
+int tm_ctx_set_stream(tm_ctx* c, void* hip_stream);
+/* Blocks until all work queued on the context stream has finished. */
+int tm_ctx_synchronize(tm_ctx* c);
+
+/* Sparse operators: EffFieldGPU::setExchangeMatOnDev (EffFieldGPU.cu:211-214),
+ * setGradientMatsOnDev (EffFieldGPU.cu:205-209), setSTTDataOnDevice matrices
+ * (EffFieldGPU.cu:253-287), GpuPotential::setGradientMatrices
+ * (GpuPotential.cu:54-83).  n x n CSR, host arrays, deep-copied. */
+int tm_ctx_set_op(tm_ctx* c, int op, const int* rowptr, const int* colidx, const double* vals);
+
+/* GpuPotential::setGradientMatrices Js argument (GpuPotential.cu:80-82) and
+ * TheLLG's invJs (TheLLG.cpp:110, 1/Js with non-finite -> 0).  n values, T. */
+int tm_ctx_set_js(tm_ctx* c, const double* Js);
+/* EffFieldGPU::setUniaxialAnisotropy (EffFieldGPU.cu:55-64): axis 3n, Ku n (J/m^3). */
+int tm_ctx_set_uniaxial(tm_ctx* c, const double* axis3n, const double* Ku);
+/* EffFieldGPU::setDMIdata (EffFieldGPU.cu:289-314): D (J/m^2 per mesh unit,
+ * already scaled, SimulationData.cpp:45-49), invNodeVol, node normals 3n,
+ * node surface area. */
+int tm_ctx_set_dmi(tm_ctx* c, const double* D, const double* invNodeVol, const double* nv3n, const double* nodeArea);
+/* EffFieldGPU::setSTTDataOnDevice vectors eta_j{x,y,z} (EffFieldGPU.cu:253-287). */
+int tm_ctx_set_stt_eta(tm_ctx* c, const double* eta_x, const double* eta_y, const double* eta_z);
+/* TheLLG::setZeemanField (TheLLG.cpp:477-483): applied field in A/m; either
+ * a uniform vector (H3n == NULL) or a per-node 3n field. */
+int tm_ctx_set_zeeman(tm_ctx* c, const double* H3n, double hx, double hy, double hz);
+
+/* EffFieldGPU::setMagDev (EffFieldGPU.cu:164-203). */
+int tm_ctx_set_mag(tm_ctx* c, const double* m3n, int where);
+int tm_ctx_get_mag(tm_ctx* c, double* m3n, int where);
+/* EffFieldGPU::NormalizeMag (EffFieldGPU.cu:625-663): normalises m in place
+ * and stores it as the context magnetisation. */
+int tm_ctx_normalize_mag(tm_ctx* c, double* m3n, int where);
+
+/* Individual fields from the context magnetisation, each n x 3 column-major,
+ * same units as the reference return values (J/m^3 before invJs for
+ * exchange/anisotropy/DMI). */
+int tm_exchange_field(tm_ctx* c, double* out3n, int where);  /* ExchangeFieldGPU (EffFieldGPU.cu:71-86) */
+int tm_uniaxial_field(tm_ctx* c, double* out3n, int where);  /* UniaxialAnisotropyField (EffFieldGPU.cu:538-560) */
+int tm_cubic_field(tm_ctx* c, double* out3n, int where);     /* CubicAnisotropyField (EffFieldGPU.cu:514-535): zeros, as the reference */
+int tm_dmi_field(tm_ctx* c, double* out3n, int where);       /* DMIField (EffFieldGPU.cu:596-622) */
+int tm_stt_uterm(tm_ctx* c, double* out3n, int where);       /* UTermSTT_GPU (EffFieldGPU.cu:107-148) */
+
+/* ClassicLLG_dev/_hst and LLG_noPrec_dev/_hst (EffFieldGPU.cu:316-430):
+ * dm/dtau from the context magnetisation and a given Heff. */
+int tm_llg_rhs(tm_ctx* c, const double* Heff3n, int hwhere, double alpha, int variant, double* out3n, int owhere);
+/* STT_term_LLG_dev (EffFieldGPU.cu:434-475): -(beta-alpha) m x u - (1+alpha beta) m x (m x u). */
+int tm_stt_llg(tm_ctx* c, const double* U3n, int uwhere, double alpha, double beta, double* out3n, int owhere);
+/* MaxTorque (EffFieldGPU.cu:484-512): max_i |m_i x H_i|. */
+int tm_max_torque(tm_ctx* c, const double* Heff3n, int hwhere, double* out);
+
+/* GpuPotential::calcDivM (GpuPotential.cu:86-106): sum_d tGrad_d (Js .* m_d)
+ * from the given magnetisation (NOT divided by mu0, as the reference). */
+int tm_div_m(tm_ctx* c, const double* m3n, int mwhere, double* b, int bwhere);
+/* GpuPotential::calcGradientField (GpuPotential.cu:109-122): H = -grad u. */
+int tm_grad_field(tm_ctx* c, const double* u, int uwhere, double* H3n, int hwhere);
+
+/* Boundary nodes (DemagField ctor, DemagField.cpp:65-89): ascending global
+ * node ids, bnx of them. */
+int tm_ctx_set_boundary(tm_ctx* c, int bnx, const int* bnodes);
+/* DemagField::dirichletBEM (DemagField.cpp:67): dense bnx x bnx column-major
+ * matrix with leading dimension ld >= bnx. */
+int tm_ctx_set_bem_dense(tm_ctx* c, const double* B, int ld, int where);
+/* Dense boundary-element matrix assembled on the GPU (BEMprocessing.cpp:358-411,
+ * Lindholm.cpp:41-128) from the boundary triangulation: bxyz (bnx x 3
+ * row-major coordinates of the boundary nodes), faces (nbel x 3 boundary-local
+ * node indices), face areas, outward unit face normals (nbel x 3 row-major)
+ * and the interior solid angle at every boundary node.  `formula` selects
+ * the solid-angle form of the Lindholm weights (TM_BEM_REFERENCE or
+ * TM_BEM_ACCURATE). */
+int tm_ctx_assemble_bem(tm_ctx* c, int nbel, const double* bxyz, const int* faces, const double* face_area,
+                        const double* face_normal, const double* solid_angle, int formula);
+/* Copies the context's dense matrix out (column-major, ld = bnx). */
+int tm_ctx_get_bem_dense(tm_ctx* c, double* B, int where);
+/* DemagField::denseMVP (DemagField.cpp:145-147): g = B u on boundary vectors. */
+int tm_bem_apply(tm_ctx* c, const double* ub, int uwhere, double* gb, int gwhere);
+
+/* DemagField::initializeSolvers (DemagField.cpp:243-282): the Neumann matrix
+ * (stiffness with row nx/2 replaced by the identity) and the Dirichlet matrix
+ * (stiffness with boundary rows replaced by the identity), CSR host arrays. */
+int tm_ctx_set_demag_solvers(tm_ctx* c, const int* rpN, const int* colN, const double* valN, const int* rpD,
+                             const int* colD, const double* valD, int precond);
+/* Solver tolerance and cap (ProgramSpecs.cpp:70 "CG tolerance"). */
+int tm_ctx_set_solver_params(tm_ctx* c, double tol, int maxit);
+/* DemagField::calcPotential (DemagField.cpp:163-181): u = u1 + u2 for the
+ * given magnetisation (m3n == NULL uses the context magnetisation).  Solves
+ * are warm-started from the previous call (AMGCLSolver.cpp:97-103). */
+int tm_demag_potential(tm_ctx* c, const double* m3n, int mwhere, double* u, int uwhere, tm_eval_stats* stats);
+/* DemagField::calcField (DemagField.cpp:92-94): Hdem = -grad u. */
+int tm_demag_field(tm_ctx* c, const double* m3n, int mwhere, double* H3n, int hwhere, tm_eval_stats* stats);
+/* Last solutions of the two solves (u1, u2), n each. */
+int tm_demag_last_potentials(tm_ctx* c, double* u1, double* u2, int where);
+
+/* One full effective-field + LLG right-hand-side evaluation with the demag
+ * field recomputed (the composition TheLLG::rhs_d -> classicVersion_GPU ->
+ * evaluateAllEffectiveFields -> totalEffectiveField -> ClassicLLG_dev of
+ * LLGWrapper.cu:78-82, 213-228 and TheLLG.cpp:238-262, with DemagField::calcField
+ * as in TheLLG::calcDemagField, TheLLG.cpp:278-283):
+ *   Heff = (Hexc + Hku + Hdmi) invJs + Hext + Hdem,  dm/dtau = LLG(m, Heff).
+ * Terms without data set on the context are zero.  m3n == NULL uses the
+ * context magnetisation.  Heff3n may be NULL. */
+int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int variant, double* dmdt3n, int dwhere,
+                double* Heff3n, int hwhere, tm_eval_stats* stats);
+
+/* ---- host-side setup helpers (no GPU needed) ---------------------------- */
+/* Kuhn-split cuboid: (nx+1)(ny+1)(nz+1) nodes, 6 nx ny nz tets, spacing h,
+ * interior nodes jittered uniformly by +-jitter*h per coordinate (seeded). */
+int tm_mesh_kuhn_cuboid_size(int nx, int ny, int nz, int* n_nodes, int* n_tets);
+int tm_mesh_kuhn_cuboid(int nx, int ny, int nz, double h, double jitter, uint64_t seed, double* xyz, int* fel);
+/* P1 operator pattern (FEMprocessing.cpp:228-286 touch exactly the node pairs
+ * sharing a tet) and values: stiffness, grad_d, tGrad_d, node volumes. */
+int tm_fem_pattern(int n, int ntet, const int* fel, int* rowptr);
+int tm_fem_assemble(int n, int ntet, const double* xyz, const int* fel, const int* rowptr, int* col, double* K,
+                    double* gx, double* gy, double* gz, double* tgx, double* tgy, double* tgz, double* nodevol,
+                    double* elvol);
+/* Boundary extraction (Boundary.cpp:47-224). */
+int tm_boundary_faces(int ntet, const int* fel, int* faces, int* face_tet, int* nbel_out);
+int tm_boundary_props(int n, const double* xyz, const int* fel, int nbel, const int* faces, const int* face_tet,
+                      int* bnodes, int* bindex, double* face_area, double* face_normal, double* node_area,
+                      double* node_normal, int* bnx_out);
+/* Interior solid angles at boundary nodes (BEMprocessing.cpp:358-372). */
+int tm_boundary_solid_angles(int n, int ntet, const double* xyz, const int* fel, const int* bindex, int bnx,
+                             double* omega);
+/* Seeded inputs. */
+int tm_init_random_unit(int n, uint64_t seed, double* m3n);
+int tm_uniform(uint64_t seed, int count, double* out);
+
+/* Library version string. */
+const char* tm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TETMAG_HIP_H_ */

@@ -1,0 +1,950 @@
+/*
+ * refcpu.c -- FP64 CPU restatement of tetmag's effective-field + LLG-RHS path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle and the CPU
+ * baseline.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load it.  The product (tetmag_amd/) never links or calls it.
+ *
+ * Every function restates one piece of the reference (tetmag v1.7.7) and cites
+ * the file:line it follows.  The reference cannot be compiled in this
+ * environment (Eigen3/Boost/VTK/AMGCL/SUNDIALS/H2Lib absent), so these are
+ * hand restatements of the formulas, written in plain C with OpenMP.
+ *
+ * Conventions (all FP64, int32 indices):
+ *   xyz   : n x 3 row-major node coordinates (mesh units)
+ *   fel   : ntet x 4 row-major tetrahedron connectivity
+ *   fields/magnetisation : "3n SoA" = x-block, y-block, z-block, i.e. the
+ *           column-major n x 3 layout of the reference state vector
+ *           (TheLLG.cpp:156, typedefs.h:39-41)
+ *   sparse: CSR with sorted column indices (Eigen RowMajor SparseMatrix,
+ *           typedefs.h:35)
+ *   B     : dense bnx x bnx column-major (BEMprocessing.cpp:375-411)
+ *
+ * Pinning status of each function is recorded in the test files; see
+ * tests/test_oracle_*.py.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define REF_PI 3.14159265358979323846264338327950288
+/* PhysicalConstants.h:39 */
+#define REF_MU0 (1.256637061435917295376 * 1.e-6)
+
+/* ------------------------------------------------------------------------ */
+/* small 3-vector helpers                                                    */
+/* ------------------------------------------------------------------------ */
+static inline void v3sub(const double* a, const double* b, double* r) {
+    r[0] = a[0] - b[0]; r[1] = a[1] - b[1]; r[2] = a[2] - b[2];
+}
+static inline double v3dot(const double* a, const double* b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static inline void v3cross(const double* a, const double* b, double* r) {
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double v3norm(const double* a) { return sqrt(v3dot(a, a)); }
+
+/* auxiliaries.cpp:42-47 -- areEqual(x, y, ulp=100) */
+static int ref_are_equal(double x, double y, int ulp) {
+    const double eps = DBL_EPSILON;
+    int almost = fabs(x - y) < (eps * fabs(x + y) * ulp);
+    if (fabs(x) < eps * ulp && fabs(y) < eps * ulp) almost = 1;
+    return almost;
+}
+
+/* 3x3 determinant, rows r0,r1,r2 */
+static inline double det3(const double* r0, const double* r1, const double* r2) {
+    return r0[0] * (r1[1] * r2[2] - r1[2] * r2[1])
+         - r0[1] * (r1[0] * r2[2] - r1[2] * r2[0])
+         + r0[2] * (r1[0] * r2[1] - r1[1] * r2[0]);
+}
+
+/* FEMprocessing.cpp:115-123 -- det of [1 x y z] rows = 6 * signed volume.
+ * Expanded along the column of ones. */
+static double signed_vol6(const double* p1, const double* p2, const double* p3, const double* p4) {
+    /* det [[1,p1],[1,p2],[1,p3],[1,p4]]: Laplace expansion along column 0 */
+    return det3(p2, p3, p4) - det3(p1, p3, p4) + det3(p1, p2, p4) - det3(p1, p2, p3);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FEM setup                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* FEMprocessing.cpp:104-113 -- element volumes |det|/6, node volume = sum V_e/4 */
+void ref_volumes(int n, int ntet, const double* xyz, const int* fel, double* elvol, double* nodevol) {
+    for (int i = 0; i < n; ++i) nodevol[i] = 0.0;
+    for (int e = 0; e < ntet; ++e) {
+        const int* t = fel + 4 * e;
+        double v = fabs(signed_vol6(xyz + 3 * t[0], xyz + 3 * t[1], xyz + 3 * t[2], xyz + 3 * t[3])) / 6.0;
+        elvol[e] = v;
+        for (int j = 0; j < 4; ++j) nodevol[t[j]] += v / 4.0;
+    }
+}
+
+/* FEMprocessing.cpp:153-195 -- shape-function gradients by cofactors.
+ * For local node a, node_mat rows are the coordinates of nodes (a+1)%4,
+ * (a+2)%4, (a+3)%4; column d is replaced by ones; grad_d N_a =
+ * (-1)^(a+1) det(.) / V6. */
+void ref_shape_grads(int ntet, const double* xyz, const int* fel, double* dNx, double* dNy, double* dNz) {
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ntet; ++e) {
+        const int* t = fel + 4 * e;
+        const double* P[4] = {xyz + 3 * t[0], xyz + 3 * t[1], xyz + 3 * t[2], xyz + 3 * t[3]};
+        double V6 = signed_vol6(P[0], P[1], P[2], P[3]);
+        for (int a = 0; a < 4; ++a) {
+            double rows[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) rows[r][c] = P[(a + 1 + r) % 4][c];
+            double sgn = ((a + 1) % 2 == 0) ? 1.0 : -1.0;
+            double out[3];
+            for (int d = 0; d < 3; ++d) {
+                double m[3][3];
+                memcpy(m, rows, sizeof(m));
+                for (int r = 0; r < 3; ++r) m[r][d] = 1.0;
+                out[d] = sgn * det3(m[0], m[1], m[2]) / V6;
+            }
+            dNx[4 * e + a] = out[0];
+            dNy[4 * e + a] = out[1];
+            dNz[4 * e + a] = out[2];
+        }
+    }
+}
+
+static int cmp_int(const void* a, const void* b) {
+    int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+
+/* node -> incident tets incidence (CSR) */
+static void node_tets(int n, int ntet, const int* fel, int** ptr_out, int** idx_out) {
+    int* ptr = (int*)calloc((size_t)n + 1, sizeof(int));
+    for (int e = 0; e < ntet; ++e)
+        for (int j = 0; j < 4; ++j) ptr[fel[4 * e + j] + 1]++;
+    for (int i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+    int* idx = (int*)malloc(sizeof(int) * (size_t)ptr[n]);
+    int* fill = (int*)malloc(sizeof(int) * (size_t)n);
+    memcpy(fill, ptr, sizeof(int) * (size_t)n);
+    for (int e = 0; e < ntet; ++e)
+        for (int j = 0; j < 4; ++j) idx[fill[fel[4 * e + j]]++] = e;
+    free(fill);
+    *ptr_out = ptr;
+    *idx_out = idx;
+}
+
+/* Sparsity of the P1 operators: (i,j) is present iff i and j share a tet.
+ * This is the set of entries touched by coeffRef in FEMprocessing.cpp:237-252
+ * and 256-264.  Returns row lengths into rowptr (size n+1, prefix-summed). */
+int ref_fem_pattern(int n, int ntet, const int* fel, int* rowptr) {
+    int *tp, *ti;
+    node_tets(n, ntet, fel, &tp, &ti);
+    rowptr[0] = 0;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+        int cnt = tp[i + 1] - tp[i];
+        int* buf = (int*)malloc(sizeof(int) * (size_t)(4 * cnt + 1));
+        int k = 0;
+        for (int q = tp[i]; q < tp[i + 1]; ++q)
+            for (int j = 0; j < 4; ++j) buf[k++] = fel[4 * ti[q] + j];
+        qsort(buf, (size_t)k, sizeof(int), cmp_int);
+        int u = 0;
+        for (int q = 0; q < k; ++q)
+            if (q == 0 || buf[q] != buf[q - 1]) ++u;
+        rowptr[i + 1] = u;
+        free(buf);
+    }
+    for (int i = 0; i < n; ++i) rowptr[i + 1] += rowptr[i];
+    free(tp);
+    free(ti);
+    return rowptr[n];
+}
+
+static inline int find_col(const int* col, int lo, int hi, int j) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (col[mid] < j) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* FEMprocessing.cpp:228-253 (gradient operators) and 266-286 (stiffness).
+ *   grad_d(i,j)  += dN_d[j] * (V_e/4) / NodeVolume(i)
+ *   tGrad_d(i,j) += dN_d[i] * (V_e/4)
+ *   K(i,j)       += V_e * sum_d dN_d[i] dN_d[j]
+ * Output on the pattern from ref_fem_pattern (col filled here). */
+void ref_fem_assemble(int n, int ntet, const double* xyz, const int* fel, const int* rowptr, int* col,
+                      double* K, double* gX, double* gY, double* gZ, double* tX, double* tY, double* tZ,
+                      double* nodevol) {
+    int *tp, *ti;
+    node_tets(n, ntet, fel, &tp, &ti);
+    double* elvol = (double*)malloc(sizeof(double) * (size_t)ntet);
+    double* dN = (double*)malloc(sizeof(double) * (size_t)ntet * 12);
+    double *dNx = dN, *dNy = dN + 4 * (size_t)ntet, *dNz = dN + 8 * (size_t)ntet;
+    ref_volumes(n, ntet, xyz, fel, elvol, nodevol);
+    ref_shape_grads(ntet, xyz, fel, dNx, dNy, dNz);
+    /* column pattern */
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+        int cnt = tp[i + 1] - tp[i];
+        int* buf = (int*)malloc(sizeof(int) * (size_t)(4 * cnt + 1));
+        int k = 0;
+        for (int q = tp[i]; q < tp[i + 1]; ++q)
+            for (int j = 0; j < 4; ++j) buf[k++] = fel[4 * ti[q] + j];
+        qsort(buf, (size_t)k, sizeof(int), cmp_int);
+        int u = rowptr[i];
+        for (int q = 0; q < k; ++q)
+            if (q == 0 || buf[q] != buf[q - 1]) col[u++] = buf[q];
+        free(buf);
+    }
+    /* row-wise accumulation: row i gathers contributions of its incident tets,
+     * in increasing tet order (the reference adds tets in increasing order). */
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+        int lo = rowptr[i], hi = rowptr[i + 1];
+        for (int p = lo; p < hi; ++p) {
+            K[p] = gX[p] = gY[p] = gZ[p] = tX[p] = tY[p] = tZ[p] = 0.0;
+        }
+        for (int q = tp[i]; q < tp[i + 1]; ++q) {
+            int e = ti[q];
+            const int* t = fel + 4 * e;
+            int li = 0;
+            while (t[li] != i) ++li;
+            double w = elvol[e] / 4.0;
+            for (int lj = 0; lj < 4; ++lj) {
+                int j = t[lj];
+                int p = find_col(col, lo, hi, j);
+                gX[p] += dNx[4 * e + lj] * w / nodevol[i];
+                gY[p] += dNy[4 * e + lj] * w / nodevol[i];
+                gZ[p] += dNz[4 * e + lj] * w / nodevol[i];
+                tX[p] += dNx[4 * e + li] * w;
+                tY[p] += dNy[4 * e + li] * w;
+                tZ[p] += dNz[4 * e + li] * w;
+                K[p] += elvol[e] * (dNx[4 * e + li] * dNx[4 * e + lj] + dNy[4 * e + li] * dNy[4 * e + lj] +
+                                    dNz[4 * e + li] * dNz[4 * e + lj]);
+            }
+        }
+    }
+    free(elvol);
+    free(dN);
+    free(tp);
+    free(ti);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boundary                                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct { int a, b, c, tet; } face_t;
+
+static int cmp_face(const void* x, const void* y) {
+    const face_t* f = (const face_t*)x;
+    const face_t* g = (const face_t*)y;
+    if (f->a != g->a) return (f->a > g->a) - (f->a < g->a);
+    if (f->b != g->b) return (f->b > g->b) - (f->b < g->b);
+    return (f->c > g->c) - (f->c < g->c);
+}
+
+/* Boundary.cpp:159-197 -- every tet contributes its 4 faces (node ids
+ * sorted); faces shared by two tets are interior, faces seen once form the
+ * boundary.  faces_out must hold 4*ntet*3 ints, face_tet_out 4*ntet ints.
+ * The reference pops a max-priority-queue, i.e. it lists the boundary faces
+ * in decreasing lexicographic order; this restatement lists them in the same
+ * decreasing order.  Returns nbel. */
+int ref_boundary_faces(int ntet, const int* fel, int* faces_out, int* face_tet_out) {
+    size_t nf = 4 * (size_t)ntet;
+    face_t* F = (face_t*)malloc(sizeof(face_t) * nf);
+    for (int e = 0; e < ntet; ++e) {
+        for (int j = 0; j < 4; ++j) {
+            int v[3], k = 0;
+            for (int q = 0; q < 4; ++q)
+                if (q != j) v[k++] = fel[4 * e + q];
+            qsort(v, 3, sizeof(int), cmp_int);
+            face_t f = {v[0], v[1], v[2], e};
+            F[4 * (size_t)e + j] = f;
+        }
+    }
+    qsort(F, nf, sizeof(face_t), cmp_face);
+    int nb = 0;
+    /* walk from the largest face down, as the priority queue does */
+    long i = (long)nf - 1;
+    while (i >= 0) {
+        if (i > 0 && cmp_face(&F[i], &F[i - 1]) == 0) {
+            i -= 2;
+        } else {
+            faces_out[3 * nb + 0] = F[i].a;
+            faces_out[3 * nb + 1] = F[i].b;
+            faces_out[3 * nb + 2] = F[i].c;
+            face_tet_out[nb] = F[i].tet;
+            ++nb;
+            i -= 1;
+        }
+    }
+    free(F);
+    return nb;
+}
+
+/* auxiliaries.cpp:58-72 */
+static double tri_area(const double* p1, const double* p2, const double* p3) {
+    double v12[3], v13[3], c[3];
+    v3sub(p1, p2, v12);
+    v3sub(p1, p3, v13);
+    v3cross(v12, v13, c);
+    return 0.5 * v3norm(c);
+}
+static void tri_unit_normal(const double* p1, const double* p2, const double* p3, double* nrm) {
+    double v12[3], v13[3], c[3];
+    v3sub(p1, p2, v12);
+    v3sub(p1, p3, v13);
+    v3cross(v12, v13, c);
+    double l = v3norm(c);
+    nrm[0] = c[0] / l; nrm[1] = c[1] / l; nrm[2] = c[2] / l;
+}
+
+/* Boundary.cpp:200-211 (sorted unique boundary nodes), 58-69 (index map),
+ * 149-156 (face areas), 214-224 (node areas = sum A_f/3), 109-125 (outward
+ * unit face normals via the owning tet's centre), 78-89 (node normals =
+ * normalised area-weighted face normals, non-finite -> 0).
+ * bnodes_out needs n ints; returns bnx. */
+int ref_boundary_props(int n, const double* xyz, const int* fel, int nbel, const int* faces, const int* face_tet,
+                       int* bnodes_out, int* bindex_out, double* face_area, double* face_normal,
+                       double* node_area, double* node_normal) {
+    char* isb = (char*)calloc((size_t)n, 1);
+    for (int f = 0; f < nbel; ++f)
+        for (int j = 0; j < 3; ++j) isb[faces[3 * f + j]] = 1;
+    int bnx = 0;
+    for (int i = 0; i < n; ++i) {
+        bindex_out[i] = -1;
+        if (isb[i]) { bindex_out[i] = bnx; bnodes_out[bnx++] = i; }
+    }
+    free(isb);
+    for (int i = 0; i < n; ++i) {
+        node_area[i] = 0.0;
+        node_normal[3 * i] = node_normal[3 * i + 1] = node_normal[3 * i + 2] = 0.0;
+    }
+    for (int f = 0; f < nbel; ++f) {
+        const double* p0 = xyz + 3 * faces[3 * f];
+        const double* p1 = xyz + 3 * faces[3 * f + 1];
+        const double* p2 = xyz + 3 * faces[3 * f + 2];
+        double A = tri_area(p0, p1, p2);
+        face_area[f] = A;
+        double nv[3];
+        tri_unit_normal(p0, p1, p2, nv);
+        const int* t = fel + 4 * face_tet[f];
+        double ctr[3] = {0, 0, 0};
+        for (int q = 0; q < 4; ++q)
+            for (int d = 0; d < 3; ++d) ctr[d] += xyz[3 * t[q] + d] / 4.0;
+        double toc[3];
+        v3sub(ctr, p0, toc);
+        double s = v3dot(nv, toc);
+        double sg = (s > 0) ? 1.0 : ((s < 0) ? -1.0 : 0.0);
+        for (int d = 0; d < 3; ++d) face_normal[3 * f + d] = -nv[d] * sg;
+    }
+    for (int f = 0; f < nbel; ++f) {
+        for (int j = 0; j < 3; ++j) {
+            int v = faces[3 * f + j];
+            node_area[v] += face_area[f] / 3.0;
+            for (int d = 0; d < 3; ++d) node_normal[3 * v + d] += face_normal[3 * f + d] * face_area[f] / 3.0;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        double l = v3norm(node_normal + 3 * i);
+        for (int d = 0; d < 3; ++d) {
+            double v = node_normal[3 * i + d] / l;
+            node_normal[3 * i + d] = isfinite(v) ? v : 0.0;
+        }
+    }
+    return bnx;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BEM                                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Lindholm.cpp:41-60 -- solid angle subtended by triangle (T1,T2,T3) at x,
+ * 2*acos(numer/denom), with the areEqual(arg, 1) clamp. */
+double ref_solid_angle(const double* x, const double* T1, const double* T2, const double* T3) {
+    double r1[3], r2[3], r3[3];
+    v3sub(T1, x, r1);
+    v3sub(T2, x, r2);
+    v3sub(T3, x, r3);
+    double r1n = v3norm(r1), r2n = v3norm(r2), r3n = v3norm(r3);
+    double r123 = r1n * r2n * r3n;
+    double r23 = v3dot(r2, r3), r12 = v3dot(r1, r2), r31 = v3dot(r3, r1);
+    double numer = r123 + r1n * r23 + r2n * r31 + r3n * r12;
+    double denom = sqrt(2.0 * (r2n * r3n + r23) * (r3n * r1n + r31) * (r1n * r2n + r12));
+    double arg = numer / denom;
+    if (ref_are_equal(arg, 1.0, 100)) arg = 1.0;
+    return 2.0 * acos(arg);
+}
+
+/* Lindholm.cpp:63-128 (isOrdered = false branch, as called from
+ * BEMprocessing.cpp:401) -- double-layer weights of the three vertices of a
+ * flat triangle seen from x0, D.A. Lindholm, IEEE Trans. Mag. 20, 2025 (1984). */
+void ref_lindholm_weights(const double* x0, const double* p1, const double* p2, const double* p3, double A,
+                          const double* nv, double* L) {
+    const double* p[3] = {p1, p2, p3};
+    for (int i = 0; i < 3; ++i) {
+        double d[3];
+        v3sub(p[i], x0, d);
+        /* Eigen isMuchSmallerThan(1.): |d|^2 <= (1e-12)^2 */
+        if (v3dot(d, d) <= 1e-12 * 1e-12) { L[0] = L[1] = L[2] = 0.0; return; }
+    }
+    double chi[3];
+    tri_unit_normal(p1, p2, p3, chi);
+    double rho_vec[3][3], xi[3][3], s[3], eta[3][3], eta0[3], rho[3];
+    for (int i = 0; i < 3; ++i) v3sub(p[i], x0, rho_vec[i]);
+    for (int i = 0; i < 3; ++i) v3sub(rho_vec[(i + 1) % 3], rho_vec[i], xi[i]);
+    for (int i = 0; i < 3; ++i) {
+        s[i] = v3norm(xi[i]);
+        for (int d = 0; d < 3; ++d) xi[i][d] /= s[i];
+        v3cross(chi, xi[i], eta[i]);
+        eta0[i] = v3dot(eta[i], rho_vec[i]);
+        rho[i] = v3norm(rho_vec[i]);
+    }
+    double chi0 = v3dot(chi, rho_vec[0]);
+    double gamma[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) gamma[i][j] = v3dot(xi[(i + 1) % 3], xi[j]);
+    double P[3];
+    for (int i = 0; i < 3; ++i) {
+        int i1 = (i + 1) % 3;
+        P[i] = log((rho[i] + rho[i1] + s[i]) / (rho[i] + rho[i1] - s[i]));
+    }
+    double Omega = ref_solid_angle(x0, p1, p2, p3);
+    if (chi0 < 0.) Omega = -Omega;
+    for (int i = 0; i < 3; ++i) {
+        int i1 = (i + 1) % 3;
+        double gP = gamma[i][0] * P[0] + gamma[i][1] * P[1] + gamma[i][2] * P[2];
+        L[i] = s[i1] / (8. * REF_PI * A) * (eta0[i1] * Omega - chi0 * gP);
+    }
+    if (v3dot(nv, chi) < 0) { L[0] = -L[0]; L[1] = -L[1]; L[2] = -L[2]; }
+}
+
+/* BEMprocessing.cpp:358-372 -- interior solid angle at each boundary node:
+ * sum over tets touching the node of the angle subtended by the opposite face. */
+void ref_solid_angles(int ntet, const double* xyz, const int* fel, const int* bindex, int bnx, double* sub) {
+    for (int i = 0; i < bnx; ++i) sub[i] = 0.0;
+    for (int e = 0; e < ntet; ++e) {
+        const int* t = fel + 4 * e;
+        for (int j = 0; j < 4; ++j) {
+            int bi = bindex[t[j]];
+            if (bi >= 0) {
+                sub[bi] += ref_solid_angle(xyz + 3 * t[j], xyz + 3 * t[(j + 1) % 4], xyz + 3 * t[(j + 2) % 4],
+                                           xyz + 3 * t[(j + 3) % 4]);
+            }
+        }
+    }
+}
+
+/* BEMprocessing.cpp:375-411 -- dense double-layer matrix, column-major
+ * bnx x bnx: B(j,j) = sub_j/(4 pi) - 1; B(n, m) += Lindholm weight of vertex
+ * m of every face not containing node n. */
+void ref_bem_matrix(int bnx, int nbel, const double* xyz, const int* bnodes, const int* bindex, const int* faces,
+                    const double* face_area, const double* face_normal, const double* sub, double* B) {
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int nrow = 0; nrow < bnx; ++nrow) {
+        for (int m = 0; m < bnx; ++m) B[(size_t)m * bnx + nrow] = 0.0;
+        B[(size_t)nrow * bnx + nrow] = sub[nrow] / (4. * REF_PI) - 1.;
+        const double* x0 = xyz + 3 * bnodes[nrow];
+        for (int f = 0; f < nbel; ++f) {
+            const int* v = faces + 3 * f;
+            int b0 = bindex[v[0]], b1 = bindex[v[1]], b2 = bindex[v[2]];
+            if (nrow == b0 || nrow == b1 || nrow == b2) continue;
+            double L[3];
+            ref_lindholm_weights(x0, xyz + 3 * v[0], xyz + 3 * v[1], xyz + 3 * v[2], face_area[f],
+                                 face_normal + 3 * f, L);
+            B[(size_t)b0 * bnx + nrow] += L[0];
+            B[(size_t)b1 * bnx + nrow] += L[1];
+            B[(size_t)b2 * bnx + nrow] += L[2];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sparse helpers                                                            */
+/* ------------------------------------------------------------------------ */
+void ref_spmv(int n, const int* rowptr, const int* col, const double* val, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) s += val[p] * x[col[p]];
+        y[i] = s;
+    }
+}
+
+/* dense column-major GEMV y = B x (DemagField.cpp:145-147, denseMVP) */
+void ref_dense_gemv(int b, const double* B, const double* x, double* y) {
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+        int nt = omp_get_num_threads(), id = omp_get_thread_num();
+#else
+        int nt = 1, id = 0;
+#endif
+        int chunk = (b + nt - 1) / nt;
+        int lo = id * chunk, hi = lo + chunk;
+        if (hi > b) hi = b;
+        for (int r = lo; r < hi; ++r) y[r] = 0.0;
+        for (int c = 0; c < b; ++c) {
+            const double* colp = B + (size_t)c * b;
+            double xc = x[c];
+            for (int r = lo; r < hi; ++r) y[r] += colp[r] * xc;
+        }
+    }
+}
+
+/* Replace the listed rows of a CSR matrix by identity rows and drop explicit
+ * zeros, on the same column pattern (prune(0,0) after the replacement).
+ * FEMprocessing.cpp:329-369 (Dirichlet: all boundary rows) and
+ * DemagField.cpp:204-220 (Neumann: row nx/2).  Output arrays must hold nnz
+ * entries; returns the new nnz. */
+int ref_replace_rows_identity(int n, const int* rowptr, const int* col, const double* val, const char* fixed,
+                              int* rp_out, int* col_out, double* val_out) {
+    int k = 0;
+    rp_out[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        if (fixed[i]) {
+            col_out[k] = i;
+            val_out[k] = 1.0;
+            ++k;
+        } else {
+            for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+                if (val[p] != 0.0) {
+                    col_out[k] = col[p];
+                    val_out[k] = val[p];
+                    ++k;
+                }
+            }
+        }
+        rp_out[i + 1] = k;
+    }
+    return k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Krylov solver: right-preconditioned restarted GMRES with ILU(0).          */
+/* The reference solves both row-replaced (non-symmetric) systems with a     */
+/* Krylov method: AMGCL smoothed-aggregation + ILU0 smoothing + GMRES         */
+/* (GPUSolver.h:53-63, GPUSolver.cu:43-48; AMGCLSolver.cpp:39-103), warm-     */
+/* started from the previous solution (AMGCLSolver.cpp:97-103).  AMGCL is     */
+/* not vendored; this restatement keeps the Krylov method (GMRES) and the     */
+/* ILU0 smoother as the preconditioner, without the AMG hierarchy.            */
+/* Convergence: ||b - A x|| <= tol * ||b||.                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    const int* rowptr;
+    const int* col;
+    const double* val;
+    double* lu;  /* ILU(0) factors on the pattern */
+    int* diag;   /* position of the diagonal entry per row */
+} ilu0_t;
+
+static int ilu0_factor(ilu0_t* f) {
+    int n = f->n;
+    f->lu = (double*)malloc(sizeof(double) * (size_t)f->rowptr[n]);
+    f->diag = (int*)malloc(sizeof(int) * (size_t)n);
+    memcpy(f->lu, f->val, sizeof(double) * (size_t)f->rowptr[n]);
+    for (int i = 0; i < n; ++i) {
+        int lo = f->rowptr[i], hi = f->rowptr[i + 1];
+        int d = find_col(f->col, lo, hi, i);
+        if (d >= hi || f->col[d] != i) return -1;
+        f->diag[i] = d;
+    }
+    int* pos = (int*)malloc(sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; ++i) pos[i] = -1;
+    for (int i = 0; i < n; ++i) {
+        int lo = f->rowptr[i], hi = f->rowptr[i + 1];
+        for (int p = lo; p < hi; ++p) pos[f->col[p]] = p;
+        for (int p = lo; p < hi && f->col[p] < i; ++p) {
+            int k = f->col[p];
+            double lik = f->lu[p] / f->lu[f->diag[k]];
+            f->lu[p] = lik;
+            for (int q = f->diag[k] + 1; q < f->rowptr[k + 1]; ++q) {
+                int j = f->col[q];
+                if (pos[j] >= 0) f->lu[pos[j]] -= lik * f->lu[q];
+            }
+        }
+        for (int p = lo; p < hi; ++p) pos[f->col[p]] = -1;
+        if (f->lu[f->diag[i]] == 0.0) { free(pos); return -2; }
+    }
+    free(pos);
+    return 0;
+}
+
+static void ilu0_apply(const ilu0_t* f, const double* r, double* z) {
+    int n = f->n;
+    for (int i = 0; i < n; ++i) {
+        double s = r[i];
+        for (int p = f->rowptr[i]; p < f->diag[i]; ++p) s -= f->lu[p] * z[f->col[p]];
+        z[i] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = z[i];
+        for (int p = f->diag[i] + 1; p < f->rowptr[i + 1]; ++p) s -= f->lu[p] * z[f->col[p]];
+        z[i] = s / f->lu[f->diag[i]];
+    }
+}
+
+static double dot_n(int n, const double* a, const double* b) {
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+typedef struct {
+    ilu0_t f;
+    int* rowptr;
+    int* col;
+    double* val;
+} ref_solver_t;
+
+/* SolverFactory::setMatrix + compute (SolverFactory.cpp:45-48;
+ * AMGCLSolver.cpp:59-95): copy the matrix and build the preconditioner once. */
+void* ref_solver_create(int n, const int* rowptr, const int* col, const double* val) {
+    ref_solver_t* s = (ref_solver_t*)calloc(1, sizeof(ref_solver_t));
+    int nnz = rowptr[n];
+    s->rowptr = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+    s->col = (int*)malloc(sizeof(int) * (size_t)nnz);
+    s->val = (double*)malloc(sizeof(double) * (size_t)nnz);
+    memcpy(s->rowptr, rowptr, sizeof(int) * (size_t)(n + 1));
+    memcpy(s->col, col, sizeof(int) * (size_t)nnz);
+    memcpy(s->val, val, sizeof(double) * (size_t)nnz);
+    s->f.n = n;
+    s->f.rowptr = s->rowptr;
+    s->f.col = s->col;
+    s->f.val = s->val;
+    if (ilu0_factor(&s->f) != 0) {
+        free(s->f.lu); free(s->f.diag); free(s->rowptr); free(s->col); free(s->val); free(s);
+        return NULL;
+    }
+    return s;
+}
+
+void ref_solver_destroy(void* h) {
+    ref_solver_t* s = (ref_solver_t*)h;
+    if (!s) return;
+    free(s->f.lu); free(s->f.diag); free(s->rowptr); free(s->col); free(s->val); free(s);
+}
+
+/* SolverFactory::solve for the CPU solvers (AMGCLSolver.cpp:97-103):
+ * x holds the initial guess on entry (warm start) and the solution on exit.
+ * Returns the number of inner iterations (>=0) when ||b - A x|| <= tol ||b||
+ * was reached, -1 when maxit was exhausted first.  *relres receives the
+ * final true relative residual. */
+int ref_solver_solve(void* h, const double* b, double* x, double tol, int maxit, int restart, double* relres) {
+    ref_solver_t* s = (ref_solver_t*)h;
+    const ilu0_t* f = &s->f;
+    int n = f->n;
+    const int *rowptr = s->rowptr, *col = s->col;
+    const double* val = s->val;
+    int m = restart;
+    double* V = (double*)malloc(sizeof(double) * (size_t)n * (m + 1));
+    double* Z = (double*)malloc(sizeof(double) * (size_t)n);
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    double* H = (double*)calloc((size_t)(m + 1) * m, sizeof(double));
+    double *cs = (double*)malloc(sizeof(double) * m), *sn = (double*)malloc(sizeof(double) * m);
+    double* g = (double*)malloc(sizeof(double) * (m + 1));
+    double* yv = (double*)malloc(sizeof(double) * m);
+    double bnorm = sqrt(dot_n(n, b, b));
+    if (bnorm == 0.0) bnorm = 1.0;
+    int it = 0, status = -1;
+    double res = 0.0;
+    for (;;) {
+        /* true residual r = b - A x */
+        ref_spmv(n, rowptr, col, val, x, w);
+        for (int i = 0; i < n; ++i) V[i] = b[i] - w[i];
+        double beta = sqrt(dot_n(n, V, V));
+        res = beta / bnorm;
+        if (res <= tol) { status = it; break; }
+        if (it >= maxit) break;
+        for (int i = 0; i < n; ++i) V[i] /= beta;
+        for (int i = 0; i <= m; ++i) g[i] = 0.0;
+        g[0] = beta;
+        int kdone = 0;
+        for (int j = 0; j < m && it < maxit; ++j) {
+            ilu0_apply(f, V + (size_t)j * n, Z);
+            ref_spmv(n, rowptr, col, val, Z, w);
+            for (int i = 0; i <= j; ++i) { /* modified Gram-Schmidt */
+                const double* vi = V + (size_t)i * n;
+                double hij = dot_n(n, w, vi);
+                H[i * m + j] = hij;
+#pragma omp parallel for schedule(static)
+                for (int q = 0; q < n; ++q) w[q] -= hij * vi[q];
+            }
+            double hn = sqrt(dot_n(n, w, w));
+            H[(j + 1) * m + j] = hn;
+            if (hn != 0.0) {
+                double* vn = V + (size_t)(j + 1) * n;
+                for (int q = 0; q < n; ++q) vn[q] = w[q] / hn;
+            }
+            for (int i = 0; i < j; ++i) { /* apply previous Givens rotations */
+                double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+                H[(i + 1) * m + j] = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+                H[i * m + j] = t;
+            }
+            double a = H[j * m + j], bb = H[(j + 1) * m + j];
+            double r = hypot(a, bb);
+            cs[j] = a / r;
+            sn[j] = bb / r;
+            H[j * m + j] = r;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++it;
+            kdone = j + 1;
+            if (fabs(g[j + 1]) / bnorm <= 0.5 * tol || hn == 0.0) break;
+        }
+        for (int i = kdone - 1; i >= 0; --i) { /* back substitution */
+            double sacc = g[i];
+            for (int q = i + 1; q < kdone; ++q) sacc -= H[i * m + q] * yv[q];
+            yv[i] = sacc / H[i * m + i];
+        }
+        for (int q = 0; q < n; ++q) w[q] = 0.0;
+        for (int i = 0; i < kdone; ++i) {
+            const double* vi = V + (size_t)i * n;
+            for (int q = 0; q < n; ++q) w[q] += yv[i] * vi[q];
+        }
+        ilu0_apply(f, w, Z); /* right preconditioning: x += M^-1 V y */
+        for (int q = 0; q < n; ++q) x[q] += Z[q];
+    }
+    if (relres) *relres = res;
+    free(V); free(Z); free(w); free(H); free(cs); free(sn); free(g); free(yv);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Effective fields                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* EffFieldCalc.cpp:59-63, 97-100 (and TheLLG.cpp:88-93 for the GPU copy):
+ * XC = -2 * diag(A .* invNodeVol) * K;  Hexc = XC * m  (J/m^3 units before
+ * the invJs scaling of TheLLG.cpp:245). */
+void ref_exchange_field(int n, const int* rowptr, const int* col, const double* K, const double* A,
+                        const double* nodevol, const double* m, double* H) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double f = -2.0 * (A[i] * (1.0 / nodevol[i]));
+        double sx = 0, sy = 0, sz = 0;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            double v = f * K[p];
+            int j = col[p];
+            sx += v * m[j];
+            sy += v * m[n + j];
+            sz += v * m[2 * n + j];
+        }
+        H[i] = sx; H[n + i] = sy; H[2 * n + i] = sz;
+    }
+}
+
+/* EffFieldCalc.cpp:71-84 -- Hani_i = 2 Ku_i (k_i . m_i) k_i; axis is 3n SoA. */
+void ref_uniaxial_field(int n, const double* Ku, const double* axis, const double* m, double* H) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double kx = axis[i], ky = axis[n + i], kz = axis[2 * n + i];
+        double d = kx * m[i] + ky * m[n + i] + kz * m[2 * n + i];
+        double f = 2.0 * Ku[i] * d;
+        H[i] = f * kx; H[n + i] = f * ky; H[2 * n + i] = f * kz;
+    }
+}
+
+/* EffFieldCalc.cpp:159-170 and 280-298 -- bulk DMI:
+ *   curl_x = tGy m_z - tGz m_y (cyclic) with tG = -tGrad,
+ *   Hdmi = (S_i (m_i x nv_i) - 2 curl_i) * invNodeVol_i * D_i.
+ * tX,tY,tZ are the tGrad operators as assembled (without the minus sign). */
+void ref_dmi_field(int n, const int* rowptr, const int* col, const double* tX, const double* tY, const double* tZ,
+                   const double* D, const double* nodevol, const double* node_normal_soa,
+                   const double* node_area, const double* m, double* H) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double gxz = 0, gyz = 0, gzy = 0, gxy = 0, gzx = 0, gyx = 0; /* (tG_d m_c) */
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            int j = col[p];
+            double ax = -tX[p], ay = -tY[p], az = -tZ[p];
+            gyz += ay * m[2 * n + j];
+            gzy += az * m[n + j];
+            gzx += az * m[j];
+            gxz += ax * m[2 * n + j];
+            gxy += ax * m[n + j];
+            gyx += ay * m[j];
+        }
+        double curl[3] = {gyz - gzy, gzx - gxz, gxy - gyx};
+        double mi[3] = {m[i], m[n + i], m[2 * n + i]};
+        double nvi[3] = {node_normal_soa[i], node_normal_soa[n + i], node_normal_soa[2 * n + i]};
+        double c[3];
+        v3cross(mi, nvi, c);
+        double f = (1.0 / nodevol[i]) * D[i];
+        for (int d = 0; d < 3; ++d) H[d * n + i] = (node_area[i] * c[d] - 2. * curl[d]) * f;
+    }
+}
+
+/* TheLLG.cpp:442-449, 477-483 -- uniform Zeeman field |B|/mu0 along (theta, phi) in radians */
+void ref_zeeman_field(int n, double Babs, double theta, double phi, double* H) {
+    double hx = sin(theta) * cos(phi), hy = sin(theta) * sin(phi), hz = cos(theta);
+    for (int i = 0; i < n; ++i) {
+        H[i] = hx * (Babs / REF_MU0);
+        H[n + i] = hy * (Babs / REF_MU0);
+        H[2 * n + i] = hz * (Babs / REF_MU0);
+    }
+}
+
+/* TheLLG.cpp:238-254 -- Heff = (Hexc + Hku + Hdmi) * invJs + Hext + Hdem,
+ * invJs = 1/Js with non-finite -> 0 (TheLLG.cpp:110).  Null pointers are
+ * treated as zero fields. */
+void ref_total_field(int n, const double* Js, const double* Hexc, const double* Hku, const double* Hdmi,
+                     const double* Hext, const double* Hdem, double* Heff) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double inv = 1.0 / Js[i];
+        if (!isfinite(inv)) inv = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            size_t q = (size_t)d * n + i;
+            double a = (Hexc ? Hexc[q] : 0.0) + (Hku ? Hku[q] : 0.0) + (Hdmi ? Hdmi[q] : 0.0);
+            Heff[q] = a * inv + (Hext ? Hext[q] : 0.0) + (Hdem ? Hdem[q] : 0.0);
+        }
+    }
+}
+
+/* TheLLG.cpp:165-175 (classic, variant 1) and 191-196 (no precession,
+ * variant 0): dm/dtau = -m x H - alpha m x (m x H)  or  -alpha m x (m x H). */
+void ref_llg_rhs(int n, double alpha, int variant, const double* m, const double* H, double* dmdt) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double mi[3] = {m[i], m[n + i], m[2 * n + i]};
+        double h[3] = {H[i], H[n + i], H[2 * n + i]};
+        double mxh[3], mxmxh[3];
+        v3cross(mi, h, mxh);
+        v3cross(mi, mxh, mxmxh);
+        for (int d = 0; d < 3; ++d)
+            dmdt[d * n + i] = (variant == 0) ? -alpha * mxmxh[d] : -mxh[d] - alpha * mxmxh[d];
+    }
+}
+
+/* TheLLG.cpp:426-439 -- max_i |m_i x H_i| (the caller multiplies by mu0). */
+double ref_max_torque(int n, const double* m, const double* H) {
+    double mx = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double mi[3] = {m[i], m[n + i], m[2 * n + i]};
+        double h[3] = {H[i], H[n + i], H[2 * n + i]};
+        double c[3];
+        v3cross(mi, h, c);
+        double t = v3norm(c);
+        if (t > mx) mx = t;
+    }
+    return mx;
+}
+
+/* EffFieldGPU.cu:625-663 -- m_i <- m_i / |m_i| */
+void ref_normalize(int n, double* m) {
+    for (int i = 0; i < n; ++i) {
+        double l = sqrt(m[i] * m[i] + m[n + i] * m[n + i] + m[2 * n + i] * m[2 * n + i]);
+        m[i] /= l; m[n + i] /= l; m[2 * n + i] /= l;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Demag (FEM-BEM)                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* DemagField.cpp:124-136 -- b = (tGx (mx.*Js) + tGy (my.*Js) + tGz (mz.*Js)) / mu0 */
+void ref_div_m(int n, const int* rowptr, const int* col, const double* tX, const double* tY, const double* tZ,
+               const double* Js, const double* m, double* b) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double sx = 0, sy = 0, sz = 0;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            int j = col[p];
+            sx += tX[p] * (m[j] * Js[j]);
+            sy += tY[p] * (m[n + j] * Js[j]);
+            sz += tZ[p] * (m[2 * n + j] * Js[j]);
+        }
+        b[i] = (sx + sy + sz) / REF_MU0;
+    }
+}
+
+/* DemagField.cpp:102-113 -- Hdem = -(gradX u, gradY u, gradZ u) */
+void ref_grad_field(int n, const int* rowptr, const int* col, const double* gX, const double* gY,
+                    const double* gZ, const double* u, double* H) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double sx = 0, sy = 0, sz = 0;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            double uj = u[col[p]];
+            sx += gX[p] * uj;
+            sy += gY[p] * uj;
+            sz += gZ[p] * uj;
+        }
+        H[i] = -sx; H[n + i] = -sy; H[2 * n + i] = -sz;
+    }
+}
+
+/* DemagField.cpp:184-201 -- gather u1 on the boundary, dense B product,
+ * scatter into an n-vector that is zero elsewhere. */
+void ref_boundary_integral(int n, int bnx, const int* bnodes, const double* B, const double* u1, double* g) {
+    double* ub = (double*)malloc(sizeof(double) * (size_t)bnx);
+    double* gb = (double*)malloc(sizeof(double) * (size_t)bnx);
+    for (int i = 0; i < bnx; ++i) ub[i] = u1[bnodes[i]];
+    ref_dense_gemv(bnx, B, ub, gb);
+    for (int i = 0; i < n; ++i) g[i] = 0.0;
+    for (int i = 0; i < bnx; ++i) g[bnodes[i]] = gb[i];
+    free(ub);
+    free(gb);
+}
+
+/* DemagField.cpp:97-99 -- E = -1/2 sum Js V Hdem . m */
+double ref_demag_energy(int n, const double* Js, const double* nodevol, const double* H, const double* m) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+        s += Js[i] * nodevol[i] * (H[i] * m[i] + H[n + i] * m[n + i] + H[2 * n + i] * m[2 * n + i]);
+    return -0.5 * s;
+}
+
+/* EffFieldCalc.cpp:65-69 (exchange) and 172-175 (DMI): -1/2 sum V H . m */
+double ref_field_energy(int n, const double* nodevol, const double* H, const double* m) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+        s += nodevol[i] * (H[i] * m[i] + H[n + i] * m[n + i] + H[2 * n + i] * m[2 * n + i]);
+    return -0.5 * s;
+}
+
+/* EffFieldCalc.cpp:86-95 -- sum V Ku - sum Ku V (k.m)^2 */
+double ref_uniaxial_energy(int n, const double* Ku, const double* axis, const double* nodevol, const double* m) {
+    double off = 0.0, s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        off += nodevol[i] * Ku[i];
+        double d = axis[i] * m[i] + axis[n + i] * m[n + i] + axis[2 * n + i] * m[2 * n + i];
+        s += Ku[i] * nodevol[i] * d * d;
+    }
+    return off - s;
+}
+
+/* TheLLG.cpp:345-348 -- E = -sum Js V Hext . m */
+double ref_zeeman_energy(int n, const double* Js, const double* nodevol, const double* Hext, const double* m) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+        s += Js[i] * nodevol[i] * (Hext[i] * m[i] + Hext[n + i] * m[n + i] + Hext[2 * n + i] * m[2 * n + i]);
+    return -s;
+}
+
+int ref_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void ref_set_num_threads(int t) {
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}

@@ -1,0 +1,186 @@
+"""Device-resident field context: the composition TheLLG + DemagField run on the GPU.
+
+:class:`FieldContext` owns one ``tm_ctx`` (include/tetmag_hip.h) holding the
+operators, material data, the dense boundary-element matrix, both demag
+solvers and the magnetisation, and exposes the hot path:
+
+* :meth:`FieldContext.eval_rhs` -- one effective-field + LLG right-hand-side
+  evaluation with the demag field recomputed (TheLLG::rhs_d ->
+  classicVersion_GPU -> evaluateAllEffectiveFields -> totalEffectiveField ->
+  ClassicLLG_dev, LLGWrapper.cu:78-82, 213-228, TheLLG.cpp:238-262).
+
+Device buffers are torch tensors on ``cuda:0``; the context runs on a torch
+stream so ordering with torch work is by stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .problem import Problem
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class FieldContext:
+    def __init__(self, prob: Problem, *, tol: float = 1e-10, maxit: int = 100000, device: int = 0,
+                 B: np.ndarray | None = None, stream=None, bem_formula: int = N.TM_BEM_REFERENCE):
+        torch = _torch()
+        L = N.lib()
+        self.L = L
+        self.prob = prob
+        self.n = prob.n
+        chosen = C.c_int()
+        N.check(L.tm_set_device(device, C.byref(chosen)), "tm_set_device")
+        self.device = torch.device("cuda", chosen.value)
+        self.stream = stream if stream is not None else torch.cuda.Stream(device=self.device)
+        h = C.c_void_p()
+        N.check(L.tm_ctx_create(self.n, C.byref(h)), "tm_ctx_create")
+        self.h = h
+        N.check(L.tm_ctx_set_stream(self.h, C.c_void_p(self.stream.cuda_stream)), "tm_ctx_set_stream")
+        ops = prob.ops
+        rp, col = N.i32(ops.rowptr), N.i32(ops.col)
+        self._set_op(N.TM_OP_EXCHANGE, rp, col, prob.exchange_values())
+        for o, v in ((N.TM_OP_TGRAD_X, ops.tgx), (N.TM_OP_TGRAD_Y, ops.tgy), (N.TM_OP_TGRAD_Z, ops.tgz),
+                     (N.TM_OP_GRAD_X, ops.gx), (N.TM_OP_GRAD_Y, ops.gy), (N.TM_OP_GRAD_Z, ops.gz)):
+            self._set_op(o, rp, col, v)
+        N.check(L.tm_ctx_set_js(self.h, N.ptr(N.f64(prob.Js))), "tm_ctx_set_js")
+        if prob.use_uniaxial:
+            N.check(L.tm_ctx_set_uniaxial(self.h, N.ptr(N.f64(prob.kaxis)), N.ptr(N.f64(prob.Ku))),
+                    "tm_ctx_set_uniaxial")
+        if prob.use_dmi:
+            # TheLLG.cpp:94-98: curl operators are -tGrad
+            for o, v in ((N.TM_OP_CURL_X, ops.tgx), (N.TM_OP_CURL_Y, ops.tgy), (N.TM_OP_CURL_Z, ops.tgz)):
+                self._set_op(o, rp, col, -v)
+            inv_v = 1.0 / ops.nodevol
+            N.check(L.tm_ctx_set_dmi(self.h, N.ptr(N.f64(prob.D)), N.ptr(N.f64(inv_v)),
+                                     N.ptr(N.f64(prob.bnd.node_normal)), N.ptr(N.f64(prob.bnd.node_area))),
+                    "tm_ctx_set_dmi")
+        hx, hy, hz = (float(v) for v in prob.hext)
+        N.check(L.tm_ctx_set_zeeman(self.h, None, hx, hy, hz), "tm_ctx_set_zeeman")
+        bnd = prob.bnd
+        N.check(L.tm_ctx_set_boundary(self.h, bnd.bnx, N.ptr(N.i32(bnd.bnodes))), "tm_ctx_set_boundary")
+        if B is not None:
+            Bf = np.asfortranarray(B, dtype=np.float64)
+            N.check(L.tm_ctx_set_bem_dense(self.h, Bf.ctypes.data, bnd.bnx, N.TM_HOST), "tm_ctx_set_bem_dense")
+        else:
+            self.assemble_bem(bem_formula)
+        rpN, colN, valN = prob.neumann_matrix()
+        rpD, colD, valD = prob.dirichlet_matrix()
+        N.check(L.tm_ctx_set_demag_solvers(self.h, N.ptr(rpN), N.ptr(colN), N.ptr(valN), N.ptr(rpD), N.ptr(colD),
+                                           N.ptr(valD), N.TM_PRECOND_JACOBI), "tm_ctx_set_demag_solvers")
+        self.set_solver_params(tol, maxit)
+        with torch.cuda.stream(self.stream):
+            self.m = torch.as_tensor(N.f64(prob.m0)).to(self.device)
+            self.dmdt = torch.empty(3 * self.n, dtype=torch.float64, device=self.device)
+            self.heff = torch.empty(3 * self.n, dtype=torch.float64, device=self.device)
+        self.stream.synchronize()
+        self.last_stats = N.EvalStats()
+
+    # -- setup -------------------------------------------------------------------
+    def _set_op(self, op, rp, col, vals):
+        N.check(self.L.tm_ctx_set_op(self.h, op, N.ptr(rp), N.ptr(col), N.ptr(N.f64(vals))), f"tm_ctx_set_op({op})")
+
+    def assemble_bem(self, formula: int = N.TM_BEM_REFERENCE):
+        """Dense double-layer matrix assembled on the GPU (BEMprocessing.cpp:358-411)."""
+        bnd = self.prob.bnd
+        bxyz = N.f64(self.prob.mesh.xyz[bnd.bnodes])
+        faces_b = N.i32(bnd.bindex[bnd.faces])
+        N.check(self.L.tm_ctx_assemble_bem(self.h, bnd.nbel, N.ptr(bxyz), N.ptr(faces_b), N.ptr(N.f64(bnd.face_area)),
+                                           N.ptr(N.f64(bnd.face_normal)), N.ptr(N.f64(bnd.solid_angle)),
+                                           int(formula)),
+                "tm_ctx_assemble_bem")
+
+    def bem_matrix(self) -> np.ndarray:
+        b = self.prob.bnd.bnx
+        B = np.empty((b, b), order="F")
+        N.check(self.L.tm_ctx_get_bem_dense(self.h, B.ctypes.data, N.TM_HOST), "tm_ctx_get_bem_dense")
+        return B
+
+    def set_solver_params(self, tol: float, maxit: int):
+        N.check(self.L.tm_ctx_set_solver_params(self.h, float(tol), int(maxit)), "tm_ctx_set_solver_params")
+
+    def set_mag(self, m):
+        """EffFieldGPU::setMagDev: m is a (3n,) numpy array or device tensor."""
+        torch = _torch()
+        if isinstance(m, np.ndarray):
+            N.check(self.L.tm_ctx_set_mag(self.h, N.ptr(N.f64(m)), N.TM_HOST), "tm_ctx_set_mag")
+            with torch.cuda.stream(self.stream):
+                self.m.copy_(torch.as_tensor(N.f64(m)), non_blocking=False)
+        else:
+            with torch.cuda.stream(self.stream):
+                self.m.copy_(m)
+            N.check(self.L.tm_ctx_set_mag(self.h, N.ptr(self.m), N.TM_DEVICE), "tm_ctx_set_mag")
+
+    # -- hot path ----------------------------------------------------------------
+    def eval_rhs(self, m=None, *, alpha=None, variant=N.TM_LLG_CLASSIC, want_heff=False, stats=True):
+        """One Heff + LLG RHS evaluation on the device.  m: (3n,) device tensor
+        (None = current context magnetisation).  Returns the dm/dtau tensor
+        (owned by the context, overwritten by the next call)."""
+        alpha = self.prob.alpha if alpha is None else alpha
+        mptr = N.ptr(m) if m is not None else None
+        st = self.last_stats if stats else None
+        status = self.L.tm_eval_rhs(self.h, mptr, N.TM_DEVICE, float(alpha), int(variant), N.ptr(self.dmdt),
+                                    N.TM_DEVICE, N.ptr(self.heff) if want_heff else None, N.TM_DEVICE,
+                                    C.byref(st) if st is not None else None)
+        N.check(status, "tm_eval_rhs")
+        return self.dmdt
+
+    def eval_rhs_host(self, m: np.ndarray, alpha=None, variant=N.TM_LLG_CLASSIC):
+        """Same evaluation with host arrays in and out (synchronous)."""
+        alpha = self.prob.alpha if alpha is None else alpha
+        n3 = 3 * self.n
+        dm = np.empty(n3)
+        H = np.empty(n3)
+        st = N.EvalStats()
+        N.check(self.L.tm_eval_rhs(self.h, N.ptr(N.f64(m)), N.TM_HOST, float(alpha), int(variant), N.ptr(dm),
+                                   N.TM_HOST, N.ptr(H), N.TM_HOST, C.byref(st)), "tm_eval_rhs")
+        self.last_stats = st
+        return dm, H, st.as_dict()
+
+    # -- individual pieces (host arrays) -----------------------------------------
+    def _field(self, fn, name):
+        out = np.empty(3 * self.n)
+        N.check(fn(self.h, N.ptr(out), N.TM_HOST), name)
+        return out
+
+    def exchange_field(self):
+        return self._field(self.L.tm_exchange_field, "tm_exchange_field")
+
+    def uniaxial_field(self):
+        return self._field(self.L.tm_uniaxial_field, "tm_uniaxial_field")
+
+    def dmi_field(self):
+        return self._field(self.L.tm_dmi_field, "tm_dmi_field")
+
+    def demag_field(self, m: np.ndarray | None = None):
+        H = np.empty(3 * self.n)
+        st = N.EvalStats()
+        mp = N.ptr(N.f64(m)) if m is not None else None
+        N.check(self.L.tm_demag_field(self.h, mp, N.TM_HOST, N.ptr(H), N.TM_HOST, C.byref(st)), "tm_demag_field")
+        return H, st.as_dict()
+
+    def potentials(self):
+        u1 = np.empty(self.n)
+        u2 = np.empty(self.n)
+        N.check(self.L.tm_demag_last_potentials(self.h, N.ptr(u1), N.ptr(u2), N.TM_HOST), "tm_demag_last_potentials")
+        return u1, u2
+
+    def synchronize(self):
+        N.check(self.L.tm_ctx_synchronize(self.h), "tm_ctx_synchronize")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

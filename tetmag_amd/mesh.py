@@ -1,0 +1,163 @@
+"""Mesh generation and P1/boundary operator preparation (host, C++ via the C ABI).
+
+This is the setup stage of the reference (preproc/FEMprocessing.cpp,
+preproc/Boundary.cpp, preproc/BEMprocessing.cpp): it runs once per mesh on
+the host and produces the operators the GPU path consumes.  The dense
+boundary-element matrix itself is assembled on the GPU by
+``FieldContext.assemble_bem``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+
+@dataclass
+class Mesh:
+    xyz: np.ndarray  # (n, 3) float64, mesh units
+    fel: np.ndarray  # (ntet, 4) int32
+    scale: float = 1e-9  # metres per mesh unit (ProgramSpecs "scale")
+
+    @property
+    def n(self) -> int:
+        return self.xyz.shape[0]
+
+    @property
+    def ntet(self) -> int:
+        return self.fel.shape[0]
+
+
+@dataclass
+class Operators:
+    """P1 operators on one shared CSR pattern (sorted columns)."""
+    rowptr: np.ndarray
+    col: np.ndarray
+    K: np.ndarray     # stiffness  (FEMprocessing.cpp:266-286)
+    gx: np.ndarray    # grad_d     (FEMprocessing.cpp:243-245)
+    gy: np.ndarray
+    gz: np.ndarray
+    tgx: np.ndarray   # tGrad_d    (FEMprocessing.cpp:247-249)
+    tgy: np.ndarray
+    tgz: np.ndarray
+    nodevol: np.ndarray
+    elvol: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+
+@dataclass
+class Boundary:
+    faces: np.ndarray        # (nbel, 3) global node ids, sorted per face
+    face_tet: np.ndarray     # (nbel,)
+    bnodes: np.ndarray       # (bnx,) ascending
+    bindex: np.ndarray       # (n,) boundary index or -1
+    face_area: np.ndarray    # (nbel,)
+    face_normal: np.ndarray  # (nbel, 3) outward unit normals
+    node_area: np.ndarray    # (n,)
+    node_normal: np.ndarray  # (3n,) SoA, zero off the boundary
+    solid_angle: np.ndarray  # (bnx,) interior solid angle
+
+    @property
+    def bnx(self) -> int:
+        return self.bnodes.shape[0]
+
+    @property
+    def nbel(self) -> int:
+        return self.faces.shape[0]
+
+
+def kuhn_cuboid(nx: int, ny: int, nz: int, h: float, jitter: float = 0.0, seed: int = 0,
+                scale: float = 1e-9) -> Mesh:
+    L = N.lib()
+    n = C.c_int()
+    t = C.c_int()
+    N.check(L.tm_mesh_kuhn_cuboid_size(nx, ny, nz, C.byref(n), C.byref(t)), "tm_mesh_kuhn_cuboid_size")
+    xyz = np.empty((n.value, 3), np.float64)
+    fel = np.empty((t.value, 4), np.int32)
+    N.check(L.tm_mesh_kuhn_cuboid(nx, ny, nz, float(h), float(jitter), int(seed), N.ptr(xyz), N.ptr(fel)),
+            "tm_mesh_kuhn_cuboid")
+    return Mesh(xyz, fel, scale)
+
+
+def fem_operators(mesh: Mesh) -> Operators:
+    L = N.lib()
+    n, nt = mesh.n, mesh.ntet
+    xyz = N.f64(mesh.xyz)
+    fel = N.i32(mesh.fel)
+    rowptr = np.empty(n + 1, np.int32)
+    N.check(L.tm_fem_pattern(n, nt, N.ptr(fel), N.ptr(rowptr)), "tm_fem_pattern")
+    nnz = int(rowptr[-1])
+    col = np.empty(nnz, np.int32)
+    arrs = [np.empty(nnz, np.float64) for _ in range(7)]
+    nodevol = np.empty(n, np.float64)
+    elvol = np.empty(nt, np.float64)
+    N.check(L.tm_fem_assemble(n, nt, N.ptr(xyz), N.ptr(fel), N.ptr(rowptr), N.ptr(col),
+                              *[N.ptr(a) for a in arrs], N.ptr(nodevol), N.ptr(elvol)), "tm_fem_assemble")
+    K, gx, gy, gz, tgx, tgy, tgz = arrs
+    return Operators(rowptr, col, K, gx, gy, gz, tgx, tgy, tgz, nodevol, elvol)
+
+
+def boundary(mesh: Mesh) -> Boundary:
+    L = N.lib()
+    n, nt = mesh.n, mesh.ntet
+    xyz = N.f64(mesh.xyz)
+    fel = N.i32(mesh.fel)
+    faces = np.empty((4 * nt, 3), np.int32)
+    face_tet = np.empty(4 * nt, np.int32)
+    nb = C.c_int()
+    N.check(L.tm_boundary_faces(nt, N.ptr(fel), N.ptr(faces), N.ptr(face_tet), C.byref(nb)), "tm_boundary_faces")
+    nbel = nb.value
+    faces = np.ascontiguousarray(faces[:nbel])
+    face_tet = np.ascontiguousarray(face_tet[:nbel])
+    bnodes = np.empty(n, np.int32)
+    bindex = np.empty(n, np.int32)
+    farea = np.empty(nbel, np.float64)
+    fnorm = np.empty((nbel, 3), np.float64)
+    narea = np.empty(n, np.float64)
+    nnorm = np.empty(3 * n, np.float64)
+    bnx = C.c_int()
+    N.check(L.tm_boundary_props(n, N.ptr(xyz), N.ptr(fel), nbel, N.ptr(faces), N.ptr(face_tet), N.ptr(bnodes),
+                                N.ptr(bindex), N.ptr(farea), N.ptr(fnorm), N.ptr(narea), N.ptr(nnorm),
+                                C.byref(bnx)), "tm_boundary_props")
+    bnodes = np.ascontiguousarray(bnodes[:bnx.value])
+    omega = np.empty(bnx.value, np.float64)
+    N.check(L.tm_boundary_solid_angles(n, nt, N.ptr(xyz), N.ptr(fel), N.ptr(bindex), bnx.value, N.ptr(omega)),
+            "tm_boundary_solid_angles")
+    return Boundary(faces, face_tet, bnodes, bindex, farea, fnorm, narea, nnorm, omega)
+
+
+def replace_rows_identity(rowptr, col, val, fixed_mask):
+    """Rows flagged in fixed_mask become identity rows; explicit zeros are
+    dropped (the row replacement + prune(0,0) of FEMprocessing.cpp:329-369 and
+    DemagField.cpp:204-220)."""
+    rowptr = np.asarray(rowptr)
+    n = rowptr.shape[0] - 1
+    row_of = np.repeat(np.arange(n, dtype=np.int32), np.diff(rowptr))
+    keep = (~fixed_mask[row_of]) & (val != 0.0)
+    rows = np.concatenate([row_of[keep], np.nonzero(fixed_mask)[0].astype(np.int32)])
+    cols = np.concatenate([col[keep], np.nonzero(fixed_mask)[0].astype(np.int32)])
+    vals = np.concatenate([val[keep], np.ones(int(fixed_mask.sum()))])
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    return rp, N.i32(cols), N.f64(vals)
+
+
+def init_random_unit(n: int, seed: int) -> np.ndarray:
+    m = np.empty(3 * n, np.float64)
+    N.check(N.lib().tm_init_random_unit(n, int(seed), N.ptr(m)), "tm_init_random_unit")
+    return m
+
+
+def uniform(seed: int, count: int) -> np.ndarray:
+    out = np.empty(count, np.float64)
+    N.check(N.lib().tm_uniform(int(seed), count, N.ptr(out)), "tm_uniform")
+    return out
