@@ -233,6 +233,27 @@ int tm_demag_last_potentials(tm_ctx* c, double* u1, double* u2, int where);
 int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int variant, double* dmdt3n, int dwhere,
                 double* Heff3n, int hwhere, tm_eval_stats* stats);
 
+/* Explicit Euler advance of the context magnetisation with renormalisation:
+ * m <- (m + dtau dm/dtau) / |m + dtau dm/dtau|  (the renormalisation of
+ * EffFieldGPU::NormalizeMag, EffFieldGPU.cu:625-663, fused with the update). */
+int tm_ctx_step_euler(tm_ctx* c, const double* dmdt3n, int where, double dtau);
+
+/* Phase timers of tm_eval_rhs / tm_demag_field, measured with HIP events on the
+ * context stream (the reference's u1 / h2 / u2 / total timers,
+ * DemagField.cpp:155-160).  Milliseconds summed over `count` evaluations. */
+typedef struct tm_phase_times {
+    double div_ms;        /* Poisson right-hand side (k_div) */
+    double neumann_ms;    /* u1 solve */
+    double bem_ms;        /* dense B product + scatter */
+    double dirichlet_ms;  /* u2 solve */
+    double fields_ms;     /* fused Heff + LLG kernel */
+    long long iters_neumann;
+    long long iters_dirichlet;
+    int count;
+} tm_phase_times;
+int tm_ctx_enable_timers(tm_ctx* c, int on);
+int tm_ctx_get_timers(tm_ctx* c, tm_phase_times* out, int reset);
+
 /* ---- host-side setup helpers (no GPU needed) ---------------------------- */
 /* Kuhn-split cuboid: (nx+1)(ny+1)(nz+1) nodes, 6 nx ny nz tets, spacing h,
  * interior nodes jittered uniformly by +-jitter*h per coordinate (seeded). */

@@ -888,7 +888,7 @@ void ref_grad_field(int n, const int* rowptr, const int* col, const double* gX, 
 /* DemagField.cpp:184-201 -- gather u1 on the boundary, dense B product,
  * scatter into an n-vector that is zero elsewhere. */
 void ref_boundary_integral(int n, int bnx, const int* bnodes, const double* B, const double* u1, double* g) {
-    double* ub = (double*)malloc(sizeof(double) * (size_t)bnx);
+    double* ub = (double*)calloc((size_t)bnx, sizeof(double));
     double* gb = (double*)malloc(sizeof(double) * (size_t)bnx);
     for (int i = 0; i < bnx; ++i) ub[i] = u1[bnodes[i]];
     ref_dense_gemv(bnx, B, ub, gb);

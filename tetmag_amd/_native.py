@@ -65,6 +65,22 @@ class EvalStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PhaseTimes(C.Structure):
+    _fields_ = [
+        ("div_ms", C.c_double),
+        ("neumann_ms", C.c_double),
+        ("bem_ms", C.c_double),
+        ("dirichlet_ms", C.c_double),
+        ("fields_ms", C.c_double),
+        ("iters_neumann", C.c_longlong),
+        ("iters_dirichlet", C.c_longlong),
+        ("count", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -124,6 +140,9 @@ _SIGNATURES = {
     "tm_boundary_solid_angles": [_I, _I, _P, _P, _P, _I, _P],
     "tm_init_random_unit": [_I, _U64, _P],
     "tm_uniform": [_U64, _I, _P],
+    "tm_ctx_step_euler": [_P, _P, _I, _D],
+    "tm_ctx_enable_timers": [_P, _I],
+    "tm_ctx_get_timers": [_P, _P, _I],
     "tm_version": [],
 }
 _RESTYPE = {"tm_version": C.c_char_p}

@@ -80,6 +80,7 @@ class FieldContext:
             self.dmdt = torch.empty(3 * self.n, dtype=torch.float64, device=self.device)
             self.heff = torch.empty(3 * self.n, dtype=torch.float64, device=self.device)
         self.stream.synchronize()
+        N.check(L.tm_ctx_set_mag(self.h, N.ptr(N.f64(prob.m0)), N.TM_HOST), "tm_ctx_set_mag")
         self.last_stats = N.EvalStats()
 
     # -- setup -------------------------------------------------------------------
@@ -95,6 +96,16 @@ class FieldContext:
                                            N.ptr(N.f64(bnd.face_normal)), N.ptr(N.f64(bnd.solid_angle)),
                                            int(formula)),
                 "tm_ctx_assemble_bem")
+
+    def bem_matrix_device(self):
+        """Copy of the dense matrix as a (b, b) column-major device tensor (B[:, j] contiguous)."""
+        torch = _torch()
+        b = self.prob.bnd.bnx
+        with torch.cuda.stream(self.stream):
+            T = torch.empty((b, b), dtype=torch.float64, device=self.device)
+        self.stream.synchronize()
+        N.check(self.L.tm_ctx_get_bem_dense(self.h, N.ptr(T), N.TM_DEVICE), "tm_ctx_get_bem_dense")
+        return T.t()
 
     def bem_matrix(self) -> np.ndarray:
         b = self.prob.bnd.bnx
@@ -170,6 +181,25 @@ class FieldContext:
         u2 = np.empty(self.n)
         N.check(self.L.tm_demag_last_potentials(self.h, N.ptr(u1), N.ptr(u2), N.TM_HOST), "tm_demag_last_potentials")
         return u1, u2
+
+    def step_euler(self, dtau: float, dmdt=None):
+        """m <- normalize(m + dtau dm/dtau) on the device (dmdt: device tensor,
+        default the last eval_rhs result)."""
+        d = self.dmdt if dmdt is None else dmdt
+        N.check(self.L.tm_ctx_step_euler(self.h, N.ptr(d), N.TM_DEVICE, float(dtau)), "tm_ctx_step_euler")
+
+    def enable_timers(self, on: bool = True):
+        N.check(self.L.tm_ctx_enable_timers(self.h, int(on)), "tm_ctx_enable_timers")
+
+    def timers(self, reset: bool = False) -> dict:
+        t = N.PhaseTimes()
+        N.check(self.L.tm_ctx_get_timers(self.h, C.byref(t), int(reset)), "tm_ctx_get_timers")
+        return t.as_dict()
+
+    def get_mag(self) -> np.ndarray:
+        out = np.empty(3 * self.n)
+        N.check(self.L.tm_ctx_get_mag(self.h, N.ptr(out), N.TM_HOST), "tm_ctx_get_mag")
+        return out
 
     def synchronize(self):
         N.check(self.L.tm_ctx_synchronize(self.h), "tm_ctx_synchronize")

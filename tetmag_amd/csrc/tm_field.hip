@@ -75,6 +75,10 @@ struct tm_ctx {
     DevBuf<double> rhsN, rhsD, u1, u2;
     double tol = 1e-10;
     int maxit = 100000;
+    // phase timers
+    bool timers = false;
+    hipEvent_t ev[6] = {};
+    tm_phase_times tsum = {};
 };
 
 namespace {
@@ -358,15 +362,30 @@ __global__ __launch_bounds__(kBlock) void k_add(int n, const double* __restrict_
     if (i < n) o[i] += a[i];
 }
 
+// m <- normalize(m + dt dm)
+__global__ __launch_bounds__(kBlock) void k_euler(int n, double* __restrict__ m, const double* __restrict__ dm,
+                                                  double dt) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double x = m[i] + dt * dm[i], y = m[n + i] + dt * dm[n + i], z = m[2 * n + i] + dt * dm[2 * n + i];
+    const double l = sqrt(x * x + y * y + z * z);
+    const double f = l > 0.0 ? 1.0 / l : 1.0;  // a zero vector stays zero (Eigen normalize())
+    m[i] = x * f;
+    m[n + i] = y * f;
+    m[2 * n + i] = z * f;
+}
+
 // normalize functor (EffFieldGPU.cu:625-663)
 __global__ __launch_bounds__(kBlock) void k_normalize(int n, double* __restrict__ m) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const double x = m[i], y = m[n + i], z = m[2 * n + i];
     const double l = sqrt(x * x + y * y + z * z);
-    m[i] = x / l;
-    m[n + i] = y / l;
-    m[2 * n + i] = z / l;
+    if (l > 0.0) {  // Eigen's normalize() leaves a zero vector unchanged
+        m[i] = x / l;
+        m[n + i] = y / l;
+        m[2 * n + i] = z / l;
+    }
 }
 
 struct EvalArgs {
@@ -602,6 +621,8 @@ int tm_ctx_create(int n, tm_ctx** out) {
 int tm_ctx_destroy(tm_ctx* c) {
     if (!c) return TM_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
     return TM_OK;
@@ -963,19 +984,26 @@ int demag_potential_dev(tm_ctx* c, const double* md, tm_eval_stats* stats) {
     if (!c->has_solvers || !c->has_B || !c->has_js) return TM_ERR_NOT_READY;
     TM_CHECK(commit_ops(c));
     if (!c->has_op[TM_OP_TGRAD_X] || !c->has_op[TM_OP_TGRAD_Y] || !c->has_op[TM_OP_TGRAD_Z]) return TM_ERR_NOT_READY;
+    if (c->timers) TM_HIP(hipEventRecord(c->ev[0], c->stream));
     // rhsPoisson (DemagField.cpp:124-136): divM / mu0
     hipLaunchKernelGGL(k_div, dim3(sell_grid(c)), dim3(kBlock), 0, c->stream, view(c->pat), c->opv[TM_OP_TGRAD_X].p,
                        c->opv[TM_OP_TGRAD_Y].p, c->opv[TM_OP_TGRAD_Z].p, c->js.p, md, 1.0 / kMu0, c->rhsN.p);
     TM_HIP(hipGetLastError());
     int itN = 0, itD = 0;
     double rN = 0, rD = 0;
+    if (c->timers) TM_HIP(hipEventRecord(c->ev[1], c->stream));
     int st = c->cgN.solve(c->rhsN.p, c->u1.p, c->tol, c->maxit, c->stream, &itN, &rN);
     if (st != TM_OK && st != TM_ERR_MAXITER) return st;
     int stN = st;
+    if (c->timers) TM_HIP(hipEventRecord(c->ev[2], c->stream));
     // boundaryIntegral (DemagField.cpp:184-201): gather, B product, scatter
     TM_CHECK(bem_gemv(c->bnx, c->bld, c->B.p, c->u1.p, c->bnodes.p, c->bpart.p, nullptr, c->bnodes.p, c->rhsD.p,
                       c->stream));
+    if (c->timers) TM_HIP(hipEventRecord(c->ev[3], c->stream));
     st = c->cgD.solve(c->rhsD.p, c->u2.p, c->tol, c->maxit, c->stream, &itD, &rD);
+    if (c->timers) TM_HIP(hipEventRecord(c->ev[4], c->stream));
+    c->tsum.iters_neumann += itN;
+    c->tsum.iters_dirichlet += itD;
     if (st != TM_OK && st != TM_ERR_MAXITER) return st;
     if (stats) {
         stats->iters_neumann = itN;
@@ -1083,6 +1111,18 @@ int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int vari
     a.counter = c->counter.p;
     a.torque = stats ? c->scal.p : nullptr;
     TM_CHECK(eval_dispatch(a, exch, demag, dmi, sell_grid(c), c->stream));
+    if (c->timers && demag) {
+        TM_HIP(hipEventRecord(c->ev[5], c->stream));
+        TM_HIP(hipEventSynchronize(c->ev[5]));
+        float t[5];
+        for (int k = 0; k < 5; ++k) TM_HIP(hipEventElapsedTime(&t[k], c->ev[k], c->ev[k + 1]));
+        c->tsum.div_ms += t[0];
+        c->tsum.neumann_ms += t[1];
+        c->tsum.bem_ms += t[2];
+        c->tsum.dirichlet_ms += t[3];
+        c->tsum.fields_ms += t[4];
+        c->tsum.count += 1;
+    }
     if (dwhere == TM_HOST) TM_CHECK(copy_out(dmdt, TM_HOST, a.dmdt, (size_t)3 * n, c->stream));
     if (Heff && hwhere == TM_HOST) TM_CHECK(copy_out(Heff, TM_HOST, a.heff, (size_t)3 * n, c->stream));
     if (stats) {
@@ -1093,6 +1133,31 @@ int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int vari
         stats->max_torque = mt;
     }
     return st;
+}
+
+int tm_ctx_step_euler(tm_ctx* c, const double* dm, int where, double dtau) {
+    TM_CHECK(ctx_ok(c));
+    const double* dd = nullptr;
+    TM_CHECK(stage_in(c, dm, where, (size_t)3 * c->n, c->w3b, &dd));
+    hipLaunchKernelGGL(k_euler, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, c->m.p, dd, dtau);
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+int tm_ctx_enable_timers(tm_ctx* c, int on) {
+    TM_CHECK(ctx_ok(c));
+    if (on && !c->ev[0])
+        for (auto& e : c->ev) TM_HIP(hipEventCreate(&e));
+    c->timers = on != 0;
+    return TM_OK;
+}
+
+int tm_ctx_get_timers(tm_ctx* c, tm_phase_times* out, int reset) {
+    TM_CHECK(ctx_ok(c));
+    if (!out) return TM_ERR_BAD_ARG;
+    *out = c->tsum;
+    if (reset) c->tsum = tm_phase_times{};
+    return TM_OK;
 }
 
 }  // extern "C"
