@@ -108,6 +108,7 @@ def main():
     torch.cuda.synchronize()
     log(f"[setup] device context + GPU BEM assembly {time.time() - t0:.1f}s")
     dtau = args.dt_fs * 1e-3 * prob.time_scale()
+    log(f"[setup] AMG levels neumann {ctx.amg_levels(0)} dirichlet {ctx.amg_levels(1)}")
 
     def step():
         ctx.eval_rhs(None)
@@ -158,7 +159,7 @@ def main():
         "data": "synthetic (seeded jittered Kuhn cuboid, vortex initial state)",
         "config": {"workload": prob.name, "nodes": n, "tets": s["tets"], "boundary_nodes": b, "nnz": nnz,
                    "cg_tol": args.tol, "cg_iters_per_eval": [round(it_n, 2), round(it_d, 2)],
-                   "precond": "jacobi", "form": "plain FP64, unbatched", "parallelism": "single GPU"},
+                   "precond": "sa-amg V(1,1) jacobi-smoothed", "form": "plain FP64, unbatched", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_bem_gemv_part+finish (dense B u1, FP64)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None,

@@ -1,0 +1,418 @@
+// tetmag_gpu.hpp -- header-only C++ facades with the reference GPU class
+// interfaces, implemented over the C ABI of tetmag_hip.h.
+//
+// The classes keep the names, method names, argument meaning and return
+// conventions of tetmag's gpu/ classes (EffFieldGPU.h:54-127,
+// GpuPotential.h:50-75, GPUSolver.h:65-87 + SolverFactory.h:42-65,
+// SpMatCUDA.h:49-79, LLGWrapper.h:57-63, SetDevice.h:36-44), so TheLLG,
+// DemagField and TheSimulation compile against them unchanged apart from the
+// type aliases below.  They are templates over the dense and sparse matrix
+// types so that no Eigen is needed to build them:
+//
+//   * Dense  : column-major matrix with Dense(rows, cols), rows(), cols(),
+//              data() (Eigen::MatrixXd fits);
+//   * Sparse : compressed sparse matrix with rows(), cols(), nonZeros(),
+//              outerIndexPtr(), innerIndexPtr(), valuePtr() and a static
+//              IsRowMajor (Eigen::SparseMatrix<double, RowMajor> fits);
+//   * DevVec : device vector with data() and size() (the facade's own
+//              tmg::DeviceVector here, thrust::device_vector in the
+//              reference).
+//
+// In tetmag:   using EffFieldGPU = tmg::EffFieldGPU_T<Eigen::MatrixXd, SpMat>;
+// Standalone:  using EffFieldGPU = tmg::EffFieldGPU_T<tmg::DenseMatrix, tmg::CsrMatrix>;
+//
+// Errors: the reference prints and continues or exits; the facades throw
+// tmg::Error carrying the TM_ERR_* status, except where the reference returns
+// a success flag (SetDevice::set, GPUSolver::compute, wasSuccessful).
+#ifndef TETMAG_GPU_HPP_
+#define TETMAG_GPU_HPP_
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tetmag_hip.h"
+
+namespace tmg {
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int st, const std::string& what) : std::runtime_error(what + " failed with status " + std::to_string(st)), status(st) {}
+};
+
+inline void check(int st, const char* what) {
+    if (st != TM_OK) throw Error(st, what);
+}
+
+// ---- standalone matrix types (used when Eigen is not available) ----------
+struct DenseMatrix {  // column-major
+    int r = 0, c = 0;
+    std::vector<double> v;
+    DenseMatrix() = default;
+    DenseMatrix(int rows_, int cols_) : r(rows_), c(cols_), v((size_t)rows_ * cols_, 0.0) {}
+    int rows() const { return r; }
+    int cols() const { return c; }
+    double* data() { return v.data(); }
+    const double* data() const { return v.data(); }
+    double& operator()(int i, int j) { return v[(size_t)j * r + i]; }
+    double operator()(int i, int j) const { return v[(size_t)j * r + i]; }
+};
+
+struct CsrMatrix {  // compressed row storage
+    static constexpr bool IsRowMajor = true;
+    int r = 0, c = 0;
+    std::vector<int> rp, ci;
+    std::vector<double> val;
+    int rows() const { return r; }
+    int cols() const { return c; }
+    int nonZeros() const { return rp.empty() ? 0 : rp.back(); }
+    const int* outerIndexPtr() const { return rp.data(); }
+    const int* innerIndexPtr() const { return ci.data(); }
+    const double* valuePtr() const { return val.data(); }
+};
+
+// Owning device vector (the thrust::device_vector<double> of the reference).
+class DeviceVector {
+    double* p_ = nullptr;
+    size_t n_ = 0;
+
+   public:
+    DeviceVector() = default;
+    explicit DeviceVector(size_t n) { resize(n); }
+    DeviceVector(const DeviceVector&) = delete;
+    DeviceVector& operator=(const DeviceVector&) = delete;
+    ~DeviceVector() {
+        if (p_) (void)hipFree(p_);
+    }
+    void resize(size_t n) {
+        if (n == n_) return;
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+        if (n && hipMalloc(&p_, n * sizeof(double)) != hipSuccess) throw Error(TM_ERR_HIP, "hipMalloc");
+        n_ = n;
+    }
+    double* data() { return p_; }
+    const double* data() const { return p_; }
+    size_t size() const { return n_; }
+    std::vector<double> host() const {
+        std::vector<double> h(n_);
+        if (n_ && hipMemcpy(h.data(), p_, n_ * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            throw Error(TM_ERR_HIP, "hipMemcpy");
+        return h;
+    }
+};
+
+namespace detail {
+// CSR arrays of a sparse matrix (row-major as is, column-major transposed).
+template <class Sparse>
+struct CsrView {
+    std::vector<int> rp, ci;
+    std::vector<double> val;
+    int rows = 0, cols = 0;
+    explicit CsrView(const Sparse& A) {
+        rows = (int)A.rows();
+        cols = (int)A.cols();
+        const int outer = Sparse::IsRowMajor ? rows : cols;
+        const int* op = A.outerIndexPtr();
+        const int* ip = A.innerIndexPtr();
+        const double* vp = A.valuePtr();
+        if (Sparse::IsRowMajor) {
+            rp.assign(op, op + outer + 1);
+            ci.assign(ip, ip + op[outer]);
+            val.assign(vp, vp + op[outer]);
+        } else {  // CSC -> CSR
+            rp.assign((size_t)rows + 1, 0);
+            for (int q = 0; q < op[outer]; ++q) rp[ip[q] + 1]++;
+            for (int i = 0; i < rows; ++i) rp[i + 1] += rp[i];
+            ci.resize(op[outer]);
+            val.resize(op[outer]);
+            std::vector<int> fill(rp.begin(), rp.end() - 1);
+            for (int j = 0; j < outer; ++j)
+                for (int q = op[j]; q < op[j + 1]; ++q) {
+                    const int p = fill[ip[q]]++;
+                    ci[p] = j;
+                    val[p] = vp[q];
+                }
+        }
+        // sorted columns within rows
+        for (int i = 0; i < rows; ++i) {
+            std::vector<std::pair<int, double>> t;
+            for (int q = rp[i]; q < rp[i + 1]; ++q) t.push_back({ci[q], val[q]});
+            if (std::is_sorted(t.begin(), t.end())) continue;
+            std::sort(t.begin(), t.end());
+            for (size_t k = 0; k < t.size(); ++k) {
+                ci[rp[i] + k] = t[k].first;
+                val[rp[i] + k] = t[k].second;
+            }
+        }
+    }
+};
+
+// Column-major n x 3 dense matrix -> 3n SoA pointer (they are the same layout).
+template <class Dense>
+const double* soa(const Dense& M) {
+    return M.data();
+}
+}  // namespace detail
+
+// ---- SetDevice (SetDevice.h:36-44) ---------------------------------------
+class SetDevice {
+    int ndev = 0;
+    int mydev;
+    bool success = false;
+
+   public:
+    explicit SetDevice(int dev) : mydev(dev) {}
+    // true on success; false when no device exists (the caller falls back to
+    // the CPU path, tetmag.cpp:82-90).  Out-of-range ids select device 0.
+    bool set() {
+        int chosen = 0;
+        const int st = tm_set_device(mydev, &chosen);
+        success = st == TM_OK;
+        if (success) mydev = chosen;
+        tm_device_count(&ndev);
+        return success;
+    }
+    int device() const { return mydev; }
+};
+
+// ---- SpMatCUDA (SpMatCUDA.h:49-79) ---------------------------------------
+template <class Sparse>
+class SpMatCUDA_T {
+    tm_spmat* h_ = nullptr;
+    int rows_ = 0, cols_ = 0;
+    std::vector<double> x_;
+
+   public:
+    const double alpha = 1.0, beta = 0.0;
+    explicit SpMatCUDA_T(const Sparse& A) {
+        detail::CsrView<Sparse> c(A);
+        rows_ = c.rows;
+        cols_ = c.cols;
+        check(tm_spmat_create(rows_, cols_, c.rp.data(), c.ci.data(), c.val.data(), &h_), "tm_spmat_create");
+    }
+    SpMatCUDA_T(const SpMatCUDA_T&) = delete;
+    SpMatCUDA_T& operator=(const SpMatCUDA_T&) = delete;
+    virtual ~SpMatCUDA_T() { tm_spmat_destroy(h_); }
+    void setX(const std::vector<double>& x) { x_ = x; }
+    // y = A x on host data (mvp + mvpResEig)
+    std::vector<double> mvp(const std::vector<double>& x) {
+        std::vector<double> y((size_t)rows_);
+        check(tm_spmat_mvp(h_, x.data(), TM_HOST, y.data(), TM_HOST), "tm_spmat_mvp");
+        return y;
+    }
+    std::vector<double> mvpResEig() { return mvp(x_); }
+    // y = A x on device data (mvp(devVecD) + mvpResDev)
+    void mvp(const DeviceVector& x, DeviceVector& y) {
+        y.resize((size_t)rows_);
+        check(tm_spmat_mvp(h_, x.data(), TM_DEVICE, y.data(), TM_DEVICE), "tm_spmat_mvp");
+    }
+};
+
+// ---- GPUSolver : SolverFactory (GPUSolver.h:65-87, SolverFactory.h:42-65) --
+template <class Sparse>
+class GPUSolver_T {
+    tm_solver* h_ = nullptr;
+    std::unique_ptr<detail::CsrView<Sparse>> A_;
+    std::vector<double> b_, x_;
+    double cgTol_ = 1e-7;  // ProgramSpecs.cpp:70 default
+    std::string preconditioner_ = "amg";
+    int iters_ = 0, status_ = TM_OK;
+    double error_ = NAN;
+
+   public:
+    int maxit = 100000;
+    GPUSolver_T() = default;
+    GPUSolver_T(const GPUSolver_T&) = delete;
+    GPUSolver_T& operator=(const GPUSolver_T&) = delete;
+    virtual ~GPUSolver_T() { tm_solver_destroy(h_); }
+    void setMatrix(const Sparse& A) { A_.reset(new detail::CsrView<Sparse>(A)); }
+    template <class Vec>
+    void setLoadVector(const Vec& b) {
+        b_.assign(b.data(), b.data() + b.size());
+    }
+    void setCGTolerance(double tol) { cgTol_ = tol; }
+    void definePreconditioner(const std::string& p) { preconditioner_ = p; }
+    // GPUSolver::compute (GPUSolver.cu:88-91): false when the matrix cannot be
+    // handled (setup throws in the reference, caught at GPUSolver.cu:74-76).
+    bool compute() {
+        if (!A_) return false;
+        const int pre = (preconditioner_ == "jacobi" || preconditioner_ == "diagonal") ? TM_PRECOND_JACOBI : TM_PRECOND_AMG;
+        tm_solver_destroy(h_);
+        h_ = nullptr;
+        status_ = tm_solver_create(A_->rows, A_->rp.data(), A_->ci.data(), A_->val.data(), pre, &h_);
+        x_.assign((size_t)A_->rows, 0.0);
+        return status_ == TM_OK;
+    }
+    // warm-started from the previous solution, as GPUSolver keeps x_d
+    void solve() {
+        status_ = tm_solver_solve(h_, b_.data(), TM_HOST, x_.data(), TM_HOST, cgTol_, maxit, 2, &iters_, &error_);
+        if (status_ != TM_OK && status_ != TM_ERR_MAXITER && status_ != TM_ERR_NONFINITE) check(status_, "tm_solver_solve");
+    }
+    const std::vector<double>& result() const { return x_; }
+    bool wasSuccessful() const { return std::isfinite(error_) && status_ == TM_OK; }
+    int iterations() const { return iters_; }
+    double error() const { return error_; }
+    int status() const { return status_; }
+};
+
+// ---- EffFieldGPU (EffFieldGPU.h:54-127) ----------------------------------
+template <class Dense, class Sparse>
+class EffFieldGPU_T {
+    int nx_;
+    tm_ctx* h_ = nullptr;
+    Dense field(int (*fn)(tm_ctx*, double*, int), const char* what) {
+        Dense M(nx_, 3);
+        check(fn(h_, M.data(), TM_HOST), what);
+        return M;
+    }
+    void setOp(int op, const Sparse& A) {
+        detail::CsrView<Sparse> c(A);
+        check(tm_ctx_set_op(h_, op, c.rp.data(), c.ci.data(), c.val.data()), "tm_ctx_set_op");
+    }
+    std::vector<double> llg(const Dense& H, double alpha, int variant) {
+        std::vector<double> out((size_t)3 * nx_);
+        check(tm_llg_rhs(h_, detail::soa(H), TM_HOST, alpha, variant, out.data(), TM_HOST), "tm_llg_rhs");
+        return out;
+    }
+
+   public:
+    explicit EffFieldGPU_T(int nx) : nx_(nx) { init(nx); }
+    EffFieldGPU_T(const EffFieldGPU_T&) = delete;
+    EffFieldGPU_T& operator=(const EffFieldGPU_T&) = delete;
+    ~EffFieldGPU_T() { tm_ctx_destroy(h_); }
+    void init(int nx) {
+        if (h_) return;
+        nx_ = nx;
+        check(tm_ctx_create(nx, &h_), "tm_ctx_create");
+    }
+    tm_ctx* handle() { return h_; }
+
+    std::vector<double> ClassicLLG_hst(const Dense& Heff, double alpha) { return llg(Heff, alpha, TM_LLG_CLASSIC); }
+    std::vector<double> LLG_noPrec_hst(const Dense& Heff, double alpha) { return llg(Heff, alpha, TM_LLG_NOPREC); }
+    std::vector<double> STT_term_LLG_hst(const Dense& U, double alpha, double beta) {
+        std::vector<double> out((size_t)3 * nx_);
+        check(tm_stt_llg(h_, detail::soa(U), TM_HOST, alpha, beta, out.data(), TM_HOST), "tm_stt_llg");
+        return out;
+    }
+
+    void setMagDev(const Dense& M) { check(tm_ctx_set_mag(h_, detail::soa(M), TM_HOST), "tm_ctx_set_mag"); }
+    void setMagDev(const std::vector<double>& m) { check(tm_ctx_set_mag(h_, m.data(), TM_HOST), "tm_ctx_set_mag"); }
+    void setMagDev(const DeviceVector& m) {
+        check(tm_ctx_set_mag(h_, m.data(), TM_DEVICE), "tm_ctx_set_mag");
+        check(tm_ctx_synchronize(h_), "tm_ctx_synchronize");
+    }
+    void setExchangeMatOnDev(const Sparse& XC) { setOp(TM_OP_EXCHANGE, XC); }
+    void setGradientMatsOnDev(const Sparse& a, const Sparse& b, const Sparse& c) {
+        setOp(TM_OP_CURL_X, a);
+        setOp(TM_OP_CURL_Y, b);
+        setOp(TM_OP_CURL_Z, c);
+    }
+
+    Dense ExchangeFieldGPU() { return field(tm_exchange_field, "tm_exchange_field"); }
+    Dense UniaxialAnisotropyField() { return field(tm_uniaxial_field, "tm_uniaxial_field"); }
+    Dense CubicAnisotropyField() { return field(tm_cubic_field, "tm_cubic_field"); }
+
+    template <class Vec>
+    void setSTTDataOnDevice(const Sparse& Gx, const Sparse& Gy, const Sparse& Gz, const Vec& ex, const Vec& ey,
+                            const Vec& ez) {
+        setOp(TM_OP_STT_GRAD_X, Gx);
+        setOp(TM_OP_STT_GRAD_Y, Gy);
+        setOp(TM_OP_STT_GRAD_Z, Gz);
+        check(tm_ctx_set_stt_eta(h_, ex.data(), ey.data(), ez.data()), "tm_ctx_set_stt_eta");
+    }
+    Dense UTermSTT_GPU() { return field(tm_stt_uterm, "tm_stt_uterm"); }
+
+    // ClassicLLG_dev / LLG_noPrec_dev: result in the caller's device vector
+    void ClassicLLG_dev(const Dense& Heff, double alpha, DeviceVector& out) {
+        out.resize((size_t)3 * nx_);
+        check(tm_llg_rhs(h_, detail::soa(Heff), TM_HOST, alpha, TM_LLG_CLASSIC, out.data(), TM_DEVICE), "tm_llg_rhs");
+        check(tm_ctx_synchronize(h_), "tm_ctx_synchronize");
+    }
+    void LLG_noPrec_dev(const Dense& Heff, double alpha, DeviceVector& out) {
+        out.resize((size_t)3 * nx_);
+        check(tm_llg_rhs(h_, detail::soa(Heff), TM_HOST, alpha, TM_LLG_NOPREC, out.data(), TM_DEVICE), "tm_llg_rhs");
+        check(tm_ctx_synchronize(h_), "tm_ctx_synchronize");
+    }
+
+    double MaxTorque(const Dense& Heff) {
+        double t = 0.0;
+        check(tm_max_torque(h_, detail::soa(Heff), TM_HOST, &t), "tm_max_torque");
+        return t;
+    }
+    template <class Vec>
+    void setUniaxialAnisotropy(const Dense& axis, const Vec& Ku) {
+        check(tm_ctx_set_uniaxial(h_, detail::soa(axis), Ku.data()), "tm_ctx_set_uniaxial");
+    }
+    template <class Axes, class Vec>
+    void setCubicAnisotropy(const Axes&, const Vec&, const Vec&) {}  // not implemented on the GPU in the reference
+    template <class Vec>
+    void setDMIdata(const Vec& D, const Vec& invNodeVol, const Dense& nv, const Vec& nodeArea) {
+        check(tm_ctx_set_dmi(h_, D.data(), invNodeVol.data(), detail::soa(nv), nodeArea.data()), "tm_ctx_set_dmi");
+    }
+    Dense DMIField() { return field(tm_dmi_field, "tm_dmi_field"); }
+    void NormalizeMag(Dense& M, int nx) {
+        (void)nx;
+        check(tm_ctx_normalize_mag(h_, M.data(), TM_HOST), "tm_ctx_normalize_mag");
+    }
+    void displayTimer() {}
+};
+
+// ---- GpuPotential (GpuPotential.h:50-75) ---------------------------------
+template <class Dense, class Sparse>
+class GpuPotential_T {
+    tm_ctx* h_ = nullptr;
+    int nx_ = 0;
+
+   public:
+    GpuPotential_T() = default;
+    GpuPotential_T(const GpuPotential_T&) = delete;
+    GpuPotential_T& operator=(const GpuPotential_T&) = delete;
+    ~GpuPotential_T() { tm_ctx_destroy(h_); }
+    template <class Vec>
+    void setGradientMatrices(const Sparse& tx, const Sparse& ty, const Sparse& tz, const Sparse& gx, const Sparse& gy,
+                             const Sparse& gz, const Vec& Js) {
+        nx_ = (int)gx.rows();
+        tm_ctx_destroy(h_);
+        h_ = nullptr;
+        check(tm_ctx_create(nx_, &h_), "tm_ctx_create");
+        const Sparse* ops[6] = {&tx, &ty, &tz, &gx, &gy, &gz};
+        const int ids[6] = {TM_OP_TGRAD_X, TM_OP_TGRAD_Y, TM_OP_TGRAD_Z, TM_OP_GRAD_X, TM_OP_GRAD_Y, TM_OP_GRAD_Z};
+        for (int k = 0; k < 6; ++k) {
+            detail::CsrView<Sparse> c(*ops[k]);
+            check(tm_ctx_set_op(h_, ids[k], c.rp.data(), c.ci.data(), c.val.data()), "tm_ctx_set_op");
+        }
+        check(tm_ctx_set_js(h_, Js.data()), "tm_ctx_set_js");
+    }
+    // sum_d tGrad_d (Js m_d), not divided by mu0 (GpuPotential.cu:86-106)
+    std::vector<double> calcDivM(const Dense& mag) {
+        std::vector<double> b((size_t)nx_);
+        check(tm_div_m(h_, detail::soa(mag), TM_HOST, b.data(), TM_HOST), "tm_div_m");
+        return b;
+    }
+    template <class Vec>
+    Dense calcGradientField(const Vec& u) {
+        Dense H(nx_, 3);
+        check(tm_grad_field(h_, u.data(), TM_HOST, H.data(), TM_HOST), "tm_grad_field");
+        return H;
+    }
+};
+
+// ---- LLGWrapper (LLGWrapper.h:57-63) --------------------------------------
+class LLGWrapper {
+    int nx3 = 0;
+
+   public:
+    void init(int nx) { nx3 = 3 * nx; }  // the scratch lives in the field context
+    int size() const { return nx3; }
+};
+
+}  // namespace tmg
+
+#endif  // TETMAG_GPU_HPP_

@@ -75,7 +75,8 @@ extern "C" {
 #define TM_BEM_ACCURATE 1  /* Van Oosterom-Strackee 2 atan2(.): full precision for distant triangles */
 
 /* ---- preconditioners for tm_solver_create ------------------------------ */
-#define TM_PRECOND_JACOBI 0
+#define TM_PRECOND_JACOBI 0 /* diagonal */
+#define TM_PRECOND_AMG 1    /* smoothed-aggregation AMG V(1,1)-cycle (the reference uses AMGCL SA, GPUSolver.h:53-63) */
 
 /* ---- device selection: SetDevice (SetDevice.h:36-44) ------------------- */
 
@@ -114,6 +115,11 @@ int tm_solver_destroy(tm_solver* s);
  * 100 GMRES iterations silently, GPUSolver.cu:47).  iters/relres may be NULL. */
 int tm_solver_solve(tm_solver* s, const double* b, int bwhere, double* x, int xwhere, double tol, int maxit,
                     int warm, int* iters, double* relres);
+
+/* Preconditioner hierarchy of an AMG solver: *nlevels receives the number of
+ * levels including the dense coarsest one, sizes[0..min(cap,nlevels)-1] their
+ * unknowns (0 levels for a Jacobi solver). */
+int tm_solver_amg_info(tm_solver* s, int* nlevels, int* sizes, int cap);
 
 /* ---- field context: EffFieldGPU + GpuPotential + DemagField state ------- */
 typedef struct tm_ctx tm_ctx;
@@ -211,6 +217,8 @@ int tm_bem_apply(tm_ctx* c, const double* ub, int uwhere, double* gb, int gwhere
  * (stiffness with boundary rows replaced by the identity), CSR host arrays. */
 int tm_ctx_set_demag_solvers(tm_ctx* c, const int* rpN, const int* colN, const double* valN, const int* rpD,
                              const int* colD, const double* valD, int precond);
+/* AMG hierarchy of the context's Neumann (which = 0) or Dirichlet (1) solver. */
+int tm_ctx_solver_amg_info(tm_ctx* c, int which, int* nlevels, int* sizes, int cap);
 /* Solver tolerance and cap (ProgramSpecs.cpp:70 "CG tolerance"). */
 int tm_ctx_set_solver_params(tm_ctx* c, double tol, int maxit);
 /* DemagField::calcPotential (DemagField.cpp:163-181): u = u1 + u2 for the
@@ -253,6 +261,43 @@ typedef struct tm_phase_times {
 } tm_phase_times;
 int tm_ctx_enable_timers(tm_ctx* c, int on);
 int tm_ctx_get_timers(tm_ctx* c, tm_phase_times* out, int reset);
+
+/* ---- batched evaluation: up to 64 magnetisation samples on one mesh ------ */
+/* No reference counterpart as one call: the reference evaluates one sample at
+ * a time (TheLLG::rhs_d, LLGWrapper.cu:213-228) and its hysteresis driver
+ * loops over field values (Hysteresis.cpp:43-169).  Samples share the
+ * context's operators, boundary matrix and AMG hierarchies; arrays are
+ * [3][n][S] (component, node, sample) for fields and [n][S] for potentials. */
+#define TM_BATCH_MAX 64
+#define TM_PREC_FP64 0  /* all-FP64 batched solves */
+#define TM_PREC_MIXED 1 /* FP32 inner CG iterations, FP64 iterative refinement to the tolerance */
+
+typedef struct tm_batch tm_batch;
+typedef struct tm_batch_stats {
+    int nsamples;
+    int outer_neumann, outer_dirichlet;  /* refinement steps */
+    int inner_neumann, inner_dirichlet;  /* inner CG iterations (all samples advance together) */
+    double max_relres_neumann;           /* worst final FP64 relative residual over the samples */
+    double max_relres_dirichlet;
+    double max_torque[TM_BATCH_MAX];     /* per sample, A/m */
+} tm_batch_stats;
+
+/* Needs a context with operators, boundary matrix and AMG demag solvers. */
+int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out);
+int tm_batch_destroy(tm_batch* b);
+/* Per-sample uniform applied field (A/m), [3][S]; default = the context's. */
+int tm_batch_set_zeeman(tm_batch* b, const double* h3s);
+int tm_batch_set_mag(tm_batch* b, const double* m, int where);
+int tm_batch_get_mag(tm_batch* b, double* m, int where);
+/* One Heff + LLG RHS evaluation of every sample (demag recomputed; solves
+ * warm-started per sample).  m == NULL uses the batch magnetisation. */
+int tm_batch_eval_rhs(tm_batch* b, const double* m, int mwhere, double alpha, int variant, double* dmdt, int dwhere,
+                      double* heff, int hwhere, tm_batch_stats* stats);
+int tm_batch_get_potentials(tm_batch* b, double* u1, double* u2, int where);
+/* m <- normalize(m + dtau dm/dtau) for every sample. */
+int tm_batch_step_euler(tm_batch* b, const double* dmdt, int where, double dtau);
+int tm_batch_enable_timers(tm_batch* b, int on);
+int tm_batch_get_timers(tm_batch* b, tm_phase_times* out, int reset);
 
 /* ---- host-side setup helpers (no GPU needed) ---------------------------- */
 /* Kuhn-split cuboid: (nx+1)(ny+1)(nz+1) nodes, 6 nx ny nz tets, spacing h,

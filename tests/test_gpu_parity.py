@@ -28,19 +28,25 @@ def _case(name):
                                B_ext=(0.01, -0.02, 0.015))
     if name == "ragged":  # n not a multiple of 64, thin in z, jittered
         return P.small_problem(nx=9, ny=3, nz=1, h=2.5, jitter=0.15, seed=5, Ku=1e5, D=-2e-3)
+    if name == "mid":  # two AMG levels above the dense coarse grid
+        return P.small_problem(nx=36, ny=30, nz=8, h=3.0, jitter=0.1, seed=21, Ku=5e4, D=0.0, init="vortex")
     raise KeyError(name)
 
 
-CASES = ["cfg0", "small_ku_dmi_zeeman", "ragged"]
+CASES = [("cfg0", "amg"), ("cfg0", "jacobi"), ("small_ku_dmi_zeeman", "amg"), ("ragged", "amg"),
+         ("mid", "amg")]
 
 
-@pytest.fixture(scope="module", params=CASES)
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: f"{c[0]}-{c[1]}")
 def pair(request, oracle_mod, gpu_available):
     from tetmag_amd.context import FieldContext
-    print(f"\n[case] {request.param}", flush=True)
-    prob = _case(request.param)
+    from tetmag_amd import _native as N
+    name, pre = request.param
+    print(f"\n[case] {name} precond={pre}", flush=True)
+    prob = _case(name)
     orc = oracle_mod.Oracle(prob.mesh.xyz, prob.mesh.fel)
-    ctx = FieldContext(prob, tol=GPU_CG_TOL)
+    ctx = FieldContext(prob, tol=GPU_CG_TOL,
+                       precond=N.TM_PRECOND_AMG if pre == "amg" else N.TM_PRECOND_JACOBI)
     yield prob, orc, ctx
     ctx.close()
 

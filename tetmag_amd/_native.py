@@ -38,6 +38,10 @@ TM_OP_STT_GRAD_X, TM_OP_STT_GRAD_Y, TM_OP_STT_GRAD_Z = 10, 11, 12
 TM_LLG_NOPREC = 0
 TM_LLG_CLASSIC = 1
 TM_PRECOND_JACOBI = 0
+TM_PRECOND_AMG = 1
+TM_PREC_FP64 = 0
+TM_PREC_MIXED = 1
+TM_BATCH_MAX = 64
 TM_BEM_REFERENCE = 0
 TM_BEM_ACCURATE = 1
 
@@ -79,6 +83,24 @@ class PhaseTimes(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("nsamples", C.c_int),
+        ("outer_neumann", C.c_int),
+        ("outer_dirichlet", C.c_int),
+        ("inner_neumann", C.c_int),
+        ("inner_dirichlet", C.c_int),
+        ("max_relres_neumann", C.c_double),
+        ("max_relres_dirichlet", C.c_double),
+        ("max_torque", C.c_double * 64),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "max_torque"}
+        d["max_torque"] = list(self.max_torque[:self.nsamples])
+        return d
 
 
 _P = C.c_void_p
@@ -143,6 +165,18 @@ _SIGNATURES = {
     "tm_ctx_step_euler": [_P, _P, _I, _D],
     "tm_ctx_enable_timers": [_P, _I],
     "tm_ctx_get_timers": [_P, _P, _I],
+    "tm_solver_amg_info": [_P, _P, _P, _I],
+    "tm_ctx_solver_amg_info": [_P, _I, _P, _P, _I],
+    "tm_batch_create": [_P, _I, _I, _PP],
+    "tm_batch_destroy": [_P],
+    "tm_batch_set_zeeman": [_P, _P],
+    "tm_batch_set_mag": [_P, _P, _I],
+    "tm_batch_get_mag": [_P, _P, _I],
+    "tm_batch_eval_rhs": [_P, _P, _I, _D, _I, _P, _I, _P, _I, _P],
+    "tm_batch_get_potentials": [_P, _P, _P, _I],
+    "tm_batch_step_euler": [_P, _P, _I, _D],
+    "tm_batch_enable_timers": [_P, _I],
+    "tm_batch_get_timers": [_P, _P, _I],
     "tm_version": [],
 }
 _RESTYPE = {"tm_version": C.c_char_p}
@@ -157,6 +191,13 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    # torch bundles its own libamdhip64.so under the same soname as the ROCm
+    # runtime this library links; whichever loads first serves the whole
+    # process.  Loading torch first keeps one runtime that both can use.
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - torch is optional for pure C-ABI use
+        pass
     if not os.path.exists(LIB_PATH):
         raise NativeLibraryError(
             f"{LIB_PATH} not found: build it with `python -m tetmag_amd.build` (no CPU fallback exists)")

@@ -29,8 +29,10 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", f"-I{IN
 SOURCES = [
     "tm_sparse.hip",
     "tm_cg.hip",
+    "tm_amg.hip",
     "tm_bem.hip",
     "tm_field.hip",
+    "tm_batch.hip",
     "tm_setup.cpp",
 ]
 
@@ -65,7 +67,7 @@ def build(verbose: bool = False) -> str:
                 print(o, out)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fopenmp", "-o", tmp, *objs,
-          "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/llvm/lib"])
+          "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/llvm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

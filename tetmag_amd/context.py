@@ -29,7 +29,8 @@ def _torch():
 
 class FieldContext:
     def __init__(self, prob: Problem, *, tol: float = 1e-10, maxit: int = 100000, device: int = 0,
-                 B: np.ndarray | None = None, stream=None, bem_formula: int = N.TM_BEM_REFERENCE):
+                 B: np.ndarray | None = None, stream=None, bem_formula: int = N.TM_BEM_REFERENCE,
+                 precond: int = N.TM_PRECOND_AMG):
         torch = _torch()
         L = N.lib()
         self.L = L
@@ -73,7 +74,7 @@ class FieldContext:
         rpN, colN, valN = prob.neumann_matrix()
         rpD, colD, valD = prob.dirichlet_matrix()
         N.check(L.tm_ctx_set_demag_solvers(self.h, N.ptr(rpN), N.ptr(colN), N.ptr(valN), N.ptr(rpD), N.ptr(colD),
-                                           N.ptr(valD), N.TM_PRECOND_JACOBI), "tm_ctx_set_demag_solvers")
+                                           N.ptr(valD), int(precond)), "tm_ctx_set_demag_solvers")
         self.set_solver_params(tol, maxit)
         with torch.cuda.stream(self.stream):
             self.m = torch.as_tensor(N.f64(prob.m0)).to(self.device)
@@ -195,6 +196,13 @@ class FieldContext:
         t = N.PhaseTimes()
         N.check(self.L.tm_ctx_get_timers(self.h, C.byref(t), int(reset)), "tm_ctx_get_timers")
         return t.as_dict()
+
+    def amg_levels(self, which: int) -> list:
+        """Unknowns per AMG level of the Neumann (0) or Dirichlet (1) solver."""
+        nl = C.c_int()
+        sizes = np.zeros(16, np.int32)
+        N.check(self.L.tm_ctx_solver_amg_info(self.h, which, C.byref(nl), N.ptr(sizes), 16), "tm_ctx_solver_amg_info")
+        return sizes[:nl.value].tolist()
 
     def get_mag(self) -> np.ndarray:
         out = np.empty(3 * self.n)

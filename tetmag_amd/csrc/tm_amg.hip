@@ -1,0 +1,552 @@
+// tm_amg.hip -- smoothed-aggregation algebraic multigrid preconditioner.
+//
+// The reference preconditions both demag solves with AMGCL's smoothed
+// aggregation (relaxation parameter 0.75, ILU0 smoothing; GPUSolver.cu:43-48,
+// AMGCLSolver.cpp:39-52).  AMGCL is not vendored, so this is a fresh
+// implementation of the published method (Vanek, Mandel, Brezina, Computing
+// 56 (1996) 179-196):
+//   * strength of connection |a_ij| >= theta sqrt(a_ii a_jj), theta = 0.08;
+//   * greedy aggregation (root nodes with unaggregated strong neighbourhoods,
+//     then attachment of the remaining nodes, then leftover aggregates);
+//   * tentative prolongator = aggregate indicator (constant near-null space),
+//     smoothed once by damped Jacobi: P = (I - w D^-1 A) P_tent,
+//     w = (4/3) / rho(D^-1 A) from a power-iteration estimate;
+//   * Galerkin coarse operators A_c = P^T A P until <= kCoarseMax unknowns,
+//     whose dense inverse is applied directly.
+// Identity rows of the solver's matrix (fixed values) are left out of the
+// hierarchy; their residual is zero at every CG iteration.
+//
+// On the device one V(1,1)-cycle with damped-Jacobi smoothing (same weight
+// before and after, so the preconditioner is symmetric) is, per level:
+//   k_amg_pre      x = w D^-1 b and r = b - A x fused (x gathered on the fly)
+//   k_amg_spmv     b_c = P^T r
+//   ...            coarser levels / dense coarse solve
+//   k_amg_prolong  x += P x_c
+//   k_amg_post     x' = x + w D^-1 (b - A x)  (+ (r, z) partial on level 0)
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+#include <vector>
+
+#include "tm_internal.hpp"
+
+namespace tmh {
+
+namespace {
+
+struct Csr {
+    int n = 0, m = 0;  // rows, cols
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+};
+
+void sort_rows(Csr& A) {
+    std::vector<std::pair<int, double>> tmp;
+    for (int i = 0; i < A.n; ++i) {
+        tmp.clear();
+        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) tmp.push_back({A.ci[q], A.v[q]});
+        std::sort(tmp.begin(), tmp.end());
+        for (size_t k = 0; k < tmp.size(); ++k) {
+            A.ci[A.rp[i] + k] = tmp[k].first;
+            A.v[A.rp[i] + k] = tmp[k].second;
+        }
+    }
+}
+
+// C = A * B (Gustavson, dense accumulator), sorted rows.
+Csr spgemm(const Csr& A, const Csr& B) {
+    Csr C;
+    C.n = A.n;
+    C.m = B.m;
+    C.rp.assign((size_t)A.n + 1, 0);
+    std::vector<double> acc((size_t)B.m, 0.0);
+    std::vector<int> mark((size_t)B.m, -1);
+    std::vector<int> touched;
+    for (int i = 0; i < A.n; ++i) {
+        touched.clear();
+        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) {
+            const int j = A.ci[q];
+            const double a = A.v[q];
+            for (int t = B.rp[j]; t < B.rp[j + 1]; ++t) {
+                const int k = B.ci[t];
+                if (mark[k] != i) {
+                    mark[k] = i;
+                    acc[k] = 0.0;
+                    touched.push_back(k);
+                }
+                acc[k] += a * B.v[t];
+            }
+        }
+        std::sort(touched.begin(), touched.end());
+        for (int k : touched) {
+            C.ci.push_back(k);
+            C.v.push_back(acc[k]);
+        }
+        C.rp[i + 1] = (int)C.ci.size();
+    }
+    return C;
+}
+
+Csr transpose(const Csr& A) {
+    Csr T;
+    T.n = A.m;
+    T.m = A.n;
+    T.rp.assign((size_t)T.n + 1, 0);
+    for (int q = 0; q < A.rp[A.n]; ++q) T.rp[A.ci[q] + 1]++;
+    for (int i = 0; i < T.n; ++i) T.rp[i + 1] += T.rp[i];
+    T.ci.resize(A.rp[A.n]);
+    T.v.resize(A.rp[A.n]);
+    std::vector<int> fill(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < A.n; ++i)
+        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) {
+            const int p = fill[A.ci[q]]++;
+            T.ci[p] = i;
+            T.v[p] = A.v[q];
+        }
+    return T;  // rows come out sorted because i increases
+}
+
+std::vector<double> diagonal(const Csr& A) {
+    std::vector<double> d((size_t)A.n, 0.0);
+    for (int i = 0; i < A.n; ++i)
+        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q)
+            if (A.ci[q] == i) d[i] = A.v[q];
+    return d;
+}
+
+// rho(D^-1 A) by power iteration from a fixed pseudo-random start.
+double spectral_radius(const Csr& A, const std::vector<double>& d) {
+    const int n = A.n;
+    std::vector<double> x((size_t)n), y((size_t)n);
+    uint64_t s = 0x9E3779B97F4A7C15ULL;
+    for (int i = 0; i < n; ++i) {
+        s ^= s << 13;
+        s ^= s >> 7;
+        s ^= s << 17;
+        x[i] = 0.5 + (double)(s >> 11) * (1.0 / 9007199254740992.0);
+    }
+    double lam = 1.0;
+    for (int it = 0; it < 20; ++it) {
+        double nrm = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double a = 0.0;
+            for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) a += A.v[q] * x[A.ci[q]];
+            y[i] = d[i] != 0.0 ? a / d[i] : 0.0;
+            nrm += y[i] * y[i];
+        }
+        nrm = std::sqrt(nrm);
+        if (nrm == 0.0) return 1.0;
+        double xn = 0.0;
+        for (int i = 0; i < n; ++i) xn += x[i] * x[i];
+        lam = nrm / std::sqrt(xn);
+        for (int i = 0; i < n; ++i) x[i] = y[i] / nrm;
+    }
+    return lam * 1.05;  // safety margin: power iteration underestimates
+}
+
+// Greedy aggregation on the strength graph; returns the number of
+// aggregates, agg[i] = aggregate of i or -1 (isolated / excluded).
+int aggregate(const Csr& A, const std::vector<double>& d, const std::vector<char>& excl, double theta,
+              std::vector<int>& agg) {
+    const int n = A.n;
+    std::vector<int> srp((size_t)n + 1, 0), sci;
+    for (int i = 0; i < n; ++i) {
+        if (!excl[i])
+            for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) {
+                const int j = A.ci[q];
+                if (j == i || excl[j]) continue;
+                if (std::fabs(A.v[q]) >= theta * std::sqrt(std::fabs(d[i] * d[j]))) sci.push_back(j);
+            }
+        srp[i + 1] = (int)sci.size();
+    }
+    agg.assign((size_t)n, -1);
+    int na = 0;
+    // pass 1: roots whose strong neighbourhood is entirely unaggregated
+    for (int i = 0; i < n; ++i) {
+        if (excl[i] || agg[i] >= 0 || srp[i + 1] == srp[i]) continue;
+        bool free_nb = true;
+        for (int q = srp[i]; q < srp[i + 1] && free_nb; ++q) free_nb = agg[sci[q]] < 0;
+        if (!free_nb) continue;
+        agg[i] = na;
+        for (int q = srp[i]; q < srp[i + 1]; ++q) agg[sci[q]] = na;
+        ++na;
+    }
+    // pass 2: attach remaining nodes to a neighbouring aggregate
+    std::vector<int> agg1 = agg;
+    for (int i = 0; i < n; ++i) {
+        if (excl[i] || agg[i] >= 0) continue;
+        for (int q = srp[i]; q < srp[i + 1]; ++q)
+            if (agg1[sci[q]] >= 0) {
+                agg[i] = agg1[sci[q]];
+                break;
+            }
+    }
+    // pass 3: leftovers with strong neighbours form new aggregates
+    for (int i = 0; i < n; ++i) {
+        if (excl[i] || agg[i] >= 0 || srp[i + 1] == srp[i]) continue;
+        agg[i] = na;
+        for (int q = srp[i]; q < srp[i + 1]; ++q)
+            if (agg[sci[q]] < 0) agg[sci[q]] = na;
+        ++na;
+    }
+    // pass 4: nodes without strong connections join the aggregate of their
+    // largest-magnitude aggregated neighbour (else become singletons), so that
+    // P reproduces constants exactly on every non-excluded node.
+    for (int i = 0; i < n; ++i) {
+        if (excl[i] || agg[i] >= 0) continue;
+        int best = -1;
+        double bv = 0.0;
+        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) {
+            const int j = A.ci[q];
+            if (j == i || excl[j] || agg[j] < 0) continue;
+            if (std::fabs(A.v[q]) > bv) {
+                bv = std::fabs(A.v[q]);
+                best = agg[j];
+            }
+        }
+        agg[i] = best >= 0 ? best : na++;
+    }
+    return na;
+}
+
+// In-place Gauss-Jordan inverse with partial pivoting (row-major n x n).
+bool dense_inverse(std::vector<double>& a, int n) {
+    std::vector<double> inv((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)p * n + c])) p = r;
+        if (a[(size_t)p * n + c] == 0.0) return false;
+        if (p != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(a[(size_t)p * n + k], a[(size_t)c * n + k]);
+                std::swap(inv[(size_t)p * n + k], inv[(size_t)c * n + k]);
+            }
+        const double piv = 1.0 / a[(size_t)c * n + c];
+        for (int k = 0; k < n; ++k) {
+            a[(size_t)c * n + k] *= piv;
+            inv[(size_t)c * n + k] *= piv;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = a[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+                inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+            }
+        }
+    }
+    a.swap(inv);
+    return true;
+}
+
+// ---------------------------------------------------------------- kernels --
+
+#define AMG_SLICE_LOOP(A)                                                      \
+    const int lane = threadIdx.x & (kWave - 1);                                \
+    const int nsl = ((A).n + kWave - 1) / kWave;                               \
+    for (int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < nsl; \
+         s += gridDim.x * (kBlock / kWave))
+
+// x = w D^-1 b ;  r = b - A x   (x gathered on the fly as w dinv_j b_j)
+__global__ __launch_bounds__(kBlock) void k_amg_pre(SellView A, const double* __restrict__ val,
+                                                    const double* __restrict__ dinv, const double* __restrict__ b,
+                                                    double w, double* __restrict__ x, double* __restrict__ r) {
+    AMG_SLICE_LOOP(A) {
+        double acc = 0.0;
+        sell_row(A, s, lane, [&](int64_t e, int c) { acc += val[e] * (w * dinv[c] * b[c]); });
+        const int row = s * kWave + lane;
+        if (row < A.n) {
+            const double bi = b[row];
+            x[row] = w * dinv[row] * bi;
+            r[row] = bi - acc;
+        }
+    }
+}
+
+
+
+// xo = x + w D^-1 (b - A x); optionally partial (b, xo) per block into part.
+__global__ __launch_bounds__(kBlock) void k_amg_post(SellView A, const double* __restrict__ val,
+                                                     const double* __restrict__ dinv, const double* __restrict__ b,
+                                                     double w, const double* __restrict__ x,
+                                                     double* __restrict__ xo, double* part) {
+    __shared__ double sp[kBlock / kWave];
+    double dot = 0.0;
+    AMG_SLICE_LOOP(A) {
+        double acc = 0.0;
+        sell_row(A, s, lane, [&](int64_t e, int c) { acc += val[e] * x[c]; });
+        const int row = s * kWave + lane;
+        if (row < A.n) {
+            const double bi = b[row];
+            const double v = x[row] + w * dinv[row] * (bi - acc);
+            xo[row] = v;
+            dot += bi * v;
+        }
+    }
+    if (!part) return;
+    dot = wave_sum(dot);
+    if ((threadIdx.x & (kWave - 1)) == 0) sp[threadIdx.x / kWave] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) t += sp[k];
+        part[blockIdx.x] = t;
+    }
+}
+
+// x = C b for the dense (row-major) coarse inverse; one wavefront per row.
+__global__ __launch_bounds__(kBlock) void k_amg_dense(int nc, const double* __restrict__ C,
+                                                      const double* __restrict__ b, double* __restrict__ x) {
+    const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (row >= nc) return;
+    double acc = 0.0;
+    for (int k = lane; k < nc; k += kWave) acc += C[(size_t)row * nc + k] * b[k];
+    acc = wave_sum(acc);
+    if (lane == 0) x[row] = acc;
+}
+
+// partial (a, b) per block (grid-stride)
+__global__ __launch_bounds__(kBlock) void k_amg_dot(int n, const double* __restrict__ a,
+                                                    const double* __restrict__ b, double* part) {
+    __shared__ double sp[kBlock / kWave];
+    double dot = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) dot += a[i] * b[i];
+    dot = wave_sum(dot);
+    if ((threadIdx.x & (kWave - 1)) == 0) sp[threadIdx.x / kWave] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) t += sp[k];
+        part[blockIdx.x] = t;
+    }
+}
+
+// ---- lanes-per-row CSR kernels (LPR lanes of a wavefront share one row) ----
+template <int LPR>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) v += __shfl_down(v, o, LPR);
+    return v;
+}
+
+// MODE 0: y = A x      MODE 1: y += A x (prolongation)
+// MODE 2: x = w dinv b, r = b - A(w dinv b)   (pre-smoothing + residual)
+// MODE 3: xo = x + w dinv (b - A x)           (post-smoothing)
+template <int LPR, int MODE>
+__global__ __launch_bounds__(kBlock) void k_csrv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                 const double* __restrict__ v, const double* __restrict__ x,
+                                                 const double* __restrict__ dinv, const double* __restrict__ b,
+                                                 double w, double* __restrict__ y, double* __restrict__ y2) {
+    const int row = (blockIdx.x * kBlock + threadIdx.x) / LPR;
+    const int sub = threadIdx.x & (LPR - 1);
+    if (row >= n) return;  // whole groups exit together (LPR divides kBlock)
+    double acc = 0.0;
+    for (int q = rp[row] + sub; q < rp[row + 1]; q += LPR) {
+        const int c = ci[q];
+        acc += MODE == 2 ? v[q] * (w * dinv[c] * b[c]) : v[q] * x[c];
+    }
+    acc = group_sum<LPR>(acc);
+    if (sub != 0) return;
+    if (MODE == 0) y[row] = acc;
+    if (MODE == 1) y[row] += acc;
+    if (MODE == 2) {
+        const double bi = b[row];
+        y[row] = w * dinv[row] * bi;
+        y2[row] = bi - acc;
+    }
+    if (MODE == 3) y2[row] = x[row] + w * dinv[row] * (b[row] - acc);
+}
+
+template <int MODE>
+void csrv_launch(const AmgPrecond::CsrDev& M, const double* x, const double* dinv, const double* b, double w,
+                 double* y, double* y2, hipStream_t s) {
+    const int rows_per_block = kBlock / M.lpr;
+    const dim3 g((M.n + rows_per_block - 1) / rows_per_block);
+    switch (M.lpr) {
+        case 4: hipLaunchKernelGGL((k_csrv<4, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        case 8: hipLaunchKernelGGL((k_csrv<8, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        case 16: hipLaunchKernelGGL((k_csrv<16, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        case 32: hipLaunchKernelGGL((k_csrv<32, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        default: hipLaunchKernelGGL((k_csrv<64, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+    }
+}
+
+int upload_csrv(const Csr& M, AmgPrecond::CsrDev& D, hipStream_t s) {
+    D.n = M.n;
+    const double avg = M.n ? (double)M.rp[M.n] / M.n : 1.0;
+    int lpr = 4;
+    while (lpr < 64 && lpr < avg) lpr *= 2;
+    D.lpr = lpr;
+    TM_CHECK(D.rp.upload(M.rp.data(), M.rp.size(), s));
+    TM_CHECK(D.ci.upload(M.ci.data(), M.ci.size(), s));
+    TM_CHECK(D.v.upload(M.v.data(), M.v.size(), s));
+    TM_HIP(hipStreamSynchronize(s));
+    return TM_OK;
+}
+
+int upload_csr(const Csr& M, SellPattern& P, DevBuf<double>& val, hipStream_t s) {
+    P.build(M.n, M.m, M.rp.data(), M.ci.data());
+    std::vector<double> sv;
+    TM_CHECK(P.scatter(M.rp.data(), M.ci.data(), M.v.data(), sv));
+    TM_CHECK(P.upload(s));
+    TM_CHECK(val.upload(sv.data(), sv.size(), s));
+    TM_HIP(hipStreamSynchronize(s));
+    return TM_OK;
+}
+
+
+}  // namespace
+
+int AmgPrecond::setup(int n, const int* rowptr, const int* colidx, const double* vals,
+                      const std::vector<unsigned char>& fixed, const SellPattern* A0, const double* A0val,
+                      const double* A0dinv, hipStream_t s) {
+    // symmetric operator seen by CG: free-free block, identity on fixed rows
+    Csr A;
+    A.n = A.m = n;
+    A.rp.assign((size_t)n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        if (fixed[i]) {
+            A.ci.push_back(i);
+            A.v.push_back(1.0);
+        } else {
+            for (int q = rowptr[i]; q < rowptr[i + 1]; ++q)
+                if (!fixed[colidx[q]] && vals[q] != 0.0) {
+                    A.ci.push_back(colidx[q]);
+                    A.v.push_back(vals[q]);
+                }
+        }
+        A.rp[i + 1] = (int)A.ci.size();
+    }
+    levels.clear();
+    std::vector<char> excl(fixed.begin(), fixed.end());
+    Csr cur = A;
+    int lev = 0;
+    while (cur.n > kCoarseMax && lev < kMaxLevels) {
+        std::vector<double> d = diagonal(cur);
+        const double rho = spectral_radius(cur, d);
+        const double w = 4.0 / (3.0 * rho);
+        std::vector<int> agg;
+        const int na = aggregate(cur, d, excl, 0.08, agg);
+        if (na == 0 || na >= cur.n) break;
+        // tentative prolongator
+        Csr T;
+        T.n = cur.n;
+        T.m = na;
+        T.rp.assign((size_t)cur.n + 1, 0);
+        for (int i = 0; i < cur.n; ++i) {
+            if (agg[i] >= 0) {
+                T.ci.push_back(agg[i]);
+                T.v.push_back(1.0);
+            }
+            T.rp[i + 1] = (int)T.ci.size();
+        }
+        // P = (I - w D^-1 A) T, excluded rows stay zero
+        Csr AT = spgemm(cur, T);
+        Csr P;
+        P.n = cur.n;
+        P.m = na;
+        P.rp.assign((size_t)cur.n + 1, 0);
+        for (int i = 0; i < cur.n; ++i) {
+            if (!excl[i]) {
+                const double f = d[i] != 0.0 ? w / d[i] : 0.0;
+                int t = T.rp[i];
+                for (int q = AT.rp[i]; q < AT.rp[i + 1]; ++q) {
+                    double v = -f * AT.v[q];
+                    if (t < T.rp[i + 1] && T.ci[t] == AT.ci[q]) v += T.v[t++];
+                    if (v != 0.0) {
+                        P.ci.push_back(AT.ci[q]);
+                        P.v.push_back(v);
+                    }
+                }
+            }
+            P.rp[i + 1] = (int)P.ci.size();
+        }
+        Csr R = transpose(P);
+        Csr Ac = spgemm(R, spgemm(cur, P));
+        Level L;
+        L.n = cur.n;
+        L.omega = w;
+        if (lev == 0 && A0) {
+            L.A_ext = A0;
+            L.Aval_ext = A0val;
+            L.dinv_ext = A0dinv;
+        } else {
+            TM_CHECK(upload_csrv(cur, L.A, s));
+            std::vector<double> di((size_t)cur.n);
+            for (int i = 0; i < cur.n; ++i) di[i] = d[i] != 0.0 ? 1.0 / d[i] : 0.0;
+            TM_CHECK(L.dinv.upload(di.data(), di.size(), s));
+        }
+        TM_CHECK(upload_csrv(P, L.P, s));
+        TM_CHECK(upload_csrv(R, L.R, s));
+        L.nnz_a = cur.rp[cur.n];
+        L.nnz_p = P.rp[P.n];
+        TM_CHECK(L.x.alloc(cur.n));
+        TM_CHECK(L.x2.alloc(cur.n));
+        TM_CHECK(L.r.alloc(cur.n));
+        TM_CHECK(L.b.alloc(na));
+        levels.push_back(std::move(L));
+        cur = std::move(Ac);
+        excl.assign((size_t)cur.n, 0);
+        ++lev;
+    }
+    // dense coarse inverse
+    nc = cur.n;
+    std::vector<double> dense((size_t)nc * nc, 0.0);
+    for (int i = 0; i < nc; ++i)
+        for (int q = cur.rp[i]; q < cur.rp[i + 1]; ++q) dense[(size_t)i * nc + cur.ci[q]] = cur.v[q];
+    if (!dense_inverse(dense, nc)) return TM_ERR_UNSUPPORTED;
+    TM_CHECK(Cinv.upload(dense.data(), dense.size(), s));
+    TM_CHECK(xc.alloc(nc));
+    TM_CHECK(bc.alloc(nc));
+    TM_HIP(hipStreamSynchronize(s));
+    return TM_OK;
+}
+
+// z = M^-1 r (one V-cycle); writes the (r, z) partials of `grid` blocks to part.
+int AmgPrecond::apply(const double* r0, double* z, double* part, int grid, hipStream_t s) {
+    const int L = (int)levels.size();
+    if (L == 0) {
+        hipLaunchKernelGGL(k_amg_dense, dim3((nc + 3) / 4), dim3(kBlock), 0, s, nc, Cinv.p, r0, z);
+        hipLaunchKernelGGL(k_amg_dot, dim3(grid), dim3(kBlock), 0, s, nc, r0, z, part);
+        TM_HIP(hipGetLastError());
+        return TM_OK;
+    }
+    const double* b = r0;
+    for (int l = 0; l < L; ++l) {
+        Level& V = levels[l];
+        if (l == 0) {
+            const SellPattern& A = *V.A_ext;
+            hipLaunchKernelGGL(k_amg_pre, dim3(grid), dim3(kBlock), 0, s, view(A), V.Aval_ext, V.dinv_ext, b, V.omega,
+                               V.x.p, V.r.p);
+        } else {
+            csrv_launch<2>(V.A, nullptr, V.dinv.p, b, V.omega, V.x.p, V.r.p, s);
+        }
+        csrv_launch<0>(V.R, V.r.p, nullptr, nullptr, 0.0, V.b.p, nullptr, s);  // b_{l+1} = P^T r
+        b = V.b.p;
+    }
+    hipLaunchKernelGGL(k_amg_dense, dim3((nc + 3) / 4), dim3(kBlock), 0, s, nc, Cinv.p, levels[L - 1].b.p, xc.p);
+    const double* xcoarse = xc.p;
+    for (int l = L - 1; l >= 0; --l) {
+        Level& V = levels[l];
+        const double* bl = l == 0 ? r0 : levels[l - 1].b.p;
+        csrv_launch<1>(V.P, xcoarse, nullptr, nullptr, 0.0, V.x.p, nullptr, s);  // x += P x_c
+        if (l == 0) {
+            const SellPattern& A = *V.A_ext;
+            hipLaunchKernelGGL(k_amg_post, dim3(grid), dim3(kBlock), 0, s, view(A), V.Aval_ext, V.dinv_ext, bl,
+                               V.omega, V.x.p, z, part);
+            xcoarse = z;
+        } else {
+            csrv_launch<3>(V.A, V.x.p, V.dinv.p, bl, V.omega, nullptr, V.x2.p, s);
+            xcoarse = V.x2.p;
+        }
+    }
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+}  // namespace tmh

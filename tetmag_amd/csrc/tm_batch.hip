@@ -1,0 +1,970 @@
+// tm_batch.hip -- many magnetisation samples on one mesh in one pass.
+//
+// The reference evaluates one magnetisation at a time (TheLLG, DemagField)
+// and its hysteresis driver runs field values one after another
+// (Hysteresis.cpp:43-169).  Here up to 64 samples share every operator read:
+// vectors are stored node-major, sample-fastest ([node][sample]); each sparse
+// product runs one wavefront per row with one lane per sample, so the row's
+// column indices and values are wavefront-uniform loads read once for all
+// samples and every gather is one contiguous access.  The boundary-element
+// product becomes a GEMM (B times the b x S boundary potentials).
+//
+// Both demag solves use the same scheme per sample:
+//   outer (FP64):  x_fixed = b_fixed;  r = b - A x  (FP64 operator and vectors)
+//   inner (T):     A d = r by AMG-preconditioned CG in precision T, from d = 0
+//   x += d, repeat until ||b - A x|| <= tol ||b|| for every sample.
+// T = double solves to the final tolerance in one outer step (the FP64
+// batched form); T = float runs the inner iterations in FP32 to a relative
+// 1e-5 and recovers FP64 accuracy by iterative refinement (the mixed form).
+// Converged samples are frozen (their step lengths are zero).
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "tm_internal.hpp"
+
+namespace tmh {
+
+namespace {
+
+constexpr int kMaxS = 64;
+constexpr int kRowsPerBlock = kBlock / kWave;  // one row per wavefront
+
+struct DCsr {
+    int n = 0;
+    DevBuf<int> rp, ci;
+    DevBuf<double> v;
+    int upload(int rows, const int* rowptr, const int* col, const double* val, hipStream_t s) {
+        n = rows;
+        TM_CHECK(rp.upload(rowptr, (size_t)rows + 1, s));
+        TM_CHECK(ci.upload(col, (size_t)rowptr[rows], s));
+        TM_CHECK(v.upload(val, (size_t)rowptr[rows], s));
+        TM_HIP(hipStreamSynchronize(s));
+        return TM_OK;
+    }
+};
+
+struct CsrArgs {
+    int n;
+    const int* __restrict__ rp;
+    const int* __restrict__ ci;
+    const double* __restrict__ v;
+};
+inline CsrArgs args(const DCsr& M) { return CsrArgs{M.n, M.rp.p, M.ci.p, M.v.p}; }
+inline CsrArgs args(const AmgPrecond::CsrDev& M) { return CsrArgs{M.n, M.rp.p, M.ci.p, M.v.p}; }
+
+// Per-sample CG state of one inner solve.
+struct BState {
+    double gam[kMaxS];    // (r, z)
+    double alpha[kMaxS];
+    double beta[kMaxS];
+    double itol2[kMaxS];  // inner tolerance^2 (absolute, on ||r||^2)
+    double rr[kMaxS];
+    int active[kMaxS];
+    int iters[kMaxS];
+};
+
+// Per-sample outer state.
+struct OState {
+    double bb[kMaxS];
+    double rr[kMaxS];
+    int done[kMaxS];
+};
+
+#define B_ROW_LOOP(NROWS)                                                            \
+    const int lane = threadIdx.x & (kWave - 1);                                      \
+    const bool on = lane < S;                                                        \
+    for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
+         row < (NROWS); row += gridDim.x * kRowsPerBlock)
+
+// Writes the block's per-sample partial sum v (lane = sample) to part[s*G + block].
+__device__ inline void b_partial(double* part, int S, double v) {
+    __shared__ double sp[kRowsPerBlock][kWave];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    sp[wv][lane] = v;
+    __syncthreads();
+    if (wv == 0 && lane < S) {
+        double t = 0.0;
+        for (int k = 0; k < kRowsPerBlock; ++k) t += sp[k][lane];
+        part[(size_t)lane * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
+}
+
+// Sum of part[s*G .. s*G+G) for sample s = blockIdx.x, returned to thread 0.
+__device__ inline double b_total(const double* part, int G) {
+    __shared__ double sr[kBlock / kWave];
+    const int s = blockIdx.x;
+    double t = 0.0;
+    for (int g = threadIdx.x; g < G; g += kBlock) t += part[(size_t)s * G + g];
+    t = wave_sum(t);
+    if ((threadIdx.x & (kWave - 1)) == 0) sr[threadIdx.x / kWave] = t;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kBlock / kWave; ++k) r += sr[k];
+    return r;
+}
+
+// ---------------------------------------------------------------- outer --
+__global__ __launch_bounds__(kBlock) void k_b_fix(int n, int S, const unsigned char* __restrict__ fixed,
+                                                  const double* __restrict__ b, double* __restrict__ x) {
+    const size_t tot = (size_t)n * S;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
+        if (fixed[e / S]) x[e] = b[e];
+}
+
+// r = b - A x (FP64), rT = (T) r, partials of (b,b) and (r,r).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_resid(CsrArgs A, int S, const double* __restrict__ b,
+                                                    const double* __restrict__ x, double* __restrict__ r64,
+                                                    T* __restrict__ rT, double* part_bb, double* part_rr) {
+    double vb = 0.0, vr = 0.0;
+    B_ROW_LOOP(A.n) {
+        double acc = 0.0;
+        for (int q = A.rp[row]; q < A.rp[row + 1]; ++q) {
+            const int c = A.ci[q];
+            if (on) acc += A.v[q] * x[(size_t)c * S + lane];
+        }
+        if (on) {
+            const size_t e = (size_t)row * S + lane;
+            const double bi = b[e], ri = bi - acc;
+            r64[e] = ri;
+            rT[e] = (T)ri;
+            vb += bi * bi;
+            vr += ri * ri;
+        }
+    }
+    b_partial(part_bb, S, vb);
+    b_partial(part_rr, S, vr);
+}
+
+// outer decision per sample; inner tolerance and initial inner state.
+__global__ void k_b_outer_red(int G, int S, const double* part_bb, const double* part_rr, double tol, double itol,
+                              OState* os, BState* bs) {
+    const int s = blockIdx.x;
+    const double bb = b_total(part_bb, G);
+    __syncthreads();
+    const double rr = b_total(part_rr, G);
+    if (threadIdx.x == 0 && s < S) {
+        os->bb[s] = bb;
+        os->rr[s] = rr;
+        const bool done = !(rr > tol * tol * bb) || !std::isfinite(rr);
+        os->done[s] = done ? 1 : 0;
+        // inner target: the final tolerance, or a relative itol for the
+        // refinement steps, whichever is looser
+        const double a2 = tol * tol * bb;
+        const double i2 = itol * itol * rr;
+        bs->itol2[s] = std::max(a2, i2);
+        bs->rr[s] = rr;
+        bs->active[s] = done ? 0 : 1;
+        bs->iters[s] = 0;
+        bs->beta[s] = 0.0;
+        bs->alpha[s] = 0.0;
+        bs->gam[s] = 0.0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_add(size_t tot, const T* __restrict__ d, double* __restrict__ x) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
+        x[e] += (double)d[e];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_zero(size_t tot, T* __restrict__ x) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) x[e] = (T)0;
+}
+
+// ------------------------------------------------------------- inner CG --
+// p = z + beta p_old (p_old not read for k = 0); q = A p; partial (p, q).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(CsrArgs A, int S, const T* __restrict__ z,
+                                                      const T* __restrict__ pold, T* __restrict__ pnew,
+                                                      T* __restrict__ q, int k, const BState* bs, double* part) {
+    const int ln = threadIdx.x & (kWave - 1);
+    const T beta = (k > 0 && ln < S) ? (T)bs->beta[ln] : (T)0;
+    double v = 0.0;
+    B_ROW_LOOP(A.n) {
+        T acc = 0;
+        for (int qq = A.rp[row]; qq < A.rp[row + 1]; ++qq) {
+            const int c = A.ci[qq];
+            if (on) {
+                const size_t e = (size_t)c * S + lane;
+                const T pc = k > 0 ? z[e] + beta * pold[e] : z[e];
+                acc += (T)A.v[qq] * pc;
+            }
+        }
+        if (on) {
+            const size_t e = (size_t)row * S + lane;
+            const T pi = k > 0 ? z[e] + beta * pold[e] : z[e];
+            pnew[e] = pi;
+            q[e] = acc;
+            v += (double)pi * (double)acc;
+        }
+    }
+    b_partial(part, S, v);
+}
+
+__global__ void k_b_red_pq(int G, int S, const double* part, BState* bs) {
+    const int s = blockIdx.x;
+    const double pq = b_total(part, G);
+    if (threadIdx.x == 0 && s < S) {
+        const bool ok = bs->active[s] && pq > 0.0 && std::isfinite(pq);
+        if (bs->active[s] && !ok) bs->active[s] = 0;  // breakdown: freeze the sample
+        bs->alpha[s] = ok ? bs->gam[s] / pq : 0.0;
+    }
+}
+
+// d += alpha p; r -= alpha q; partial (r, r)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_cg_update(int n, int S, const T* __restrict__ p,
+                                                        const T* __restrict__ q, T* __restrict__ d,
+                                                        T* __restrict__ r, const BState* bs, double* part) {
+    const int ln = threadIdx.x & (kWave - 1);
+    const T alpha = ln < S ? (T)bs->alpha[ln] : (T)0;
+    double v = 0.0;
+    B_ROW_LOOP(n) {
+        if (on) {
+            const size_t e = (size_t)row * S + lane;
+            d[e] += alpha * p[e];
+            const T ri = r[e] - alpha * q[e];
+            r[e] = ri;
+            v += (double)ri * (double)ri;
+        }
+    }
+    b_partial(part, S, v);
+}
+
+// after z = M r: gamma, convergence, beta
+__global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* part_rr, int k, BState* bs) {
+    const int s = blockIdx.x;
+    const double rz = b_total(part_rz, G);
+    __syncthreads();
+    const double rr = k >= 0 ? b_total(part_rr, G) : 0.0;
+    if (threadIdx.x == 0 && s < S) {
+        if (k >= 0) bs->rr[s] = rr;  // k < 0: initial call, rr comes from the outer residual
+        const double r2 = bs->rr[s];
+        bool act = bs->active[s] && r2 > bs->itol2[s] && std::isfinite(rz) && std::isfinite(r2);
+        if (k >= 0 && bs->active[s]) bs->iters[s] += 1;
+        bs->beta[s] = (act && k >= 0 && bs->gam[s] != 0.0) ? rz / bs->gam[s] : 0.0;
+        bs->gam[s] = rz;
+        bs->active[s] = act ? 1 : 0;
+    }
+}
+
+// --------------------------------------------------------- AMG V-cycle --
+// x = w dinv b; r = b - A x
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs A, int S, const double* __restrict__ dinv,
+                                                  const T* __restrict__ b, double w, T* __restrict__ x,
+                                                  T* __restrict__ r) {
+    B_ROW_LOOP(A.n) {
+        T acc = 0;
+        for (int q = A.rp[row]; q < A.rp[row + 1]; ++q) {
+            const int c = A.ci[q];
+            if (on) acc += (T)A.v[q] * ((T)(w * dinv[c]) * b[(size_t)c * S + lane]);
+        }
+        if (on) {
+            const size_t e = (size_t)row * S + lane;
+            const T bi = b[e];
+            x[e] = (T)(w * dinv[row]) * bi;
+            r[e] = bi - acc;
+        }
+    }
+}
+
+// y = M x (ACC: y += M x)
+template <typename T, bool ACC>
+__global__ __launch_bounds__(kBlock) void k_b_spmm(CsrArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
+    B_ROW_LOOP(M.n) {
+        T acc = 0;
+        for (int q = M.rp[row]; q < M.rp[row + 1]; ++q) {
+            const int c = M.ci[q];
+            if (on) acc += (T)M.v[q] * x[(size_t)c * S + lane];
+        }
+        if (on) {
+            const size_t e = (size_t)row * S + lane;
+            if (ACC) y[e] += acc;
+            else y[e] = acc;
+        }
+    }
+}
+
+// xo = x + w dinv (b - A x); optional partial (b, xo)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_post(CsrArgs A, int S, const double* __restrict__ dinv,
+                                                   const T* __restrict__ b, double w, const T* __restrict__ x,
+                                                   T* __restrict__ xo, double* part) {
+    double v = 0.0;
+    B_ROW_LOOP(A.n) {
+        T acc = 0;
+        for (int q = A.rp[row]; q < A.rp[row + 1]; ++q) {
+            const int c = A.ci[q];
+            if (on) acc += (T)A.v[q] * x[(size_t)c * S + lane];
+        }
+        if (on) {
+            const size_t e = (size_t)row * S + lane;
+            const T bi = b[e];
+            const T zi = x[e] + (T)(w * dinv[row]) * (bi - acc);
+            xo[e] = zi;
+            v += (double)bi * (double)zi;
+        }
+    }
+    if (part) b_partial(part, S, v);
+}
+
+// x = C b for the dense coarse inverse (row-major nc x nc)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_dense(int nc, int S, const double* __restrict__ C,
+                                                    const T* __restrict__ b, T* __restrict__ x) {
+    B_ROW_LOOP(nc) {
+        double acc = 0.0;
+        for (int k = 0; k < nc; ++k)
+            if (on) acc += C[(size_t)row * nc + k] * (double)b[(size_t)k * S + lane];
+        if (on) x[(size_t)row * S + lane] = (T)acc;
+    }
+}
+
+// ------------------------------------------------------ fields and LLG --
+__global__ __launch_bounds__(kBlock) void k_b_div(CsrArgs tx, const double* __restrict__ ty,
+                                                  const double* __restrict__ tz, int S, int n,
+                                                  const double* __restrict__ js, const double* __restrict__ m,
+                                                  double scale, double* __restrict__ out) {
+    const size_t NS = (size_t)n * S;
+    B_ROW_LOOP(n) {
+        double sx = 0, sy = 0, sz = 0;
+        for (int q = tx.rp[row]; q < tx.rp[row + 1]; ++q) {
+            const int c = tx.ci[q];
+            if (on) {
+                const size_t e = (size_t)c * S + lane;
+                const double j = js[c];
+                sx += tx.v[q] * (m[e] * j);
+                sy += ty[q] * (m[NS + e] * j);
+                sz += tz[q] * (m[2 * NS + e] * j);
+            }
+        }
+        if (on) out[(size_t)row * S + lane] = (sx + sy + sz) * scale;
+    }
+}
+
+struct BEvalArgs {
+    CsrArgs A;                         // pattern + exchange values (v may be null)
+    const double* gx; const double* gy; const double* gz;
+    const double* cx; const double* cy; const double* cz;
+    int S, n;
+    const double* m; const double* u1; const double* u2;
+    const double* invjs; const double* ku; const double* kaxis;
+    const double* dmi_fac; const double* nvs;
+    const double* hext;   // [3][S] per-sample uniform applied field
+    double alpha;
+    int variant;
+    double* dmdt; double* heff;
+    double* torque;       // [S], zeroed before
+};
+
+// Fused Heff + LLG for every (node, sample): TheLLG.cpp:238-254 and
+// EffFieldGPU.cu:316-362, with the demag field -grad(u1 + u2).
+template <bool DMI>
+__global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
+    const int S = a.S;
+    const size_t NS = (size_t)a.n * S;
+    const CsrArgs A = a.A;
+    double tmax = 0.0;
+    B_ROW_LOOP(a.n) {
+        double ex = 0, ey = 0, ez = 0, dx = 0, dy = 0, dz = 0, cxs = 0, cys = 0, czs = 0;
+        for (int q = A.rp[row]; q < A.rp[row + 1]; ++q) {
+            const int c = A.ci[q];
+            if (!on) continue;
+            const size_t e = (size_t)c * S + lane;
+            const double mx = a.m[e], my = a.m[NS + e], mz = a.m[2 * NS + e];
+            if (A.v) {
+                const double xc = A.v[q];
+                ex += xc * mx;
+                ey += xc * my;
+                ez += xc * mz;
+            }
+            const double u = a.u1[e] + a.u2[e];
+            dx += a.gx[q] * u;
+            dy += a.gy[q] * u;
+            dz += a.gz[q] * u;
+            if (DMI) {
+                const double X = a.cx[q], Y = a.cy[q], Z = a.cz[q];
+                cxs += Y * mz - Z * my;
+                cys += Z * mx - X * mz;
+                czs += X * my - Y * mx;
+            }
+        }
+        if (!on) continue;
+        const size_t e = (size_t)row * S + lane;
+        const double mx = a.m[e], my = a.m[NS + e], mz = a.m[2 * NS + e];
+        double hx = ex, hy = ey, hz = ez;
+        if (a.ku) {
+            const double kx = a.kaxis[row], ky = a.kaxis[a.n + row], kz = a.kaxis[2 * a.n + row];
+            const double f = 2.0 * a.ku[row] * (kx * mx + ky * my + kz * mz);
+            hx += f * kx;
+            hy += f * ky;
+            hz += f * kz;
+        }
+        if (DMI) {
+            const double sx = a.nvs[row], sy = a.nvs[a.n + row], sz = a.nvs[2 * a.n + row];
+            const double f = a.dmi_fac[row];
+            hx += ((my * sz - mz * sy) - 2.0 * cxs) * f;
+            hy += ((mz * sx - mx * sz) - 2.0 * cys) * f;
+            hz += ((mx * sy - my * sx) - 2.0 * czs) * f;
+        }
+        const double ij = a.invjs ? a.invjs[row] : 1.0;
+        hx = hx * ij + a.hext[lane] - dx;
+        hy = hy * ij + a.hext[S + lane] - dy;
+        hz = hz * ij + a.hext[2 * S + lane] - dz;
+        if (a.heff) {
+            a.heff[e] = hx;
+            a.heff[NS + e] = hy;
+            a.heff[2 * NS + e] = hz;
+        }
+        const double tx = my * hz - mz * hy, ty = mz * hx - mx * hz, tz = mx * hy - my * hx;
+        const double bx = my * tz - mz * ty, by = mz * tx - mx * tz, bz = mx * ty - my * tx;
+        if (a.variant == TM_LLG_NOPREC) {
+            a.dmdt[e] = -a.alpha * bx;
+            a.dmdt[NS + e] = -a.alpha * by;
+            a.dmdt[2 * NS + e] = -a.alpha * bz;
+        } else {
+            a.dmdt[e] = -tx - a.alpha * bx;
+            a.dmdt[NS + e] = -ty - a.alpha * by;
+            a.dmdt[2 * NS + e] = -tz - a.alpha * bz;
+        }
+        tmax = fmax(tmax, sqrt(tx * tx + ty * ty + tz * tz));
+    }
+    if (!a.torque) return;
+    __shared__ double st[kRowsPerBlock][kWave];
+    const int ln = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    st[wv][ln] = tmax;
+    __syncthreads();
+    if (wv == 0 && ln < S) {
+        double t = st[0][ln];
+        for (int k = 1; k < kRowsPerBlock; ++k) t = fmax(t, st[k][ln]);
+        atomic_max_nonneg(a.torque + ln, t);
+    }
+}
+
+// U[j][s] = u[bn[j]][s]
+__global__ __launch_bounds__(kBlock) void k_b_gather(int b, int S, const int* __restrict__ bn,
+                                                     const double* __restrict__ u, double* __restrict__ U) {
+    const size_t tot = (size_t)b * S;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
+        U[e] = u[(size_t)bn[e / S] * S + e % S];
+}
+
+// out[bn[j]][s] = G[j][s]
+__global__ __launch_bounds__(kBlock) void k_b_scatter(int b, int S, const int* __restrict__ bn,
+                                                      const double* __restrict__ G, double* __restrict__ out) {
+    const size_t tot = (size_t)b * S;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
+        out[(size_t)bn[e / S] * S + e % S] = G[e];
+}
+
+// m <- normalize(m + dt dm) per (node, sample)
+__global__ __launch_bounds__(kBlock) void k_b_euler(size_t NS, double* __restrict__ m, const double* __restrict__ dm,
+                                                    double dt) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock) {
+        const double x = m[e] + dt * dm[e], y = m[NS + e] + dt * dm[NS + e], z = m[2 * NS + e] + dt * dm[2 * NS + e];
+        const double l = sqrt(x * x + y * y + z * z);
+        const double f = l > 0.0 ? 1.0 / l : 1.0;
+        m[e] = x * f;
+        m[NS + e] = y * f;
+        m[2 * NS + e] = z * f;
+    }
+}
+
+__global__ void k_b_zero_small(int cnt, double* p) {
+    if ((int)threadIdx.x < cnt) p[threadIdx.x] = 0.0;
+}
+
+inline int elem_grid(size_t tot) { return (int)std::min<size_t>(4096, (tot + kBlock - 1) / kBlock); }
+inline int row_grid(int rows) { return std::max(1, std::min(2048, (rows + kRowsPerBlock - 1) / kRowsPerBlock)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batched solver for one demag system, sharing the context solver's operator
+// and AMG hierarchy.
+template <typename T>
+struct BatchSolver {
+    const CgSolver* cg = nullptr;
+    int n = 0, S = 0, G = 0;
+    DCsr A;                       // level-0 operator (FP64 values)
+    DevBuf<double> dinv;
+    DevBuf<unsigned char> fixed;
+    DevBuf<double> r64;
+    DevBuf<T> rT, d, z, p0, p1, q;
+    struct Lv {
+        DevBuf<T> x, x2, r, b;
+    };
+    std::vector<Lv> lv;
+    DevBuf<T> xc;
+    DevBuf<double> part;          // 4 slots x S x G
+    DevBuf<OState> ost;
+    DevBuf<BState> bst;
+    OState* h_ost = nullptr;
+    BState* h_bst = nullptr;
+    int last_inner = 8;
+
+    ~BatchSolver() {
+        if (h_ost) (void)hipHostFree(h_ost);
+        if (h_bst) (void)hipHostFree(h_bst);
+    }
+
+    int setup(const CgSolver& solver, int S_, hipStream_t s) {
+        cg = &solver;
+        n = solver.n;
+        S = S_;
+        G = row_grid(n);
+        TM_CHECK(A.upload(n, solver.pat.h_rowptr.data(), solver.pat.h_csrcol.data(), solver.h_val.data(), s));
+        TM_CHECK(dinv.upload(solver.h_dinv.data(), n, s));
+        TM_CHECK(fixed.upload(solver.h_fixed.data(), n, s));
+        const size_t NS = (size_t)n * S;
+        TM_CHECK(r64.alloc(NS));
+        TM_CHECK(rT.alloc(NS));
+        TM_CHECK(d.alloc(NS));
+        TM_CHECK(z.alloc(NS));
+        TM_CHECK(p0.alloc(NS));
+        TM_CHECK(p1.alloc(NS));
+        TM_CHECK(q.alloc(NS));
+        lv.clear();
+        lv.resize(solver.amg.levels.size());
+        for (size_t l = 0; l < solver.amg.levels.size(); ++l) {
+            const auto& L = solver.amg.levels[l];
+            TM_CHECK(lv[l].x.alloc((size_t)L.n * S));
+            TM_CHECK(lv[l].x2.alloc((size_t)L.n * S));
+            TM_CHECK(lv[l].r.alloc((size_t)L.n * S));
+            TM_CHECK(lv[l].b.alloc((size_t)L.R.n * S));
+        }
+        TM_CHECK(xc.alloc((size_t)std::max(1, solver.amg.nc) * S));
+        TM_CHECK(part.alloc((size_t)4 * S * G));
+        TM_CHECK(ost.alloc(1));
+        TM_CHECK(bst.alloc(1));
+        if (!h_ost) TM_HIP(hipHostMalloc((void**)&h_ost, sizeof(OState), hipHostMallocDefault));
+        if (!h_bst) TM_HIP(hipHostMalloc((void**)&h_bst, sizeof(BState), hipHostMallocDefault));
+        TM_HIP(hipStreamSynchronize(s));
+        return TM_OK;
+    }
+
+    // z = M r (AMG V-cycle, batched), partial (r, z) into prz
+    int precond(const T* r, T* zout, double* prz, hipStream_t s) {
+        const AmgPrecond& M = cg->amg;
+        const int L = (int)M.levels.size();
+        if (L == 0) {
+            hipLaunchKernelGGL((k_b_dense<T>), dim3(row_grid(M.nc)), dim3(kBlock), 0, s, M.nc, S, M.Cinv.p, r, zout);
+            // (r, z) partial via a post-sweep with w = 0 on the identity: reuse k_b_cg_update-free path
+            return TM_ERR_UNSUPPORTED;
+        }
+        const T* b = r;
+        for (int l = 0; l < L; ++l) {
+            const auto& V = M.levels[l];
+            if (l == 0)
+                hipLaunchKernelGGL((k_b_pre<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, dinv.p, b, V.omega,
+                                   lv[0].x.p, lv[0].r.p);
+            else
+                hipLaunchKernelGGL((k_b_pre<T>), dim3(row_grid(V.n)), dim3(kBlock), 0, s, args(V.A), S, V.dinv.p, b,
+                                   V.omega, lv[l].x.p, lv[l].r.p);
+            hipLaunchKernelGGL((k_b_spmm<T, false>), dim3(row_grid(V.R.n)), dim3(kBlock), 0, s, args(V.R), S,
+                               (const T*)lv[l].r.p, lv[l].b.p);
+            b = lv[l].b.p;
+        }
+        hipLaunchKernelGGL((k_b_dense<T>), dim3(row_grid(M.nc)), dim3(kBlock), 0, s, M.nc, S, M.Cinv.p, b, xc.p);
+        const T* xcoarse = xc.p;
+        for (int l = L - 1; l >= 0; --l) {
+            const auto& V = M.levels[l];
+            const T* bl = l == 0 ? r : lv[l - 1].b.p;
+            hipLaunchKernelGGL((k_b_spmm<T, true>), dim3(row_grid(V.P.n)), dim3(kBlock), 0, s, args(V.P), S, xcoarse,
+                               lv[l].x.p);
+            if (l == 0) {
+                hipLaunchKernelGGL((k_b_post<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, dinv.p, bl, V.omega,
+                                   (const T*)lv[0].x.p, zout, prz);
+                xcoarse = zout;
+            } else {
+                hipLaunchKernelGGL((k_b_post<T>), dim3(row_grid(V.n)), dim3(kBlock), 0, s, args(V.A), S, V.dinv.p, bl,
+                                   V.omega, (const T*)lv[l].x.p, lv[l].x2.p, (double*)nullptr);
+                xcoarse = lv[l].x2.p;
+            }
+        }
+        TM_HIP(hipGetLastError());
+        return TM_OK;
+    }
+
+    // Solves A x = b for all samples ([n][S] FP64); x is the warm start.
+    int solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* outer_out, int* inner_out,
+              double* maxrel) {
+        if (cg->precond != TM_PRECOND_AMG || cg->amg.levels.empty()) return TM_ERR_UNSUPPORTED;
+        const size_t NS = (size_t)n * S;
+        double* pbb = part.p;
+        double* prr = part.p + (size_t)S * G;
+        double* prz = part.p + (size_t)2 * S * G;
+        double* ppq = part.p + (size_t)3 * S * G;
+        const double itol = sizeof(T) == 4 ? 1e-5 : 0.0;
+        int outer = 0, inner_total = 0;
+        hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
+        for (;;) {
+            hipLaunchKernelGGL((k_b_resid<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, b, (const double*)x, r64.p,
+                               rT.p, pbb, prr);
+            hipLaunchKernelGGL(k_b_outer_red, dim3(S), dim3(kBlock), 0, s, G, S, pbb, prr, tol, itol, ost.p, bst.p);
+            TM_HIP(hipMemcpyAsync(h_ost, ost.p, sizeof(OState), hipMemcpyDeviceToHost, s));
+            TM_HIP(hipStreamSynchronize(s));
+            bool all = true;
+            double worst = 0.0;
+            for (int k = 0; k < S; ++k) {
+                all = all && h_ost->done[k];
+                if (h_ost->bb[k] > 0) worst = std::max(worst, std::sqrt(h_ost->rr[k] / h_ost->bb[k]));
+            }
+            if (maxrel) *maxrel = worst;
+            if (all || outer >= 8 || inner_total >= maxit) break;
+            ++outer;
+            // inner solve A d = rT from d = 0
+            hipLaunchKernelGGL((k_b_zero<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, d.p);
+            TM_CHECK(precond(rT.p, z.p, prz, s));
+            hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, -1, bst.p);
+            int enq = 0;
+            int chunk = std::max(4, last_inner);
+            T* pc = p0.p;
+            T* po = p1.p;
+            for (;;) {
+                for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
+                    hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, (const T*)z.p,
+                                       (const T*)po, pc, q.p, enq, (const BState*)bst.p, ppq);
+                    hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, ppq, bst.p);
+                    hipLaunchKernelGGL((k_b_cg_update<T>), dim3(G), dim3(kBlock), 0, s, n, S, (const T*)pc,
+                                       (const T*)q.p, d.p, rT.p, (const BState*)bst.p, prr);
+                    TM_CHECK(precond(rT.p, z.p, prz, s));
+                    hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, enq, bst.p);
+                    std::swap(pc, po);
+                }
+                TM_HIP(hipGetLastError());
+                TM_HIP(hipMemcpyAsync(h_bst, bst.p, sizeof(BState), hipMemcpyDeviceToHost, s));
+                TM_HIP(hipStreamSynchronize(s));
+                bool any = false;
+                for (int k = 0; k < S; ++k) any = any || h_bst->active[k];
+                if (!any || enq >= maxit) break;
+                chunk = 2;
+            }
+            last_inner = std::max(4, enq - 1);
+            inner_total += enq;
+            hipLaunchKernelGGL((k_b_add<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const T*)d.p, x);
+        }
+        if (outer_out) *outer_out = outer;
+        if (inner_out) *inner_out = inner_total;
+        TM_HIP(hipGetLastError());
+        bool all = true;
+        for (int k = 0; k < S; ++k) all = all && h_ost->done[k];
+        return all ? TM_OK : TM_ERR_MAXITER;
+    }
+};
+
+}  // namespace tmh
+
+using namespace tmh;
+
+// tm_ctx internals needed here (defined in tm_field.hip)
+namespace tmh {
+}  // namespace tmh
+
+struct tm_batch {
+    int n = 0, S = 0, prec = TM_PREC_FP64;
+    tm_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    CtxView cv{};
+    DCsr ops;                          // union pattern with exchange values
+    DevBuf<double> gx, gy, gz, tx, ty, tz, cx, cy, cz;
+    bool exch = false, dmi = false;
+    DevBuf<double> m, dmdt, heff, u1, u2, rhsN, rhsD, U, Gm, hext, torque;
+    BatchSolver<double> sN64, sD64;
+    BatchSolver<float> sN32, sD32;
+    bool timers = false;
+    hipEvent_t ev[6] = {};
+    tm_phase_times tsum = {};
+};
+
+namespace {
+
+// CSR values of an operator on the union pattern (pattern entries it lacks are 0).
+int csr_on_pattern(const SellPattern& P, const std::vector<int>& rp, const std::vector<int>& col,
+                   const std::vector<double>& val, std::vector<double>& out) {
+    out.assign(P.h_csrcol.size(), 0.0);
+    for (int i = 0; i < P.n; ++i) {
+        int k = P.h_rowptr[i];
+        for (int q = rp[i]; q < rp[i + 1]; ++q) {
+            while (k < P.h_rowptr[i + 1] && P.h_csrcol[k] < col[q]) ++k;
+            if (k >= P.h_rowptr[i + 1] || P.h_csrcol[k] != col[q]) return TM_ERR_BAD_ARG;
+            out[k] += val[q];
+        }
+    }
+    return TM_OK;
+}
+
+int upload_op(tm_batch* bt, int op, DevBuf<double>& dst) {
+    std::vector<double> v;
+    TM_CHECK(csr_on_pattern(*bt->cv.pat, *bt->cv.hop_rp[op], *bt->cv.hop_col[op], *bt->cv.hop_val[op], v));
+    TM_CHECK(dst.upload(v.data(), v.size(), bt->stream));
+    TM_HIP(hipStreamSynchronize(bt->stream));
+    return TM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
+    if (!c || !out || nsamples < 1 || nsamples > kMaxS) return TM_ERR_BAD_ARG;
+    if (precision != TM_PREC_FP64 && precision != TM_PREC_MIXED) return TM_ERR_BAD_ARG;
+    tm_batch* bt = new tm_batch();
+    int st = ctx_view(c, &bt->cv);
+    if (st != TM_OK) {
+        delete bt;
+        return st;
+    }
+    const CtxView& v = bt->cv;
+    if (!v.cgN || !v.cgD || !v.B || !v.js || !v.hop_set[TM_OP_GRAD_X] || !v.hop_set[TM_OP_TGRAD_X]) {
+        delete bt;
+        return TM_ERR_NOT_READY;
+    }
+    if (v.cgN->precond != TM_PRECOND_AMG || v.cgD->precond != TM_PRECOND_AMG) {
+        delete bt;
+        return TM_ERR_UNSUPPORTED;
+    }
+    bt->ctx = c;
+    bt->n = v.n;
+    bt->S = nsamples;
+    bt->prec = precision;
+    bt->stream = v.stream;
+    auto fail = [&](int s) {
+        tm_batch_destroy(bt);
+        return s;
+    };
+    if (rocblas_create_handle(&bt->blas) != rocblas_status_success) return fail(TM_ERR_HIP);
+    rocblas_set_stream(bt->blas, bt->stream);
+    const SellPattern& P = *v.pat;
+    std::vector<double> xc(P.h_csrcol.size(), 0.0);
+    bt->exch = v.hop_set[TM_OP_EXCHANGE];
+    if (bt->exch) {
+        if ((st = csr_on_pattern(P, *v.hop_rp[TM_OP_EXCHANGE], *v.hop_col[TM_OP_EXCHANGE], *v.hop_val[TM_OP_EXCHANGE],
+                                 xc)) != TM_OK)
+            return fail(st);
+    }
+    if ((st = bt->ops.upload(P.n, P.h_rowptr.data(), P.h_csrcol.data(), xc.data(), bt->stream)) != TM_OK)
+        return fail(st);
+    int ops_ok = TM_OK;
+    ops_ok |= upload_op(bt, TM_OP_GRAD_X, bt->gx);
+    ops_ok |= upload_op(bt, TM_OP_GRAD_Y, bt->gy);
+    ops_ok |= upload_op(bt, TM_OP_GRAD_Z, bt->gz);
+    ops_ok |= upload_op(bt, TM_OP_TGRAD_X, bt->tx);
+    ops_ok |= upload_op(bt, TM_OP_TGRAD_Y, bt->ty);
+    ops_ok |= upload_op(bt, TM_OP_TGRAD_Z, bt->tz);
+    bt->dmi = v.has_dmi && v.hop_set[TM_OP_CURL_X] && v.hop_set[TM_OP_CURL_Y] && v.hop_set[TM_OP_CURL_Z];
+    if (bt->dmi) {
+        ops_ok |= upload_op(bt, TM_OP_CURL_X, bt->cx);
+        ops_ok |= upload_op(bt, TM_OP_CURL_Y, bt->cy);
+        ops_ok |= upload_op(bt, TM_OP_CURL_Z, bt->cz);
+    }
+    if (ops_ok != TM_OK) return fail(TM_ERR_BAD_ARG);
+    const size_t NS = (size_t)bt->n * nsamples;
+    int a = TM_OK;
+    a |= bt->m.alloc(3 * NS);
+    a |= bt->dmdt.alloc(3 * NS);
+    a |= bt->heff.alloc(3 * NS);
+    a |= bt->u1.alloc(NS);
+    a |= bt->u2.alloc(NS);
+    a |= bt->rhsN.alloc(NS);
+    a |= bt->rhsD.alloc(NS);
+    a |= bt->U.alloc((size_t)v.bnx * nsamples);
+    a |= bt->Gm.alloc((size_t)v.bnx * nsamples);
+    a |= bt->hext.alloc((size_t)3 * nsamples);
+    a |= bt->torque.alloc(nsamples);
+    if (a != TM_OK) return fail(TM_ERR_HIP);
+    (void)hipMemsetAsync(bt->u1.p, 0, NS * sizeof(double), bt->stream);
+    (void)hipMemsetAsync(bt->u2.p, 0, NS * sizeof(double), bt->stream);
+    (void)hipMemsetAsync(bt->rhsD.p, 0, NS * sizeof(double), bt->stream);
+    (void)hipMemsetAsync(bt->m.p, 0, 3 * NS * sizeof(double), bt->stream);
+    std::vector<double> h((size_t)3 * nsamples);
+    for (int s = 0; s < nsamples; ++s)
+        for (int d = 0; d < 3; ++d) h[(size_t)d * nsamples + s] = v.hext[d];
+    if ((st = bt->hext.upload(h.data(), h.size(), bt->stream)) != TM_OK) return fail(st);
+    if (precision == TM_PREC_FP64) {
+        if ((st = bt->sN64.setup(*v.cgN, nsamples, bt->stream)) != TM_OK) return fail(st);
+        if ((st = bt->sD64.setup(*v.cgD, nsamples, bt->stream)) != TM_OK) return fail(st);
+    } else {
+        if ((st = bt->sN32.setup(*v.cgN, nsamples, bt->stream)) != TM_OK) return fail(st);
+        if ((st = bt->sD32.setup(*v.cgD, nsamples, bt->stream)) != TM_OK) return fail(st);
+    }
+    if (hipStreamSynchronize(bt->stream) != hipSuccess) return fail(TM_ERR_HIP);
+    *out = bt;
+    return TM_OK;
+}
+
+int tm_batch_destroy(tm_batch* bt) {
+    if (!bt) return TM_OK;
+    if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+    if (bt->blas) rocblas_destroy_handle(bt->blas);
+    for (auto& e : bt->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete bt;
+    return TM_OK;
+}
+
+int tm_batch_set_zeeman(tm_batch* bt, const double* h3s) {
+    if (!bt || !h3s) return TM_ERR_BAD_ARG;
+    TM_CHECK(bt->hext.upload(h3s, (size_t)3 * bt->S, bt->stream));
+    TM_HIP(hipStreamSynchronize(bt->stream));
+    return TM_OK;
+}
+
+int tm_batch_set_mag(tm_batch* bt, const double* m, int where) {
+    if (!bt) return TM_ERR_BAD_ARG;
+    return copy_in(bt->m.p, m, where, (size_t)3 * bt->n * bt->S, bt->stream);
+}
+
+int tm_batch_get_mag(tm_batch* bt, double* m, int where) {
+    if (!bt) return TM_ERR_BAD_ARG;
+    return copy_out(m, where, bt->m.p, (size_t)3 * bt->n * bt->S, bt->stream);
+}
+
+int tm_batch_get_potentials(tm_batch* bt, double* u1, double* u2, int where) {
+    if (!bt) return TM_ERR_BAD_ARG;
+    const size_t NS = (size_t)bt->n * bt->S;
+    if (u1) TM_CHECK(copy_out(u1, where, bt->u1.p, NS, bt->stream));
+    if (u2) TM_CHECK(copy_out(u2, where, bt->u2.p, NS, bt->stream));
+    return TM_OK;
+}
+
+int tm_batch_enable_timers(tm_batch* bt, int on) {
+    if (!bt) return TM_ERR_BAD_ARG;
+    if (on && !bt->ev[0])
+        for (auto& e : bt->ev) TM_HIP(hipEventCreate(&e));
+    bt->timers = on != 0;
+    return TM_OK;
+}
+
+int tm_batch_get_timers(tm_batch* bt, tm_phase_times* out, int reset) {
+    if (!bt || !out) return TM_ERR_BAD_ARG;
+    *out = bt->tsum;
+    if (reset) bt->tsum = tm_phase_times{};
+    return TM_OK;
+}
+
+int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, int variant, double* dmdt, int dwhere,
+                      double* heff, int hwhere, tm_batch_stats* stats) {
+    if (!bt || !dmdt || (variant != TM_LLG_CLASSIC && variant != TM_LLG_NOPREC)) return TM_ERR_BAD_ARG;
+    if ((dwhere != TM_HOST && dwhere != TM_DEVICE) || (heff && hwhere != TM_HOST && hwhere != TM_DEVICE))
+        return TM_ERR_BAD_ARG;
+    const int n = bt->n, S = bt->S;
+    const size_t NS = (size_t)n * S;
+    hipStream_t s = bt->stream;
+    const CtxView& v = bt->cv;
+    if (m) TM_CHECK(copy_in(bt->m.p, m, mwhere, 3 * NS, s));
+    if (bt->timers) TM_HIP(hipEventRecord(bt->ev[0], s));
+    // Poisson right-hand side (DemagField.cpp:124-136), all samples
+    hipLaunchKernelGGL(k_b_div, dim3(row_grid(n)), dim3(kBlock), 0, s,
+                       CsrArgs{n, bt->ops.rp.p, bt->ops.ci.p, bt->tx.p}, (const double*)bt->ty.p,
+                       (const double*)bt->tz.p, S, n, v.js, (const double*)bt->m.p, 1.0 / kMu0, bt->rhsN.p);
+    TM_HIP(hipGetLastError());
+    if (bt->timers) TM_HIP(hipEventRecord(bt->ev[1], s));
+    int oN = 0, iN = 0, oD = 0, iD = 0;
+    double rN = 0, rD = 0;
+    int stN = bt->prec == TM_PREC_FP64 ? bt->sN64.solve(bt->rhsN.p, bt->u1.p, v.tol, v.maxit, s, &oN, &iN, &rN)
+                                       : bt->sN32.solve(bt->rhsN.p, bt->u1.p, v.tol, v.maxit, s, &oN, &iN, &rN);
+    if (stN != TM_OK && stN != TM_ERR_MAXITER) return stN;
+    if (bt->timers) TM_HIP(hipEventRecord(bt->ev[2], s));
+    // boundary integral (DemagField.cpp:184-201) as a GEMM: G = B U
+    const int b = v.bnx;
+    hipLaunchKernelGGL(k_b_gather, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
+                       (const double*)bt->u1.p, bt->U.p);
+    const double one = 1.0, zero = 0.0;
+    // column-major: Gt (S x b) = Ut (S x b) * B^T
+    if (rocblas_dgemm(bt->blas, rocblas_operation_none, rocblas_operation_transpose, S, b, b, &one, bt->U.p, S, v.B,
+                      v.bld, &zero, bt->Gm.p, S) != rocblas_status_success)
+        return TM_ERR_HIP;
+    hipLaunchKernelGGL(k_b_scatter, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
+                       (const double*)bt->Gm.p, bt->rhsD.p);
+    if (bt->timers) TM_HIP(hipEventRecord(bt->ev[3], s));
+    int stD = bt->prec == TM_PREC_FP64 ? bt->sD64.solve(bt->rhsD.p, bt->u2.p, v.tol, v.maxit, s, &oD, &iD, &rD)
+                                       : bt->sD32.solve(bt->rhsD.p, bt->u2.p, v.tol, v.maxit, s, &oD, &iD, &rD);
+    if (stD != TM_OK && stD != TM_ERR_MAXITER) return stD;
+    if (bt->timers) TM_HIP(hipEventRecord(bt->ev[4], s));
+    // fused fields + LLG
+    BEvalArgs a = {};
+    a.A = CsrArgs{n, bt->ops.rp.p, bt->ops.ci.p, bt->exch ? bt->ops.v.p : nullptr};
+    a.gx = bt->gx.p;
+    a.gy = bt->gy.p;
+    a.gz = bt->gz.p;
+    if (bt->dmi) {
+        a.cx = bt->cx.p;
+        a.cy = bt->cy.p;
+        a.cz = bt->cz.p;
+        a.dmi_fac = v.dmi_fac;
+        a.nvs = v.nvs;
+    }
+    a.S = S;
+    a.n = n;
+    a.m = bt->m.p;
+    a.u1 = bt->u1.p;
+    a.u2 = bt->u2.p;
+    a.invjs = v.invjs;
+    a.ku = v.has_ku ? v.ku : nullptr;
+    a.kaxis = v.has_ku ? v.kaxis : nullptr;
+    a.hext = bt->hext.p;
+    a.alpha = alpha;
+    a.variant = variant;
+    a.dmdt = dwhere == TM_DEVICE ? dmdt : bt->dmdt.p;
+    a.heff = heff ? (hwhere == TM_DEVICE ? heff : bt->heff.p) : nullptr;
+    a.torque = stats ? bt->torque.p : nullptr;
+    if (stats) hipLaunchKernelGGL(k_b_zero_small, dim3(1), dim3(kMaxS), 0, s, S, bt->torque.p);
+    if (bt->dmi) hipLaunchKernelGGL(k_b_eval<true>, dim3(row_grid(n)), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_b_eval<false>, dim3(row_grid(n)), dim3(kBlock), 0, s, a);
+    TM_HIP(hipGetLastError());
+    if (bt->timers) {
+        TM_HIP(hipEventRecord(bt->ev[5], s));
+        TM_HIP(hipEventSynchronize(bt->ev[5]));
+        float t[5];
+        for (int k = 0; k < 5; ++k) TM_HIP(hipEventElapsedTime(&t[k], bt->ev[k], bt->ev[k + 1]));
+        bt->tsum.div_ms += t[0];
+        bt->tsum.neumann_ms += t[1];
+        bt->tsum.bem_ms += t[2];
+        bt->tsum.dirichlet_ms += t[3];
+        bt->tsum.fields_ms += t[4];
+        bt->tsum.iters_neumann += iN;
+        bt->tsum.iters_dirichlet += iD;
+        bt->tsum.count += 1;
+    }
+    if (dwhere == TM_HOST) TM_CHECK(copy_out(dmdt, TM_HOST, a.dmdt, 3 * NS, s));
+    if (heff && hwhere == TM_HOST) TM_CHECK(copy_out(heff, TM_HOST, a.heff, 3 * NS, s));
+    if (stats) {
+        stats->outer_neumann = oN;
+        stats->outer_dirichlet = oD;
+        stats->inner_neumann = iN;
+        stats->inner_dirichlet = iD;
+        stats->max_relres_neumann = rN;
+        stats->max_relres_dirichlet = rD;
+        stats->nsamples = S;
+        TM_HIP(hipMemcpyAsync(stats->max_torque, bt->torque.p, sizeof(double) * S, hipMemcpyDeviceToHost, s));
+        TM_HIP(hipStreamSynchronize(s));
+    }
+    return stN != TM_OK ? stN : stD;
+}
+
+int tm_batch_step_euler(tm_batch* bt, const double* dm, int where, double dtau) {
+    if (!bt) return TM_ERR_BAD_ARG;
+    const size_t NS = (size_t)bt->n * bt->S;
+    const double* dd = dm;
+    if (where == TM_HOST) {
+        TM_CHECK(copy_in(bt->heff.p, dm, TM_HOST, 3 * NS, bt->stream));
+        dd = bt->heff.p;
+    } else if (where != TM_DEVICE) {
+        return TM_ERR_BAD_ARG;
+    }
+    hipLaunchKernelGGL(k_b_euler, dim3(elem_grid(NS)), dim3(kBlock), 0, bt->stream, NS, bt->m.p, dd, dtau);
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+}  // extern "C"

@@ -10,12 +10,16 @@
 // reference's systems and returns the reference's solution (same gauge for
 // the Neumann problem).
 //
-// Each iteration is two kernels with no host round trip:
-//   k_cg_spmv   : p = z + beta p_old (fused into the gathers), q = A p, (p,q)
-//   k_cg_update : x += alpha p, r -= alpha q, z = D^-1 r, (r,z), (r,r)
-// Dot products finish inside the launch (last-block reduction) and the
-// scalars stay in device memory.  The host checks convergence once per
-// chunk of iterations.
+// Each iteration is two kernels with no host round trip and no device-scope
+// fence:
+//   k_cg_spmv(k)   : gamma_k, (r,r) from the update's partials; beta; p = z +
+//                    beta p_old fused into the gathers; q = A p; partial (p,q)
+//   k_cg_update(k) : alpha = gamma_k / (p,q); x += alpha p; r -= alpha q;
+//                    z = D^-1 r; partials (r,z), (r,r)
+// Every block reduces the producer's partials itself (tm_internal.hpp), so all
+// blocks take the same convergence decision; block 0 publishes the scalars.
+// Grids are at most kMaxCgBlocks blocks (grid-stride), which keeps the
+// partial arrays short.  The host checks the state once per chunk.
 #include <algorithm>
 #include <cmath>
 
@@ -23,169 +27,166 @@
 
 namespace tmh {
 
-int spmv(const SellPattern& P, const double* val, const double* x, double* y, hipStream_t s);
-
 namespace {
 
-__global__ __launch_bounds__(kBlock) void k_cg_init_fixed(int n, const unsigned char* __restrict__ fixed,
-                                                          const double* __restrict__ b, double* __restrict__ x,
-                                                          double tol, int maxit, double* partials, CgState* st) {
-    __shared__ double smem[kBlock / kWave];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[1] = {0.0};
-    if (i < n) {
-        const double bi = b[i];
-        if (fixed[i]) x[i] = bi;
-        v[0] = bi * bi;
+// Slices handled by this block's wavefronts: s = wave_global, += total waves.
+#define CG_SLICE_LOOP(A)                                                      \
+    const int lane = threadIdx.x & (kWave - 1);                               \
+    const int nsl = ((A).n + kWave - 1) / kWave;                              \
+    for (int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < nsl; \
+         s += gridDim.x * (kBlock / kWave))
+
+__device__ inline void block_partials(double* part, int stride, double v0, double v1, int nv) {
+    __shared__ double sp[2][kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    v0 = wave_sum(v0);
+    if (nv > 1) v1 = wave_sum(v1);
+    if (lane == 0) {
+        sp[0][wid] = v0;
+        sp[1][wid] = v1;
     }
-    block_sum<1>(v, smem);
-    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
-    if (last_block_arrive(&st->counter)) {
-        double tot[1];
-        block_reduce_partials<1>(partials, 0, gridDim.x, tot, smem);
-        if (threadIdx.x == 0) {
-            st->bb = tot[0];
-            st->tol2bb = tol * tol * tot[0];
-            st->iters = 0;
-            st->done = 0;
-            st->beta = 0.0;
-            st->alpha = 0.0;
-            st->pad = maxit;
-            atomicExch(&st->counter, 0u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) {
+            a += sp[0][k];
+            b += sp[1][k];
         }
+        part[blockIdx.x] = a;
+        if (nv > 1) part[stride + blockIdx.x] = b;
     }
 }
 
+// part layout: [0, G) bb | [G, 2G) rz | [2G, 3G) rr | [3G, 4G) pq
+__global__ __launch_bounds__(kBlock) void k_cg_init_fixed(int n, const unsigned char* __restrict__ fixed,
+                                                          const double* __restrict__ b, double* __restrict__ x,
+                                                          double* part) {
+    double v = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double bi = b[i];
+        if (fixed[i]) x[i] = bi;
+        v += bi * bi;
+    }
+    block_partials(part, 0, v, 0.0, 1);
+}
+
+template <bool JAC>
 __global__ __launch_bounds__(kBlock) void k_cg_init_res(SellView A, const double* __restrict__ val,
                                                         const double* __restrict__ dinv,
                                                         const double* __restrict__ b, const double* __restrict__ x,
                                                         double* __restrict__ r, double* __restrict__ z,
-                                                        double* partials, CgState* st) {
-    __shared__ double smem[2 * kBlock / kWave];
-    const int row = blockIdx.x * kBlock + threadIdx.x;
-    const int s = row / kWave, lane = row & (kWave - 1);
-    double v[2] = {0.0, 0.0};
-    if (s < (A.n + kWave - 1) / kWave) {
-        const int64_t base = A.off[s] + lane;
-        const int w = A.w[s];
+                                                        double* part, double tol, CgState* st) {
+    const int G = gridDim.x;
+    double bb[1];
+    reduce_partials_all<1>(part, G, G, bb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->bb = bb[0];
+        st->tol2bb = tol * tol * bb[0];
+        st->iters = 0;
+        st->done = 0;
+        st->rr = 0.0;
+    }
+    double vrz = 0.0, vrr = 0.0;
+    CG_SLICE_LOOP(A) {
         double acc = 0.0;
-        for (int k = 0; k < w; ++k) {
-            const int64_t q = base + (int64_t)k * kWave;
-            acc += val[q] * x[A.col[q]];
-        }
+        sell_row(A, s, lane, [&](int64_t e, int c) { acc += val[e] * x[c]; });
+        const int row = s * kWave + lane;
         if (row < A.n) {
             const double ri = b[row] - acc;
-            const double zi = dinv[row] * ri;
             r[row] = ri;
-            z[row] = zi;
-            v[0] = ri * zi;
-            v[1] = ri * ri;
+            vrr += ri * ri;
+            if (JAC) {
+                const double zi = dinv[row] * ri;
+                z[row] = zi;
+                vrz += ri * zi;
+            }
         }
     }
-    block_sum<2>(v, smem);
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = v[0];
-        partials[gridDim.x + blockIdx.x] = v[1];
-    }
-    if (last_block_arrive(&st->counter)) {
-        double tot[2];
-        block_reduce_partials<2>(partials, gridDim.x, gridDim.x, tot, smem);
-        if (threadIdx.x == 0) {
-            st->gamma = tot[0];
-            st->rr = tot[1];
-            if (!isfinite(tot[0]) || !isfinite(tot[1])) st->done = 2;
-            else if (tot[1] <= st->tol2bb) st->done = 1;
-            atomicExch(&st->counter, 0u);
-        }
-    }
+    if (JAC) block_partials(part + G, G, vrz, vrr, 2);
+    else block_partials(part + 2 * G, G, vrr, 0.0, 1);
 }
 
-// p_new = z + beta p_old; q = A p_new; partial (p_new, q).
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(SellView A, const double* __restrict__ val,
                                                     const double* __restrict__ z, const double* __restrict__ p_old,
-                                                    double* __restrict__ p_new, double* __restrict__ q,
-                                                    int first, double* partials, CgState* st) {
-    __shared__ double smem[kBlock / kWave];
+                                                    double* __restrict__ p_new, double* __restrict__ q, int k,
+                                                    double* part, CgState* st) {
     if (st->done) return;
-    const double beta = first ? 0.0 : st->beta;
-    const int row = blockIdx.x * kBlock + threadIdx.x;
-    const int s = row / kWave, lane = row & (kWave - 1);
-    double v[1] = {0.0};
-    if (s < (A.n + kWave - 1) / kWave) {
-        const int64_t base = A.off[s] + lane;
-        const int w = A.w[s];
-        double acc = 0.0;
-        if (first) {
-            for (int k = 0; k < w; ++k) {
-                const int64_t e = base + (int64_t)k * kWave;
-                acc += val[e] * z[A.col[e]];
-            }
-        } else {
-            for (int k = 0; k < w; ++k) {
-                const int64_t e = base + (int64_t)k * kWave;
-                const int c = A.col[e];
-                acc += val[e] * (z[c] + beta * p_old[c]);
-            }
+    const int G = gridDim.x;
+    double t[2];
+    reduce_partials_all<2>(part + G, G, G, t);  // (r,z), (r,r)
+    const double gam = t[0], rr = t[1];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!isfinite(gam) || !isfinite(rr)) {
+        if (lead) st->done = 2;
+        return;
+    }
+    if (rr <= st->tol2bb) {
+        if (lead) {
+            st->done = 1;
+            st->rr = rr;
+            st->iters = k;
         }
+        return;
+    }
+    const double beta = k > 0 ? gam / st->gam[(k - 1) & 1] : 0.0;
+    if (lead) {
+        st->gam[k & 1] = gam;
+        st->rr = rr;
+    }
+    double v = 0.0;
+    CG_SLICE_LOOP(A) {
+        double acc = 0.0;
+        if (k == 0) sell_row(A, s, lane, [&](int64_t e, int c) { acc += val[e] * z[c]; });
+        else sell_row(A, s, lane, [&](int64_t e, int c) { acc += val[e] * (z[c] + beta * p_old[c]); });
+        const int row = s * kWave + lane;
         if (row < A.n) {
-            const double pi = first ? z[row] : z[row] + beta * p_old[row];
+            const double pi = k == 0 ? z[row] : z[row] + beta * p_old[row];
             p_new[row] = pi;
             q[row] = acc;
-            v[0] = pi * acc;
+            v += pi * acc;
         }
     }
-    block_sum<1>(v, smem);
-    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
-    if (last_block_arrive(&st->counter)) {
-        double tot[1];
-        block_reduce_partials<1>(partials, 0, gridDim.x, tot, smem);
-        if (threadIdx.x == 0) {
-            const double pq = tot[0];
-            if (!(pq > 0.0) || !isfinite(pq)) st->done = 2;
-            else st->alpha = st->gamma / pq;
-            atomicExch(&st->counter, 0u);
-        }
-    }
+    block_partials(part + 3 * G, G, v, 0.0, 1);
 }
 
+template <bool JAC>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int n, const double* __restrict__ dinv,
                                                       const double* __restrict__ p, const double* __restrict__ q,
                                                       double* __restrict__ x, double* __restrict__ r,
-                                                      double* __restrict__ z, double* partials, CgState* st) {
-    __shared__ double smem[2 * kBlock / kWave];
+                                                      double* __restrict__ z, int k, double* part, CgState* st) {
     if (st->done) return;
-    const double alpha = st->alpha;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (i < n) {
+    const int G = gridDim.x;
+    double t[1];
+    reduce_partials_all<1>(part + 3 * G, G, G, t);
+    const double pq = t[0];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!(pq > 0.0) || !isfinite(pq)) {
+        if (lead) st->done = 2;
+        return;
+    }
+    const double alpha = st->gam[k & 1] / pq;
+    double vrz = 0.0, vrr = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += G * kBlock) {
         x[i] += alpha * p[i];
         const double ri = r[i] - alpha * q[i];
-        const double zi = dinv[i] * ri;
         r[i] = ri;
-        z[i] = zi;
-        v[0] = ri * zi;
-        v[1] = ri * ri;
-    }
-    block_sum<2>(v, smem);
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = v[0];
-        partials[gridDim.x + blockIdx.x] = v[1];
-    }
-    if (last_block_arrive(&st->counter)) {
-        double tot[2];
-        block_reduce_partials<2>(partials, gridDim.x, gridDim.x, tot, smem);
-        if (threadIdx.x == 0) {
-            const double g = tot[0];
-            st->beta = g / st->gamma;
-            st->gamma = g;
-            st->rr = tot[1];
-            st->iters += 1;
-            if (!isfinite(g) || !isfinite(tot[1])) st->done = 2;
-            else if (tot[1] <= st->tol2bb) st->done = 1;
-            else if (st->iters >= st->pad) st->done = 3;
-            atomicExch(&st->counter, 0u);
+        vrr += ri * ri;
+        if (JAC) {
+            const double zi = dinv[i] * ri;
+            z[i] = zi;
+            vrz += ri * zi;
         }
     }
+    if (lead) st->iters = k + 1;
+    if (JAC) block_partials(part + G, G, vrz, vrr, 2);
+    else block_partials(part + 2 * G, G, vrr, 0.0, 1);
+}
+
+// (r, r) after the last update when the loop stopped on maxit.
+__global__ __launch_bounds__(kBlock) void k_cg_final(int G, double* part, CgState* st) {
+    double t[2];
+    reduce_partials_all<2>(part + G, G, G, t);
+    if (threadIdx.x == 0 && st->done == 0) st->rr = t[1];
 }
 
 }  // namespace
@@ -195,11 +196,12 @@ CgSolver::~CgSolver() {
     h_st = nullptr;
 }
 
-int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* vals, int precond,
+int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* vals, int precond_,
                     hipStream_t s) {
-    if (precond != TM_PRECOND_JACOBI) return TM_ERR_BAD_ARG;
+    if (precond_ != TM_PRECOND_JACOBI && precond_ != TM_PRECOND_AMG) return TM_ERR_BAD_ARG;
+    precond = precond_;
     n = n_;
-    // identity rows: a single stored entry on the diagonal equal to 1
+    // identity rows: a single stored nonzero on the diagonal equal to 1
     std::vector<unsigned char> fx((size_t)n, 0);
     std::vector<double> di((size_t)n, 0.0);
     nfixed = 0;
@@ -238,6 +240,9 @@ int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* 
         else return TM_ERR_UNSUPPORTED;  // free block must be positive definite
     }
     pat.build(n, n, rowptr, colidx);
+    h_val.assign(vals, vals + rowptr[n]);
+    h_dinv = di;
+    h_fixed = fx;
     std::vector<double> sv;
     TM_CHECK(pat.scatter(rowptr, colidx, vals, sv));
     TM_CHECK(pat.upload(s));
@@ -249,42 +254,50 @@ int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* 
     TM_CHECK(q.alloc(n));
     TM_CHECK(p0.alloc(n));
     TM_CHECK(p1.alloc(n));
-    TM_CHECK(xw.alloc(n));
-    TM_HIP(hipMemsetAsync(xw.p, 0, sizeof(double) * n, s));
-    const int nb = grid_for((int64_t)pat.nslices * kWave);
-    TM_CHECK(partials.alloc((size_t)2 * std::max(nb, grid_for(n))));
+    grid = std::max(1, std::min(kMaxCgBlocks, (pat.nslices + kBlock / kWave - 1) / (kBlock / kWave)));
+    TM_CHECK(part.alloc((size_t)4 * grid));
     TM_CHECK(st.alloc(1));
     TM_HIP(hipMemsetAsync(st.p, 0, sizeof(CgState), s));
     if (!h_st) TM_HIP(hipHostMalloc((void**)&h_st, sizeof(CgState), hipHostMallocDefault));
     TM_HIP(hipStreamSynchronize(s));
+    if (precond == TM_PRECOND_AMG) TM_CHECK(amg.setup(n, rowptr, colidx, vals, fx, &pat, val.p, dinv.p, s));
     last_iters = 0;
-    has_solution = false;
     return TM_OK;
 }
 
 int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* iters, double* relres) {
     if (n == 0) return TM_ERR_NOT_READY;
     if (!(tol > 0.0) || maxit < 1) return TM_ERR_BAD_ARG;
-    const int gv = grid_for(n);
-    const int gs = grid_for((int64_t)pat.nslices * kWave);
+    const int G = grid;
     const SellView A = view(pat);
-    hipLaunchKernelGGL(k_cg_init_fixed, dim3(gv), dim3(kBlock), 0, s, n, fixed.p, b, x, tol, maxit, partials.p,
-                       st.p);
-    hipLaunchKernelGGL(k_cg_init_res, dim3(gs), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p, partials.p,
-                       st.p);
+    hipLaunchKernelGGL(k_cg_init_fixed, dim3(G), dim3(kBlock), 0, s, n, fixed.p, b, x, part.p);
+    const bool jac = precond == TM_PRECOND_JACOBI;
+    if (jac)
+        hipLaunchKernelGGL(k_cg_init_res<true>, dim3(G), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p, part.p,
+                           tol, st.p);
+    else
+        hipLaunchKernelGGL(k_cg_init_res<false>, dim3(G), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p,
+                           part.p, tol, st.p);
     TM_HIP(hipGetLastError());
+    if (!jac) TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s));
     // First chunk sized from the previous solve (warm-started solves along a
-    // trajectory need similar counts); then small chunks.
+    // trajectory need similar counts), then small chunks.
     int enq = 0;
     int chunk = std::max(4, std::min(maxit, last_iters > 8 ? last_iters - 2 : 8));
     double* pcur = p0.p;
     double* pold = p1.p;
     for (;;) {
-        for (int k = 0; k < chunk && enq < maxit; ++k, ++enq) {
-            hipLaunchKernelGGL(k_cg_spmv, dim3(gs), dim3(kBlock), 0, s, A, val.p, z.p, pold, pcur, q.p,
-                               enq == 0 ? 1 : 0, partials.p, st.p);
-            hipLaunchKernelGGL(k_cg_update, dim3(gv), dim3(kBlock), 0, s, n, dinv.p, pcur, q.p, x, r.p, z.p,
-                               partials.p, st.p);
+        for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
+            hipLaunchKernelGGL(k_cg_spmv, dim3(G), dim3(kBlock), 0, s, A, val.p, z.p, pold, pcur, q.p, enq, part.p,
+                               st.p);
+            if (jac) {
+                hipLaunchKernelGGL(k_cg_update<true>, dim3(G), dim3(kBlock), 0, s, n, dinv.p, pcur, q.p, x, r.p, z.p,
+                                   enq, part.p, st.p);
+            } else {
+                hipLaunchKernelGGL(k_cg_update<false>, dim3(G), dim3(kBlock), 0, s, n, dinv.p, pcur, q.p, x, r.p,
+                                   z.p, enq, part.p, st.p);
+                TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s));
+            }
             std::swap(pcur, pold);
         }
         TM_HIP(hipGetLastError());
@@ -293,14 +306,19 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
         if (h_st->done || enq >= maxit) break;
         chunk = 4;
     }
+    if (!h_st->done) {
+        // one more convergence test on the last residual
+        hipLaunchKernelGGL(k_cg_final, dim3(1), dim3(kBlock), 0, s, G, part.p, st.p);
+        TM_HIP(hipMemcpyAsync(h_st, st.p, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        TM_HIP(hipStreamSynchronize(s));
+    }
     const CgState hs = *h_st;
     if (hs.bb == 0.0) TM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
     last_iters = hs.iters;
-    has_solution = true;
     if (iters) *iters = hs.iters;
     if (relres) *relres = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
     if (hs.done == 2) return TM_ERR_NONFINITE;
-    if (hs.done == 1 || hs.bb == 0.0) return TM_OK;
+    if (hs.done == 1 || hs.bb == 0.0 || hs.rr <= hs.tol2bb) return TM_OK;
     return TM_ERR_MAXITER;
 }
 
@@ -361,3 +379,22 @@ int tm_solver_solve(tm_solver* S, const double* b, int bwhere, double* x, int xw
 }
 
 }  // extern "C"
+
+namespace tmh {
+int amg_info(const CgSolver& cg, int* nlevels, int* sizes, int cap) {
+    if (!nlevels || (cap > 0 && !sizes)) return TM_ERR_BAD_ARG;
+    if (cg.precond != TM_PRECOND_AMG) {
+        *nlevels = 0;
+        return TM_OK;
+    }
+    const int L = (int)cg.amg.levels.size();
+    *nlevels = L + 1;
+    for (int l = 0; l < L + 1 && l < cap; ++l) sizes[l] = l < L ? cg.amg.levels[l].n : cg.amg.nc;
+    return TM_OK;
+}
+}  // namespace tmh
+
+extern "C" int tm_solver_amg_info(tm_solver* s, int* nlevels, int* sizes, int cap) {
+    if (!s) return TM_ERR_BAD_ARG;
+    return tmh::amg_info(s->cg, nlevels, sizes, cap);
+}

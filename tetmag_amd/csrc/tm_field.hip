@@ -30,6 +30,7 @@ int bem_assemble(int b, int ld, int nbel, const double* bxyz, const int* faces, 
 int bem_gemv(int b, int ld, const double* B, const double* src, const int* gidx, double* part, double* y,
              const int* sidx, double* dst, hipStream_t s);
 int bem_gemv_splits(int b);
+int amg_info(const CgSolver& cg, int* nlevels, int* sizes, int cap);
 }  // namespace tmh
 
 using namespace tmh;
@@ -60,8 +61,6 @@ struct tm_ctx {
     // state and work
     DevBuf<double> m, w3a, w3b;    // 3n
     DevBuf<double> wn;             // n
-    DevBuf<double> red;            // reduction partials
-    DevBuf<unsigned> counter;      // reduction handshake
     DevBuf<double> scal;           // device scalars (max torque)
     // demag
     int bnx = 0;
@@ -131,8 +130,6 @@ int commit_ops(tm_ctx* c) {
     }
     TM_HIP(hipStreamSynchronize(c->stream));
     c->ops_dirty = false;
-    const int nb = grid_for((int64_t)c->pat.nslices * kWave);
-    TM_CHECK(c->red.alloc((size_t)std::max(nb, grid_for(n)) * 2));
     return TM_OK;
 }
 
@@ -243,14 +240,14 @@ __global__ __launch_bounds__(kBlock) void k_div(SellView A, const double* __rest
     SELL_ROW_PROLOGUE
     const int n = A.n;
     double sx = 0, sy = 0, sz = 0;
-    for (int k = 0; k < w; ++k) {
-        const int64_t q = base + (int64_t)k * kWave;
-        const int cc = A.col[q];
+    (void)base;
+    (void)w;
+    sell_row(A, s, lane, [&](int64_t q, int cc) {
         const double j = js[cc];
         sx += tx[q] * (m[cc] * j);
         sy += ty[q] * (m[n + cc] * j);
         sz += tz[q] * (m[2 * n + cc] * j);
-    }
+    });
     if (row < n) out[row] = (sx + sy + sz) * scale;
 }
 
@@ -319,12 +316,11 @@ __global__ __launch_bounds__(kBlock) void k_stt_llg(int n, const double* __restr
     out[2 * n + i] = -(beta - alpha) * az - (1.0 + alpha * beta) * bz;
 }
 
-// max_i |m_i x H_i| (EffFieldGPU.cu:484-512), finished by the last block.
+// max_i |m_i x H_i| (EffFieldGPU.cu:484-512): block max, then one integer
+// atomic max per block into *out (zeroed beforehand).
 __global__ __launch_bounds__(kBlock) void k_max_torque(int n, const double* __restrict__ m,
-                                                       const double* __restrict__ h, double* partials,
-                                                       unsigned* counter, double* out) {
+                                                       const double* __restrict__ h, double* out) {
     __shared__ double smem[kBlock / kWave];
-    __shared__ double res;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double t = 0.0;
     if (i < n) {
@@ -339,22 +335,11 @@ __global__ __launch_bounds__(kBlock) void k_max_torque(int n, const double* __re
     if (threadIdx.x == 0) {
         double v = smem[0];
         for (int k = 1; k < kBlock / kWave; ++k) v = fmax(v, smem[k]);
-        partials[blockIdx.x] = v;
-    }
-    if (last_block_arrive(counter)) {
-        double v = 0.0;
-        for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock) v = fmax(v, partials[k]);
-        v = wave_max(v);
-        if ((threadIdx.x & (kWave - 1)) == 0) smem[threadIdx.x / kWave] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            res = smem[0];
-            for (int k = 1; k < kBlock / kWave; ++k) res = fmax(res, smem[k]);
-            *out = res;
-            atomicExch(counter, 0u);
-        }
+        atomic_max_nonneg(out, v);
     }
 }
+
+__global__ void k_zero1(double* p) { *p = 0.0; }
 
 // o += a
 __global__ __launch_bounds__(kBlock) void k_add(int n, const double* __restrict__ a, double* __restrict__ o) {
@@ -404,9 +389,7 @@ struct EvalArgs {
     int variant;
     double* dmdt;               // may be null
     double* heff;               // may be null
-    double* partials;
-    unsigned* counter;
-    double* torque;             // may be null
+    double* torque;             // may be null (zeroed before the launch)
 };
 
 // Fused Heff + LLG (TheLLG.cpp:238-254 and EffFieldGPU.cu:316-362):
@@ -422,13 +405,9 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a) {
     const int s = row / kWave, lane = row & (kWave - 1);
     double t = 0.0;
     if (s < (n + kWave - 1) / kWave) {
-        const int64_t base = A.off[s] + lane;
-        const int w = A.w[s];
         const double* __restrict__ m = a.m;
         double ex = 0, ey = 0, ez = 0, dx = 0, dy = 0, dz = 0, cx = 0, cy = 0, cz = 0;
-        for (int k = 0; k < w; ++k) {
-            const int64_t q = base + (int64_t)k * kWave;
-            const int cc = A.col[q];
+        sell_row(A, s, lane, [&](int64_t q, int cc) {
             double mx = 0, my = 0, mz = 0;
             if (EXCH || DMI) {
                 mx = m[cc];
@@ -453,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a) {
                 cy += Z * mx - X * mz;
                 cz += X * my - Y * mx;
             }
-        }
+        });
         if (row < n) {
             const double mx = m[row], my = m[n + row], mz = m[2 * n + row];
             double hx = ex, hy = ey, hz = ez;  // J/m^3 terms
@@ -508,21 +487,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a) {
     if (threadIdx.x == 0) {
         double v = smem[0];
         for (int k = 1; k < kBlock / kWave; ++k) v = fmax(v, smem[k]);
-        a.partials[blockIdx.x] = v;
-    }
-    if (last_block_arrive(a.counter)) {
-        double v = 0.0;
-        for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock) v = fmax(v, a.partials[k]);
-        v = wave_max(v);
-        __syncthreads();
-        if ((threadIdx.x & (kWave - 1)) == 0) smem[threadIdx.x / kWave] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = smem[0];
-            for (int k = 1; k < kBlock / kWave; ++k) r = fmax(r, smem[k]);
-            *a.torque = r;
-            atomicExch(a.counter, 0u);
-        }
+        atomic_max_nonneg(a.torque, v);
     }
 }
 
@@ -604,10 +569,7 @@ int tm_ctx_create(int n, tm_ctx** out) {
     if (st == TM_OK) st = c->w3a.alloc((size_t)3 * n);
     if (st == TM_OK) st = c->w3b.alloc((size_t)3 * n);
     if (st == TM_OK) st = c->wn.alloc(n);
-    if (st == TM_OK) st = c->red.alloc((size_t)2 * grid_for(n) + 64);
-    if (st == TM_OK) st = c->counter.alloc(4);
     if (st == TM_OK) st = c->scal.alloc(4);
-    if (st == TM_OK && hipMemsetAsync(c->counter.p, 0, 4 * sizeof(unsigned), c->stream) != hipSuccess) st = TM_ERR_HIP;
     if (st == TM_OK && hipMemsetAsync(c->m.p, 0, 3 * sizeof(double) * n, c->stream) != hipSuccess) st = TM_ERR_HIP;
     if (st == TM_OK && hipStreamSynchronize(c->stream) != hipSuccess) st = TM_ERR_HIP;
     if (st != TM_OK) {
@@ -831,8 +793,8 @@ int tm_max_torque(tm_ctx* c, const double* H, int hwhere, double* out) {
     if (!out) return TM_ERR_BAD_ARG;
     const double* hd = nullptr;
     TM_CHECK(stage_in(c, H, hwhere, (size_t)3 * c->n, c->w3b, &hd));
-    hipLaunchKernelGGL(k_max_torque, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, c->m.p, hd, c->red.p,
-                       c->counter.p, c->scal.p);
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, c->stream, c->scal.p);
+    hipLaunchKernelGGL(k_max_torque, dim3(grid_for(c->n)), dim3(kBlock), 0, c->stream, c->n, c->m.p, hd, c->scal.p);
     TM_HIP(hipGetLastError());
     TM_HIP(hipMemcpyAsync(out, c->scal.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TM_HIP(hipStreamSynchronize(c->stream));
@@ -964,6 +926,13 @@ int tm_ctx_set_demag_solvers(tm_ctx* c, const int* rpN, const int* colN, const d
     TM_HIP(hipStreamSynchronize(c->stream));
     c->has_solvers = true;
     return TM_OK;
+}
+
+int tm_ctx_solver_amg_info(tm_ctx* c, int which, int* nlevels, int* sizes, int cap) {
+    TM_CHECK(ctx_ok(c));
+    if (!c->has_solvers) return TM_ERR_NOT_READY;
+    if (which != 0 && which != 1) return TM_ERR_BAD_ARG;
+    return amg_info(which == 0 ? c->cgN : c->cgD, nlevels, sizes, cap);
 }
 
 int tm_ctx_set_solver_params(tm_ctx* c, double tol, int maxit) {
@@ -1107,9 +1076,8 @@ int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int vari
     a.variant = variant;
     a.dmdt = dwhere == TM_DEVICE ? dmdt : c->w3a.p;
     a.heff = Heff ? (hwhere == TM_DEVICE ? Heff : c->w3b.p) : nullptr;
-    a.partials = c->red.p;
-    a.counter = c->counter.p;
     a.torque = stats ? c->scal.p : nullptr;
+    if (stats) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, c->stream, c->scal.p);
     TM_CHECK(eval_dispatch(a, exch, demag, dmi, sell_grid(c), c->stream));
     if (c->timers && demag) {
         TM_HIP(hipEventRecord(c->ev[5], c->stream));
@@ -1161,3 +1129,38 @@ int tm_ctx_get_timers(tm_ctx* c, tm_phase_times* out, int reset) {
 }
 
 }  // extern "C"
+
+namespace tmh {
+// Read-only view of a context for the batched evaluator (tm_batch.hip).
+int ctx_view(tm_ctx* c, CtxView* v) {
+    TM_CHECK(ctx_ok(c));
+    TM_CHECK(commit_ops(c));
+    v->n = c->n;
+    v->stream = c->stream;
+    v->pat = &c->pat;
+    for (int o = 0; o < TM_NUM_OPS; ++o) {
+        v->hop_set[o] = c->hop[o].set;
+        v->hop_rp[o] = &c->hop[o].rp;
+        v->hop_col[o] = &c->hop[o].col;
+        v->hop_val[o] = &c->hop[o].val;
+    }
+    v->js = c->has_js ? c->js.p : nullptr;
+    v->invjs = c->has_js ? c->invjs.p : nullptr;
+    v->has_ku = c->has_ku;
+    v->ku = c->ku.p;
+    v->kaxis = c->kaxis.p;
+    v->has_dmi = c->has_dmi;
+    v->dmi_fac = c->dmi_fac.p;
+    v->nvs = c->nvs.p;
+    v->bnx = c->bnx;
+    v->bld = c->bld;
+    v->bnodes = c->bnodes.p;
+    v->B = c->has_B ? c->B.p : nullptr;
+    v->cgN = c->has_solvers ? &c->cgN : nullptr;
+    v->cgD = c->has_solvers ? &c->cgD : nullptr;
+    v->tol = c->tol;
+    v->maxit = c->maxit;
+    for (int d = 0; d < 3; ++d) v->hext[d] = c->hext_u[d];
+    return TM_OK;
+}
+}  // namespace tmh

@@ -40,6 +40,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -101,6 +115,29 @@ struct SellView {
 
 inline SellView view(const SellPattern& P) { return SellView{P.n, P.off.p, P.w.p, P.col.p}; }
 
+// Calls f(entry, column) for every stored entry of the row owned by `lane` in
+// slice s.  Unrolled by four so the column loads of four entries (and the
+// dependent value/gather loads) are in flight together.
+template <class F>
+__device__ __forceinline__ void sell_row(const SellView& A, int s, int lane, F&& f) {
+    const int64_t base = A.off[s] + lane;
+    const int w = A.w[s];
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+        const int64_t e0 = base + (int64_t)k * kWave;
+        const int64_t e1 = e0 + kWave, e2 = e1 + kWave, e3 = e2 + kWave;
+        const int c0 = A.col[e0], c1 = A.col[e1], c2 = A.col[e2], c3 = A.col[e3];
+        f(e0, c0);
+        f(e1, c1);
+        f(e2, c2);
+        f(e3, c3);
+    }
+    for (; k < w; ++k) {
+        const int64_t e = base + (int64_t)k * kWave;
+        f(e, A.col[e]);
+    }
+}
+
 // Block-wide reductions for kBlock threads.
 __device__ inline double wave_sum(double v) {
     for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_down(v, o, kWave);
@@ -129,37 +166,42 @@ __device__ inline void block_sum(double (&v)[NV], double* smem /* NV * kBlock/kW
     }
 }
 
-// "Last block finishes the reduction" handshake.  Every block writes its
-// partials, fences, and bumps a counter; the block that sees count == grid-1
-// reduces all partials (in block order, so the result is deterministic) and
-// resets the counter for the next launch.
-__device__ inline bool last_block_arrive(unsigned* counter) {
-    __shared__ bool am_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        am_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (am_last) __threadfence();
-    return am_last;
-}
-
-// Sums NV partial arrays (stride `stride`) of `count` entries inside one block.
+// Every block of a consumer kernel reduces the partials written by the
+// producer kernel (one per producer block, at most a few hundred) itself; the
+// sum is then known to every thread of the block.  Kernel boundaries order
+// producer and consumer, so no device-scope fence is needed, and every block
+// computes bit-identical totals (same data, same order), so decisions taken on
+// them are uniform across the grid.
 template <int NV>
-__device__ inline void block_reduce_partials(const double* partials, int stride, int count, double (&out)[NV],
-                                             double* smem) {
+__device__ inline void reduce_partials_all(const double* partials, int stride, int count, double (&out)[NV]) {
+    __shared__ double sred[NV][kBlock / kWave];
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
     for (int i = threadIdx.x; i < count; i += kBlock) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) acc[q] += __builtin_nontemporal_load(partials + (size_t)q * stride + i);
+        for (int q = 0; q < NV; ++q) acc[q] += partials[(size_t)q * stride + i];
     }
-    block_sum<NV>(acc, smem);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
-    for (int q = 0; q < NV; ++q) out[q] = acc[q];
+    for (int q = 0; q < NV; ++q) {
+        const double v = wave_sum(acc[q]);
+        if (lane == 0) sred[q][wid] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double v = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) v += sred[q][k];
+        out[q] = v;
+    }
+    __syncthreads();
+}
+
+// Atomic max of a non-negative double (the IEEE bit patterns of non-negative
+// doubles order like unsigned integers).
+__device__ inline void atomic_max_nonneg(double* addr, double v) {
+    atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
 }
 
 inline int grid_for(int64_t n, int block = kBlock) { return (int)((n + block - 1) / block); }
@@ -172,37 +214,94 @@ int copy_out(double* dst, int where, const double* src_dev, size_t count, hipStr
 // Preconditioned CG on "symmetric matrix with identity rows" systems.
 // ---------------------------------------------------------------------------
 struct CgState {
-    double gamma;    // (r, z) of the current residual
-    double alpha;    // step length of the current iteration
-    double beta;     // p update coefficient for the next iteration
-    double rr;       // (r, r)
+    double gam[2];   // (r, z) of iterations k (slot k & 1)
     double bb;       // (b, b)
-    double tol2bb;   // tol^2 * (b, b)
-    int iters;
-    int done;        // 1 = converged, 2 = breakdown/non-finite
-    unsigned counter;
-    int pad;
+    double tol2bb;   // tol^2 (b, b)
+    double rr;       // (r, r) at exit
+    int iters;       // completed iterations
+    int done;        // 0 running, 1 converged, 2 breakdown / non-finite
+};
+
+// Smoothed-aggregation AMG V-cycle used as CG preconditioner (tm_amg.hip).
+struct AmgPrecond {
+    static constexpr int kCoarseMax = 512;
+    static constexpr int kMaxLevels = 8;
+    // CSR on the device, processed with `lpr` lanes per row.
+    struct CsrDev {
+        int n = 0;
+        int lpr = 4;
+        DevBuf<int> rp, ci;
+        DevBuf<double> v;
+    };
+    struct Level {
+        int n = 0;
+        double omega = 0.0;
+        int64_t nnz_a = 0, nnz_p = 0;
+        // level 0 reuses the solver's SELL matrix (not owned)
+        const SellPattern* A_ext = nullptr;
+        const double* Aval_ext = nullptr;
+        const double* dinv_ext = nullptr;
+        CsrDev A, P, R;  // A used on levels >= 1
+        DevBuf<double> dinv;
+        DevBuf<double> x, x2, r, b;  // b: restricted right-hand side of the next level
+    };
+    std::vector<Level> levels;
+    int nc = 0;
+    DevBuf<double> Cinv, xc, bc;
+    int setup(int n, const int* rowptr, const int* colidx, const double* vals, const std::vector<unsigned char>& fixed,
+              const SellPattern* A0, const double* A0val, const double* A0dinv, hipStream_t s);
+    int apply(const double* r, double* z, double* part, int grid, hipStream_t s);
 };
 
 struct CgSolver {
     int n = 0;
+    int grid = 0;            // blocks of every CG kernel (<= kMaxCgBlocks)
     SellPattern pat;
     DevBuf<double> val;      // matrix values in SELL order
+    std::vector<double> h_val;  // matrix values in CSR order (pat.h_rowptr / h_csrcol)
+    std::vector<double> h_dinv;
+    std::vector<unsigned char> h_fixed;
     DevBuf<double> dinv;     // inverse diagonal (1 on fixed rows)
     DevBuf<unsigned char> fixed;  // 1 on identity rows
-    DevBuf<double> r, z, q, p0, p1, xw;  // work vectors (xw: solver-owned solution)
-    DevBuf<double> partials;  // 3 * nblocks
+    DevBuf<double> r, z, q, p0, p1;
+    DevBuf<double> part;     // partial sums: bb | rz | rr | pq, grid entries each
     DevBuf<CgState> st;
     CgState* h_st = nullptr;  // pinned mirror for the convergence check
+    int precond = TM_PRECOND_JACOBI;
+    AmgPrecond amg;
     int nfixed = 0;
     int last_iters = 0;
-    bool has_solution = false;
     ~CgSolver();
     int setup(int n, const int* rowptr, const int* colidx, const double* vals, int precond, hipStream_t s);
     // Solves A x = b for device vectors b, x (x is the warm start).  tol is
     // relative to ||b||.  Returns TM_OK, TM_ERR_MAXITER or TM_ERR_NONFINITE.
     int solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* iters, double* relres);
 };
+
+constexpr int kMaxCgBlocks = 512;
+
+// Read-only view of a field context's device data (tm_field.hip), used by
+// the batched evaluator.
+struct CtxView {
+    int n;
+    hipStream_t stream;
+    const SellPattern* pat;
+    const std::vector<int>* hop_rp[TM_NUM_OPS];
+    const std::vector<int>* hop_col[TM_NUM_OPS];
+    const std::vector<double>* hop_val[TM_NUM_OPS];
+    bool hop_set[TM_NUM_OPS];
+    const double *js, *invjs, *ku, *kaxis, *dmi_fac, *nvs;
+    bool has_ku, has_dmi;
+    int bnx, bld;
+    const int* bnodes;
+    const double* B;
+    const CgSolver* cgN;
+    const CgSolver* cgD;
+    double tol;
+    int maxit;
+    double hext[3];
+};
+int ctx_view(tm_ctx* c, CtxView* v);
 
 }  // namespace tmh
 

@@ -163,6 +163,32 @@ def config1(seed: int = 2025) -> Problem:
     return cuboid_problem("cuboid-128x128x16-h3", 128, 128, 16, 3.0, 0.1, seed, mat, "vortex")
 
 
+def smooth_random_fields(mesh: M.Mesh, count: int, seed: int, modes: int = 8, lam_min: float = 20.0,
+                         lam_max: float = 200.0) -> list:
+    """Seeded smooth fields m = normalise(sum_k a_k cos(q_k . r + phi_k)), |q_k| =
+    2 pi / lambda_k with lambda_k ~ U[lam_min, lam_max] (mesh units), random
+    directions, amplitudes a_k (3-vectors) and phases."""
+    out = []
+    xyz = mesh.xyz
+    for s in range(count):
+        u = M.uniform(seed * 1000003 + s, modes * 9)
+        acc = np.zeros((mesh.n, 3))
+        for k in range(modes):
+            v = u[9 * k:9 * k + 9]
+            lam = lam_min + (lam_max - lam_min) * v[0]
+            phi_d = 2 * math.pi * v[1]
+            ct = 2 * v[2] - 1
+            st = math.sqrt(max(0.0, 1 - ct * ct))
+            q = (2 * math.pi / lam) * np.array([st * math.cos(phi_d), st * math.sin(phi_d), ct])
+            ph = 2 * math.pi * v[3]
+            a = 2 * v[4:7] - 1
+            acc += np.cos(xyz @ q + ph)[:, None] * a[None, :]
+        nrm = np.linalg.norm(acc, axis=1, keepdims=True)
+        nrm[nrm == 0] = 1.0
+        out.append(np.ascontiguousarray((acc / nrm).T.reshape(-1)))
+    return out
+
+
 def small_problem(nx=6, ny=5, nz=3, h=4.0, jitter=0.1, seed=7, Ku=3e4, D=0.0, B_ext=(0.0, 0.0, 0.0),
                   init="random") -> Problem:
     """Small seeded problem for parity tests (all field terms switchable)."""
