@@ -405,12 +405,38 @@ class GpuPotential_T {
 };
 
 // ---- LLGWrapper (LLGWrapper.h:57-63) --------------------------------------
+// The reference's device right-hand side for CVODE (LLGWrapper.cu:104-107,
+// 213-228) copies u into the field object, evaluates the fields, sums them on
+// the host and returns the LLG right-hand side in a device vector.  Here one
+// call evaluates the whole right-hand side on the device of the context that
+// owns the operators (the EffFieldGPU's), including demag when the context
+// has its boundary, BEM matrix and demag solvers set.
 class LLGWrapper {
     int nx3 = 0;
+    tm_ctx* ctx_ = nullptr;
+    double alpha_ = 0.0;
+    int variant_ = TM_LLG_CLASSIC;
 
    public:
-    void init(int nx) { nx3 = 3 * nx; }  // the scratch lives in the field context
+    void init(int nx) { nx3 = 3 * nx; }
     int size() const { return nx3; }
+    // selectLLGTypeGPU (LLGWrapper.cu:109-120): classic or no-precession form.
+    void bind(tm_ctx* ctx, double alpha, bool removePrecession) {
+        ctx_ = ctx;
+        alpha_ = alpha;
+        variant_ = removePrecession ? TM_LLG_NOPREC : TM_LLG_CLASSIC;
+    }
+    // rhs_d: u, udot are 3n device pointers (SoA x|y|z).
+    void rhs_d(const double* u, double* udot, tm_eval_stats* stats = nullptr) {
+        if (!ctx_) throw Error(TM_ERR_NOT_READY, "LLGWrapper::rhs_d before bind");
+        check(tm_eval_rhs(ctx_, u, TM_DEVICE, alpha_, variant_, udot, TM_DEVICE, nullptr, TM_DEVICE, stats),
+              "tm_eval_rhs");
+    }
+    // operator()(dev_vec) (LLGWrapper.cu:104-107)
+    void operator()(const DeviceVector& u, DeviceVector& udot) {
+        if (udot.size() != u.size()) udot.resize(u.size());
+        rhs_d(u.data(), udot.data());
+    }
 };
 
 }  // namespace tmg

@@ -129,6 +129,21 @@ int main() {
     ref_llg_rhs(n, 0.5, 1, m.data(), H.data(), ref.data());
     expect("ClassicLLG_dev", rel(dh.data(), ref.data(), ref.size()), 1e-14);
 
+    // LLGWrapper::rhs_d: whole right-hand side on the device (no Js set, so
+    // invJs = 1 and Heff = Hx + Hk; no demag on this context).
+    {
+        tmg::LLGWrapper w;
+        w.init(n);
+        w.bind(gpucalc.handle(), 0.5, false);
+        tmg::DeviceVector u((size_t)3 * n), udot;
+        if (hipMemcpy(u.data(), m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return 1;
+        w(u, udot);
+        std::vector<double> got = udot.host(), Hs((size_t)3 * n);
+        for (size_t q = 0; q < Hs.size(); ++q) Hs[q] = Hx.v[q] + Hk.v[q];
+        ref_llg_rhs(n, 0.5, 1, m.data(), Hs.data(), ref.data());
+        expect("LLGWrapper::rhs_d", rel(got.data(), ref.data(), ref.size()), 1e-13);
+    }
+
     // GpuPotential (DemagField::initializeSolvers, DemagField.cpp:243-250)
     GpuPotential gp;
     gp.setGradientMatrices(sparse(tX, 1), sparse(tY, 1), sparse(tZ, 1), sparse(gX, 1), sparse(gY, 1), sparse(gZ, 1), Js);
