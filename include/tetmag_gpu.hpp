@@ -426,11 +426,14 @@ class LLGWrapper {
         alpha_ = alpha;
         variant_ = removePrecession ? TM_LLG_NOPREC : TM_LLG_CLASSIC;
     }
-    // rhs_d: u, udot are 3n device pointers (SoA x|y|z).
+    // rhs_d: u, udot are 3n device pointers (SoA x|y|z).  Synchronous like
+    // the reference (udot is complete on return, whatever stream the caller
+    // reads it from).
     void rhs_d(const double* u, double* udot, tm_eval_stats* stats = nullptr) {
         if (!ctx_) throw Error(TM_ERR_NOT_READY, "LLGWrapper::rhs_d before bind");
         check(tm_eval_rhs(ctx_, u, TM_DEVICE, alpha_, variant_, udot, TM_DEVICE, nullptr, TM_DEVICE, stats),
               "tm_eval_rhs");
+        check(tm_ctx_synchronize(ctx_), "tm_ctx_synchronize");
     }
     // operator()(dev_vec) (LLGWrapper.cu:104-107)
     void operator()(const DeviceVector& u, DeviceVector& udot) {

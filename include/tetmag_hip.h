@@ -241,6 +241,36 @@ int tm_demag_last_potentials(tm_ctx* c, double* u1, double* u2, int where);
 int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int variant, double* dmdt3n, int dwhere,
                 double* Heff3n, int hwhere, tm_eval_stats* stats);
 
+/* Energies as densities (J/m^3, divided by the total volume) and volume
+ * averages, as TheSimulation::calculateEnergies (TheSimulation.cpp:384-396)
+ * and getMeanM / TheLLG::getMeanH (TheSimulation.cpp:100-102,
+ * TheLLG.cpp:421-423) report them; cubic and surface terms are zero, as on
+ * the reference's GPU path. */
+typedef struct tm_energies {
+    double demag;     /* -1/2 sum Js V Hdem.m    (DemagField.cpp:97-99) */
+    double exchange;  /* -1/2 sum V Hxc.m        (EffFieldCalc.cpp:65-69) */
+    double uniaxial;  /* sum V Ku (1 - (k.m)^2)  (EffFieldCalc.cpp:86-95) */
+    double zeeman;    /* -sum Js V Hext.m        (TheLLG.cpp:345-348) */
+    double cubic;
+    double surface;
+    double dmi;       /* -1/2 sum V Hdmi.m       (EffFieldCalc.cpp:172-175) */
+    double total;
+    double mean_m[3];
+    double mean_h[3]; /* applied field average, A/m */
+    double volume;    /* sum of node volumes (mesh units^3) */
+} tm_energies;
+
+/* Node volumes (MeshData NodeVolume) used by the energies. */
+int tm_ctx_set_node_volume(tm_ctx* c, const double* V);
+/* Energies of m (null = the context's m).  recompute_demag = 0 reuses the
+ * last potentials (TheSimulation::calculateEnergies with calcDemag false). */
+int tm_energies_eval(tm_ctx* c, const double* m3n, int mwhere, int recompute_demag, tm_energies* out);
+/* "freeze demag" (ProgramSpecs.cpp:81, TheSimulation.cpp:235-236): when on,
+ * tm_eval_rhs reuses the potentials of the last demag solve, refreshed by
+ * tm_ctx_refresh_demag (or by any call that solves). */
+int tm_ctx_set_freeze_demag(tm_ctx* c, int on);
+int tm_ctx_refresh_demag(tm_ctx* c, const double* m3n, int mwhere, tm_eval_stats* stats);
+
 /* Explicit Euler advance of the context magnetisation with renormalisation:
  * m <- (m + dtau dm/dtau) / |m + dtau dm/dtau|  (the renormalisation of
  * EffFieldGPU::NormalizeMag, EffFieldGPU.cu:625-663, fused with the update). */

@@ -80,7 +80,9 @@ def lib():
 
 
 def p(a):
-    return None if a is None else a.ctypes.data
+    # ndarray.ctypes carries _as_parameter_ and a reference to the array, so a
+    # temporary (p(f64(x))) stays alive until the ctypes call has returned.
+    return None if a is None else a.ctypes
 
 
 def f64(a):

@@ -96,3 +96,21 @@ def test_gpupotential(setup):
     assert rel_l2(gp.calcDivM(m.reshape(3, n).T), o.div(m, prob.Js) * mu0) < 1e-13
     u = np.random.default_rng(8).normal(size=n)
     assert rel_l2(gp.calcGradientField(u).T.reshape(-1), o.grad_field(u)) < 1e-13
+
+
+def test_temporaries_survive_the_call(setup):
+    """Arrays converted on the fly (non-contiguous or non-float64 inputs) stay
+    alive until the C call has read them."""
+    import gc
+    from tetmag_amd.gpu import EffFieldGPU
+    prob, o = setup
+    n = o.n
+    g = EffFieldGPU(n)
+    m = prob.m0
+    g.setMagDev(m.reshape(3, n).T)
+    H = np.random.default_rng(3).normal(size=(n, 3))
+    for _ in range(5):
+        out = g.ClassicLLG_hst(np.asfortranarray(H)[:, :], 0.2)  # converted copy inside the wrapper
+        gc.collect()
+        ref = o.llg(m, np.ascontiguousarray(H.T).reshape(-1), 0.2, variant=1)
+        assert rel_l2(out, ref) < 1e-14

@@ -145,3 +145,32 @@ def test_llg_invariants(pair):
     dm, H, _ = ctx.eval_rhs_host(m)
     dots = dm[:n] * m[:n] + dm[n:2 * n] * m[n:2 * n] + dm[2 * n:] * m[2 * n:]
     assert np.max(np.abs(dots)) <= 1e-12 * np.max(np.abs(dm))
+
+
+def test_energies(pair):
+    """tm_energies_eval against the oracle's energy functions on the oracle's
+    fields (densities, J/m^3), plus volume averages of m and of the applied field."""
+    prob, orc, ctx = pair
+    n = prob.n
+    m = prob.m0
+    ref = _oracle_eval(prob, orc, m)
+    fields = {"Hexc": ref["Hexc"], "Hdem": ref["Hdem"]}
+    if "Hdmi" in ref:
+        fields["Hdmi"] = ref["Hdmi"]
+    fields["Hext"] = np.concatenate([np.full(n, c) for c in prob.hext])
+    eo = orc.energies(m, fields, Js=prob.Js, Ku=prob.Ku, axis=prob.kaxis)
+    V = orc.nodevol.sum()
+    eg = ctx.energies(m)
+    assert eg["volume"] == pytest.approx(V, rel=1e-13)
+    scale = max(abs(v) for v in eo.values()) / V
+    for k, v in eo.items():
+        assert abs(eg[k] - v / V) <= 1e-10 * scale, (k, eg[k], v / V)
+    tot = sum(eo.values()) / V
+    assert abs(eg["total"] - tot) <= 1e-10 * scale
+    w = orc.nodevol / V
+    for d in range(3):
+        assert eg["mean_m"][d] == pytest.approx(float(w @ m[d * n:(d + 1) * n]), abs=1e-13)
+        assert eg["mean_h"][d] == pytest.approx(prob.hext[d], abs=1e-9 * (1 + abs(prob.hext[d])))
+    # frozen potentials: reusing the last solve gives the same demag energy
+    eg2 = ctx.energies(None, recompute_demag=False)
+    assert eg2["demag"] == pytest.approx(eg["demag"], rel=1e-12)

@@ -85,6 +85,19 @@ class PhaseTimes(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Energies(C.Structure):
+    _fields_ = [("demag", C.c_double), ("exchange", C.c_double), ("uniaxial", C.c_double),
+                ("zeeman", C.c_double), ("cubic", C.c_double), ("surface", C.c_double), ("dmi", C.c_double),
+                ("total", C.c_double), ("mean_m", C.c_double * 3), ("mean_h", C.c_double * 3),
+                ("volume", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("mean_m", "mean_h")}
+        d["mean_m"] = list(self.mean_m)
+        d["mean_h"] = list(self.mean_h)
+        return d
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("nsamples", C.c_int),
@@ -167,6 +180,10 @@ _SIGNATURES = {
     "tm_ctx_get_timers": [_P, _P, _I],
     "tm_solver_amg_info": [_P, _P, _P, _I],
     "tm_ctx_solver_amg_info": [_P, _I, _P, _P, _I],
+    "tm_ctx_set_node_volume": [_P, _P],
+    "tm_energies_eval": [_P, _P, _I, _I, _P],
+    "tm_ctx_set_freeze_demag": [_P, _I],
+    "tm_ctx_refresh_demag": [_P, _P, _I, _P],
     "tm_batch_create": [_P, _I, _I, _PP],
     "tm_batch_destroy": [_P],
     "tm_batch_set_zeeman": [_P, _P],
@@ -219,18 +236,36 @@ def check(status: int, what: str) -> int:
     return status
 
 
-def ptr(a) -> int:
-    """Address of a C-contiguous numpy array or a torch tensor (host or device)."""
+class _Buf:
+    """A buffer address for a ctypes call that keeps its owner alive.
+
+    ``ptr(f64(x))`` may be handed a temporary array; returning a bare integer
+    would let Python free the array before the C function reads it.  ctypes
+    converts this object through ``_as_parameter_`` while the argument (and so
+    the owner) is alive for the duration of the call."""
+    __slots__ = ("_as_parameter_", "owner")
+
+    def __init__(self, addr: int, owner):
+        self._as_parameter_ = C.c_void_p(addr)
+        self.owner = owner
+
+    def __int__(self):
+        return self._as_parameter_.value or 0
+
+
+def ptr(a):
+    """Address of a C-contiguous numpy array or a torch tensor (host or device),
+    as an object that keeps the array or tensor alive while it is referenced."""
     if a is None:
         return None
     if isinstance(a, np.ndarray):
         if not a.flags["C_CONTIGUOUS"]:
             raise ValueError("array must be C-contiguous")
-        return a.ctypes.data
+        return _Buf(a.ctypes.data, a)
     if hasattr(a, "data_ptr"):
         if not a.is_contiguous():
             raise ValueError("tensor must be contiguous")
-        return a.data_ptr()
+        return _Buf(a.data_ptr(), a)
     raise TypeError(f"unsupported buffer type {type(a)}")
 
 

@@ -51,6 +51,7 @@ class FieldContext:
                      (N.TM_OP_GRAD_X, ops.gx), (N.TM_OP_GRAD_Y, ops.gy), (N.TM_OP_GRAD_Z, ops.gz)):
             self._set_op(o, rp, col, v)
         N.check(L.tm_ctx_set_js(self.h, N.ptr(N.f64(prob.Js))), "tm_ctx_set_js")
+        N.check(L.tm_ctx_set_node_volume(self.h, N.ptr(N.f64(prob.ops.nodevol))), "tm_ctx_set_node_volume")
         if prob.use_uniaxial:
             N.check(L.tm_ctx_set_uniaxial(self.h, N.ptr(N.f64(prob.kaxis)), N.ptr(N.f64(prob.Ku))),
                     "tm_ctx_set_uniaxial")
@@ -203,6 +204,27 @@ class FieldContext:
         sizes = np.zeros(16, np.int32)
         N.check(self.L.tm_ctx_solver_amg_info(self.h, which, C.byref(nl), N.ptr(sizes), 16), "tm_ctx_solver_amg_info")
         return sizes[:nl.value].tolist()
+
+    def energies(self, m: np.ndarray | None = None, recompute_demag: bool = True) -> dict:
+        """Energy densities (J/m^3) and volume averages of m (None = current m)."""
+        e = N.Energies()
+        mp = N.ptr(N.f64(m)) if m is not None else None
+        N.check(self.L.tm_energies_eval(self.h, mp, N.TM_HOST, int(recompute_demag), C.byref(e)), "tm_energies_eval")
+        return e.as_dict()
+
+    def set_freeze_demag(self, on: bool):
+        N.check(self.L.tm_ctx_set_freeze_demag(self.h, int(on)), "tm_ctx_set_freeze_demag")
+
+    def refresh_demag(self, m=None):
+        st = N.EvalStats()
+        if m is None:
+            N.check(self.L.tm_ctx_refresh_demag(self.h, None, N.TM_DEVICE, C.byref(st)), "tm_ctx_refresh_demag")
+        elif isinstance(m, np.ndarray):
+            N.check(self.L.tm_ctx_refresh_demag(self.h, N.ptr(N.f64(m)), N.TM_HOST, C.byref(st)),
+                    "tm_ctx_refresh_demag")
+        else:
+            N.check(self.L.tm_ctx_refresh_demag(self.h, N.ptr(m), N.TM_DEVICE, C.byref(st)), "tm_ctx_refresh_demag")
+        return st.as_dict()
 
     def get_mag(self) -> np.ndarray:
         out = np.empty(3 * self.n)

@@ -55,9 +55,13 @@ struct tm_ctx {
     DevBuf<double> ku, kaxis;      // n, 3n
     DevBuf<double> dmi_fac, nvs;   // n (D / V), 3n (node normal * node area)
     DevBuf<double> eta;            // 3n
+    DevBuf<double> vol;            // n, node volumes (energies, mean m / mean H)
+    DevBuf<double> epart, eout;    // energy partial sums and totals
     DevBuf<double> hext;           // 3n, when per-node
     double hext_u[3] = {0, 0, 0};
-    bool has_js = false, has_ku = false, has_dmi = false, has_eta = false, hext_field = false;
+    bool has_js = false, has_ku = false, has_dmi = false, has_eta = false, hext_field = false, has_vol = false;
+    bool have_u = false;           // u1/u2 hold the potentials of an earlier demag solve
+    bool freeze_demag = false;     // reuse u1/u2 in tm_eval_rhs (ProgramSpecs.cpp:81)
     // state and work
     DevBuf<double> m, w3a, w3b;    // 3n
     DevBuf<double> wn;             // n
@@ -507,6 +511,129 @@ int eval_dispatch(const EvalArgs& a, bool e, bool d, bool m, int grid, hipStream
         case 5: launch_eval<true, false, true>(a, grid, s); break;
         case 6: launch_eval<true, true, false>(a, grid, s); break;
         default: launch_eval<true, true, true>(a, grid, s); break;
+    }
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+// Energies and volume averages (TheSimulation.cpp:100-102, 384-396):
+//   exchange  -1/2 sum V Hxc . m          (EffFieldCalc.cpp:65-69, Hxc in J/m^3)
+//   uniaxial  sum V Ku - sum V Ku (k.m)^2 (EffFieldCalc.cpp:86-95)
+//   zeeman    -sum Js V Hext . m          (TheLLG.cpp:345-348)
+//   dmi       -1/2 sum V Hdmi . m         (EffFieldCalc.cpp:172-175)
+//   demag     -1/2 sum Js V Hdem . m      (DemagField.cpp:97-99)
+//   <m> = sum V m / sum V,  <H> = sum V Hext / sum V  (TheLLG.cpp:421-423)
+// One SELL pass recomputes the row terms; each block writes kEnergyTerms
+// partial sums, and k_energy_finish adds them in a fixed order (FP64).
+constexpr int kEnergyTerms = 12;  // xc, ku, zee, dmi, dem, m(3), hext(3), vol
+
+template <bool EXCH, bool DEMAG, bool DMI>
+__global__ __launch_bounds__(kBlock) void k_energy(EvalArgs a, const double* __restrict__ vol,
+                                                   const double* __restrict__ js, double* __restrict__ part) {
+    __shared__ double smem[kEnergyTerms * (kBlock / kWave)];
+    const SellView A = a.A;
+    const int n = A.n;
+    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int s = row / kWave, lane = row & (kWave - 1);
+    double e[kEnergyTerms];
+#pragma unroll
+    for (int q = 0; q < kEnergyTerms; ++q) e[q] = 0.0;
+    if (s < (n + kWave - 1) / kWave) {
+        const double* __restrict__ m = a.m;
+        double ex = 0, ey = 0, ez = 0, dx = 0, dy = 0, dz = 0, cx = 0, cy = 0, cz = 0;
+        sell_row(A, s, lane, [&](int64_t q, int cc) {
+            double mx = 0, my = 0, mz = 0;
+            if (EXCH || DMI) {
+                mx = m[cc];
+                my = m[n + cc];
+                mz = m[2 * n + cc];
+            }
+            if (EXCH) {
+                const double v = a.xc[q];
+                ex += v * mx;
+                ey += v * my;
+                ez += v * mz;
+            }
+            if (DEMAG) {
+                const double u = a.u1[cc] + a.u2[cc];
+                dx += a.gx[q] * u;
+                dy += a.gy[q] * u;
+                dz += a.gz[q] * u;
+            }
+            if (DMI) {
+                const double X = a.cx[q], Y = a.cy[q], Z = a.cz[q];
+                cx += Y * mz - Z * my;
+                cy += Z * mx - X * mz;
+                cz += X * my - Y * mx;
+            }
+        });
+        if (row < n) {
+            const double V = vol[row];
+            const double J = js ? js[row] : 0.0;
+            const double mx = m[row], my = m[n + row], mz = m[2 * n + row];
+            if (EXCH) e[0] = V * (ex * mx + ey * my + ez * mz);
+            if (a.ku) {
+                const double kx = a.kaxis[row], ky = a.kaxis[n + row], kz = a.kaxis[2 * n + row];
+                const double d = kx * mx + ky * my + kz * mz;
+                e[1] = V * a.ku[row] - a.ku[row] * V * d * d;
+            }
+            double hx = a.hx, hy = a.hy, hz = a.hz;
+            if (a.hext) {
+                hx += a.hext[row];
+                hy += a.hext[n + row];
+                hz += a.hext[2 * n + row];
+            }
+            e[2] = J * V * (hx * mx + hy * my + hz * mz);
+            if (DMI) {
+                const double sx = a.nvs[row], sy = a.nvs[n + row], sz = a.nvs[2 * n + row];
+                const double f = a.dmi_fac[row];
+                const double px = ((my * sz - mz * sy) - 2.0 * cx) * f;
+                const double py = ((mz * sx - mx * sz) - 2.0 * cy) * f;
+                const double pz = ((mx * sy - my * sx) - 2.0 * cz) * f;
+                e[3] = V * (px * mx + py * my + pz * mz);
+            }
+            if (DEMAG) e[4] = -J * V * (dx * mx + dy * my + dz * mz);  // Hdem = -grad u
+            e[5] = V * mx;
+            e[6] = V * my;
+            e[7] = V * mz;
+            e[8] = V * hx;
+            e[9] = V * hy;
+            e[10] = V * hz;
+            e[11] = V;
+        }
+    }
+    block_sum<kEnergyTerms>(e, smem);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < kEnergyTerms; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = e[q];
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_finish(int G, const double* __restrict__ part,
+                                                          double* __restrict__ out) {
+    double t[kEnergyTerms];
+    reduce_partials_all<kEnergyTerms>(part, G, G, t);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < kEnergyTerms; ++q) out[q] = t[q];
+}
+
+template <bool E, bool D, bool M>
+void launch_energy(const EvalArgs& a, const double* vol, const double* js, double* part, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_energy<E, D, M>), dim3(grid), dim3(kBlock), 0, s, a, vol, js, part);
+}
+
+int energy_dispatch(const EvalArgs& a, bool e, bool d, bool m, const double* vol, const double* js, double* part,
+                    int grid, hipStream_t s) {
+    const int key = (e ? 4 : 0) | (d ? 2 : 0) | (m ? 1 : 0);
+    switch (key) {
+        case 0: launch_energy<false, false, false>(a, vol, js, part, grid, s); break;
+        case 1: launch_energy<false, false, true>(a, vol, js, part, grid, s); break;
+        case 2: launch_energy<false, true, false>(a, vol, js, part, grid, s); break;
+        case 3: launch_energy<false, true, true>(a, vol, js, part, grid, s); break;
+        case 4: launch_energy<true, false, false>(a, vol, js, part, grid, s); break;
+        case 5: launch_energy<true, false, true>(a, vol, js, part, grid, s); break;
+        case 6: launch_energy<true, true, false>(a, vol, js, part, grid, s); break;
+        default: launch_energy<true, true, true>(a, vol, js, part, grid, s); break;
     }
     TM_HIP(hipGetLastError());
     return TM_OK;
@@ -974,6 +1101,7 @@ int demag_potential_dev(tm_ctx* c, const double* md, tm_eval_stats* stats) {
     c->tsum.iters_neumann += itN;
     c->tsum.iters_dirichlet += itD;
     if (st != TM_OK && st != TM_ERR_MAXITER) return st;
+    c->have_u = true;
     if (stats) {
         stats->iters_neumann = itN;
         stats->iters_dirichlet = itD;
@@ -1039,9 +1167,10 @@ int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int vari
     TM_CHECK(commit_ops(c));
     tm_eval_stats local = {};
     const bool demag = c->has_solvers && c->has_B;
+    const bool solve = demag && !(c->freeze_demag && c->have_u);
     int st = TM_OK;
     if (demag) {
-        st = demag_potential_dev(c, c->m.p, &local);
+        if (solve) st = demag_potential_dev(c, c->m.p, &local);
         if (st != TM_OK && st != TM_ERR_MAXITER) return st;
         if (!c->has_op[TM_OP_GRAD_X] || !c->has_op[TM_OP_GRAD_Y] || !c->has_op[TM_OP_GRAD_Z]) return TM_ERR_NOT_READY;
     }
@@ -1079,7 +1208,7 @@ int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int vari
     a.torque = stats ? c->scal.p : nullptr;
     if (stats) hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, c->stream, c->scal.p);
     TM_CHECK(eval_dispatch(a, exch, demag, dmi, sell_grid(c), c->stream));
-    if (c->timers && demag) {
+    if (c->timers && solve) {
         TM_HIP(hipEventRecord(c->ev[5], c->stream));
         TM_HIP(hipEventSynchronize(c->ev[5]));
         float t[5];
@@ -1099,6 +1228,92 @@ int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int vari
         TM_HIP(hipStreamSynchronize(c->stream));
         *stats = local;
         stats->max_torque = mt;
+    }
+    return st;
+}
+
+int tm_ctx_set_node_volume(tm_ctx* c, const double* V) {
+    TM_CHECK(ctx_ok(c));
+    TM_CHECK(upload_vec(c, c->vol, V, c->n));
+    c->has_vol = true;
+    return TM_OK;
+}
+
+int tm_ctx_set_freeze_demag(tm_ctx* c, int on) {
+    TM_CHECK(ctx_ok(c));
+    c->freeze_demag = on != 0;
+    return TM_OK;
+}
+
+int tm_ctx_refresh_demag(tm_ctx* c, const double* m3n, int mwhere, tm_eval_stats* stats) {
+    TM_CHECK(ctx_ok(c));
+    if (m3n) TM_CHECK(copy_in(c->m.p, m3n, mwhere, (size_t)3 * c->n, c->stream));
+    int st = demag_potential_dev(c, c->m.p, stats);
+    if (st == TM_OK || st == TM_ERR_MAXITER) TM_HIP(hipStreamSynchronize(c->stream));
+    return st;
+}
+
+int tm_energies_eval(tm_ctx* c, const double* m3n, int mwhere, int recompute_demag, tm_energies* out) {
+    TM_CHECK(ctx_ok(c));
+    if (!out) return TM_ERR_BAD_ARG;
+    if (!c->has_vol) return TM_ERR_NOT_READY;
+    const int n = c->n;
+    if (m3n) TM_CHECK(copy_in(c->m.p, m3n, mwhere, (size_t)3 * n, c->stream));
+    TM_CHECK(commit_ops(c));
+    const bool demag = c->has_solvers && c->has_B;
+    int st = TM_OK;
+    if (demag) {
+        if (recompute_demag || !c->have_u) st = demag_potential_dev(c, c->m.p, nullptr);
+        if (st != TM_OK && st != TM_ERR_MAXITER) return st;
+        if (!c->has_op[TM_OP_GRAD_X] || !c->has_op[TM_OP_GRAD_Y] || !c->has_op[TM_OP_GRAD_Z]) return TM_ERR_NOT_READY;
+    }
+    const bool exch = c->has_op[TM_OP_EXCHANGE];
+    const bool dmi = c->has_dmi && c->has_op[TM_OP_CURL_X] && c->has_op[TM_OP_CURL_Y] && c->has_op[TM_OP_CURL_Z];
+    EvalArgs a = {};
+    a.A = view(c->pat);
+    a.xc = exch ? c->opv[TM_OP_EXCHANGE].p : nullptr;
+    if (demag) {
+        a.gx = c->opv[TM_OP_GRAD_X].p;
+        a.gy = c->opv[TM_OP_GRAD_Y].p;
+        a.gz = c->opv[TM_OP_GRAD_Z].p;
+        a.u1 = c->u1.p;
+        a.u2 = c->u2.p;
+    }
+    if (dmi) {
+        a.cx = c->opv[TM_OP_CURL_X].p;
+        a.cy = c->opv[TM_OP_CURL_Y].p;
+        a.cz = c->opv[TM_OP_CURL_Z].p;
+        a.dmi_fac = c->dmi_fac.p;
+        a.nvs = c->nvs.p;
+    }
+    a.m = c->m.p;
+    a.ku = c->has_ku ? c->ku.p : nullptr;
+    a.kaxis = c->has_ku ? c->kaxis.p : nullptr;
+    a.hext = c->hext_field ? c->hext.p : nullptr;
+    a.hx = c->hext_u[0];
+    a.hy = c->hext_u[1];
+    a.hz = c->hext_u[2];
+    const int G = sell_grid(c);
+    TM_CHECK(c->epart.alloc(std::max(c->epart.n, (size_t)kEnergyTerms * G)));
+    TM_CHECK(c->eout.alloc(std::max(c->eout.n, (size_t)kEnergyTerms)));
+    TM_CHECK(energy_dispatch(a, exch, demag, dmi, c->vol.p, c->has_js ? c->js.p : nullptr, c->epart.p, G, c->stream));
+    hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(kBlock), 0, c->stream, G, (const double*)c->epart.p, c->eout.p);
+    TM_HIP(hipGetLastError());
+    double t[kEnergyTerms];
+    TM_HIP(hipMemcpyAsync(t, c->eout.p, sizeof(t), hipMemcpyDeviceToHost, c->stream));
+    TM_HIP(hipStreamSynchronize(c->stream));
+    const double V = t[11];
+    *out = tm_energies{};
+    out->volume = V;
+    out->exchange = -0.5 * t[0] / V;
+    out->uniaxial = t[1] / V;
+    out->zeeman = -t[2] / V;
+    out->dmi = -0.5 * t[3] / V;
+    out->demag = -0.5 * t[4] / V;
+    out->total = out->exchange + out->uniaxial + out->zeeman + out->dmi + out->demag;
+    for (int d = 0; d < 3; ++d) {
+        out->mean_m[d] = t[5 + d] / V;
+        out->mean_h[d] = t[8 + d] / V;
     }
     return st;
 }
