@@ -119,6 +119,12 @@ int tm_solver_solve(tm_solver* s, const double* b, int bwhere, double* x, int xw
 /* Preconditioner hierarchy of an AMG solver: *nlevels receives the number of
  * levels including the dense coarsest one, sizes[0..min(cap,nlevels)-1] their
  * unknowns (0 levels for a Jacobi solver). */
+/* Solves A^T x = b for the solver's matrix A (symmetric apart from identity
+ * rows): CG on the free block, then the identity rows' values from the
+ * coupling.  Warm start: a separate solution vector from tm_solver_solve.
+ * The adjoint of GPUSolver::solve (no counterpart in the reference). */
+int tm_solver_solve_transposed(tm_solver* s, const double* b, int bwhere, double* x, int xwhere, double tol,
+                               int maxit, int warm, int* iters, double* relres);
 int tm_solver_amg_info(tm_solver* s, int* nlevels, int* sizes, int cap);
 
 /* ---- field context: EffFieldGPU + GpuPotential + DemagField state ------- */
@@ -270,6 +276,25 @@ int tm_energies_eval(tm_ctx* c, const double* m3n, int mwhere, int recompute_dem
  * tm_ctx_refresh_demag (or by any call that solves). */
 int tm_ctx_set_freeze_demag(tm_ctx* c, int on);
 int tm_ctx_refresh_demag(tm_ctx* c, const double* m3n, int mwhere, tm_eval_stats* stats);
+
+/* ---- adjoints (no counterpart in the reference; gradients of trajectory
+ * objectives with respect to m, applied fields, Ku and D) -------------------
+ * Vector-Jacobian product of the right-hand side f(m) = LLG(m, Heff(m)) of
+ * tm_eval_rhs: g_m = (df/dm)^T lam, including the transposed demag chain
+ * (transposed gradient, Dirichlet and Neumann solves with the transposed
+ * row-replaced matrices, B^T) unless demag is frozen.  The forward is
+ * re-evaluated at m, which becomes the context's magnetisation.  Optional
+ * outputs (null to skip): g_hext3 (host, 3) = gradient w.r.t. a uniform
+ * applied field; g_hfield3n = w.r.t. a per-node applied field; g_ku (n) =
+ * w.r.t. per-node Ku; g_d (n) = w.r.t. per-node bulk D (needs node volumes).
+ * g_m, g_hfield3n, g_ku and g_d are written where gwhere says. */
+int tm_eval_rhs_vjp(tm_ctx* c, const double* m3n, int mwhere, double alpha, int variant, const double* lam3n,
+                    int lwhere, double* gm3n, int gwhere, double* g_hext3, double* g_hfield3n, double* g_ku,
+                    double* g_d, tm_eval_stats* stats);
+/* g = (dHdem/dm)^T v (Hdem is linear in m). */
+int tm_demag_field_vjp(tm_ctx* c, const double* v3n, int vwhere, double* g3n, int gwhere, tm_eval_stats* stats);
+/* x_b = B^T y_b on the boundary (the transpose of tm_bem_apply). */
+int tm_bem_apply_transposed(tm_ctx* c, const double* yb, int ywhere, double* xb, int xwhere);
 
 /* Explicit Euler advance of the context magnetisation with renormalisation:
  * m <- (m + dtau dm/dtau) / |m + dtau dm/dtau|  (the renormalisation of

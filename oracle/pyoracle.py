@@ -296,6 +296,88 @@ class Oracle:
         out["max_torque"] = self.max_torque(m, out["Heff"])
         return out
 
+    # -- FP64 CPU adjoint (no reference counterpart) ---------------------------
+    # Restated from the same operators with scipy: transposed CSR products and
+    # GMRES/ILU(0) on the transposed row-replaced matrices.  Pinned by central
+    # finite differences of eval_rhs (tests/test_oracle_adjoint.py).
+    def _csr(self, vals):
+        import scipy.sparse as sp
+        return sp.csr_matrix((vals, self.col, self.rowptr), shape=(self.n, self.n))
+
+    def transposed_solvers(self):
+        if getattr(self, "_tsolvers", None) is None:
+            import scipy.sparse as sp
+            out = []
+            for rp, col, val in (self.neumann_matrix(), self.dirichlet_matrix()):
+                T = sp.csr_matrix((val, col, rp), shape=(self.n, self.n)).T.tocsr()
+                T.sort_indices()
+                out.append(Solver(i32(T.indptr), i32(T.indices), f64(T.data)))
+            self._tsolvers = tuple(out)
+        return self._tsolvers
+
+    def demag_vjp(self, v, Js, tol=1e-14):
+        """(dHdem/dm)^T v for Hdem = -grad(u1 + u2) (DemagField.cpp:92-240)."""
+        n = self.n
+        v = f64(v).reshape(3, n)
+        G = [self._csr(g) for g in (self.gX, self.gY, self.gZ)]
+        T = [self._csr(t) for t in (self.tX, self.tY, self.tZ)]
+        w0 = -(G[0].T @ v[0] + G[1].T @ v[1] + G[2].T @ v[2])
+        sNt, sDt = self.transposed_solvers()
+        y2 = sDt.solve(w0, tol=tol, warm=False)
+        t = w0.copy()
+        t[self.bnodes] += self.B.T @ y2[self.bnodes]
+        y1 = sNt.solve(t, tol=tol, warm=False)
+        return np.concatenate([f64(Js) / MU0 * (T[d].T @ y1) for d in range(3)])
+
+    def eval_rhs_vjp(self, m, lam, *, Js, A, Ku=None, axis=None, D=None, Hext=None, alpha=0.5, variant=1,
+                     demag=True, tol=1e-14):
+        """lam^T d f / d(m, Hext, Ku, D) for f = eval_rhs(m)["dmdt"]."""
+        n = self.n
+        m = f64(m)
+        fw = self.eval_rhs(m, Js=Js, A=A, Ku=Ku, axis=axis, D=D, Hext=Hext, alpha=alpha, variant=variant,
+                           demag=demag, tol=tol, warm=False)
+        M = m.reshape(3, n).T
+        H = fw["Heff"].reshape(3, n).T
+        L = f64(lam).reshape(3, n).T
+        ml = np.cross(M, L)
+        gH = -alpha * np.cross(M, ml)
+        gm = alpha * (np.cross(L, np.cross(M, H)) - np.cross(H, np.cross(L, M)))
+        if variant == 1:
+            gH += ml
+            gm += np.cross(L, H)
+        Jsv = f64(Js)
+        with np.errstate(divide="ignore"):
+            invjs = np.where(Jsv != 0, 1.0 / Jsv, 0.0)
+        gp = gH * invjs[:, None]
+        V = self.nodevol
+        Kop = self._csr(self.K)
+        XC = Kop.multiply((-2.0 * f64(A) / V)[:, None]).tocsr()
+        gm += np.stack([XC.T @ gp[:, c] for c in range(3)], axis=1)
+        out = {"gH": np.ascontiguousarray(gH.T.reshape(-1)), "g_hext": gH.sum(axis=0)}
+        if Ku is not None and np.any(Ku != 0):
+            k = f64(axis).reshape(3, n).T
+            kg = np.sum(k * gp, axis=1)
+            gm += (2.0 * f64(Ku) * kg)[:, None] * k
+            out["g_ku"] = 2.0 * np.sum(k * M, axis=1) * kg
+        if D is not None and np.any(D != 0):
+            f = f64(D) / V
+            S = (self.node_normal.reshape(3, n) * self.node_area).T
+            C = [-self._csr(t) for t in (self.tX, self.tY, self.tZ)]
+            gm += f[:, None] * np.cross(S, gp)
+            w = -2.0 * f[:, None] * gp
+            gm[:, 0] += C[2].T @ w[:, 1] - C[1].T @ w[:, 2]
+            gm[:, 1] += C[0].T @ w[:, 2] - C[2].T @ w[:, 0]
+            gm[:, 2] += C[1].T @ w[:, 0] - C[0].T @ w[:, 1]
+            curl = np.stack([C[1] @ M[:, 2] - C[2] @ M[:, 1], C[2] @ M[:, 0] - C[0] @ M[:, 2],
+                             C[0] @ M[:, 1] - C[1] @ M[:, 0]], axis=1)
+            out["g_d"] = np.sum(gp * (np.cross(M, S) - 2.0 * curl), axis=1) / V
+        g = np.ascontiguousarray(gm.T.reshape(-1))
+        if demag:
+            g += self.demag_vjp(out["gH"], Js, tol=tol)
+        out["gm"] = g
+        out["dmdt"] = fw["dmdt"]
+        return out
+
     # -- energies (EffFieldCalc.cpp, DemagField.cpp:97-99, TheLLG.cpp:345-348) --
     def energies(self, m, fields, *, Js, Ku=None, axis=None):
         L = lib()

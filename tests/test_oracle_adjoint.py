@@ -1,0 +1,92 @@
+"""Pinning of the oracle's FP64 CPU adjoint (Oracle.eval_rhs_vjp, demag_vjp)
+by central finite differences of the oracle's forward right-hand side, on a
+small jittered mesh with exchange, uniaxial anisotropy, bulk DMI, Zeeman and
+demag.  f is a cubic polynomial in m and linear in Hext, Ku and D, so the
+central difference error is O(eps^2); solves run to 1e-14."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+EPS = 1e-5
+
+
+@pytest.fixture(scope="module")
+def setup(oracle_mod):
+    from tetmag_amd import problem as P
+    prob = P.small_problem(nx=6, ny=5, nz=3, h=3.0, jitter=0.1, seed=13, Ku=4e4, D=3e-3, B_ext=(0.02, -0.01, 0.03))
+    o = oracle_mod.Oracle(prob.mesh.xyz, prob.mesh.fel)
+    rng = np.random.default_rng(17)
+    lam = rng.normal(size=3 * prob.n)
+    return prob, o, lam
+
+
+def _kw(prob, **over):
+    kw = dict(Js=prob.Js, A=prob.A, Ku=prob.Ku, axis=prob.kaxis, D=prob.D, Hext=prob.hext, alpha=prob.alpha,
+              variant=1, tol=1e-14)
+    kw.update(over)
+    return kw
+
+
+def _phi(o, prob, lam, m, **over):
+    return float(lam @ o.eval_rhs(m, **_kw(prob, **over), warm=False)["dmdt"])
+
+
+@pytest.mark.parametrize("variant", [1, 0])
+def test_vjp_wrt_m(setup, variant):
+    prob, o, lam = setup
+    m = prob.m0
+    g = o.eval_rhs_vjp(m, lam, **_kw(prob, variant=variant))["gm"]
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        d = rng.normal(size=m.size)
+        fd = (_phi(o, prob, lam, m + EPS * d, variant=variant) - _phi(o, prob, lam, m - EPS * d, variant=variant)) / (2 * EPS)
+        assert abs(g @ d - fd) <= 1e-7 * (abs(fd) + np.abs(g).sum() * 1e-3), (g @ d, fd)
+
+
+def test_vjp_wrt_parameters(setup):
+    """f is linear in Hext, Ku and D, so a central difference with a finite
+    step is exact (up to rounding and solver noise, which a large step keeps
+    small)."""
+    prob, o, lam = setup
+    m = prob.m0
+    r = o.eval_rhs_vjp(m, lam, **_kw(prob))
+    for c in range(3):
+        e = np.zeros(3)
+        e[c] = 1e3
+        fd = (_phi(o, prob, lam, m, Hext=prob.hext + e) - _phi(o, prob, lam, m, Hext=prob.hext - e)) / 2
+        assert r["g_hext"] @ e == pytest.approx(fd, rel=1e-9)
+    rng = np.random.default_rng(9)
+    dK = rng.normal(size=prob.n) * 1e4
+    fd = (_phi(o, prob, lam, m, Ku=prob.Ku + dK) - _phi(o, prob, lam, m, Ku=prob.Ku - dK)) / 2
+    assert r["g_ku"] @ dK == pytest.approx(fd, rel=1e-9)
+    dD = rng.normal(size=prob.n) * prob.D[0] * 0.3
+    fd = (_phi(o, prob, lam, m, D=prob.D + dD) - _phi(o, prob, lam, m, D=prob.D - dD)) / 2
+    assert r["g_d"] @ dD == pytest.approx(fd, rel=1e-9)
+
+
+def test_demag_vjp_dot_product(setup):
+    """Hdem is linear in m: <v, Hdem(m)> = <Hdem^T v, m> for any v, m."""
+    prob, o, lam = setup
+    m = prob.m0
+    H, _, _ = o.demag(m, prob.Js, tol=1e-14, warm=False)
+    g = o.demag_vjp(lam, prob.Js)
+    assert g @ m == pytest.approx(lam @ H, rel=1e-11)
+
+
+def test_vjp_detects_a_planted_sign_error(setup, monkeypatch):
+    """A sign error in the adjoint's exchange transpose fails the FD check."""
+    prob, o, lam = setup
+    m = prob.m0
+    import pyoracle as O
+    orig = O.Oracle._csr
+
+    def flipped(self, vals):
+        return orig(self, -vals) if vals is self.K else orig(self, vals)
+
+    monkeypatch.setattr(O.Oracle, "_csr", flipped)
+    g = o.eval_rhs_vjp(m, lam, **_kw(prob))["gm"]
+    monkeypatch.setattr(O.Oracle, "_csr", orig)
+    d = np.random.default_rng(5).normal(size=m.size)
+    fd = (_phi(o, prob, lam, m + EPS * d) - _phi(o, prob, lam, m - EPS * d)) / (2 * EPS)
+    assert abs(g @ d - fd) > 1e-3 * abs(fd)

@@ -33,6 +33,7 @@ SOURCES = [
     "tm_bem.hip",
     "tm_field.hip",
     "tm_batch.hip",
+    "tm_adjoint.hip",
     "tm_setup.cpp",
 ]
 

@@ -30,7 +30,7 @@ def _torch():
 class FieldContext:
     def __init__(self, prob: Problem, *, tol: float = 1e-10, maxit: int = 100000, device: int = 0,
                  B: np.ndarray | None = None, stream=None, bem_formula: int = N.TM_BEM_REFERENCE,
-                 precond: int = N.TM_PRECOND_AMG):
+                 precond: int = N.TM_PRECOND_AMG, uniaxial: bool | None = None, dmi: bool | None = None):
         torch = _torch()
         L = N.lib()
         self.L = L
@@ -52,10 +52,12 @@ class FieldContext:
             self._set_op(o, rp, col, v)
         N.check(L.tm_ctx_set_js(self.h, N.ptr(N.f64(prob.Js))), "tm_ctx_set_js")
         N.check(L.tm_ctx_set_node_volume(self.h, N.ptr(N.f64(prob.ops.nodevol))), "tm_ctx_set_node_volume")
-        if prob.use_uniaxial:
+        self.has_uniaxial = prob.use_uniaxial if uniaxial is None else bool(uniaxial)
+        self.has_dmi = prob.use_dmi if dmi is None else bool(dmi)
+        if self.has_uniaxial:
             N.check(L.tm_ctx_set_uniaxial(self.h, N.ptr(N.f64(prob.kaxis)), N.ptr(N.f64(prob.Ku))),
                     "tm_ctx_set_uniaxial")
-        if prob.use_dmi:
+        if self.has_dmi:
             # TheLLG.cpp:94-98: curl operators are -tGrad
             for o, v in ((N.TM_OP_CURL_X, ops.tgx), (N.TM_OP_CURL_Y, ops.tgy), (N.TM_OP_CURL_Z, ops.tgz)):
                 self._set_op(o, rp, col, -v)
@@ -225,6 +227,79 @@ class FieldContext:
         else:
             N.check(self.L.tm_ctx_refresh_demag(self.h, N.ptr(m), N.TM_DEVICE, C.byref(st)), "tm_ctx_refresh_demag")
         return st.as_dict()
+
+    # -- material / field parameters (host arrays) --------------------------------
+    def set_hext(self, h3):
+        """Uniform applied field (A/m)."""
+        hx, hy, hz = (float(v) for v in h3)
+        N.check(self.L.tm_ctx_set_zeeman(self.h, None, hx, hy, hz), "tm_ctx_set_zeeman")
+
+    def set_ku(self, Ku):
+        """Per-node uniaxial constant (J/m^3) along the problem's axes."""
+        if not self.has_uniaxial:
+            raise RuntimeError("context built without uniaxial anisotropy (pass uniaxial=True)")
+        Kd = N.f64(Ku)
+        N.check(self.L.tm_ctx_set_uniaxial(self.h, N.ptr(N.f64(self.prob.kaxis)), N.ptr(Kd)), "tm_ctx_set_uniaxial")
+
+    def set_d(self, D):
+        """Per-node bulk DMI constant (J/m^2 divided by the mesh scale)."""
+        if not self.has_dmi:
+            raise RuntimeError("context built without DMI (pass dmi=True)")
+        ops, bnd = self.prob.ops, self.prob.bnd
+        N.check(self.L.tm_ctx_set_dmi(self.h, N.ptr(N.f64(D)), N.ptr(N.f64(1.0 / ops.nodevol)),
+                                      N.ptr(N.f64(bnd.node_normal)), N.ptr(N.f64(bnd.node_area))), "tm_ctx_set_dmi")
+
+    # -- adjoint ------------------------------------------------------------------
+    def vjp_host(self, m: np.ndarray, lam: np.ndarray, alpha=None, variant=N.TM_LLG_CLASSIC, want_ku=None,
+                 want_d=None) -> dict:
+        """lam^T d(dm/dtau)/d(m, Hext, Ku, D) at m (host arrays, synchronous)."""
+        alpha = self.prob.alpha if alpha is None else alpha
+        n = self.n
+        gm = np.empty(3 * n)
+        gh = np.empty(3)
+        gf = np.empty(3 * n)
+        want_ku = self.has_uniaxial if want_ku is None else want_ku
+        want_d = self.has_dmi if want_d is None else want_d
+        gku = np.empty(n) if want_ku else None
+        gd = np.empty(n) if want_d else None
+        st = N.EvalStats()
+        N.check(self.L.tm_eval_rhs_vjp(self.h, N.ptr(N.f64(m)), N.TM_HOST, float(alpha), int(variant),
+                                       N.ptr(N.f64(lam)), N.TM_HOST, N.ptr(gm), N.TM_HOST, N.ptr(gh), N.ptr(gf),
+                                       N.ptr(gku), N.ptr(gd), C.byref(st)), "tm_eval_rhs_vjp")
+        out = {"gm": gm, "g_hext": gh, "gH": gf, "stats": st.as_dict()}
+        if want_ku:
+            out["g_ku"] = gku
+        if want_d:
+            out["g_d"] = gd
+        return out
+
+    def vjp(self, m, lam, alpha=None, variant=N.TM_LLG_CLASSIC, want_ku=False, want_d=False):
+        """Device form: m, lam are (3n,) float64 tensors on the context's device.
+        Returns (g_m tensor, g_hext numpy (3,), g_ku tensor | None, g_d tensor | None)."""
+        torch = _torch()
+        alpha = self.prob.alpha if alpha is None else alpha
+        n = self.n
+        with torch.cuda.stream(self.stream):
+            gm = torch.empty(3 * n, dtype=torch.float64, device=self.device)
+            gku = torch.empty(n, dtype=torch.float64, device=self.device) if want_ku else None
+            gd = torch.empty(n, dtype=torch.float64, device=self.device) if want_d else None
+        gh = np.empty(3)
+        N.check(self.L.tm_eval_rhs_vjp(self.h, N.ptr(m), N.TM_DEVICE, float(alpha), int(variant), N.ptr(lam),
+                                       N.TM_DEVICE, N.ptr(gm), N.TM_DEVICE, N.ptr(gh), None, N.ptr(gku), N.ptr(gd),
+                                       None), "tm_eval_rhs_vjp")
+        return gm, gh, gku, gd
+
+    def demag_vjp_host(self, v: np.ndarray) -> np.ndarray:
+        g = np.empty(3 * self.n)
+        N.check(self.L.tm_demag_field_vjp(self.h, N.ptr(N.f64(v)), N.TM_HOST, N.ptr(g), N.TM_HOST, None),
+                "tm_demag_field_vjp")
+        return g
+
+    def bem_apply(self, x: np.ndarray, transposed: bool = False) -> np.ndarray:
+        y = np.empty(self.prob.bnd.bnx)
+        fn = self.L.tm_bem_apply_transposed if transposed else self.L.tm_bem_apply
+        N.check(fn(self.h, N.ptr(N.f64(x)), N.TM_HOST, N.ptr(y), N.TM_HOST), "tm_bem_apply")
+        return y
 
     def get_mag(self) -> np.ndarray:
         out = np.empty(3 * self.n)

@@ -217,6 +217,62 @@ __global__ __launch_bounds__(kBlock) void k_bem_gemv_finish(int b, int ld, int s
     if (dst) dst[sidx[r]] = s;
 }
 
+// Transposed product y = B^T x for the adjoint of the boundary integral
+// (x, y on the boundary).  Column-major B makes each output a column dot
+// product: block = 16 columns (4 per wavefront), x staged in LDS in chunks of
+// 2048 rows and shared by the 16 columns; lanes read 16-byte pairs of rows so
+// each wavefront load covers 1 KB of one column.  With dst != NULL the result
+// is also added to dst[sidx[j]] (the scatter back to the nodes).
+constexpr int kGemvTCols = 16;
+constexpr int kGemvTChunk = 2048;
+
+__global__ __launch_bounds__(kBlock) void k_bem_gemvT(int b, int ld, const double* __restrict__ B,
+                                                      const double* __restrict__ x, double* __restrict__ y,
+                                                      const int* __restrict__ sidx, double* __restrict__ dst) {
+    __shared__ double xs[kGemvTChunk];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int j0 = blockIdx.x * kGemvTCols + wave * 4;
+    dbl2 acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = dbl2{0.0, 0.0};
+    for (int r0 = 0; r0 < b; r0 += kGemvTChunk) {
+        const int rn = min(kGemvTChunk, b - r0);
+        __syncthreads();  // the previous chunk has been consumed
+        for (int i = threadIdx.x; i < kGemvTChunk; i += kBlock) xs[i] = i < rn ? x[r0 + i] : 0.0;
+        __syncthreads();
+        for (int i = 2 * lane; i < rn; i += 2 * kWave) {
+            // rows r0+i, r0+i+1: the second is a zero padding row or (x = 0)
+            // the next column's first entry only when rn is odd, which needs
+            // b odd and therefore ld > b.
+            const double x0 = xs[i], x1 = xs[i + 1];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int j = j0 + c;
+                if (j < b) {
+                    const dbl2 v = __builtin_nontemporal_load((const dbl2*)(B + (size_t)j * ld + r0 + i));
+                    acc[c].x += v.x * x0;
+                    acc[c].y += v.y * x1;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double sum = wave_sum(acc[c].x + acc[c].y);
+        const int j = j0 + c;
+        if (lane == 0 && j < b) {
+            if (y) y[j] = sum;
+            if (dst) dst[sidx[j]] += sum;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bem_gather(int b, const double* __restrict__ src,
+                                                       const int* __restrict__ gidx, double* __restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < b) out[i] = src[gidx[i]];
+}
+
 }  // namespace
 
 int bem_assemble(int b, int ld, int nbel, const double* bxyz, const int* faces, const double* farea,
@@ -251,6 +307,22 @@ int bem_gemv(int b, int ld, const double* B, const double* src, const int* gidx,
     const int cps = (b + splits - 1) / splits;
     hipLaunchKernelGGL(k_bem_gemv_part, dim3(tiles, splits), dim3(kBlock), 0, s, b, ld, B, src, gidx, cps, part);
     hipLaunchKernelGGL(k_bem_gemv_finish, dim3(grid_for(b)), dim3(kBlock), 0, s, b, ld, splits, part, y, sidx, dst);
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+// y = B^T x with x optionally gathered through gidx (into xbuf, b doubles),
+// the result optionally added to dst[sidx[j]].
+int bem_gemvT(int b, int ld, const double* B, const double* src, const int* gidx, double* xbuf, double* y,
+              const int* sidx, double* dst, hipStream_t s) {
+    if (ld % 2 != 0 || ld < b) return TM_ERR_BAD_ARG;
+    const double* x = src;
+    if (gidx) {
+        hipLaunchKernelGGL(k_bem_gather, dim3(grid_for(b)), dim3(kBlock), 0, s, b, src, gidx, xbuf);
+        x = xbuf;
+    }
+    hipLaunchKernelGGL(k_bem_gemvT, dim3((b + kGemvTCols - 1) / kGemvTCols), dim3(kBlock), 0, s, b, ld, B, x, y,
+                       sidx, dst);
     TM_HIP(hipGetLastError());
     return TM_OK;
 }

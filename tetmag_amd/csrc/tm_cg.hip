@@ -191,6 +191,26 @@ __global__ __launch_bounds__(kBlock) void k_cg_final(int G, double* part, CgStat
 
 }  // namespace
 
+// Transposed solve helpers: zero the fixed entries of the right-hand side,
+// then y_j = w_j - sum_i A_ij y_i over the free rows i coupled to fixed j.
+__global__ __launch_bounds__(kBlock) void k_adj_zero_fixed(int n, const unsigned char* __restrict__ fixed,
+                                                           const double* __restrict__ w, double* __restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) out[i] = fixed[i] ? 0.0 : w[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_fix_rows(int nf, const int* __restrict__ idx,
+                                                         const int* __restrict__ rp, const int* __restrict__ ci,
+                                                         const double* __restrict__ v, const double* __restrict__ w,
+                                                         double* __restrict__ y) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= nf) return;
+    const int j = idx[r];
+    double acc = w[j];
+    for (int q = rp[r]; q < rp[r + 1]; ++q) acc -= v[q] * y[ci[q]];
+    y[j] = acc;
+}
+
 CgSolver::~CgSolver() {
     if (h_st) (void)hipHostFree(h_st);
     h_st = nullptr;
@@ -238,6 +258,40 @@ int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* 
         if (fx[i]) di[i] = 1.0;
         else if (d > 0.0) di[i] = 1.0 / d;
         else return TM_ERR_UNSUPPORTED;  // free block must be positive definite
+    }
+    // fixed-row coupling for the transposed solve
+    {
+        std::vector<int> fidx, frp(1, 0), fci, pos((size_t)n, -1);
+        for (int i = 0; i < n; ++i)
+            if (fx[i]) {
+                pos[i] = (int)fidx.size();
+                fidx.push_back(i);
+            }
+        std::vector<std::vector<std::pair<int, double>>> rows(fidx.size());
+        for (int i = 0; i < n; ++i) {
+            if (fx[i]) continue;
+            for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+                const int j = colidx[q];
+                if (fx[j] && vals[q] != 0.0) rows[pos[j]].push_back({i, vals[q]});
+            }
+        }
+        std::vector<double> fv;
+        for (auto& r : rows) {
+            for (auto& e : r) {
+                fci.push_back(e.first);
+                fv.push_back(e.second);
+            }
+            frp.push_back((int)fci.size());
+        }
+        if (!fidx.empty()) {
+            TM_CHECK(fix_idx.upload(fidx.data(), fidx.size(), s));
+            TM_CHECK(fix_rp.upload(frp.data(), frp.size(), s));
+            if (!fci.empty()) {
+                TM_CHECK(fix_ci.upload(fci.data(), fci.size(), s));
+                TM_CHECK(fix_v.upload(fv.data(), fv.size(), s));
+            }
+            TM_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+        }
     }
     pat.build(n, n, rowptr, colidx);
     h_val.assign(vals, vals + rowptr[n]);
@@ -322,6 +376,22 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
     return TM_ERR_MAXITER;
 }
 
+int CgSolver::solve_transposed(const double* w, double* y, double tol, int maxit, hipStream_t s, int* iters,
+                               double* relres) {
+    if (n == 0) return TM_ERR_NOT_READY;
+    TM_CHECK(tb.alloc(n));
+    hipLaunchKernelGGL(k_adj_zero_fixed, dim3(grid_for(n)), dim3(kBlock), 0, s, n, fixed.p, w, tb.p);
+    TM_HIP(hipGetLastError());
+    const int st = solve(tb.p, y, tol, maxit, s, iters, relres);
+    if (st != TM_OK && st != TM_ERR_MAXITER) return st;
+    if (nfixed > 0) {
+        hipLaunchKernelGGL(k_adj_fix_rows, dim3(grid_for(nfixed)), dim3(kBlock), 0, s, nfixed, fix_idx.p, fix_rp.p,
+                           fix_ci.p, fix_v.p, w, y);
+        TM_HIP(hipGetLastError());
+    }
+    return st;
+}
+
 }  // namespace tmh
 
 using namespace tmh;
@@ -340,7 +410,9 @@ int tm_solver_create(int n, const int* rowptr, const int* colidx, const double* 
     int st = S->cg.setup(n, rowptr, colidx, vals, precond, S->stream);
     if (st == TM_OK) st = S->b.alloc(n);
     if (st == TM_OK) st = S->x.alloc(n);
+    if (st == TM_OK) st = S->xt.alloc(n);
     if (st == TM_OK && hipMemsetAsync(S->x.p, 0, sizeof(double) * n, S->stream) != hipSuccess) st = TM_ERR_HIP;
+    if (st == TM_OK && hipMemsetAsync(S->xt.p, 0, sizeof(double) * n, S->stream) != hipSuccess) st = TM_ERR_HIP;
     if (st == TM_OK && hipStreamSynchronize(S->stream) != hipSuccess) st = TM_ERR_HIP;
     if (st != TM_OK) {
         tm_solver_destroy(S);
@@ -373,6 +445,26 @@ int tm_solver_solve(tm_solver* S, const double* b, int bwhere, double* x, int xw
     if (warm == 1) TM_CHECK(copy_in(xd, x, xwhere, n, S->stream));
     else if (warm == 0) TM_HIP(hipMemsetAsync(xd, 0, sizeof(double) * n, S->stream));
     int st = S->cg.solve(bd, xd, tol, maxit, S->stream, iters, relres);
+    int st2 = copy_out(x, xwhere, xd, n, S->stream);
+    if (xwhere == TM_DEVICE) TM_HIP(hipStreamSynchronize(S->stream));
+    return st != TM_OK ? st : st2;
+}
+
+int tm_solver_solve_transposed(tm_solver* S, const double* b, int bwhere, double* x, int xwhere, double tol,
+                               int maxit, int warm, int* iters, double* relres) {
+    if (!S || !b || !x) return TM_ERR_BAD_ARG;
+    if ((bwhere != TM_HOST && bwhere != TM_DEVICE) || (xwhere != TM_HOST && xwhere != TM_DEVICE))
+        return TM_ERR_BAD_ARG;
+    const int n = S->cg.n;
+    const double* bd = b;
+    if (bwhere == TM_HOST) {
+        TM_CHECK(copy_in(S->b.p, b, TM_HOST, n, S->stream));
+        bd = S->b.p;
+    }
+    double* xd = S->xt.p;
+    if (warm == 1) TM_CHECK(copy_in(xd, x, xwhere, n, S->stream));
+    else if (warm == 0) TM_HIP(hipMemsetAsync(xd, 0, sizeof(double) * n, S->stream));
+    int st = S->cg.solve_transposed(bd, xd, tol, maxit, S->stream, iters, relres);
     int st2 = copy_out(x, xwhere, xd, n, S->stream);
     if (xwhere == TM_DEVICE) TM_HIP(hipStreamSynchronize(S->stream));
     return st != TM_OK ? st : st2;

@@ -21,7 +21,7 @@
 #include <cstring>
 #include <vector>
 
-#include "tm_internal.hpp"
+#include "tm_ctx.hpp"
 
 namespace tmh {
 int spmv(const SellPattern& P, const double* val, const double* x, double* y, hipStream_t s);
@@ -35,58 +35,10 @@ int amg_info(const CgSolver& cg, int* nlevels, int* sizes, int cap);
 
 using namespace tmh;
 
-struct tm_ctx {
-    int n = 0;
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    // sparse operators: host CSR copies until committed
-    struct HostCsr {
-        bool set = false;
-        std::vector<int> rp, col;
-        std::vector<double> val;
-    } hop[TM_NUM_OPS];
-    bool ops_dirty = false;
-    SellPattern pat;
-    DevBuf<double> opv[TM_NUM_OPS];
-    bool has_op[TM_NUM_OPS] = {};
-    // per-node data
-    DevBuf<double> js, invjs;      // n
-    DevBuf<double> ku, kaxis;      // n, 3n
-    DevBuf<double> dmi_fac, nvs;   // n (D / V), 3n (node normal * node area)
-    DevBuf<double> eta;            // 3n
-    DevBuf<double> vol;            // n, node volumes (energies, mean m / mean H)
-    DevBuf<double> epart, eout;    // energy partial sums and totals
-    DevBuf<double> hext;           // 3n, when per-node
-    double hext_u[3] = {0, 0, 0};
-    bool has_js = false, has_ku = false, has_dmi = false, has_eta = false, hext_field = false, has_vol = false;
-    bool have_u = false;           // u1/u2 hold the potentials of an earlier demag solve
-    bool freeze_demag = false;     // reuse u1/u2 in tm_eval_rhs (ProgramSpecs.cpp:81)
-    // state and work
-    DevBuf<double> m, w3a, w3b;    // 3n
-    DevBuf<double> wn;             // n
-    DevBuf<double> scal;           // device scalars (max torque)
-    // demag
-    int bnx = 0;
-    DevBuf<int> bnodes;
-    DevBuf<double> B;
-    int bld = 0;
-    bool has_B = false;
-    DevBuf<double> bpart, bvec_a, bvec_b;
-    CgSolver cgN, cgD;
-    bool has_solvers = false;
-    DevBuf<double> rhsN, rhsD, u1, u2;
-    double tol = 1e-10;
-    int maxit = 100000;
-    // phase timers
-    bool timers = false;
-    hipEvent_t ev[6] = {};
-    tm_phase_times tsum = {};
-};
 
-namespace {
+namespace tmh {
 
-inline int ctx_ok(tm_ctx* c) { return c && c->n > 0 ? TM_OK : TM_ERR_BAD_ARG; }
+int ctx_ok(tm_ctx* c) { return c && c->n > 0 ? TM_OK : TM_ERR_BAD_ARG; }
 
 int commit_ops(tm_ctx* c) {
     if (!c->ops_dirty) return TM_OK;
@@ -134,8 +86,13 @@ int commit_ops(tm_ctx* c) {
     }
     TM_HIP(hipStreamSynchronize(c->stream));
     c->ops_dirty = false;
+    ++c->ops_gen;  // transposed values built for the previous operators are stale
     return TM_OK;
 }
+
+}  // namespace tmh
+
+namespace {
 
 // --------------------------------------------------------------------------
 // kernels
@@ -639,7 +596,10 @@ int energy_dispatch(const EvalArgs& a, bool e, bool d, bool m, const double* vol
     return TM_OK;
 }
 
-inline int sell_grid(const tm_ctx* c) { return grid_for((int64_t)c->pat.nslices * kWave); }
+}  // namespace
+
+namespace tmh {
+int sell_grid(const tm_ctx* c) { return grid_for((int64_t)c->pat.nslices * kWave); }
 
 // Stage a caller pointer as a device pointer (copying host data into `tmp`).
 int stage_in(tm_ctx* c, const double* src, int where, size_t count, DevBuf<double>& tmp, const double** out) {
@@ -665,7 +625,7 @@ int finish_out(tm_ctx* c, double* dst, int where, const double* dev, size_t coun
 }
 
 // Output target: the caller's device buffer or the context scratch w3a.
-inline double* out_target(tm_ctx* c, double* dst, int where) { return where == TM_DEVICE ? dst : c->w3a.p; }
+double* out_target(tm_ctx* c, double* dst, int where) { return where == TM_DEVICE ? dst : c->w3a.p; }
 
 int upload_vec(tm_ctx* c, DevBuf<double>& d, const double* h, size_t count) {
     if (!h) return TM_ERR_BAD_ARG;
@@ -674,8 +634,9 @@ int upload_vec(tm_ctx* c, DevBuf<double>& d, const double* h, size_t count) {
     return TM_OK;
 }
 
-int demag_potential_dev(tm_ctx* c, const double* mdev, tm_eval_stats* stats);
+}  // namespace tmh
 
+namespace {
 }  // namespace
 
 extern "C" {
@@ -1072,7 +1033,7 @@ int tm_ctx_set_solver_params(tm_ctx* c, double tol, int maxit) {
 
 }  // extern "C"
 
-namespace {
+namespace tmh {
 
 // DemagField::calcPotential (DemagField.cpp:163-181) on device data; leaves
 // u1, u2 in the context.

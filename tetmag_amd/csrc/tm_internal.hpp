@@ -271,11 +271,20 @@ struct CgSolver {
     AmgPrecond amg;
     int nfixed = 0;
     int last_iters = 0;
+    // Coupling of the identity rows to the free unknowns, for the transposed
+    // solve: row r (node fix_idx[r]) holds A_ij over free rows i that have a
+    // column j = fix_idx[r].
+    DevBuf<int> fix_idx, fix_rp, fix_ci;
+    DevBuf<double> fix_v, tb;
     ~CgSolver();
     int setup(int n, const int* rowptr, const int* colidx, const double* vals, int precond, hipStream_t s);
     // Solves A x = b for device vectors b, x (x is the warm start).  tol is
     // relative to ||b||.  Returns TM_OK, TM_ERR_MAXITER or TM_ERR_NONFINITE.
     int solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* iters, double* relres);
+    // Solves A^T y = w for the same matrix (identity rows on the fixed set F):
+    // y_free = A_ff^-1 w_free (A_ff symmetric), y_F = w_F - A_{free,F}^T y_free.
+    int solve_transposed(const double* w, double* y, double tol, int maxit, hipStream_t s, int* iters,
+                         double* relres);
 };
 
 constexpr int kMaxCgBlocks = 512;
@@ -316,6 +325,6 @@ struct tm_spmat {
 
 struct tm_solver {
     tmh::CgSolver cg;
-    tmh::DevBuf<double> b, x;
+    tmh::DevBuf<double> b, x, xt;  // xt: warm start of the transposed solves
     hipStream_t stream = nullptr;
 };
