@@ -266,6 +266,15 @@ typedef struct tm_energies {
     double volume;    /* sum of node volumes (mesh units^3) */
 } tm_energies;
 
+/* Interfacial DMI with interface normal z (no counterpart in the reference,
+ * which has bulk DMI only): D per node (J/m^2 divided by the mesh scale), null
+ * to switch it off.  The field is the gradient of the lumped P1 energy
+ * sum_i D_i [m_z (Q_x m_x + Q_y m_y) - m_x Q_x m_z - m_y Q_y m_z]_i with
+ * Q_d = tGrad_d^T; it needs the node volumes and the GRAD_X/Y, TGRAD_X/Y
+ * operators.  Adds to Heff and to the DMI energy; works beside bulk DMI. */
+int tm_ctx_set_dmi_interfacial(tm_ctx* c, const double* D);
+int tm_idmi_field(tm_ctx* c, double* out3n, int where);
+
 /* Node volumes (MeshData NodeVolume) used by the energies. */
 int tm_ctx_set_node_volume(tm_ctx* c, const double* V);
 /* Energies of m (null = the context's m).  recompute_demag = 0 reuses the
@@ -286,11 +295,12 @@ int tm_ctx_refresh_demag(tm_ctx* c, const double* m3n, int mwhere, tm_eval_stats
  * re-evaluated at m, which becomes the context's magnetisation.  Optional
  * outputs (null to skip): g_hext3 (host, 3) = gradient w.r.t. a uniform
  * applied field; g_hfield3n = w.r.t. a per-node applied field; g_ku (n) =
- * w.r.t. per-node Ku; g_d (n) = w.r.t. per-node bulk D (needs node volumes).
+ * w.r.t. per-node Ku; g_d (n) = w.r.t. per-node bulk D (needs node volumes);
+ * g_di (n) = w.r.t. per-node interfacial D.
  * g_m, g_hfield3n, g_ku and g_d are written where gwhere says. */
 int tm_eval_rhs_vjp(tm_ctx* c, const double* m3n, int mwhere, double alpha, int variant, const double* lam3n,
                     int lwhere, double* gm3n, int gwhere, double* g_hext3, double* g_hfield3n, double* g_ku,
-                    double* g_d, tm_eval_stats* stats);
+                    double* g_d, double* g_di, tm_eval_stats* stats);
 /* g = (dHdem/dm)^T v (Hdem is linear in m). */
 int tm_demag_field_vjp(tm_ctx* c, const double* v3n, int vwhere, double* g3n, int gwhere, tm_eval_stats* stats);
 /* x_b = B^T y_b on the boundary (the transpose of tm_bem_apply). */

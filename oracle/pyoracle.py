@@ -56,6 +56,7 @@ def lib():
             "ref_exchange_field": ([I, P, P, P, P, P, P, P], None),
             "ref_uniaxial_field": ([I, P, P, P, P], None),
             "ref_dmi_field": ([I, P, P, P, P, P, P, P, P, P, P, P], None),
+            "ref_idmi_field": ([I, P, P, P, P, P, P, P, P, P, P], None),
             "ref_zeeman_field": ([I, D, D, D, P], None),
             "ref_total_field": ([I, P, P, P, P, P, P, P], None),
             "ref_llg_rhs": ([I, D, I, P, P, P], None),
@@ -233,6 +234,13 @@ class Oracle:
                             p(self.nodevol), p(self.node_normal), p(self.node_area), p(f64(m)), p(H))
         return H
 
+    def idmi(self, m, D):
+        """Interfacial DMI field (J/m^3), interface normal z (no reference counterpart)."""
+        H = np.empty(3 * self.n)
+        lib().ref_idmi_field(self.n, p(self.rowptr), p(self.col), p(self.gX), p(self.gY), p(self.tX), p(self.tY),
+                             p(f64(D)), p(self.nodevol), p(f64(m)), p(H))
+        return H
+
     def div(self, m, Js):
         b = np.empty(self.n)
         lib().ref_div_m(self.n, p(self.rowptr), p(self.col), p(self.tX), p(self.tY), p(self.tZ), p(f64(Js)),
@@ -275,7 +283,7 @@ class Oracle:
         return lib().ref_max_torque(self.n, p(f64(m)), p(f64(H)))
 
     def eval_rhs(self, m, *, Js, A, Ku=None, axis=None, D=None, Hext=None, alpha=0.5, variant=1, demag=True,
-                 tol=1e-14, warm=True):
+                 tol=1e-14, warm=True, Di=None):
         """TheLLG::classicVersion with every term of the composition
         (TheLLG.cpp:150-178, 238-254); Hext is a uniform (3,) field in A/m.
         Material arrays are per node and already scaled (A / scale^2, D / scale)."""
@@ -286,6 +294,9 @@ class Oracle:
             out["Hku"] = self.uniaxial(m, Ku, axis)
         if D is not None and np.any(D != 0):
             out["Hdmi"] = self.dmi(m, D)
+        if Di is not None and np.any(Di != 0):  # interfacial DMI, summed with the bulk term
+            out["Hidmi"] = self.idmi(m, Di)
+            out["Hdmi"] = out["Hidmi"] + out.get("Hdmi", 0.0)
         if Hext is not None and np.any(np.asarray(Hext) != 0):
             out["Hext"] = np.concatenate([np.full(n, Hext[0]), np.full(n, Hext[1]), np.full(n, Hext[2])])
         if demag:
@@ -330,12 +341,12 @@ class Oracle:
         return np.concatenate([f64(Js) / MU0 * (T[d].T @ y1) for d in range(3)])
 
     def eval_rhs_vjp(self, m, lam, *, Js, A, Ku=None, axis=None, D=None, Hext=None, alpha=0.5, variant=1,
-                     demag=True, tol=1e-14):
-        """lam^T d f / d(m, Hext, Ku, D) for f = eval_rhs(m)["dmdt"]."""
+                     demag=True, tol=1e-14, Di=None):
+        """lam^T d f / d(m, Hext, Ku, D, Di) for f = eval_rhs(m)["dmdt"]."""
         n = self.n
         m = f64(m)
         fw = self.eval_rhs(m, Js=Js, A=A, Ku=Ku, axis=axis, D=D, Hext=Hext, alpha=alpha, variant=variant,
-                           demag=demag, tol=tol, warm=False)
+                           demag=demag, tol=tol, warm=False, Di=Di)
         M = m.reshape(3, n).T
         H = fw["Heff"].reshape(3, n).T
         L = f64(lam).reshape(3, n).T
@@ -371,6 +382,19 @@ class Oracle:
             curl = np.stack([C[1] @ M[:, 2] - C[2] @ M[:, 1], C[2] @ M[:, 0] - C[0] @ M[:, 2],
                              C[0] @ M[:, 1] - C[1] @ M[:, 0]], axis=1)
             out["g_d"] = np.sum(gp * (np.cross(M, S) - 2.0 * curl), axis=1) / V
+        if Di is not None and np.any(Di != 0):
+            Dv = f64(Di)
+            Gx, Gy = self._csr(self.gX), self._csr(self.gY)
+            Tx, Ty = self._csr(self.tX), self._csr(self.tY)
+            w = gp
+            gm[:, 0] += -(Gx.T @ (Dv * w[:, 2])) + Dv * (Tx.T @ (w[:, 2] / V))
+            gm[:, 1] += -(Gy.T @ (Dv * w[:, 2])) + Dv * (Ty.T @ (w[:, 2] / V))
+            gm[:, 2] += (Gx.T @ (Dv * w[:, 0]) + Gy.T @ (Dv * w[:, 1])
+                         - Dv * (Tx.T @ (w[:, 0] / V) + Ty.T @ (w[:, 1] / V)))
+            out["g_di"] = (w[:, 0] * (Gx @ M[:, 2]) + w[:, 1] * (Gy @ M[:, 2])
+                           - w[:, 2] * (Gx @ M[:, 0] + Gy @ M[:, 1])
+                           - M[:, 2] * (Tx.T @ (w[:, 0] / V) + Ty.T @ (w[:, 1] / V))
+                           + M[:, 0] * (Tx.T @ (w[:, 2] / V)) + M[:, 1] * (Ty.T @ (w[:, 2] / V)))
         g = np.ascontiguousarray(gm.T.reshape(-1))
         if demag:
             g += self.demag_vjp(out["gH"], Js, tol=tol)

@@ -785,6 +785,39 @@ void ref_dmi_field(int n, const int* rowptr, const int* col, const double* tX, c
     }
 }
 
+/* Interfacial DMI (interface normal z), no reference counterpart.  Energy
+ * density D [m_z div m - (m . grad) m_z] with P1 m and the lumped (nodal)
+ * non-derivative factor:
+ *   E_h = sum_i D_i [m_z,i (Q_x m_x + Q_y m_y)_i - m_x,i (Q_x m_z)_i - m_y,i (Q_y m_z)_i],
+ *   Q_d(i,j) = int N_i d_d N_j = tGrad_d(j,i) = V_i grad_d(i,j),
+ * and H = -(1/V) dE_h/dm (J/m^3, before invJs):
+ *   H_x = D grad_x m_z - (1/V) tGrad_x (D m_z)
+ *   H_y = D grad_y m_z - (1/V) tGrad_y (D m_z)
+ *   H_z = -D (grad_x m_x + grad_y m_y) + (1/V) (tGrad_x (D m_x) + tGrad_y (D m_y)). */
+void ref_idmi_field(int n, const int* rowptr, const int* col, const double* gX, const double* gY, const double* tX,
+                    const double* tY, const double* D, const double* nodevol, const double* m, double* H) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double gxz = 0, gyz = 0, gxx = 0, gyy = 0, txz = 0, tyz = 0, txx = 0, tyy = 0;
+        for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            const int j = col[p];
+            const double mx = m[j], my = m[n + j], mz = m[2 * n + j];
+            gxz += gX[p] * mz;
+            gyz += gY[p] * mz;
+            gxx += gX[p] * mx;
+            gyy += gY[p] * my;
+            txz += tX[p] * D[j] * mz;
+            tyz += tY[p] * D[j] * mz;
+            txx += tX[p] * D[j] * mx;
+            tyy += tY[p] * D[j] * my;
+        }
+        const double iv = 1.0 / nodevol[i];
+        H[i] = D[i] * gxz - iv * txz;
+        H[n + i] = D[i] * gyz - iv * tyz;
+        H[2 * n + i] = -D[i] * (gxx + gyy) + iv * (txx + tyy);
+    }
+}
+
 /* TheLLG.cpp:442-449, 477-483 -- uniform Zeeman field |B|/mu0 along (theta, phi) in radians */
 void ref_zeeman_field(int n, double Babs, double theta, double phi, double* H) {
     double hx = sin(theta) * cos(phi), hy = sin(theta) * sin(phi), hz = cos(theta);

@@ -28,13 +28,18 @@ def _case(name):
                                B_ext=(0.01, -0.02, 0.015))
     if name == "ragged":  # n not a multiple of 64, thin in z, jittered
         return P.small_problem(nx=9, ny=3, nz=1, h=2.5, jitter=0.15, seed=5, Ku=1e5, D=-2e-3)
+    if name == "idmi":  # interfacial DMI film with bulk DMI off
+        p = P.small_problem(nx=10, ny=8, nz=2, h=2.0, jitter=0.1, seed=31, Ku=8e5, D=0.0, B_ext=(0.0, 0.01, 0.0))
+        p.mat.Di = 3e-3
+        p.Di = np.full(p.n, 3e-3 / p.mesh.scale) * (1 + 0.2 * np.random.default_rng(3).random(p.n))
+        return p
     if name == "mid":  # two AMG levels above the dense coarse grid
         return P.small_problem(nx=36, ny=30, nz=8, h=3.0, jitter=0.1, seed=21, Ku=5e4, D=0.0, init="vortex")
     raise KeyError(name)
 
 
 CASES = [("cfg0", "amg"), ("cfg0", "jacobi"), ("small_ku_dmi_zeeman", "amg"), ("ragged", "amg"),
-         ("mid", "amg")]
+         ("idmi", "amg"), ("mid", "amg")]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=lambda c: f"{c[0]}-{c[1]}")
@@ -53,7 +58,7 @@ def pair(request, oracle_mod, gpu_available):
 
 def _oracle_eval(prob, orc, m):
     return orc.eval_rhs(m, Js=prob.Js, A=prob.A, Ku=prob.Ku, axis=prob.kaxis, D=prob.D, Hext=prob.hext,
-                        alpha=prob.alpha, variant=1, tol=ORACLE_TOL)
+                        alpha=prob.alpha, variant=1, tol=ORACLE_TOL, Di=prob.Di)
 
 
 def test_bem_matrix_matches_oracle(pair):
@@ -89,6 +94,8 @@ def test_individual_fields(pair):
         assert rel_l2(ctx.uniaxial_field(), orc.uniaxial(m, prob.Ku, prob.kaxis)) < FIELD_TOL
     if prob.use_dmi:
         assert rel_l2(ctx.dmi_field(), orc.dmi(m, prob.D)) < FIELD_TOL
+    if prob.use_idmi:
+        assert rel_l2(ctx.idmi_field(), orc.idmi(m, prob.Di)) < FIELD_TOL
 
 
 def test_demag_field_and_potentials(pair):

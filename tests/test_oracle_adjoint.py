@@ -90,3 +90,19 @@ def test_vjp_detects_a_planted_sign_error(setup, monkeypatch):
     d = np.random.default_rng(5).normal(size=m.size)
     fd = (_phi(o, prob, lam, m + EPS * d) - _phi(o, prob, lam, m - EPS * d)) / (2 * EPS)
     assert abs(g @ d - fd) > 1e-3 * abs(fd)
+
+
+def test_vjp_interfacial_dmi(setup):
+    """Interfacial DMI term of the CPU adjoint (w.r.t. m and per-node Di)."""
+    prob, o, lam = setup
+    m = prob.m0
+    Di = np.full(prob.n, 2e-3 / 1e-9) * (1 + 0.2 * np.random.default_rng(4).random(prob.n))
+    kw = _kw(prob, Di=Di)
+    r = o.eval_rhs_vjp(m, lam, **kw)
+    phi = lambda mm, **ov: float(lam @ o.eval_rhs(mm, **{**kw, **ov}, warm=False)["dmdt"])
+    d = np.random.default_rng(6).normal(size=m.size)
+    fd = (phi(m + EPS * d) - phi(m - EPS * d)) / (2 * EPS)
+    assert r["gm"] @ d == pytest.approx(fd, rel=1e-7)
+    dD = np.random.default_rng(7).normal(size=prob.n) * Di[0] * 0.3
+    fd = (phi(m, Di=Di + dD) - phi(m, Di=Di - dD)) / 2
+    assert r["g_di"] @ dD == pytest.approx(fd, rel=1e-9)

@@ -189,3 +189,46 @@ def test_gmres_against_dense(oracle_mod):
         xd = np.linalg.solve(A, b)
         assert rel_l2(x, xd) < 1e-11
         assert s.last_relres <= 1e-14
+
+
+def test_interfacial_dmi_cycloid(oracle_mod):
+    """Neel cycloid m = (sin kx, 0, cos kx): interior field -2 D k m and
+    energy density D k (within the P1 error)."""
+    nx, ny, nz, h = 40, 4, 2, 1.0
+    o, mesh = _oracle(oracle_mod, nx, ny, nz, h)
+    n = o.n
+    Dv = np.full(n, 3e-3 / 1e-9)
+    k = 2 * np.pi / (nx * h)
+    x = mesh.xyz[:, 0]
+    m = np.concatenate([np.sin(k * x), np.zeros(n), np.cos(k * x)])
+    H = o.idmi(m, Dv)
+    bn = np.zeros(n, bool)
+    bn[o.bnodes] = True
+    inner = ~bn  # surface nodes carry the natural-boundary terms of the energy
+    expect = -2 * Dv[0] * k
+    for c in (0, 2):
+        assert np.max(np.abs(H[c * n:(c + 1) * n][inner] - expect * m[c * n:(c + 1) * n][inner])) < 0.006 * abs(expect)
+    e = -0.5 * np.sum(o.nodevol * (H.reshape(3, n) * m.reshape(3, n)).sum(axis=0)) / o.nodevol.sum()
+    assert e == pytest.approx(Dv[0] * k, rel=0.03)
+
+
+def test_interfacial_dmi_is_the_energy_gradient(oracle_mod):
+    """H = -(1/V) dE_h/dm for E_h = sum_i D_i [m_z (Q_x m_x + Q_y m_y) - m_x Q_x m_z
+    - m_y Q_y m_z]_i with Q_d = tGrad_d^T, written here independently."""
+    import scipy.sparse as sp
+    o, mesh = _oracle(oracle_mod, 5, 4, 3, 2.0, 0.1, 7)
+    n = o.n
+    rng = np.random.default_rng(1)
+    Dv = 3e6 * (1 + 0.3 * rng.random(n))
+    m = rng.normal(size=3 * n)
+    Qx = sp.csr_matrix((o.tX, o.col, o.rowptr), shape=(n, n)).T.tocsr()
+    Qy = sp.csr_matrix((o.tY, o.col, o.rowptr), shape=(n, n)).T.tocsr()
+
+    def E(v):
+        mx, my, mz = v.reshape(3, n)
+        return np.sum(Dv * (mz * (Qx @ mx + Qy @ my) - mx * (Qx @ mz) - my * (Qy @ mz)))
+
+    H = o.idmi(m, Dv)
+    d = rng.normal(size=3 * n)
+    dE = (E(m + d) - E(m - d)) / 2  # E is quadratic: exact
+    assert -np.sum(np.tile(o.nodevol, 3) * H * d) == pytest.approx(dE, rel=1e-11)

@@ -181,10 +181,12 @@ _SIGNATURES = {
     "tm_solver_amg_info": [_P, _P, _P, _I],
     "tm_ctx_solver_amg_info": [_P, _I, _P, _P, _I],
     "tm_ctx_set_node_volume": [_P, _P],
+    "tm_ctx_set_dmi_interfacial": [_P, _P],
+    "tm_idmi_field": [_P, _P, _I],
     "tm_energies_eval": [_P, _P, _I, _I, _P],
     "tm_ctx_set_freeze_demag": [_P, _I],
     "tm_ctx_refresh_demag": [_P, _P, _I, _P],
-    "tm_eval_rhs_vjp": [_P, _P, _I, _D, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P],
+    "tm_eval_rhs_vjp": [_P, _P, _I, _D, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P],
     "tm_demag_field_vjp": [_P, _P, _I, _P, _I, _P],
     "tm_bem_apply_transposed": [_P, _P, _I, _P, _I],
     "tm_solver_solve_transposed": [_P, _P, _I, _P, _I, _D, _I, _I, _P, _P],

@@ -1,9 +1,10 @@
 """torch.autograd bindings of the right-hand side and its adjoint.
 
-``rhs(fc, m, hext, ku, d)`` evaluates dm/dtau = LLG(m, Heff(m; hext, Ku, D))
-through ``tm_eval_rhs`` and differentiates through ``tm_eval_rhs_vjp``:
-gradients flow to m, to a uniform applied field ``hext`` (3,), and to per-node
-``ku`` / ``d`` (n,).  ``node_from_element`` maps per-element Ku or D to nodes
+``rhs(fc, m, hext, ku, d, di=...)`` evaluates dm/dtau = LLG(m, Heff(m; hext,
+Ku, D, Di)) through ``tm_eval_rhs`` and differentiates through
+``tm_eval_rhs_vjp``: gradients flow to m, to a uniform applied field ``hext``
+(3,), and to per-node ``ku`` / ``d`` (bulk DMI) / ``di`` (interfacial DMI)
+(n,).  ``ElementToNode`` maps per-element Ku or D to nodes
 (volume-weighted, the lumped form of an element-wise material), so gradients
 with respect to element values follow by autograd.  ``integrate`` advances m
 with classical RK4 steps (renormalised) in checkpointed segments: the forward
@@ -31,7 +32,9 @@ def _sync_out(fc: FieldContext):
     torch.cuda.current_stream(fc.device).wait_stream(fc.stream)
 
 
-def _apply_params(fc: FieldContext, hext, ku, d):
+def _apply_params(fc: FieldContext, hext, ku, d, di=None):
+    if di is not None:
+        fc.set_di(di.detach().double().cpu().numpy())
     if hext is not None:
         fc.set_hext(hext.detach().double().cpu().numpy())
     if ku is not None:
@@ -44,8 +47,8 @@ class LLGRhs(torch.autograd.Function):
     """dm/dtau of the field context ``fc`` (see module docstring)."""
 
     @staticmethod
-    def forward(ctx, m, hext, ku, d, fc, alpha, variant):
-        _apply_params(fc, hext, ku, d)
+    def forward(ctx, m, hext, ku, d, di, fc, alpha, variant):
+        _apply_params(fc, hext, ku, d, di)
         m = m.detach().contiguous()
         out = torch.empty_like(m)
         _sync_in(fc)
@@ -53,8 +56,8 @@ class LLGRhs(torch.autograd.Function):
                                  None, N.TM_DEVICE, None), "tm_eval_rhs")
         _sync_out(fc)
         ctx.fc, ctx.alpha, ctx.variant = fc, alpha, variant
-        ctx.has = (hext is not None, ku is not None, d is not None)
-        ctx.save_for_backward(m, *(t for t in (hext, ku, d) if t is not None))
+        ctx.has = (hext is not None, ku is not None, d is not None, di is not None)
+        ctx.save_for_backward(m, *(t for t in (hext, ku, d, di) if t is not None))
         return out
 
     @staticmethod
@@ -65,28 +68,31 @@ class LLGRhs(torch.autograd.Function):
         hext = saved.pop(0) if ctx.has[0] else None
         ku = saved.pop(0) if ctx.has[1] else None
         d = saved.pop(0) if ctx.has[2] else None
-        _apply_params(fc, hext, ku, d)
+        di = saved.pop(0) if ctx.has[3] else None
+        _apply_params(fc, hext, ku, d, di)
         g = g.detach().contiguous()
         gm = torch.empty_like(m)
         want_ku = ku is not None and ctx.needs_input_grad[2]
         want_d = d is not None and ctx.needs_input_grad[3]
         gku = torch.empty(fc.n, dtype=torch.float64, device=m.device) if want_ku else None
         gd = torch.empty(fc.n, dtype=torch.float64, device=m.device) if want_d else None
+        want_di = di is not None and ctx.needs_input_grad[4]
+        gdi = torch.empty(fc.n, dtype=torch.float64, device=m.device) if want_di else None
         gh = np.empty(3)
         _sync_in(fc)
         N.check(fc.L.tm_eval_rhs_vjp(fc.h, N.ptr(m), N.TM_DEVICE, float(ctx.alpha), int(ctx.variant), N.ptr(g),
                                      N.TM_DEVICE, N.ptr(gm), N.TM_DEVICE, N.ptr(gh), None, N.ptr(gku), N.ptr(gd),
-                                     None), "tm_eval_rhs_vjp")
+                                     N.ptr(gdi), None), "tm_eval_rhs_vjp")
         _sync_out(fc)
         g_h = None
         if hext is not None and ctx.needs_input_grad[1]:
             g_h = torch.as_tensor(gh, dtype=hext.dtype).to(hext.device)
-        return gm, g_h, gku, gd, None, None, None
+        return gm, g_h, gku, gd, gdi, None, None, None
 
 
-def rhs(fc: FieldContext, m, hext=None, ku=None, d=None, alpha=None, variant=N.TM_LLG_CLASSIC):
+def rhs(fc: FieldContext, m, hext=None, ku=None, d=None, alpha=None, variant=N.TM_LLG_CLASSIC, di=None):
     alpha = fc.prob.alpha if alpha is None else alpha
-    return LLGRhs.apply(m, hext, ku, d, fc, alpha, variant)
+    return LLGRhs.apply(m, hext, ku, d, di, fc, alpha, variant)
 
 
 # -- per-element material --------------------------------------------------------
@@ -125,8 +131,8 @@ def normalize(m, n):
     return (M / torch.linalg.vector_norm(M, dim=0, keepdim=True)).reshape(-1)
 
 
-def rk4_step(fc, m, dtau, hext=None, ku=None, d=None, alpha=None, variant=N.TM_LLG_CLASSIC):
-    f = lambda x: rhs(fc, x, hext, ku, d, alpha, variant)
+def rk4_step(fc, m, dtau, hext=None, ku=None, d=None, alpha=None, variant=N.TM_LLG_CLASSIC, di=None):
+    f = lambda x: rhs(fc, x, hext, ku, d, alpha, variant, di)
     k1 = f(m)
     k2 = f(m + 0.5 * dtau * k1)
     k3 = f(m + 0.5 * dtau * k2)
@@ -135,15 +141,15 @@ def rk4_step(fc, m, dtau, hext=None, ku=None, d=None, alpha=None, variant=N.TM_L
 
 
 def integrate(fc, m0, dtau, nsteps, hext_of_step=None, ku=None, d=None, alpha=None, variant=N.TM_LLG_CLASSIC,
-              checkpoint_every: int = 100):
+              checkpoint_every: int = 100, di=None):
     """RK4 trajectory from m0; ``hext_of_step(k)`` gives the (3,) applied field
     of step k (a tensor, possibly depending on control parameters).  Segments
     of ``checkpoint_every`` steps are recomputed in the backward pass."""
     from torch.utils.checkpoint import checkpoint
 
-    def segment(m, k0, k1, ku_, d_, *hs):
+    def segment(m, k0, k1, ku_, d_, di_, *hs):
         for j, k in enumerate(range(k0, k1)):
-            m = rk4_step(fc, m, dtau, hs[j] if hs else None, ku_, d_, alpha, variant)
+            m = rk4_step(fc, m, dtau, hs[j] if hs else None, ku_, d_, alpha, variant, di_)
         return m
 
     m = m0
@@ -152,8 +158,8 @@ def integrate(fc, m0, dtau, nsteps, hext_of_step=None, ku=None, d=None, alpha=No
         k1 = min(nsteps, k + checkpoint_every)
         hs = [hext_of_step(j) for j in range(k, k1)] if hext_of_step is not None else []
         if torch.is_grad_enabled():
-            m = checkpoint(segment, m, k, k1, ku, d, *hs, use_reentrant=False)
+            m = checkpoint(segment, m, k, k1, ku, d, di, *hs, use_reentrant=False)
         else:
-            m = segment(m, k, k1, ku, d, *hs)
+            m = segment(m, k, k1, ku, d, di, *hs)
         k = k1
     return m

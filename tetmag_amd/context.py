@@ -30,7 +30,8 @@ def _torch():
 class FieldContext:
     def __init__(self, prob: Problem, *, tol: float = 1e-10, maxit: int = 100000, device: int = 0,
                  B: np.ndarray | None = None, stream=None, bem_formula: int = N.TM_BEM_REFERENCE,
-                 precond: int = N.TM_PRECOND_AMG, uniaxial: bool | None = None, dmi: bool | None = None):
+                 precond: int = N.TM_PRECOND_AMG, uniaxial: bool | None = None, dmi: bool | None = None,
+                 idmi: bool | None = None):
         torch = _torch()
         L = N.lib()
         self.L = L
@@ -65,6 +66,9 @@ class FieldContext:
             N.check(L.tm_ctx_set_dmi(self.h, N.ptr(N.f64(prob.D)), N.ptr(N.f64(inv_v)),
                                      N.ptr(N.f64(prob.bnd.node_normal)), N.ptr(N.f64(prob.bnd.node_area))),
                     "tm_ctx_set_dmi")
+        self.has_idmi = prob.use_idmi if idmi is None else bool(idmi)
+        if self.has_idmi:
+            N.check(L.tm_ctx_set_dmi_interfacial(self.h, N.ptr(N.f64(prob.Di))), "tm_ctx_set_dmi_interfacial")
         hx, hy, hz = (float(v) for v in prob.hext)
         N.check(L.tm_ctx_set_zeeman(self.h, None, hx, hy, hz), "tm_ctx_set_zeeman")
         bnd = prob.bnd
@@ -249,9 +253,18 @@ class FieldContext:
         N.check(self.L.tm_ctx_set_dmi(self.h, N.ptr(N.f64(D)), N.ptr(N.f64(1.0 / ops.nodevol)),
                                       N.ptr(N.f64(bnd.node_normal)), N.ptr(N.f64(bnd.node_area))), "tm_ctx_set_dmi")
 
+    def set_di(self, Di):
+        """Per-node interfacial DMI constant (J/m^2 divided by the mesh scale)."""
+        if not self.has_idmi:
+            raise RuntimeError("context built without interfacial DMI (pass idmi=True)")
+        N.check(self.L.tm_ctx_set_dmi_interfacial(self.h, N.ptr(N.f64(Di))), "tm_ctx_set_dmi_interfacial")
+
+    def idmi_field(self):
+        return self._field(self.L.tm_idmi_field, "tm_idmi_field")
+
     # -- adjoint ------------------------------------------------------------------
     def vjp_host(self, m: np.ndarray, lam: np.ndarray, alpha=None, variant=N.TM_LLG_CLASSIC, want_ku=None,
-                 want_d=None) -> dict:
+                 want_d=None, want_di=None) -> dict:
         """lam^T d(dm/dtau)/d(m, Hext, Ku, D) at m (host arrays, synchronous)."""
         alpha = self.prob.alpha if alpha is None else alpha
         n = self.n
@@ -262,15 +275,19 @@ class FieldContext:
         want_d = self.has_dmi if want_d is None else want_d
         gku = np.empty(n) if want_ku else None
         gd = np.empty(n) if want_d else None
+        want_di = self.has_idmi if want_di is None else want_di
+        gdi = np.empty(n) if want_di else None
         st = N.EvalStats()
         N.check(self.L.tm_eval_rhs_vjp(self.h, N.ptr(N.f64(m)), N.TM_HOST, float(alpha), int(variant),
                                        N.ptr(N.f64(lam)), N.TM_HOST, N.ptr(gm), N.TM_HOST, N.ptr(gh), N.ptr(gf),
-                                       N.ptr(gku), N.ptr(gd), C.byref(st)), "tm_eval_rhs_vjp")
+                                       N.ptr(gku), N.ptr(gd), N.ptr(gdi), C.byref(st)), "tm_eval_rhs_vjp")
         out = {"gm": gm, "g_hext": gh, "gH": gf, "stats": st.as_dict()}
         if want_ku:
             out["g_ku"] = gku
         if want_d:
             out["g_d"] = gd
+        if want_di:
+            out["g_di"] = gdi
         return out
 
     def vjp(self, m, lam, alpha=None, variant=N.TM_LLG_CLASSIC, want_ku=False, want_d=False):
@@ -286,7 +303,7 @@ class FieldContext:
         gh = np.empty(3)
         N.check(self.L.tm_eval_rhs_vjp(self.h, N.ptr(m), N.TM_DEVICE, float(alpha), int(variant), N.ptr(lam),
                                        N.TM_DEVICE, N.ptr(gm), N.TM_DEVICE, N.ptr(gh), None, N.ptr(gku), N.ptr(gd),
-                                       None), "tm_eval_rhs_vjp")
+                                       None, None), "tm_eval_rhs_vjp")
         return gm, gh, gku, gd
 
     def demag_vjp_host(self, v: np.ndarray) -> np.ndarray:

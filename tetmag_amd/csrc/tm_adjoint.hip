@@ -18,6 +18,7 @@
 //   per-node applied field  g_H
 //   per-node Ku             2 invJs (k . m)(k . g_H)
 //   per-node D              invJs / V g_H . (m x s - 2 curl m)
+//   per-node interfacial D  the transposes of its two D factors (row, column)
 // The transposed solves run on the same identity-row matrices
 // (CgSolver::solve_transposed); B^T is a column-dot-product kernel.
 #include <algorithm>
@@ -126,6 +127,10 @@ struct VjpArgs {
     double* w0;                                 // -G^T g_H (demag) or null
     double* g_ku;                               // n or null
     double* g_d;                                // n or null
+    // interfacial DMI (idmi null: off): D, transposed grad_x/y and tGrad_x/y,
+    // forward grad_x/y (Di gradient), node volumes
+    const double *idmi, *igxT, *igyT, *itxT, *ityT, *igx, *igy, *ivol;
+    double* g_di;                               // n or null
     double* part;                               // 3 x gridDim partial sums of g_H
 };
 
@@ -141,9 +146,32 @@ __global__ __launch_bounds__(kBlock) void k_vjp_lin(VjpArgs a) {
         const double* __restrict__ gH = a.gH;
         const double* __restrict__ ij = a.invjs;
         double ex = 0, ey = 0, ez = 0, dx = 0, dy = 0, dz = 0, w = 0, cx = 0, cy = 0, cz = 0;
+        double ib[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // interfacial DMI row sums
+        const bool idmi = a.idmi != nullptr;
         sell_row(A, s, lane, [&](int64_t q, int cc) {
             const double gx = gH[cc], gy = gH[n + cc], gz = gH[2 * n + cc];
             const double sj = ij ? ij[cc] : 1.0;
+            if (idmi) {
+                // transposes of H_x = D G_x m_z - T_x (D m_z) / V (and y), and
+                // H_z = -D (G_x m_x + G_y m_y) + (T_x D m_x + T_y D m_y) / V
+                const double wx = sj * gx, wy = sj * gy, wz = sj * gz;
+                const double Dc = a.idmi[cc], ivc = 1.0 / a.ivol[cc];
+                const double GX = a.igxT[q], GY = a.igyT[q], TX = a.itxT[q], TY = a.ityT[q];
+                ib[0] += GX * Dc * wz;
+                ib[1] += GY * Dc * wz;
+                ib[2] += (GX * wx + GY * wy) * Dc;
+                ib[3] += TX * wz * ivc;
+                ib[4] += TY * wz * ivc;
+                ib[5] += TX * wx * ivc;
+                ib[6] += TY * wy * ivc;
+                if (a.g_di) {
+                    const double Gx = a.igx[q], Gy = a.igy[q];
+                    const double mx = a.m[cc], my = a.m[n + cc], mz = a.m[2 * n + cc];
+                    ib[7] += Gx * mz;
+                    ib[8] += Gy * mz;
+                    ib[9] += Gx * mx + Gy * my;
+                }
+            }
             if (EXCH) {
                 const double e = a.xcT[q];
                 ex += e * sj * gx;
@@ -202,6 +230,17 @@ __global__ __launch_bounds__(kBlock) void k_vjp_lin(VjpArgs a) {
                 }
             } else if (a.g_d) {
                 a.g_d[row] = 0.0;
+            }
+            if (idmi) {
+                const double D = a.idmi[row];
+                ox += -ib[0] + D * ib[3];
+                oy += -ib[1] + D * ib[4];
+                oz += ib[2] - D * (ib[5] + ib[6]);
+                if (a.g_di)
+                    a.g_di[row] = px * ib[7] + py * ib[8] - pz * ib[9] - mz * (ib[5] + ib[6]) + mx * ib[3] +
+                                  my * ib[4];
+            } else if (a.g_di) {
+                a.g_di[row] = 0.0;
             }
             a.gm[row] += ox;
             a.gm[n + row] += oy;
@@ -341,7 +380,7 @@ extern "C" {
 
 int tm_eval_rhs_vjp(tm_ctx* c, const double* m3n, int mwhere, double alpha, int variant, const double* lam3n,
                     int lwhere, double* gm3n, int gwhere, double* g_hext3, double* g_hfield3n, double* g_ku,
-                    double* g_d, tm_eval_stats* stats) {
+                    double* g_d, double* g_di, tm_eval_stats* stats) {
     TM_CHECK(ctx_ok(c));
     if (!lam3n || !gm3n || (variant != TM_LLG_CLASSIC && variant != TM_LLG_NOPREC)) return TM_ERR_BAD_ARG;
     if ((gwhere != TM_HOST && gwhere != TM_DEVICE) || (lwhere != TM_HOST && lwhere != TM_DEVICE))
@@ -353,7 +392,11 @@ int tm_eval_rhs_vjp(tm_ctx* c, const double* m3n, int mwhere, double alpha, int 
     const bool exch = c->has_op[TM_OP_EXCHANGE];
     const bool dmi = c->has_dmi && c->has_op[TM_OP_CURL_X] && c->has_op[TM_OP_CURL_Y] && c->has_op[TM_OP_CURL_Z];
     if (g_d && (!dmi || !c->has_vol)) return TM_ERR_NOT_READY;
+    const bool idmi = idmi_ready(c);
+    if (g_di && !idmi) return TM_ERR_NOT_READY;
     TM_CHECK(ensure_adjoint_buffers(c, demag));
+    if (idmi)
+        for (int op : {TM_OP_GRAD_X, TM_OP_GRAD_Y, TM_OP_TGRAD_X, TM_OP_TGRAD_Y}) TM_CHECK(ensure_transposed(c, op));
     if (exch) TM_CHECK(ensure_transposed(c, TM_OP_EXCHANGE));
     if (dmi)
         for (int op : {TM_OP_CURL_X, TM_OP_CURL_Y, TM_OP_CURL_Z}) TM_CHECK(ensure_transposed(c, op));
@@ -400,8 +443,21 @@ int tm_eval_rhs_vjp(tm_ctx* c, const double* m3n, int mwhere, double alpha, int 
     // scratch (a_dm, free after the forward) when host output is asked for
     if (g_ku) gku_d = gwhere == TM_DEVICE ? g_ku : c->a_dm.p;
     if (g_d) gd_d = gwhere == TM_DEVICE ? g_d : c->a_dm.p + n;
+    double* gdi_d = nullptr;
+    if (g_di) gdi_d = gwhere == TM_DEVICE ? g_di : c->a_dm.p + 2 * (size_t)n;
     a.g_ku = gku_d;
     a.g_d = gd_d;
+    a.g_di = gdi_d;
+    if (idmi) {
+        a.idmi = c->idmi.p;
+        a.igxT = c->opvT[TM_OP_GRAD_X].p;
+        a.igyT = c->opvT[TM_OP_GRAD_Y].p;
+        a.itxT = c->opvT[TM_OP_TGRAD_X].p;
+        a.ityT = c->opvT[TM_OP_TGRAD_Y].p;
+        a.igx = c->opv[TM_OP_GRAD_X].p;
+        a.igy = c->opv[TM_OP_GRAD_Y].p;
+        a.ivol = c->vol.p;
+    }
     const int G = sell_grid(c);
     a.part = c->a_part.p;
     TM_CHECK(lin_dispatch(a, exch, dmi, demag, g_d != nullptr, G, c->stream));
@@ -417,6 +473,7 @@ int tm_eval_rhs_vjp(tm_ctx* c, const double* m3n, int mwhere, double alpha, int 
         TM_CHECK(copy_out(gm3n, TM_HOST, g, (size_t)3 * n, c->stream));
         if (g_ku) TM_CHECK(copy_out(g_ku, TM_HOST, gku_d, n, c->stream));
         if (g_d) TM_CHECK(copy_out(g_d, TM_HOST, gd_d, n, c->stream));
+        if (g_di) TM_CHECK(copy_out(g_di, TM_HOST, gdi_d, n, c->stream));
     }
     if (g_hfield3n) TM_CHECK(finish_out(c, g_hfield3n, gwhere, c->a_gh.p, (size_t)3 * n));
     if (g_hext3) {

@@ -729,7 +729,9 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
         delete bt;
         return TM_ERR_NOT_READY;
     }
-    if (v.cgN->precond != TM_PRECOND_AMG || v.cgD->precond != TM_PRECOND_AMG) {
+    // the batched kernels cover exchange, uniaxial, bulk DMI, Zeeman and
+    // demag; interfacial DMI contexts are evaluated per sample
+    if (v.cgN->precond != TM_PRECOND_AMG || v.cgD->precond != TM_PRECOND_AMG || v.has_idmi) {
         delete bt;
         return TM_ERR_UNSUPPORTED;
     }

@@ -32,10 +32,12 @@ struct tm_ctx {
     DevBuf<double> dmi_fac, nvs;   // n (D / V), 3n (node normal * node area)
     DevBuf<double> eta;            // 3n
     DevBuf<double> vol;            // n, node volumes (energies, mean m / mean H)
+    DevBuf<double> idmi;           // n, interfacial DMI constant per node
     DevBuf<double> epart, eout;    // energy partial sums and totals
     DevBuf<double> hext;           // 3n, when per-node
     double hext_u[3] = {0, 0, 0};
-    bool has_js = false, has_ku = false, has_dmi = false, has_eta = false, hext_field = false, has_vol = false;
+    bool has_js = false, has_ku = false, has_dmi = false, has_eta = false, hext_field = false, has_vol = false,
+         has_idmi = false;
     bool have_u = false;           // u1/u2 hold the potentials of an earlier demag solve
     bool freeze_demag = false;     // reuse u1/u2 in tm_eval_rhs (ProgramSpecs.cpp:81)
     // state and work
@@ -80,6 +82,9 @@ int finish_out(tm_ctx* c, double* dst, int where, const double* dev, size_t coun
 // Output target: the caller's device buffer or the context scratch w3a.
 double* out_target(tm_ctx* c, double* dst, int where);
 int upload_vec(tm_ctx* c, DevBuf<double>& d, const double* h, size_t count);
+// Interfacial DMI configured with everything it needs (D, node volumes,
+// grad_x/y, tGrad_x/y).
+bool idmi_ready(const tm_ctx* c);
 // DemagField::calcPotential on device data; leaves u1, u2 in the context.
 int demag_potential_dev(tm_ctx* c, const double* mdev, tm_eval_stats* stats);
 }  // namespace tmh

@@ -344,6 +344,10 @@ struct EvalArgs {
     const double* invjs;        // may be null: invJs = 1
     const double* ku; const double* kaxis;
     const double* dmi_fac; const double* nvs;
+    // interfacial DMI (null idmi: off): D per node, grad_x/y and tGrad_x/y
+    // values, node volumes
+    const double* idmi; const double* igx; const double* igy; const double* itx; const double* ity;
+    const double* ivol;
     const double* hext;         // per-node (or null)
     double hx, hy, hz;          // uniform applied field
     double alpha;
@@ -352,6 +356,33 @@ struct EvalArgs {
     double* heff;               // may be null
     double* torque;             // may be null (zeroed before the launch)
 };
+
+// Interfacial DMI (interface normal z; no reference counterpart), the
+// gradient of the lumped P1 energy sum_i D_i [m_z (Q_x m_x + Q_y m_y) -
+// m_x Q_x m_z - m_y Q_y m_z]_i with Q_d = tGrad_d^T = diag(V) grad_d:
+//   H_x = D grad_x m_z - tGrad_x (D m_z) / V,  H_y likewise with y,
+//   H_z = -D (grad_x m_x + grad_y m_y) + (tGrad_x (D m_x) + tGrad_y (D m_y)) / V.
+// ia: grad_x m_z, grad_y m_z, grad_x m_x + grad_y m_y, tGrad_x(D m_z),
+//     tGrad_y(D m_z), tGrad_x(D m_x) + tGrad_y(D m_y)
+__device__ inline void idmi_accumulate(const EvalArgs& a, int64_t q, int cc, double mx, double my, double mz,
+                                       double (&ia)[8]) {
+    const double gx = a.igx[q], gy = a.igy[q], tx = a.itx[q], ty = a.ity[q];
+    const double Dc = a.idmi[cc];
+    ia[0] += gx * mz;
+    ia[1] += gy * mz;
+    ia[2] += gx * mx + gy * my;
+    ia[3] += tx * Dc * mz;
+    ia[4] += ty * Dc * mz;
+    ia[5] += (tx * mx + ty * my) * Dc;
+}
+
+__device__ inline void idmi_field(const EvalArgs& a, int row, const double (&ia)[8], double& hx, double& hy,
+                                  double& hz) {
+    const double D = a.idmi[row], iv = 1.0 / a.ivol[row];
+    hx = D * ia[0] - iv * ia[3];
+    hy = D * ia[1] - iv * ia[4];
+    hz = -D * ia[2] + iv * ia[5];
+}
 
 // Fused Heff + LLG (TheLLG.cpp:238-254 and EffFieldGPU.cu:316-362):
 // one pass over the shared pattern computes exchange, demag gradient and DMI
@@ -368,13 +399,16 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a) {
     if (s < (n + kWave - 1) / kWave) {
         const double* __restrict__ m = a.m;
         double ex = 0, ey = 0, ez = 0, dx = 0, dy = 0, dz = 0, cx = 0, cy = 0, cz = 0;
+        double ia[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // interfacial DMI row sums
+        const bool idmi = a.idmi != nullptr;
         sell_row(A, s, lane, [&](int64_t q, int cc) {
             double mx = 0, my = 0, mz = 0;
-            if (EXCH || DMI) {
+            if (EXCH || DMI || idmi) {
                 mx = m[cc];
                 my = m[n + cc];
                 mz = m[2 * n + cc];
             }
+            if (idmi) idmi_accumulate(a, q, cc, mx, my, mz, ia);
             if (EXCH) {
                 const double e = a.xc[q];
                 ex += e * mx;
@@ -410,6 +444,13 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a) {
                 hx += ((my * sz - mz * sy) - 2.0 * cx) * f;
                 hy += ((mz * sx - mx * sz) - 2.0 * cy) * f;
                 hz += ((mx * sy - my * sx) - 2.0 * cz) * f;
+            }
+            if (idmi) {
+                double qx, qy, qz;
+                idmi_field(a, row, ia, qx, qy, qz);
+                hx += qx;
+                hy += qy;
+                hz += qz;
             }
             const double ij = a.invjs ? a.invjs[row] : 1.0;
             hx = hx * ij + a.hx;
@@ -498,13 +539,16 @@ __global__ __launch_bounds__(kBlock) void k_energy(EvalArgs a, const double* __r
     if (s < (n + kWave - 1) / kWave) {
         const double* __restrict__ m = a.m;
         double ex = 0, ey = 0, ez = 0, dx = 0, dy = 0, dz = 0, cx = 0, cy = 0, cz = 0;
+        double ia[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // interfacial DMI row sums
+        const bool idmi = a.idmi != nullptr;
         sell_row(A, s, lane, [&](int64_t q, int cc) {
             double mx = 0, my = 0, mz = 0;
-            if (EXCH || DMI) {
+            if (EXCH || DMI || idmi) {
                 mx = m[cc];
                 my = m[n + cc];
                 mz = m[2 * n + cc];
             }
+            if (idmi) idmi_accumulate(a, q, cc, mx, my, mz, ia);
             if (EXCH) {
                 const double v = a.xc[q];
                 ex += v * mx;
@@ -548,6 +592,11 @@ __global__ __launch_bounds__(kBlock) void k_energy(EvalArgs a, const double* __r
                 const double py = ((mz * sx - mx * sz) - 2.0 * cy) * f;
                 const double pz = ((mx * sy - my * sx) - 2.0 * cz) * f;
                 e[3] = V * (px * mx + py * my + pz * mz);
+            }
+            if (idmi) {
+                double qx, qy, qz;
+                idmi_field(a, row, ia, qx, qy, qz);
+                e[3] += V * (qx * mx + qy * my + qz * mz);
             }
             if (DEMAG) e[4] = -J * V * (dx * mx + dy * my + dz * mz);  // Hdem = -grad u
             e[5] = V * mx;
@@ -626,6 +675,21 @@ int finish_out(tm_ctx* c, double* dst, int where, const double* dev, size_t coun
 
 // Output target: the caller's device buffer or the context scratch w3a.
 double* out_target(tm_ctx* c, double* dst, int where) { return where == TM_DEVICE ? dst : c->w3a.p; }
+
+bool idmi_ready(const tm_ctx* c) {
+    return c->has_idmi && c->has_vol && c->has_op[TM_OP_GRAD_X] && c->has_op[TM_OP_GRAD_Y] &&
+           c->has_op[TM_OP_TGRAD_X] && c->has_op[TM_OP_TGRAD_Y];
+}
+
+void fill_idmi(const tm_ctx* c, EvalArgs& a) {
+    if (!idmi_ready(c)) return;
+    a.idmi = c->idmi.p;
+    a.igx = c->opv[TM_OP_GRAD_X].p;
+    a.igy = c->opv[TM_OP_GRAD_Y].p;
+    a.itx = c->opv[TM_OP_TGRAD_X].p;
+    a.ity = c->opv[TM_OP_TGRAD_Y].p;
+    a.ivol = c->vol.p;
+}
 
 int upload_vec(tm_ctx* c, DevBuf<double>& d, const double* h, size_t count) {
     if (!h) return TM_ERR_BAD_ARG;
@@ -1154,6 +1218,7 @@ int tm_eval_rhs(tm_ctx* c, const double* m3n, int mwhere, double alpha, int vari
         a.dmi_fac = c->dmi_fac.p;
         a.nvs = c->nvs.p;
     }
+    fill_idmi(c, a);
     a.m = c->m.p;
     a.invjs = c->has_js ? c->invjs.p : nullptr;
     a.ku = c->has_ku ? c->ku.p : nullptr;
@@ -1198,6 +1263,32 @@ int tm_ctx_set_node_volume(tm_ctx* c, const double* V) {
     TM_CHECK(upload_vec(c, c->vol, V, c->n));
     c->has_vol = true;
     return TM_OK;
+}
+
+int tm_ctx_set_dmi_interfacial(tm_ctx* c, const double* D) {
+    TM_CHECK(ctx_ok(c));
+    if (!D) {
+        c->has_idmi = false;
+        return TM_OK;
+    }
+    TM_CHECK(upload_vec(c, c->idmi, D, c->n));
+    c->has_idmi = true;
+    return TM_OK;
+}
+
+int tm_idmi_field(tm_ctx* c, double* out, int where) {
+    TM_CHECK(ctx_ok(c));
+    if (!out) return TM_ERR_BAD_ARG;
+    TM_CHECK(commit_ops(c));
+    if (!idmi_ready(c)) return TM_ERR_NOT_READY;
+    EvalArgs a = {};
+    a.A = view(c->pat);
+    fill_idmi(c, a);
+    a.m = c->m.p;
+    double* o = out_target(c, out, where);
+    a.heff = o;
+    TM_CHECK(eval_dispatch(a, false, false, false, sell_grid(c), c->stream));
+    return finish_out(c, out, where, o, (size_t)3 * c->n);
 }
 
 int tm_ctx_set_freeze_demag(tm_ctx* c, int on) {
@@ -1247,6 +1338,7 @@ int tm_energies_eval(tm_ctx* c, const double* m3n, int mwhere, int recompute_dem
         a.dmi_fac = c->dmi_fac.p;
         a.nvs = c->nvs.p;
     }
+    fill_idmi(c, a);
     a.m = c->m.p;
     a.ku = c->has_ku ? c->ku.p : nullptr;
     a.kaxis = c->has_ku ? c->kaxis.p : nullptr;
@@ -1326,6 +1418,7 @@ int ctx_view(tm_ctx* c, CtxView* v) {
     v->ku = c->ku.p;
     v->kaxis = c->kaxis.p;
     v->has_dmi = c->has_dmi;
+    v->has_idmi = c->has_idmi;
     v->dmi_fac = c->dmi_fac.p;
     v->nvs = c->nvs.p;
     v->bnx = c->bnx;

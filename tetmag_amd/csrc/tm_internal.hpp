@@ -300,7 +300,7 @@ struct CtxView {
     const std::vector<double>* hop_val[TM_NUM_OPS];
     bool hop_set[TM_NUM_OPS];
     const double *js, *invjs, *ku, *kaxis, *dmi_fac, *nvs;
-    bool has_ku, has_dmi;
+    bool has_ku, has_dmi, has_idmi;
     int bnx, bld;
     const int* bnodes;
     const double* B;

@@ -29,6 +29,7 @@ class Material:
     Ku: float = 0.0            # J/m^3
     ku_axis: tuple = (0.0, 0.0, 1.0)
     D: float = 0.0             # J/m^2, bulk DMI (EffFieldCalc.cpp:159-170)
+    Di: float = 0.0            # J/m^2, interfacial DMI, interface normal z (no reference counterpart)
     alpha: float = 0.5
     gamma: float = 2.211e5     # m/(A s)
     B_ext: tuple = (0.0, 0.0, 0.0)  # applied field in T
@@ -50,6 +51,7 @@ class Problem:
     Ku: np.ndarray = field(default=None)       # (n,)
     kaxis: np.ndarray = field(default=None)    # (3n,) SoA
     D: np.ndarray = field(default=None)        # (n,)  J/m^2 / scale
+    Di: np.ndarray = field(default=None)       # (n,)  J/m^2 / scale (interfacial)
     hext: np.ndarray = field(default=None)     # (3,)  A/m
     m0: np.ndarray = field(default=None)       # (3n,) initial magnetisation
 
@@ -64,6 +66,7 @@ class Problem:
         ax = ax / np.linalg.norm(ax)
         self.kaxis = np.concatenate([np.full(n, ax[0]), np.full(n, ax[1]), np.full(n, ax[2])])
         self.D = np.full(n, mat.D / s)                  # SimulationData.cpp:48
+        self.Di = np.full(n, mat.Di / s)
         self.hext = np.asarray(mat.B_ext, np.float64) / MU0
 
     @property
@@ -106,6 +109,10 @@ class Problem:
     @property
     def use_dmi(self) -> bool:       # TheLLG.cpp:84
         return bool(np.any(self.D != 0))
+
+    @property
+    def use_idmi(self) -> bool:
+        return bool(np.any(self.Di != 0))
 
     def summary(self) -> dict:
         return {"n": self.n, "tets": self.mesh.ntet, "bnx": self.bnd.bnx, "nbel": self.bnd.nbel,
