@@ -83,7 +83,7 @@ def grad_csr(prob):
     import scipy.sparse as sp
     o = prob.ops
     n = prob.n
-    return [sp.csr_matrix((g, o.col, o.rowptr), shape=(n, n)) for g in (o.gX, o.gY, o.gZ)]
+    return [sp.csr_matrix((g, o.col, o.rowptr), shape=(n, n)) for g in (o.gx, o.gy, o.gz)]
 
 
 def demag_energy_from_u(prob, G, u, m):

@@ -306,6 +306,29 @@ int tm_demag_field_vjp(tm_ctx* c, const double* v3n, int vwhere, double* g3n, in
 /* x_b = B^T y_b on the boundary (the transpose of tm_bem_apply). */
 int tm_bem_apply_transposed(tm_ctx* c, const double* yb, int ywhere, double* xb, int xwhere);
 
+/* ---- on-device time stepping (the reference integrates with CVODE on the
+ * host: LLGWrapper.cu:139-228, TheLLG.cpp:578-711) --------------------------- */
+typedef struct tm_rk4_stats {
+    int sync_steps;          /* steps run with host-polled solves (the first, or use_graph = 0) */
+    int graph_steps;         /* steps replayed from the captured graph */
+    int redone_steps;        /* replays in which a solve missed its budget, redone */
+    int captures;            /* graph captures */
+    int budget_neumann;      /* CG iteration budgets of the graph */
+    int budget_dirichlet;
+    int max_iters_neumann;   /* largest CG iteration counts seen */
+    int max_iters_dirichlet;
+    double ms;               /* device time of the call */
+} tm_rk4_stats;
+/* nsteps classical RK4 steps of dtau (tau units) on the context's
+ * magnetisation, renormalised after each step; demag recomputed at every
+ * stage unless frozen.  use_graph = 1 captures one step in a HIP graph (CG
+ * with fixed iteration budgets that stop early on the device; every replay is
+ * checked and a step whose solve missed the tolerance is redone with
+ * host-polled solves and larger budgets).  The field parameters in effect at
+ * the call are used for all steps. */
+int tm_ctx_run_rk4(tm_ctx* c, double dtau, int nsteps, double alpha, int variant, int use_graph,
+                   tm_rk4_stats* stats);
+
 /* Explicit Euler advance of the context magnetisation with renormalisation:
  * m <- (m + dtau dm/dtau) / |m + dtau dm/dtau|  (the renormalisation of
  * EffFieldGPU::NormalizeMag, EffFieldGPU.cu:625-663, fused with the update). */

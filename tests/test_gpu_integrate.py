@@ -1,0 +1,65 @@
+"""On-device RK4 (tm_ctx_run_rk4): graph replay and host-polled steps agree
+with each other and with RK4 written in torch over the autograd right-hand
+side; a tolerance tightened between calls forces budget misses, which are
+redone (the failure is forced from the host, no kernel is made to fail)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup(gpu_available):
+    from tetmag_amd import problem as P
+    prob = P.small_problem(nx=12, ny=10, nz=3, h=3.0, jitter=0.1, seed=29, Ku=4e4, D=2e-3, B_ext=(0.01, 0.0, 0.02))
+    print(f"\n[case] rk4 n={prob.n}", flush=True)
+    return prob
+
+
+def test_rk4_graph_matches_sync_and_torch(setup):
+    import torch
+    from tetmag_amd.context import FieldContext
+    from tetmag_amd import autograd as AG
+    prob = setup
+    dtau = 30e-3 * prob.time_scale()
+    a = FieldContext(prob, tol=1e-12)
+    b = FieldContext(prob, tol=1e-12)
+    sa = a.run_rk4(dtau, 6, use_graph=True)
+    sb = b.run_rk4(dtau, 6, use_graph=False)
+    print(sa, sb)
+    assert sa["graph_steps"] == 5 and sa["sync_steps"] == 1 and sa["captures"] == 1
+    assert sb["sync_steps"] == 6 and sb["graph_steps"] == 0
+    ma, mb = a.get_mag(), b.get_mag()
+    assert rel_l2(ma, mb) < 1e-10
+    n = prob.n
+    assert np.allclose(np.linalg.norm(ma.reshape(3, n), axis=0), 1.0, atol=1e-13)
+    c = FieldContext(prob, tol=1e-12)
+    with torch.no_grad():
+        m = torch.as_tensor(prob.m0, device=c.device)
+        for _ in range(6):
+            m = AG.rk4_step(c, m, dtau)
+    assert rel_l2(m.cpu().numpy(), ma) < 1e-10
+    for f in (a, b, c):
+        f.close()
+
+
+def test_rk4_budget_miss_is_redone(setup):
+    from tetmag_amd.context import FieldContext
+    prob = setup
+    dtau = 30e-3 * prob.time_scale()
+    a = FieldContext(prob, tol=1e-8)
+    b = FieldContext(prob, tol=1e-8)
+    a.run_rk4(dtau, 3, use_graph=True)       # budgets learnt at 1e-8
+    b.run_rk4(dtau, 3, use_graph=False)
+    a.set_solver_params(1e-13, 100000)       # tighter: the learnt budgets fall short
+    b.set_solver_params(1e-13, 100000)
+    sa = a.run_rk4(dtau, 4, use_graph=True)
+    sb = b.run_rk4(dtau, 4, use_graph=False)
+    print(sa)
+    assert sa["redone_steps"] >= 1
+    assert sa["graph_steps"] + sa["redone_steps"] == 4
+    assert rel_l2(a.get_mag(), b.get_mag()) < 1e-9
+    a.close()
+    b.close()

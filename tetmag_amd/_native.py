@@ -98,6 +98,15 @@ class Energies(C.Structure):
         return d
 
 
+class RK4Stats(C.Structure):
+    _fields_ = [("sync_steps", C.c_int), ("graph_steps", C.c_int), ("redone_steps", C.c_int), ("captures", C.c_int),
+                ("budget_neumann", C.c_int), ("budget_dirichlet", C.c_int), ("max_iters_neumann", C.c_int),
+                ("max_iters_dirichlet", C.c_int), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("nsamples", C.c_int),
@@ -190,6 +199,7 @@ _SIGNATURES = {
     "tm_demag_field_vjp": [_P, _P, _I, _P, _I, _P],
     "tm_bem_apply_transposed": [_P, _P, _I, _P, _I],
     "tm_solver_solve_transposed": [_P, _P, _I, _P, _I, _D, _I, _I, _P, _P],
+    "tm_ctx_run_rk4": [_P, _D, _I, _D, _I, _I, _P],
     "tm_batch_create": [_P, _I, _I, _PP],
     "tm_batch_destroy": [_P],
     "tm_batch_set_zeeman": [_P, _P],

@@ -34,6 +34,7 @@ SOURCES = [
     "tm_field.hip",
     "tm_batch.hip",
     "tm_adjoint.hip",
+    "tm_integrate.hip",
     "tm_setup.cpp",
 ]
 

@@ -196,6 +196,16 @@ class FieldContext:
         d = self.dmdt if dmdt is None else dmdt
         N.check(self.L.tm_ctx_step_euler(self.h, N.ptr(d), N.TM_DEVICE, float(dtau)), "tm_ctx_step_euler")
 
+    def run_rk4(self, dtau: float, nsteps: int, use_graph: bool = True, alpha=None,
+                variant=N.TM_LLG_CLASSIC) -> dict:
+        """nsteps classical RK4 steps of the context magnetisation on the device
+        (tm_ctx_run_rk4); returns the step statistics."""
+        alpha = self.prob.alpha if alpha is None else alpha
+        st = N.RK4Stats()
+        N.check(self.L.tm_ctx_run_rk4(self.h, float(dtau), int(nsteps), float(alpha), int(variant), int(use_graph),
+                                      C.byref(st)), "tm_ctx_run_rk4")
+        return st.as_dict()
+
     def enable_timers(self, on: bool = True):
         N.check(self.L.tm_ctx_enable_timers(self.h, int(on)), "tm_ctx_enable_timers")
 

@@ -252,9 +252,10 @@ bool dense_inverse(std::vector<double>& a, int n) {
          s += gridDim.x * (kBlock / kWave))
 
 // x = w D^-1 b ;  r = b - A x   (x gathered on the fly as w dinv_j b_j)
-__global__ __launch_bounds__(kBlock) void k_amg_pre(SellView A, const double* __restrict__ val,
+__global__ __launch_bounds__(kBlock) void k_amg_pre(const CgState* st, SellView A, const double* __restrict__ val,
                                                     const double* __restrict__ dinv, const double* __restrict__ b,
                                                     double w, double* __restrict__ x, double* __restrict__ r) {
+    if (st && st->done) return;
     AMG_SLICE_LOOP(A) {
         double acc = 0.0;
         sell_row(A, s, lane, [&](int64_t e, int c) { acc += val[e] * (w * dinv[c] * b[c]); });
@@ -270,11 +271,12 @@ __global__ __launch_bounds__(kBlock) void k_amg_pre(SellView A, const double* __
 
 
 // xo = x + w D^-1 (b - A x); optionally partial (b, xo) per block into part.
-__global__ __launch_bounds__(kBlock) void k_amg_post(SellView A, const double* __restrict__ val,
+__global__ __launch_bounds__(kBlock) void k_amg_post(const CgState* st, SellView A, const double* __restrict__ val,
                                                      const double* __restrict__ dinv, const double* __restrict__ b,
                                                      double w, const double* __restrict__ x,
                                                      double* __restrict__ xo, double* part) {
     __shared__ double sp[kBlock / kWave];
+    if (st && st->done) return;
     double dot = 0.0;
     AMG_SLICE_LOOP(A) {
         double acc = 0.0;
@@ -299,8 +301,9 @@ __global__ __launch_bounds__(kBlock) void k_amg_post(SellView A, const double* _
 }
 
 // x = C b for the dense (row-major) coarse inverse; one wavefront per row.
-__global__ __launch_bounds__(kBlock) void k_amg_dense(int nc, const double* __restrict__ C,
+__global__ __launch_bounds__(kBlock) void k_amg_dense(const CgState* st, int nc, const double* __restrict__ C,
                                                       const double* __restrict__ b, double* __restrict__ x) {
+    if (st && st->done) return;
     const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (row >= nc) return;
@@ -311,9 +314,10 @@ __global__ __launch_bounds__(kBlock) void k_amg_dense(int nc, const double* __re
 }
 
 // partial (a, b) per block (grid-stride)
-__global__ __launch_bounds__(kBlock) void k_amg_dot(int n, const double* __restrict__ a,
+__global__ __launch_bounds__(kBlock) void k_amg_dot(const CgState* st, int n, const double* __restrict__ a,
                                                     const double* __restrict__ b, double* part) {
     __shared__ double sp[kBlock / kWave];
+    if (st && st->done) return;
     double dot = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) dot += a[i] * b[i];
     dot = wave_sum(dot);
@@ -338,10 +342,12 @@ __device__ inline double group_sum(double v) {
 // MODE 2: x = w dinv b, r = b - A(w dinv b)   (pre-smoothing + residual)
 // MODE 3: xo = x + w dinv (b - A x)           (post-smoothing)
 template <int LPR, int MODE>
-__global__ __launch_bounds__(kBlock) void k_csrv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                 const double* __restrict__ v, const double* __restrict__ x,
-                                                 const double* __restrict__ dinv, const double* __restrict__ b,
-                                                 double w, double* __restrict__ y, double* __restrict__ y2) {
+__global__ __launch_bounds__(kBlock) void k_csrv(const CgState* st, int n, const int* __restrict__ rp,
+                                                 const int* __restrict__ ci, const double* __restrict__ v,
+                                                 const double* __restrict__ x, const double* __restrict__ dinv,
+                                                 const double* __restrict__ b, double w, double* __restrict__ y,
+                                                 double* __restrict__ y2) {
+    if (st && st->done) return;  // the CG this V-cycle serves has converged
     const int row = (blockIdx.x * kBlock + threadIdx.x) / LPR;
     const int sub = threadIdx.x & (LPR - 1);
     if (row >= n) return;  // whole groups exit together (LPR divides kBlock)
@@ -363,16 +369,16 @@ __global__ __launch_bounds__(kBlock) void k_csrv(int n, const int* __restrict__ 
 }
 
 template <int MODE>
-void csrv_launch(const AmgPrecond::CsrDev& M, const double* x, const double* dinv, const double* b, double w,
-                 double* y, double* y2, hipStream_t s) {
+void csrv_launch(const CgState* st, const AmgPrecond::CsrDev& M, const double* x, const double* dinv,
+                 const double* b, double w, double* y, double* y2, hipStream_t s) {
     const int rows_per_block = kBlock / M.lpr;
     const dim3 g((M.n + rows_per_block - 1) / rows_per_block);
     switch (M.lpr) {
-        case 4: hipLaunchKernelGGL((k_csrv<4, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
-        case 8: hipLaunchKernelGGL((k_csrv<8, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
-        case 16: hipLaunchKernelGGL((k_csrv<16, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
-        case 32: hipLaunchKernelGGL((k_csrv<32, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
-        default: hipLaunchKernelGGL((k_csrv<64, MODE>), g, dim3(kBlock), 0, s, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        case 4: hipLaunchKernelGGL((k_csrv<4, MODE>), g, dim3(kBlock), 0, s, st, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        case 8: hipLaunchKernelGGL((k_csrv<8, MODE>), g, dim3(kBlock), 0, s, st, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        case 16: hipLaunchKernelGGL((k_csrv<16, MODE>), g, dim3(kBlock), 0, s, st, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        case 32: hipLaunchKernelGGL((k_csrv<32, MODE>), g, dim3(kBlock), 0, s, st, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
+        default: hipLaunchKernelGGL((k_csrv<64, MODE>), g, dim3(kBlock), 0, s, st, M.n, M.rp.p, M.ci.p, M.v.p, x, dinv, b, w, y, y2); break;
     }
 }
 
@@ -508,11 +514,11 @@ int AmgPrecond::setup(int n, const int* rowptr, const int* colidx, const double*
 }
 
 // z = M^-1 r (one V-cycle); writes the (r, z) partials of `grid` blocks to part.
-int AmgPrecond::apply(const double* r0, double* z, double* part, int grid, hipStream_t s) {
+int AmgPrecond::apply(const double* r0, double* z, double* part, int grid, hipStream_t s, const CgState* st) {
     const int L = (int)levels.size();
     if (L == 0) {
-        hipLaunchKernelGGL(k_amg_dense, dim3((nc + 3) / 4), dim3(kBlock), 0, s, nc, Cinv.p, r0, z);
-        hipLaunchKernelGGL(k_amg_dot, dim3(grid), dim3(kBlock), 0, s, nc, r0, z, part);
+        hipLaunchKernelGGL(k_amg_dense, dim3((nc + 3) / 4), dim3(kBlock), 0, s, st, nc, Cinv.p, r0, z);
+        hipLaunchKernelGGL(k_amg_dot, dim3(grid), dim3(kBlock), 0, s, st, nc, r0, z, part);
         TM_HIP(hipGetLastError());
         return TM_OK;
     }
@@ -521,27 +527,27 @@ int AmgPrecond::apply(const double* r0, double* z, double* part, int grid, hipSt
         Level& V = levels[l];
         if (l == 0) {
             const SellPattern& A = *V.A_ext;
-            hipLaunchKernelGGL(k_amg_pre, dim3(grid), dim3(kBlock), 0, s, view(A), V.Aval_ext, V.dinv_ext, b, V.omega,
+            hipLaunchKernelGGL(k_amg_pre, dim3(grid), dim3(kBlock), 0, s, st, view(A), V.Aval_ext, V.dinv_ext, b, V.omega,
                                V.x.p, V.r.p);
         } else {
-            csrv_launch<2>(V.A, nullptr, V.dinv.p, b, V.omega, V.x.p, V.r.p, s);
+            csrv_launch<2>(st, V.A, nullptr, V.dinv.p, b, V.omega, V.x.p, V.r.p, s);
         }
-        csrv_launch<0>(V.R, V.r.p, nullptr, nullptr, 0.0, V.b.p, nullptr, s);  // b_{l+1} = P^T r
+        csrv_launch<0>(st, V.R, V.r.p, nullptr, nullptr, 0.0, V.b.p, nullptr, s);  // b_{l+1} = P^T r
         b = V.b.p;
     }
-    hipLaunchKernelGGL(k_amg_dense, dim3((nc + 3) / 4), dim3(kBlock), 0, s, nc, Cinv.p, levels[L - 1].b.p, xc.p);
+    hipLaunchKernelGGL(k_amg_dense, dim3((nc + 3) / 4), dim3(kBlock), 0, s, st, nc, Cinv.p, levels[L - 1].b.p, xc.p);
     const double* xcoarse = xc.p;
     for (int l = L - 1; l >= 0; --l) {
         Level& V = levels[l];
         const double* bl = l == 0 ? r0 : levels[l - 1].b.p;
-        csrv_launch<1>(V.P, xcoarse, nullptr, nullptr, 0.0, V.x.p, nullptr, s);  // x += P x_c
+        csrv_launch<1>(st, V.P, xcoarse, nullptr, nullptr, 0.0, V.x.p, nullptr, s);  // x += P x_c
         if (l == 0) {
             const SellPattern& A = *V.A_ext;
-            hipLaunchKernelGGL(k_amg_post, dim3(grid), dim3(kBlock), 0, s, view(A), V.Aval_ext, V.dinv_ext, bl,
+            hipLaunchKernelGGL(k_amg_post, dim3(grid), dim3(kBlock), 0, s, st, view(A), V.Aval_ext, V.dinv_ext, bl,
                                V.omega, V.x.p, z, part);
             xcoarse = z;
         } else {
-            csrv_launch<3>(V.A, V.x.p, V.dinv.p, bl, V.omega, nullptr, V.x2.p, s);
+            csrv_launch<3>(st, V.A, V.x.p, V.dinv.p, bl, V.omega, nullptr, V.x2.p, s);
             xcoarse = V.x2.p;
         }
     }

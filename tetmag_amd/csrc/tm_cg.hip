@@ -189,6 +189,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_final(int G, double* part, CgStat
     if (threadIdx.x == 0 && st->done == 0) st->rr = t[1];
 }
 
+// Fixed-budget solves (CgSolver::enqueue): b = 0 gives x = 0 at once.
+__global__ __launch_bounds__(kBlock) void k_cg_trivial(int n, double* __restrict__ x, CgState* st) {
+    if (st->bb != 0.0) return;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->done = 1;
+        st->rr = 0.0;
+    }
+}
+
+// End of a fixed-budget solve: a residual that met the tolerance on the last
+// update counts as converged; anything else increments *fails.  *imax keeps
+// the largest iteration count seen.
+__global__ __launch_bounds__(kBlock) void k_cg_check(int G, int budget, const double* part, CgState* st,
+                                                     unsigned* fails, int* imax) {
+    double t[2];
+    reduce_partials_all<2>(part + G, G, G, t);
+    if (threadIdx.x != 0) return;
+    if (st->done == 0) {
+        st->rr = t[1];
+        if (t[1] <= st->tol2bb) {
+            st->done = 1;
+            st->iters = budget;
+        }
+    }
+    if (st->done != 1) fails[0] += 1u;
+    if (st->iters > imax[0]) imax[0] = st->iters;
+}
+
 }  // namespace
 
 // Transposed solve helpers: zero the fixed entries of the right-hand side,
@@ -333,7 +362,7 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
         hipLaunchKernelGGL(k_cg_init_res<false>, dim3(G), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p,
                            part.p, tol, st.p);
     TM_HIP(hipGetLastError());
-    if (!jac) TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s));
+    if (!jac) TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p));
     // First chunk sized from the previous solve (warm-started solves along a
     // trajectory need similar counts), then small chunks.
     int enq = 0;
@@ -350,7 +379,7 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
             } else {
                 hipLaunchKernelGGL(k_cg_update<false>, dim3(G), dim3(kBlock), 0, s, n, dinv.p, pcur, q.p, x, r.p,
                                    z.p, enq, part.p, st.p);
-                TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s));
+                TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p));
             }
             std::swap(pcur, pold);
         }
@@ -374,6 +403,41 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
     if (hs.done == 2) return TM_ERR_NONFINITE;
     if (hs.done == 1 || hs.bb == 0.0 || hs.rr <= hs.tol2bb) return TM_OK;
     return TM_ERR_MAXITER;
+}
+
+int CgSolver::enqueue(const double* b, double* x, double tol, int budget, hipStream_t s, unsigned* fails,
+                      int* imax) {
+    if (n == 0) return TM_ERR_NOT_READY;
+    if (!(tol > 0.0) || budget < 1 || !fails || !imax) return TM_ERR_BAD_ARG;
+    const int G = grid;
+    const SellView A = view(pat);
+    const bool jac = precond == TM_PRECOND_JACOBI;
+    hipLaunchKernelGGL(k_cg_init_fixed, dim3(G), dim3(kBlock), 0, s, n, fixed.p, b, x, part.p);
+    if (jac)
+        hipLaunchKernelGGL(k_cg_init_res<true>, dim3(G), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p, part.p,
+                           tol, st.p);
+    else
+        hipLaunchKernelGGL(k_cg_init_res<false>, dim3(G), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p,
+                           part.p, tol, st.p);
+    hipLaunchKernelGGL(k_cg_trivial, dim3(G), dim3(kBlock), 0, s, n, x, st.p);
+    if (!jac) TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p));
+    double* pcur = p0.p;
+    double* pold = p1.p;
+    for (int k = 0; k < budget; ++k) {
+        hipLaunchKernelGGL(k_cg_spmv, dim3(G), dim3(kBlock), 0, s, A, val.p, z.p, pold, pcur, q.p, k, part.p, st.p);
+        if (jac) {
+            hipLaunchKernelGGL(k_cg_update<true>, dim3(G), dim3(kBlock), 0, s, n, dinv.p, pcur, q.p, x, r.p, z.p, k,
+                               part.p, st.p);
+        } else {
+            hipLaunchKernelGGL(k_cg_update<false>, dim3(G), dim3(kBlock), 0, s, n, dinv.p, pcur, q.p, x, r.p, z.p, k,
+                               part.p, st.p);
+            TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p));
+        }
+        std::swap(pcur, pold);
+    }
+    hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kBlock), 0, s, G, budget, (const double*)part.p, st.p, fails, imax);
+    TM_HIP(hipGetLastError());
+    return TM_OK;
 }
 
 int CgSolver::solve_transposed(const double* w, double* y, double tol, int maxit, hipStream_t s, int* iters,

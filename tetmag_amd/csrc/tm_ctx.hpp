@@ -64,6 +64,12 @@ struct tm_ctx {
     DevBuf<double> a_dm, a_h, a_gh;      // 3n
     DevBuf<double> a_w0, a_t, a_y1, a_y2;  // n
     DevBuf<double> a_xb, a_part, a_red;  // b, partial sums, totals
+    // on-device RK4 (tm_integrate.hip): stages, backup of the step's start,
+    // device failure counter and iteration maxima of the fixed-budget solves
+    DevBuf<double> rk_k[4], rk_tmp, rk_bak;
+    DevBuf<unsigned> rk_fail;
+    DevBuf<int> rk_imax;
+    int rk_budget[2] = {0, 0};
     // phase timers
     bool timers = false;
     hipEvent_t ev[6] = {};
@@ -87,4 +93,8 @@ int upload_vec(tm_ctx* c, DevBuf<double>& d, const double* h, size_t count);
 bool idmi_ready(const tm_ctx* c);
 // DemagField::calcPotential on device data; leaves u1, u2 in the context.
 int demag_potential_dev(tm_ctx* c, const double* mdev, tm_eval_stats* stats);
+// Right-hand side from a device magnetisation; budgetN > 0: fixed-budget
+// (graph-capturable) solves, else host-polled solves reporting into stats.
+int eval_device(tm_ctx* c, const double* md, double* dmdt, double alpha, int variant, int budgetN, int budgetD,
+                unsigned* fails, int* imax2, tm_eval_stats* stats);
 }  // namespace tmh

@@ -1138,6 +1138,74 @@ int demag_potential_dev(tm_ctx* c, const double* md, tm_eval_stats* stats) {
 
 }  // namespace
 
+namespace tmh {
+// One right-hand side evaluation from a device magnetisation `md` into dmdt,
+// on the context's stream.  budgetN > 0 selects fixed-budget solves (no host
+// synchronisation; failures counted on the device), otherwise the solves poll
+// the host and `stats` receives the iteration counts.  Demag is skipped
+// (frozen potentials kept) when freeze_demag is on and potentials exist.
+int eval_device(tm_ctx* c, const double* md, double* dmdt, double alpha, int variant, int budgetN, int budgetD,
+                unsigned* fails, int* imax2, tm_eval_stats* stats) {
+    TM_CHECK(commit_ops(c));
+    const bool demag = c->has_solvers && c->has_B;
+    const bool solve = demag && !(c->freeze_demag && c->have_u);
+    if (demag && (!c->has_op[TM_OP_GRAD_X] || !c->has_op[TM_OP_GRAD_Y] || !c->has_op[TM_OP_GRAD_Z]))
+        return TM_ERR_NOT_READY;
+    int st = TM_OK;
+    if (solve) {
+        if (budgetN > 0) {
+            if (!c->has_js || !c->has_op[TM_OP_TGRAD_X] || !c->has_op[TM_OP_TGRAD_Y] || !c->has_op[TM_OP_TGRAD_Z])
+                return TM_ERR_NOT_READY;
+            hipLaunchKernelGGL(k_div, dim3(sell_grid(c)), dim3(kBlock), 0, c->stream, view(c->pat),
+                               c->opv[TM_OP_TGRAD_X].p, c->opv[TM_OP_TGRAD_Y].p, c->opv[TM_OP_TGRAD_Z].p, c->js.p, md,
+                               1.0 / kMu0, c->rhsN.p);
+            TM_HIP(hipGetLastError());
+            TM_CHECK(c->cgN.enqueue(c->rhsN.p, c->u1.p, c->tol, budgetN, c->stream, fails, imax2));
+            TM_CHECK(bem_gemv(c->bnx, c->bld, c->B.p, c->u1.p, c->bnodes.p, c->bpart.p, nullptr, c->bnodes.p,
+                              c->rhsD.p, c->stream));
+            TM_CHECK(c->cgD.enqueue(c->rhsD.p, c->u2.p, c->tol, budgetD, c->stream, fails, imax2 + 1));
+            c->have_u = true;
+        } else {
+            st = demag_potential_dev(c, md, stats);
+            if (st != TM_OK && st != TM_ERR_MAXITER) return st;
+        }
+    }
+    const bool exch = c->has_op[TM_OP_EXCHANGE];
+    const bool dmi = c->has_dmi && c->has_op[TM_OP_CURL_X] && c->has_op[TM_OP_CURL_Y] && c->has_op[TM_OP_CURL_Z];
+    EvalArgs a = {};
+    a.A = view(c->pat);
+    a.xc = exch ? c->opv[TM_OP_EXCHANGE].p : nullptr;
+    if (demag) {
+        a.gx = c->opv[TM_OP_GRAD_X].p;
+        a.gy = c->opv[TM_OP_GRAD_Y].p;
+        a.gz = c->opv[TM_OP_GRAD_Z].p;
+        a.u1 = c->u1.p;
+        a.u2 = c->u2.p;
+    }
+    if (dmi) {
+        a.cx = c->opv[TM_OP_CURL_X].p;
+        a.cy = c->opv[TM_OP_CURL_Y].p;
+        a.cz = c->opv[TM_OP_CURL_Z].p;
+        a.dmi_fac = c->dmi_fac.p;
+        a.nvs = c->nvs.p;
+    }
+    fill_idmi(c, a);
+    a.m = md;
+    a.invjs = c->has_js ? c->invjs.p : nullptr;
+    a.ku = c->has_ku ? c->ku.p : nullptr;
+    a.kaxis = c->has_ku ? c->kaxis.p : nullptr;
+    a.hext = c->hext_field ? c->hext.p : nullptr;
+    a.hx = c->hext_u[0];
+    a.hy = c->hext_u[1];
+    a.hz = c->hext_u[2];
+    a.alpha = alpha;
+    a.variant = variant;
+    a.dmdt = dmdt;
+    TM_CHECK(eval_dispatch(a, exch, demag, dmi, sell_grid(c), c->stream));
+    return st;
+}
+}  // namespace tmh
+
 extern "C" {
 
 int tm_demag_potential(tm_ctx* c, const double* m3n, int mwhere, double* u, int uwhere, tm_eval_stats* stats) {

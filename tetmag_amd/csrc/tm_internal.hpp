@@ -250,7 +250,9 @@ struct AmgPrecond {
     DevBuf<double> Cinv, xc, bc;
     int setup(int n, const int* rowptr, const int* colidx, const double* vals, const std::vector<unsigned char>& fixed,
               const SellPattern* A0, const double* A0val, const double* A0dinv, hipStream_t s);
-    int apply(const double* r, double* z, double* part, int grid, hipStream_t s);
+    // z = M^-1 r (one V-cycle); every kernel returns at once when st != NULL
+    // and st->done is set (the CG it serves has stopped).
+    int apply(const double* r, double* z, double* part, int grid, hipStream_t s, const CgState* st = nullptr);
 };
 
 struct CgSolver {
@@ -281,6 +283,11 @@ struct CgSolver {
     // Solves A x = b for device vectors b, x (x is the warm start).  tol is
     // relative to ||b||.  Returns TM_OK, TM_ERR_MAXITER or TM_ERR_NONFINITE.
     int solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* iters, double* relres);
+    // Enqueues a solve of exactly `budget` iterations with no host
+    // synchronisation (capturable in a HIP graph); every kernel returns at
+    // once after convergence.  At the end *fails is incremented on the device
+    // if the tolerance was not met, and *imax keeps the largest count.
+    int enqueue(const double* b, double* x, double tol, int budget, hipStream_t s, unsigned* fails, int* imax);
     // Solves A^T y = w for the same matrix (identity rows on the fixed set F):
     // y_free = A_ff^-1 w_free (A_ff symmetric), y_F = w_F - A_{free,F}^T y_free.
     int solve_transposed(const double* w, double* y, double tol, int maxit, hipStream_t s, int* iters,
