@@ -141,19 +141,26 @@ def vortex(mesh: M.Mesh, core_radius: float = 10.0) -> np.ndarray:
     return np.ascontiguousarray(v.T.reshape(-1))
 
 
-def cuboid_problem(name: str, nx: int, ny: int, nz: int, h: float, jitter: float = 0.0, seed: int = 0,
-                   mat: Material | None = None, init: str = "random") -> Problem:
-    mesh = M.kuhn_cuboid(nx, ny, nz, h, jitter, seed)
+def mesh_problem(name: str, mesh: M.Mesh, mat: Material | None = None, init="random", seed: int = 0) -> Problem:
+    """Problem on any tetrahedral mesh (e.g. one read by meshio.read_vtk);
+    init is "random", "vortex" or a (3n,) array."""
     ops = M.fem_operators(mesh)
     bnd = M.boundary(mesh)
     p = Problem(name, mesh, ops, bnd, mat or Material())
-    if init == "random":
+    if isinstance(init, np.ndarray):
+        p.m0 = np.ascontiguousarray(init, dtype=np.float64)
+    elif init == "random":
         p.m0 = M.init_random_unit(mesh.n, seed + 1)
     elif init == "vortex":
         p.m0 = vortex(mesh)
     else:
         raise ValueError(init)
     return p
+
+
+def cuboid_problem(name: str, nx: int, ny: int, nz: int, h: float, jitter: float = 0.0, seed: int = 0,
+                   mat: Material | None = None, init: str = "random") -> Problem:
+    return mesh_problem(name, M.kuhn_cuboid(nx, ny, nz, h, jitter, seed), mat, init, seed)
 
 
 def config0(seed: int = 2024) -> Problem:
