@@ -49,9 +49,9 @@ def test_rk4_budget_miss_is_redone(setup):
     from tetmag_amd.context import FieldContext
     prob = setup
     dtau = 30e-3 * prob.time_scale()
-    a = FieldContext(prob, tol=1e-8)
-    b = FieldContext(prob, tol=1e-8)
-    a.run_rk4(dtau, 3, use_graph=True)       # budgets learnt at 1e-8
+    a = FieldContext(prob, tol=1e-3)
+    b = FieldContext(prob, tol=1e-3)
+    a.run_rk4(dtau, 3, use_graph=True)       # budgets learnt at a loose tolerance
     b.run_rk4(dtau, 3, use_graph=False)
     a.set_solver_params(1e-13, 100000)       # tighter: the learnt budgets fall short
     b.set_solver_params(1e-13, 100000)

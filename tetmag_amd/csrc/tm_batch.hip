@@ -79,6 +79,37 @@ struct OState {
     for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
          row < (NROWS); row += gridDim.x * kRowsPerBlock)
 
+// Row entries are walked in chunks of kCh: the chunk's column indices first,
+// then kCh independent gathers, then the multiply-adds in the original entry
+// order (so sums are unchanged).  Slots past the row end repeat the row's
+// first column (an in-bounds address) and are skipped in the sums.
+constexpr int kCh = 8;
+
+template <int CH>
+__device__ inline void chunk_cols(const int* __restrict__ ci, int q, int q1, int (&c)[CH]) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) c[k] = ci[q + k < q1 ? q + k : q];
+}
+
+// acc = sum_q v[q] x[ci[q]][lane] over the row (x zero-extended for lanes >= S)
+template <typename T, typename V>
+__device__ inline T row_dot(const CsrArgs& A, const V* __restrict__ v, int row, int S, int lane, bool on,
+                            const T* __restrict__ x) {
+    const int q0 = A.rp[row], q1 = A.rp[row + 1];
+    T acc = 0;
+    for (int q = q0; q < q1; q += kCh) {
+        int c[kCh];
+        chunk_cols<kCh>(A.ci, q, q1, c);
+        T xv[kCh];
+#pragma unroll
+        for (int k = 0; k < kCh; ++k) xv[k] = on ? x[(size_t)c[k] * S + lane] : (T)0;
+#pragma unroll
+        for (int k = 0; k < kCh; ++k)
+            if (q + k < q1) acc += (T)v[q + k] * xv[k];
+    }
+    return acc;
+}
+
 // Writes the block's per-sample partial sum v (lane = sample) to part[s*G + block].
 __device__ inline void b_partial(double* part, int S, double v) {
     __shared__ double sp[kRowsPerBlock][kWave];
@@ -123,11 +154,7 @@ __global__ __launch_bounds__(kBlock) void k_b_resid(CsrArgs A, int S, const doub
                                                     T* __restrict__ rT, double* part_bb, double* part_rr) {
     double vb = 0.0, vr = 0.0;
     B_ROW_LOOP(A.n) {
-        double acc = 0.0;
-        for (int q = A.rp[row]; q < A.rp[row + 1]; ++q) {
-            const int c = A.ci[q];
-            if (on) acc += A.v[q] * x[(size_t)c * S + lane];
-        }
+        const double acc = row_dot<double>(A, A.v, row, S, lane, on, x);
         if (on) {
             const size_t e = (size_t)row * S + lane;
             const double bi = b[e], ri = bi - acc;
@@ -189,13 +216,19 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_spmv(CsrArgs A, int S, const T*
     double v = 0.0;
     B_ROW_LOOP(A.n) {
         T acc = 0;
-        for (int qq = A.rp[row]; qq < A.rp[row + 1]; ++qq) {
-            const int c = A.ci[qq];
-            if (on) {
-                const size_t e = (size_t)c * S + lane;
-                const T pc = k > 0 ? z[e] + beta * pold[e] : z[e];
-                acc += (T)A.v[qq] * pc;
+        const int q0 = A.rp[row], q1 = A.rp[row + 1];
+        for (int qq = q0; qq < q1; qq += kCh) {
+            int c[kCh];
+            chunk_cols<kCh>(A.ci, qq, q1, c);
+            T pc[kCh];
+#pragma unroll
+            for (int j = 0; j < kCh; ++j) {
+                const size_t e = (size_t)c[j] * S + lane;
+                pc[j] = on ? (k > 0 ? z[e] + beta * pold[e] : z[e]) : (T)0;
             }
+#pragma unroll
+            for (int j = 0; j < kCh; ++j)
+                if (qq + j < q1) acc += (T)A.v[qq + j] * pc[j];
         }
         if (on) {
             const size_t e = (size_t)row * S + lane;
@@ -263,9 +296,16 @@ __global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs A, int S, const double
                                                   T* __restrict__ r) {
     B_ROW_LOOP(A.n) {
         T acc = 0;
-        for (int q = A.rp[row]; q < A.rp[row + 1]; ++q) {
-            const int c = A.ci[q];
-            if (on) acc += (T)A.v[q] * ((T)(w * dinv[c]) * b[(size_t)c * S + lane]);
+        const int q0 = A.rp[row], q1 = A.rp[row + 1];
+        for (int q = q0; q < q1; q += kCh) {
+            int c[kCh];
+            chunk_cols<kCh>(A.ci, q, q1, c);
+            T bv[kCh];
+#pragma unroll
+            for (int k = 0; k < kCh; ++k) bv[k] = on ? b[(size_t)c[k] * S + lane] : (T)0;
+#pragma unroll
+            for (int k = 0; k < kCh; ++k)
+                if (q + k < q1) acc += (T)A.v[q + k] * ((T)(w * dinv[c[k]]) * bv[k]);
         }
         if (on) {
             const size_t e = (size_t)row * S + lane;
@@ -280,11 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs A, int S, const double
 template <typename T, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_b_spmm(CsrArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
     B_ROW_LOOP(M.n) {
-        T acc = 0;
-        for (int q = M.rp[row]; q < M.rp[row + 1]; ++q) {
-            const int c = M.ci[q];
-            if (on) acc += (T)M.v[q] * x[(size_t)c * S + lane];
-        }
+        const T acc = row_dot<T>(M, M.v, row, S, lane, on, x);
         if (on) {
             const size_t e = (size_t)row * S + lane;
             if (ACC) y[e] += acc;
@@ -300,11 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_b_post(CsrArgs A, int S, const doubl
                                                    T* __restrict__ xo, double* part) {
     double v = 0.0;
     B_ROW_LOOP(A.n) {
-        T acc = 0;
-        for (int q = A.rp[row]; q < A.rp[row + 1]; ++q) {
-            const int c = A.ci[q];
-            if (on) acc += (T)A.v[q] * x[(size_t)c * S + lane];
-        }
+        const T acc = row_dot<T>(A, A.v, row, S, lane, on, x);
         if (on) {
             const size_t e = (size_t)row * S + lane;
             const T bi = b[e];
@@ -336,15 +368,26 @@ __global__ __launch_bounds__(kBlock) void k_b_div(CsrArgs tx, const double* __re
     const size_t NS = (size_t)n * S;
     B_ROW_LOOP(n) {
         double sx = 0, sy = 0, sz = 0;
-        for (int q = tx.rp[row]; q < tx.rp[row + 1]; ++q) {
-            const int c = tx.ci[q];
-            if (on) {
-                const size_t e = (size_t)c * S + lane;
-                const double j = js[c];
-                sx += tx.v[q] * (m[e] * j);
-                sy += ty[q] * (m[NS + e] * j);
-                sz += tz[q] * (m[2 * NS + e] * j);
+        const int q0 = tx.rp[row], q1 = tx.rp[row + 1];
+        for (int q = q0; q < q1; q += 4) {
+            int c[4];
+            chunk_cols<4>(tx.ci, q, q1, c);
+            double mv[4][3];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const size_t e = (size_t)c[k] * S + lane;
+                mv[k][0] = on ? m[e] : 0.0;
+                mv[k][1] = on ? m[NS + e] : 0.0;
+                mv[k][2] = on ? m[2 * NS + e] : 0.0;
             }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (q + k < q1) {
+                    const double j = js[c[k]];
+                    sx += tx.v[q + k] * (mv[k][0] * j);
+                    sy += ty[q + k] * (mv[k][1] * j);
+                    sz += tz[q + k] * (mv[k][2] * j);
+                }
         }
         if (on) out[(size_t)row * S + lane] = (sx + sy + sz) * scale;
     }
@@ -375,26 +418,38 @@ __global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
     double tmax = 0.0;
     B_ROW_LOOP(a.n) {
         double ex = 0, ey = 0, ez = 0, dx = 0, dy = 0, dz = 0, cxs = 0, cys = 0, czs = 0;
-        for (int q = A.rp[row]; q < A.rp[row + 1]; ++q) {
-            const int c = A.ci[q];
-            if (!on) continue;
-            const size_t e = (size_t)c * S + lane;
-            const double mx = a.m[e], my = a.m[NS + e], mz = a.m[2 * NS + e];
-            if (A.v) {
-                const double xc = A.v[q];
-                ex += xc * mx;
-                ey += xc * my;
-                ez += xc * mz;
+        const int q0 = A.rp[row], q1 = A.rp[row + 1];
+        for (int q = q0; q < q1; q += 4) {
+            int c[4];
+            chunk_cols<4>(A.ci, q, q1, c);
+            double mv[4][3], uv[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const size_t e = (size_t)c[k] * S + lane;
+                mv[k][0] = on ? a.m[e] : 0.0;
+                mv[k][1] = on ? a.m[NS + e] : 0.0;
+                mv[k][2] = on ? a.m[2 * NS + e] : 0.0;
+                uv[k] = on ? a.u1[e] + a.u2[e] : 0.0;
             }
-            const double u = a.u1[e] + a.u2[e];
-            dx += a.gx[q] * u;
-            dy += a.gy[q] * u;
-            dz += a.gz[q] * u;
-            if (DMI) {
-                const double X = a.cx[q], Y = a.cy[q], Z = a.cz[q];
-                cxs += Y * mz - Z * my;
-                cys += Z * mx - X * mz;
-                czs += X * my - Y * mx;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (q + k >= q1) continue;
+                const double mx = mv[k][0], my = mv[k][1], mz = mv[k][2];
+                if (A.v) {
+                    const double xc = A.v[q + k];
+                    ex += xc * mx;
+                    ey += xc * my;
+                    ez += xc * mz;
+                }
+                dx += a.gx[q + k] * uv[k];
+                dy += a.gy[q + k] * uv[k];
+                dz += a.gz[q + k] * uv[k];
+                if (DMI) {
+                    const double X = a.cx[q + k], Y = a.cy[q + k], Z = a.cz[q + k];
+                    cxs += Y * mz - Z * my;
+                    cys += Z * mx - X * mz;
+                    czs += X * my - Y * mx;
+                }
             }
         }
         if (!on) continue;
@@ -449,20 +504,28 @@ __global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
     }
 }
 
-// U[j][s] = u[bn[j]][s]
+// U[j][s] = u[bn[j]][s]  (S x b column-major, ld S)
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_gather(int b, int S, const int* __restrict__ bn,
-                                                     const double* __restrict__ u, double* __restrict__ U) {
+                                                     const double* __restrict__ u, T* __restrict__ U) {
     const size_t tot = (size_t)b * S;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
-        U[e] = u[(size_t)bn[e / S] * S + e % S];
+        U[e] = (T)u[(size_t)bn[e / S] * S + e % S];
 }
 
-// out[bn[j]][s] = G[j][s]
+// out[bn[j]][s] = G[j + s b]  (G = B U^T, b x S column-major)
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_scatter(int b, int S, const int* __restrict__ bn,
-                                                      const double* __restrict__ G, double* __restrict__ out) {
+                                                      const T* __restrict__ G, double* __restrict__ out) {
     const size_t tot = (size_t)b * S;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
-        out[(size_t)bn[e / S] * S + e % S] = G[e];
+        out[(size_t)bn[e % b] * S + e / b] = (double)G[e];
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_to_float(size_t tot, const double* __restrict__ a,
+                                                       float* __restrict__ o) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
+        o[e] = (float)a[e];
 }
 
 // m <- normalize(m + dt dm) per (node, sample)
@@ -679,6 +742,7 @@ struct tm_batch {
     DevBuf<double> gx, gy, gz, tx, ty, tz, cx, cy, cz;
     bool exch = false, dmi = false;
     DevBuf<double> m, dmdt, heff, u1, u2, rhsN, rhsD, U, Gm, hext, torque;
+    DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product of the mixed form
     BatchSolver<double> sN64, sD64;
     BatchSolver<float> sN32, sD32;
     bool timers = false;
@@ -779,11 +843,22 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     a |= bt->u2.alloc(NS);
     a |= bt->rhsN.alloc(NS);
     a |= bt->rhsD.alloc(NS);
-    a |= bt->U.alloc((size_t)v.bnx * nsamples);
-    a |= bt->Gm.alloc((size_t)v.bnx * nsamples);
+    if (precision == TM_PREC_MIXED) {
+        a |= bt->Uf.alloc((size_t)v.bnx * nsamples);
+        a |= bt->Gf.alloc((size_t)v.bnx * nsamples);
+        a |= bt->Bf.alloc((size_t)v.bld * v.bnx);
+    } else {
+        a |= bt->U.alloc((size_t)v.bnx * nsamples);
+        a |= bt->Gm.alloc((size_t)v.bnx * nsamples);
+    }
     a |= bt->hext.alloc((size_t)3 * nsamples);
     a |= bt->torque.alloc(nsamples);
     if (a != TM_OK) return fail(TM_ERR_HIP);
+    if (precision == TM_PREC_MIXED) {
+        const size_t tot = (size_t)v.bld * v.bnx;
+        hipLaunchKernelGGL(k_b_to_float, dim3(elem_grid(tot)), dim3(kBlock), 0, bt->stream, tot, v.B, bt->Bf.p);
+        if (hipGetLastError() != hipSuccess) return fail(TM_ERR_HIP);
+    }
     (void)hipMemsetAsync(bt->u1.p, 0, NS * sizeof(double), bt->stream);
     (void)hipMemsetAsync(bt->u2.p, 0, NS * sizeof(double), bt->stream);
     (void)hipMemsetAsync(bt->rhsD.p, 0, NS * sizeof(double), bt->stream);
@@ -879,15 +954,27 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[2], s));
     // boundary integral (DemagField.cpp:184-201) as a GEMM: G = B U
     const int b = v.bnx;
-    hipLaunchKernelGGL(k_b_gather, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
-                       (const double*)bt->u1.p, bt->U.p);
-    const double one = 1.0, zero = 0.0;
-    // column-major: Gt (S x b) = Ut (S x b) * B^T
-    if (rocblas_dgemm(bt->blas, rocblas_operation_none, rocblas_operation_transpose, S, b, b, &one, bt->U.p, S, v.B,
-                      v.bld, &zero, bt->Gm.p, S) != rocblas_status_success)
-        return TM_ERR_HIP;
-    hipLaunchKernelGGL(k_b_scatter, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
-                       (const double*)bt->Gm.p, bt->rhsD.p);
+    // column-major G (b x S) = B (b x b, streamed in its storage order) * U^T,
+    // U = gathered boundary potentials (S x b); FP32 in the mixed form
+    if (bt->prec == TM_PREC_MIXED) {
+        hipLaunchKernelGGL(k_b_gather<float>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
+                           (const double*)bt->u1.p, bt->Uf.p);
+        const float one = 1.0f, zero = 0.0f;
+        if (rocblas_sgemm(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, &one, bt->Bf.p,
+                          v.bld, bt->Uf.p, S, &zero, bt->Gf.p, b) != rocblas_status_success)
+            return TM_ERR_HIP;
+        hipLaunchKernelGGL(k_b_scatter<float>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
+                           (const float*)bt->Gf.p, bt->rhsD.p);
+    } else {
+        hipLaunchKernelGGL(k_b_gather<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
+                           (const double*)bt->u1.p, bt->U.p);
+        const double one = 1.0, zero = 0.0;
+        if (rocblas_dgemm(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, &one, v.B, v.bld,
+                          bt->U.p, S, &zero, bt->Gm.p, b) != rocblas_status_success)
+            return TM_ERR_HIP;
+        hipLaunchKernelGGL(k_b_scatter<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
+                           (const double*)bt->Gm.p, bt->rhsD.p);
+    }
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[3], s));
     int stD = bt->prec == TM_PREC_FP64 ? bt->sD64.solve(bt->rhsD.p, bt->u2.p, v.tol, v.maxit, s, &oD, &iD, &rD)
                                        : bt->sD32.solve(bt->rhsD.p, bt->u2.p, v.tol, v.maxit, s, &oD, &iD, &rD);

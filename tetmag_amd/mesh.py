@@ -85,6 +85,59 @@ def kuhn_cuboid(nx: int, ny: int, nz: int, h: float, jitter: float = 0.0, seed: 
     return Mesh(xyz, fel, scale)
 
 
+def disk_delaunay(diameter: float, thickness: float, h: float, jitter: float = 0.15, seed: int = 0,
+                  scale: float = 1e-9) -> Mesh:
+    """Seeded jittered Delaunay tetrahedralisation of a disk (cylinder of the
+    given diameter and thickness, mesh units): layers of points every ~h in z,
+    a square lattice of spacing h inside each layer plus a ring of points on
+    the rim, interior points jittered by +-jitter*h (surface points stay on
+    their surface).  Tetrahedra are oriented positively; flat ones (volume
+    below 1e-9 h^3, from coplanar surface points) are dropped, then unused
+    points.  Points are ordered layer by layer for locality."""
+    from scipy.spatial import Delaunay
+    R = 0.5 * diameter
+    nz = max(1, int(round(thickness / h)))
+    zs = np.linspace(0.0, thickness, nz + 1)
+    g = np.arange(-R, R + 1e-9 * h, h)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    X, Y = X.reshape(-1), Y.reshape(-1)
+    inside = X * X + Y * Y < (R - 0.5 * h) ** 2
+    X, Y = X[inside], Y[inside]
+    nr = max(8, int(np.ceil(2 * np.pi * R / h)))
+    ang = 2 * np.pi * np.arange(nr) / nr
+    RX, RY = R * np.cos(ang), R * np.sin(ang)
+    u = uniform(seed, 3 * (X.size + nr) * (nz + 1))
+    pts = []
+    k = 0
+    for iz, z in enumerate(zs):
+        top = iz == 0 or iz == nz
+        jx = jitter * h * (2 * u[k:k + X.size] - 1)
+        jy = jitter * h * (2 * u[k + X.size:k + 2 * X.size] - 1)
+        jz = np.zeros(X.size) if top else jitter * h * (2 * u[k + 2 * X.size:k + 3 * X.size] - 1)
+        k += 3 * X.size
+        px, py = X + jx, Y + jy
+        r = np.hypot(px, py)
+        f = np.where(r > R - 0.25 * h, (R - 0.25 * h) / np.maximum(r, 1e-30), 1.0)  # keep off the rim ring
+        pts.append(np.stack([px * f, py * f, np.full(X.size, z) + jz], axis=1))
+        pts.append(np.stack([RX, RY, np.full(nr, z)], axis=1))
+    P = np.concatenate(pts)
+    P = P[np.lexsort((P[:, 0], P[:, 1], np.round(P[:, 2] / h).astype(np.int64)))]
+    tri = Delaunay(P)
+    T = tri.simplices.astype(np.int64)
+    a = P[T[:, 1]] - P[T[:, 0]]
+    b = P[T[:, 2]] - P[T[:, 0]]
+    c = P[T[:, 3]] - P[T[:, 0]]
+    vol = np.einsum("ij,ij->i", a, np.cross(b, c)) / 6.0
+    keep = np.abs(vol) > 1e-9 * h ** 3
+    T, vol = T[keep], vol[keep]
+    neg = vol < 0
+    T[neg] = T[neg][:, [0, 2, 1, 3]]
+    used = np.unique(T)
+    remap = -np.ones(P.shape[0], np.int64)
+    remap[used] = np.arange(used.size)
+    return Mesh(np.ascontiguousarray(P[used]), np.ascontiguousarray(remap[T].astype(np.int32)), scale)
+
+
 def fem_operators(mesh: Mesh) -> Operators:
     L = N.lib()
     n, nt = mesh.n, mesh.ntet
