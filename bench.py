@@ -158,6 +158,39 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
     return out
 
 
+def kernel_only(ctx, prob, reps):
+    """Launch the dense boundary product (k_bem_gemv_part + k_bem_gemv_finish)
+    and a plain 1 GiB device-to-device copy `reps` times each, timed with HIP
+    events on the stream they run on.  Algorithmic bytes: 8 b^2 + 16 b per
+    product, 2 GiB per copy (read + write)."""
+    import torch
+    from tetmag_amd import _native as N
+    b = prob.bnd.bnx
+    with torch.cuda.stream(ctx.stream):
+        x = torch.rand(b, dtype=torch.float64, device=ctx.device)
+        y = torch.empty(b, dtype=torch.float64, device=ctx.device)
+        src = torch.rand(1 << 27, dtype=torch.float64, device=ctx.device)
+        dst = torch.empty_like(src)
+    ctx.stream.synchronize()
+    L = N.lib()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(ctx.stream)
+    for _ in range(reps):
+        N.check(L.tm_bem_apply(ctx.h, N.ptr(x), N.TM_DEVICE, N.ptr(y), N.TM_DEVICE), "tm_bem_apply")
+    e1.record(ctx.stream)
+    with torch.cuda.stream(ctx.stream):
+        for _ in range(reps):
+            dst.copy_(src)
+    e2.record(ctx.stream)
+    torch.cuda.synchronize()
+    gemv = e0.elapsed_time(e1) / reps
+    cp = e1.elapsed_time(e2) / reps
+    log(f"[kernel-only] b={b}: boundary product {gemv:.3f} ms = {(8.0 * b * b + 16 * b) / gemv / 1e6:.0f} GB/s; "
+        f"1 GiB copy {cp:.3f} ms = {2 * (1 << 30) / cp / 1e6:.0f} GB/s")
+    print(json.dumps({"kernel_only": {"b": b, "reps": reps, "gemv_ms": gemv, "copy_ms": cp,
+                                      "gemv_bytes": 8.0 * b * b + 16 * b, "copy_bytes": 2.0 * (1 << 30)}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +205,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the per-sample error check of the batched forms")
+    ap.add_argument("--kernel-only", type=int, default=0, metavar="REPS",
+                    help="only launch the dense boundary product REPS times, then a 1 GiB device copy REPS times "
+                         "(the calibration kernel for hardware byte counters), and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -192,6 +228,9 @@ def main():
     log(f"[setup] device context + GPU BEM assembly {time.time() - t0:.1f}s")
     dtau = args.dt_fs * 1e-3 * prob.time_scale()
     log(f"[setup] AMG levels neumann {ctx.amg_levels(0)} dirichlet {ctx.amg_levels(1)}")
+    if args.kernel_only:
+        kernel_only(ctx, prob, args.kernel_only)
+        return
 
     def step():
         ctx.eval_rhs(None)

@@ -1,0 +1,58 @@
+"""Seeded workloads beyond the Kuhn cuboids: jittered Delaunay disks
+(hysteresis-sweep batch and thin-film skyrmion), checked for mesh validity
+and the stated field / state parameters."""
+import numpy as np
+import pytest
+
+from tetmag_amd import mesh as M
+from tetmag_amd import problem as P
+
+MU0 = P.MU0
+
+
+def _closed_surface(bnd):
+    from collections import Counter
+    edges = Counter()
+    for f in bnd.faces:
+        a, b, c = sorted(int(v) for v in f)
+        edges.update([(a, b), (b, c), (a, c)])
+    return all(v == 2 for v in edges.values())
+
+
+def test_disk_delaunay_mesh():
+    mesh = M.disk_delaunay(40.0, 4.0, 2.0, 0.15, 3)
+    x, f = mesh.xyz, mesh.fel
+    vol = np.einsum("ij,ij->i", x[f[:, 1]] - x[f[:, 0]], np.cross(x[f[:, 2]] - x[f[:, 0]], x[f[:, 3]] - x[f[:, 0]])) / 6
+    assert np.all(vol > 0)
+    exact = np.pi * 20.0 ** 2 * 4.0
+    assert abs(vol.sum() - exact) < 0.01 * exact  # polygonal rim
+    assert np.all(np.abs(x[:, 2] - np.clip(x[:, 2], 0, 4)) < 1e-12)
+    bnd = M.boundary(mesh)
+    assert _closed_surface(bnd)
+    mesh2 = M.disk_delaunay(40.0, 4.0, 2.0, 0.15, 3)
+    assert np.array_equal(mesh.xyz, mesh2.xyz) and np.array_equal(mesh.fel, mesh2.fel)  # seeded
+
+
+def test_hysteresis_sweep_batch():
+    prob, H, m0 = P.config3(h=6.0, nsamples=16)
+    Bk = np.linalg.norm(H, axis=0) * MU0 * np.sign(H[0])
+    assert np.allclose(Bk, np.linspace(-0.08, 0.08, 16))
+    ang = np.arctan2(np.abs(H[1]), np.abs(H[0]))
+    assert np.all(ang <= np.deg2rad(5.0) + 1e-12) and np.all(H[2] == 0)
+    assert np.allclose(np.linalg.norm(m0, axis=0), 1.0)
+    assert np.all(m0[0] > 0.8)
+
+
+def test_thin_film_skyrmion():
+    prob = P.config4(h=2.0)
+    n = prob.n
+    m = prob.m0.reshape(3, n)
+    xyz = prob.mesh.xyz
+    r = np.hypot(xyz[:, 0] - 0.5 * (xyz[:, 0].min() + xyz[:, 0].max()),
+                 xyz[:, 1] - 0.5 * (xyz[:, 1].min() + xyz[:, 1].max()))
+    assert np.allclose(np.linalg.norm(m, axis=0), 1.0)
+    assert np.all(m[2][r < 3.0] < -0.9) and np.all(m[2][r > 50.0] > 0.7)
+    assert np.allclose(m[2][np.abs(r - 15.0) < 0.5], 0.0, atol=0.05)
+    assert prob.use_idmi and not prob.use_dmi
+    assert prob.Di[0] == pytest.approx(3e-3 / 1e-9)
+    assert prob.Ku[0] == 8e5 and np.allclose(prob.kaxis.reshape(3, n)[2], 1.0)

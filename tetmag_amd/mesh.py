@@ -122,7 +122,7 @@ def disk_delaunay(diameter: float, thickness: float, h: float, jitter: float = 0
         pts.append(np.stack([RX, RY, np.full(nr, z)], axis=1))
     P = np.concatenate(pts)
     P = P[np.lexsort((P[:, 0], P[:, 1], np.round(P[:, 2] / h).astype(np.int64)))]
-    tri = Delaunay(P)
+    tri = Delaunay(P, qhull_options="QJ Qbb Q12")  # joggled input: no degenerate cospherical sets
     T = tri.simplices.astype(np.int64)
     a = P[T[:, 1]] - P[T[:, 0]]
     b = P[T[:, 2]] - P[T[:, 0]]

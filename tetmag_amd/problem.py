@@ -203,6 +203,54 @@ def smooth_random_fields(mesh: M.Mesh, count: int, seed: int, modes: int = 8, la
     return out
 
 
+def neel_skyrmion(mesh: M.Mesh, radius: float, chirality: float = 1.0) -> np.ndarray:
+    """Neel skyrmion about the xy-centre: core down in an up background,
+    theta(r) = 2 atan(radius / r) (theta = pi/2 at r = radius), in-plane part
+    along +-r_hat (chirality +1 or -1)."""
+    xyz = mesh.xyz
+    cx = 0.5 * (xyz[:, 0].min() + xyz[:, 0].max())
+    cy = 0.5 * (xyz[:, 1].min() + xyz[:, 1].max())
+    x, y = xyz[:, 0] - cx, xyz[:, 1] - cy
+    r = np.hypot(x, y)
+    th = 2.0 * np.arctan2(radius, r)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ux = np.where(r > 0, x / r, 0.0)
+        uy = np.where(r > 0, y / r, 0.0)
+    m = np.stack([chirality * np.sin(th) * ux, chirality * np.sin(th) * uy, np.cos(th)], axis=1)
+    m /= np.linalg.norm(m, axis=1, keepdims=True)
+    return np.ascontiguousarray(m.T.reshape(-1))
+
+
+def config3(seed: int = 2027, h: float = 2.5, nsamples: int = 64):
+    """Disk 200 nm x 10 nm, seeded jittered Delaunay at target edge h; config-0
+    material; `nsamples` states, each under B_ext = mu0 H_k (cos t_i, sin t_i, 0)
+    with mu0 H_k spaced from -80 to +80 mT and t_i seeded in [0, 5 deg],
+    starting from a slightly perturbed saturated state along x.
+    Returns (problem, fields (3, S) in A/m, m0 (3, n, S))."""
+    mesh = M.disk_delaunay(200.0, 10.0, h, 0.15, seed)
+    prob = mesh_problem(f"disk-200x10-h{h}", mesh, Material(), "random", seed)
+    u = M.uniform(seed + 11, nsamples)
+    th = np.deg2rad(5.0) * u
+    Bk = np.linspace(-0.08, 0.08, nsamples)
+    H = np.stack([Bk * np.cos(th), Bk * np.sin(th), np.zeros(nsamples)]) / MU0
+    n = prob.n
+    m0 = np.empty((3, n, nsamples))
+    for s in range(nsamples):
+        p = M.init_random_unit(n, seed + 100 + s).reshape(3, n)
+        v = np.array([1.0, 0.0, 0.0])[:, None] + 0.05 * p
+        m0[:, :, s] = v / np.linalg.norm(v, axis=0, keepdims=True)
+    return prob, H, m0
+
+
+def config4(seed: int = 2028, h: float = 0.6) -> Problem:
+    """Disk 120 nm x 1 nm (seeded jittered Delaunay, edge h) with interfacial
+    DMI 3e-3 J/m^2, Ku 8e5 J/m^3 along z, Ms 1.1e6 A/m, A 1.6e-11 J/m,
+    alpha 0.3; Neel skyrmion of radius 15 nm."""
+    mesh = M.disk_delaunay(120.0, 1.0, h, 0.15, seed)
+    mat = Material(Ms=1.1e6, A=1.6e-11, Ku=8e5, ku_axis=(0.0, 0.0, 1.0), alpha=0.3, Di=3e-3)
+    return mesh_problem(f"disk-120x1-h{h}", mesh, mat, neel_skyrmion(mesh, 15.0), seed)
+
+
 def small_problem(nx=6, ny=5, nz=3, h=4.0, jitter=0.1, seed=7, Ku=3e4, D=0.0, B_ext=(0.0, 0.0, 0.0),
                   init="random") -> Problem:
     """Small seeded problem for parity tests (all field terms switchable)."""
