@@ -104,7 +104,7 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
     import torch
     from tetmag_amd import _native as N
     from tetmag_amd.batch import BatchContext, to_batch, from_batch
-    name = "mixed" if precision == N.TM_PREC_MIXED else "fp64"
+    name = {N.TM_PREC_FP64: "fp64", N.TM_PREC_MIXED: "mixed", N.TM_PREC_MIXED_BEM32: "mixed_bem32"}[precision]
     bc = BatchContext(ctx, nsamples, precision)
     mb = to_batch(fields[:nsamples])
     out = {"form": name, "samples": nsamples}
@@ -275,17 +275,21 @@ def main():
     if not args.no_batched and not args.small:
         from tetmag_amd import _native as N
         fields = P.smooth_random_fields(prob.mesh, args.batch, seed=2026)
-        for prec in (N.TM_PREC_FP64, N.TM_PREC_MIXED):
+        for prec in (N.TM_PREC_FP64, N.TM_PREC_MIXED, N.TM_PREC_MIXED_BEM32):
             leg = batched_leg(prob, ctx, dtau, prec, args.batch, args.batch_steps, 2, fields, not args.no_check)
             legs[leg["form"]] = leg
-    # headline: the mixed/batched form if it meets the 1e-6 field error bar,
-    # else the FP64 batched form, else the plain form
+    # headline: the fastest batched form whose field error against FP64 meets
+    # the 1e-6 bar (the FP64 batched form always does), else the plain form
     head, form, dtype = plain["evals_per_s"], "plain FP64, unbatched", "f64"
-    mixed, b64 = legs.get("mixed"), legs.get("fp64")
-    if mixed and (args.no_check or mixed["err_vs_fp64"]["heff_rel_l2_max"] <= 1e-6):
-        head, form, dtype = mixed["evals_per_s"], f"mixed FP32 inner CG + FP64 refinement, batched x{args.batch}", "f32+f64"
-    elif b64:
-        head, form, dtype = b64["evals_per_s"], f"FP64, batched x{args.batch}", "f64"
+    names = {"fp64": (f"FP64, batched x{args.batch}", "f64"),
+             "mixed": (f"FP32 inner CG + FP64 refinement, FP64 boundary product, batched x{args.batch}", "f32+f64"),
+             "mixed_bem32": (f"FP32 inner CG + FP64 refinement, FP32 boundary product, batched x{args.batch}",
+                             "f32+f64")}
+    for key, leg in legs.items():
+        ok = key == "fp64" or (not args.no_check and leg["err_vs_fp64"]["heff_rel_l2_max"] <= 1e-6)
+        leg["meets_1e-6"] = bool(ok)
+        if ok and leg["evals_per_s"] > head:
+            head, (form, dtype) = leg["evals_per_s"], names[key]
     result = {
         "metric": "Heff+LLG RHS evals/s @283k nodes: FP64 plain vs mixed/batched at 1e-6 rel err",
         "value": head,

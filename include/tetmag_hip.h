@@ -359,6 +359,9 @@ int tm_ctx_get_timers(tm_ctx* c, tm_phase_times* out, int reset);
 #define TM_BATCH_MAX 64
 #define TM_PREC_FP64 0  /* all-FP64 batched solves */
 #define TM_PREC_MIXED 1 /* FP32 inner CG iterations, FP64 iterative refinement to the tolerance */
+#define TM_PREC_MIXED_BEM32 2 /* as MIXED, and the boundary product in FP32 (FP32 B copy, FP32
+                               * accumulation): faster, but ~1e-5 relative field error on a 41k-node
+                               * boundary, outside a 1e-6 bar */
 
 typedef struct tm_batch tm_batch;
 typedef struct tm_batch_stats {
@@ -384,6 +387,12 @@ int tm_batch_eval_rhs(tm_batch* b, const double* m, int mwhere, double alpha, in
 int tm_batch_get_potentials(tm_batch* b, double* u1, double* u2, int where);
 /* m <- normalize(m + dtau dm/dtau) for every sample. */
 int tm_batch_step_euler(tm_batch* b, const double* dmdt, int where, double dtau);
+/* nsteps classical RK4 steps of every sample's magnetisation (renormalised
+ * after each step), each stage a full batched evaluation; stats of the last
+ * evaluation in `last` (may be null).  The hysteresis driver of the
+ * reference steps one field value at a time through CVODE
+ * (Hysteresis.cpp:43-169, TheLLG.cpp:578-711). */
+int tm_batch_run_rk4(tm_batch* b, double dtau, int nsteps, double alpha, int variant, tm_batch_stats* last);
 int tm_batch_enable_timers(tm_batch* b, int on);
 int tm_batch_get_timers(tm_batch* b, tm_phase_times* out, int reset);
 

@@ -41,6 +41,7 @@ TM_PRECOND_JACOBI = 0
 TM_PRECOND_AMG = 1
 TM_PREC_FP64 = 0
 TM_PREC_MIXED = 1
+TM_PREC_MIXED_BEM32 = 2
 TM_BATCH_MAX = 64
 TM_BEM_REFERENCE = 0
 TM_BEM_ACCURATE = 1
@@ -208,6 +209,7 @@ _SIGNATURES = {
     "tm_batch_eval_rhs": [_P, _P, _I, _D, _I, _P, _I, _P, _I, _P],
     "tm_batch_get_potentials": [_P, _P, _P, _I],
     "tm_batch_step_euler": [_P, _P, _I, _D],
+    "tm_batch_run_rk4": [_P, _D, _I, _D, _I, _P],
     "tm_batch_enable_timers": [_P, _I],
     "tm_batch_get_timers": [_P, _P, _I],
     "tm_version": [],

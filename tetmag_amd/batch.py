@@ -78,6 +78,14 @@ class BatchContext:
         d = self.dmdt if dmdt is None else dmdt
         N.check(N.lib().tm_batch_step_euler(self.h, N.ptr(d), N.TM_DEVICE, float(dtau)), "tm_batch_step_euler")
 
+    def run_rk4(self, dtau: float, nsteps: int, alpha=None, variant=N.TM_LLG_CLASSIC) -> dict:
+        """nsteps RK4 steps of every sample on the device (tm_batch_run_rk4)."""
+        alpha = self.ctx.prob.alpha if alpha is None else alpha
+        st = N.BatchStats()
+        N.check(N.lib().tm_batch_run_rk4(self.h, float(dtau), int(nsteps), float(alpha), int(variant), C.byref(st)),
+                "tm_batch_run_rk4")
+        return st.as_dict()
+
     def enable_timers(self, on: bool = True):
         N.check(N.lib().tm_batch_enable_timers(self.h, int(on)), "tm_batch_enable_timers")
 

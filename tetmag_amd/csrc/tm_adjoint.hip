@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_lin(VjpArgs a) {
     __shared__ double smem[3 * (kBlock / kWave)];
     const SellView A = a.A;
     const int n = A.n;
-    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int row = xcd_block() * kBlock + threadIdx.x;
     const int s = row / kWave, lane = row & (kWave - 1);
     double hs[3] = {0.0, 0.0, 0.0};
     if (s < (n + kWave - 1) / kWave) {
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_div_T(SellView A, const double* 
                                                       const double* __restrict__ js, double scale,
                                                       const double* __restrict__ y, double* __restrict__ g) {
     const int n = A.n;
-    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int row = xcd_block() * kBlock + threadIdx.x;
     const int s = row / kWave, lane = row & (kWave - 1);
     if (s >= (n + kWave - 1) / kWave) return;
     double ax = 0, ay = 0, az = 0;

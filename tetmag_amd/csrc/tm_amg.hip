@@ -245,11 +245,13 @@ bool dense_inverse(std::vector<double>& a, int n) {
 
 // ---------------------------------------------------------------- kernels --
 
-#define AMG_SLICE_LOOP(A)                                                      \
-    const int lane = threadIdx.x & (kWave - 1);                                \
-    const int nsl = ((A).n + kWave - 1) / kWave;                               \
-    for (int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < nsl; \
-         s += gridDim.x * (kBlock / kWave))
+#define AMG_SLICE_LOOP(A)                                                                             \
+    const int lane = threadIdx.x & (kWave - 1);                                                       \
+    const int nsl = ((A).n + kWave - 1) / kWave;                                                      \
+    const int spb_ = ((nsl + (int)gridDim.x - 1) / (int)gridDim.x + kBlock / kWave - 1) / (kBlock / kWave) * \
+                     (kBlock / kWave);                                                                \
+    const int s0_ = xcd_block() * spb_, s1_ = min(nsl, s0_ + spb_);                                   \
+    for (int s = s0_ + (int)threadIdx.x / kWave; s < s1_; s += kBlock / kWave)
 
 // x = w D^-1 b ;  r = b - A x   (x gathered on the fly as w dinv_j b_j)
 __global__ __launch_bounds__(kBlock) void k_amg_pre(const CgState* st, SellView A, const double* __restrict__ val,

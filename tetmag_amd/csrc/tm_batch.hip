@@ -73,11 +73,20 @@ struct OState {
     int done[kMaxS];
 };
 
-#define B_ROW_LOOP(NROWS)                                                            \
-    const int lane = threadIdx.x & (kWave - 1);                                      \
-    const bool on = lane < S;                                                        \
-    for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
-         row < (NROWS); row += gridDim.x * kRowsPerBlock)
+// Rows are split into gridDim.x contiguous ranges.  Consecutive blocks are
+// dispatched to the 8 XCDs in turn, so block b takes range
+// (b % 8) * (G / 8) + b / 8 (G a multiple of 8): each XCD walks one contiguous
+// eighth of the rows, and the gathers of neighbouring rows (which share
+// columns) hit that XCD's L2 instead of every L2 fetching every vector.
+#define B_ROW_LOOP(NROWS)                                                                              \
+    const int lane = threadIdx.x & (kWave - 1);                                                        \
+    const bool on = lane < S;                                                                          \
+    const int rpb_ = (((NROWS) + (int)gridDim.x - 1) / (int)gridDim.x + kRowsPerBlock - 1) / kRowsPerBlock * \
+                     kRowsPerBlock;                                                                    \
+    const int r0_ = xcd_block() * rpb_;                                                                \
+    const int r1_ = min((NROWS), r0_ + rpb_);                                                          \
+    for (int row = __builtin_amdgcn_readfirstlane(r0_ + (int)(threadIdx.x / kWave)); row < r1_;        \
+         row += kRowsPerBlock)
 
 // Row entries are walked in chunks of kCh: the chunk's column indices first,
 // then kCh independent gathers, then the multiply-adds in the original entry
@@ -545,8 +554,37 @@ __global__ void k_b_zero_small(int cnt, double* p) {
     if ((int)threadIdx.x < cnt) p[threadIdx.x] = 0.0;
 }
 
+// RK4 helpers over [3][n][S]: out = base + c k;  m = normalise(base + dt/6 (k1 + 2k2 + 2k3 + k4))
+__global__ __launch_bounds__(kBlock) void k_b_axpy(size_t tot, const double* __restrict__ base,
+                                                   const double* __restrict__ k, double c, double* __restrict__ out) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
+        out[e] = base[e] + c * k[e];
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_rk_final(size_t NS, const double* __restrict__ base,
+                                                       const double* __restrict__ k1, const double* __restrict__ k2,
+                                                       const double* __restrict__ k3, const double* __restrict__ k4,
+                                                       double dt, double* __restrict__ m) {
+    const double f = dt / 6.0;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock) {
+        double v[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t q = (size_t)d * NS + e;
+            v[d] = base[q] + f * (k1[q] + 2.0 * k2[q] + 2.0 * k3[q] + k4[q]);
+        }
+        const double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        const double g = l > 0.0 ? 1.0 / l : 1.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) m[(size_t)d * NS + e] = v[d] * g;
+    }
+}
+
 inline int elem_grid(size_t tot) { return (int)std::min<size_t>(4096, (tot + kBlock - 1) / kBlock); }
-inline int row_grid(int rows) { return std::max(1, std::min(2048, (rows + kRowsPerBlock - 1) / kRowsPerBlock)); }
+// a multiple of the XCD count (see B_ROW_LOOP)
+inline int row_grid(int rows) {
+    return xcd_grid(std::min(2048, (rows + kRowsPerBlock - 1) / kRowsPerBlock));
+}
 
 }  // namespace
 
@@ -743,6 +781,7 @@ struct tm_batch {
     bool exch = false, dmi = false;
     DevBuf<double> m, dmdt, heff, u1, u2, rhsN, rhsD, U, Gm, hext, torque;
     DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product of the mixed form
+    DevBuf<double> rk_k[4], rk_base;   // RK4 stages and the step's start state
     BatchSolver<double> sN64, sD64;
     BatchSolver<float> sN32, sD32;
     bool timers = false;
@@ -781,7 +820,8 @@ extern "C" {
 
 int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     if (!c || !out || nsamples < 1 || nsamples > kMaxS) return TM_ERR_BAD_ARG;
-    if (precision != TM_PREC_FP64 && precision != TM_PREC_MIXED) return TM_ERR_BAD_ARG;
+    if (precision != TM_PREC_FP64 && precision != TM_PREC_MIXED && precision != TM_PREC_MIXED_BEM32)
+        return TM_ERR_BAD_ARG;
     tm_batch* bt = new tm_batch();
     int st = ctx_view(c, &bt->cv);
     if (st != TM_OK) {
@@ -843,7 +883,7 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     a |= bt->u2.alloc(NS);
     a |= bt->rhsN.alloc(NS);
     a |= bt->rhsD.alloc(NS);
-    if (precision == TM_PREC_MIXED) {
+    if (precision == TM_PREC_MIXED_BEM32) {
         a |= bt->Uf.alloc((size_t)v.bnx * nsamples);
         a |= bt->Gf.alloc((size_t)v.bnx * nsamples);
         a |= bt->Bf.alloc((size_t)v.bld * v.bnx);
@@ -854,7 +894,7 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     a |= bt->hext.alloc((size_t)3 * nsamples);
     a |= bt->torque.alloc(nsamples);
     if (a != TM_OK) return fail(TM_ERR_HIP);
-    if (precision == TM_PREC_MIXED) {
+    if (precision == TM_PREC_MIXED_BEM32) {
         const size_t tot = (size_t)v.bld * v.bnx;
         hipLaunchKernelGGL(k_b_to_float, dim3(elem_grid(tot)), dim3(kBlock), 0, bt->stream, tot, v.B, bt->Bf.p);
         if (hipGetLastError() != hipSuccess) return fail(TM_ERR_HIP);
@@ -955,8 +995,11 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     // boundary integral (DemagField.cpp:184-201) as a GEMM: G = B U
     const int b = v.bnx;
     // column-major G (b x S) = B (b x b, streamed in its storage order) * U^T,
-    // U = gathered boundary potentials (S x b); FP32 in the mixed form
-    if (bt->prec == TM_PREC_MIXED) {
+    // U = gathered boundary potentials (S x b).  FP64, except for
+    // TM_PREC_MIXED_BEM32 (an FP32 copy of B and FP32 accumulation: ~1e-5
+    // relative field error at 41k boundary nodes, from cancellation in the
+    // long row sums)
+    if (bt->prec == TM_PREC_MIXED_BEM32) {
         hipLaunchKernelGGL(k_b_gather<float>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
                            (const double*)bt->u1.p, bt->Uf.p);
         const float one = 1.0f, zero = 0.0f;
@@ -1039,6 +1082,37 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
         TM_HIP(hipStreamSynchronize(s));
     }
     return stN != TM_OK ? stN : stD;
+}
+
+int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int variant, tm_batch_stats* last) {
+    if (!bt || nsteps < 0 || !(dtau > 0.0)) return TM_ERR_BAD_ARG;
+    const size_t NS = (size_t)bt->n * bt->S, N3 = 3 * NS;
+    for (auto& k : bt->rk_k) TM_CHECK(k.alloc(N3));
+    TM_CHECK(bt->rk_base.alloc(N3));
+    hipStream_t s = bt->stream;
+    const dim3 g(elem_grid(N3));
+    const double cst[4] = {0.0, 0.5 * dtau, 0.5 * dtau, dtau};
+    int status = TM_OK;
+    for (int step = 0; step < nsteps; ++step) {
+        TM_HIP(hipMemcpyAsync(bt->rk_base.p, bt->m.p, N3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+        for (int st = 0; st < 4; ++st) {
+            // the batch magnetisation holds the stage state
+            if (st > 0)
+                hipLaunchKernelGGL(k_b_axpy, g, dim3(kBlock), 0, s, N3, (const double*)bt->rk_base.p,
+                                   (const double*)bt->rk_k[st - 1].p, cst[st], bt->m.p);
+            const bool fin = step == nsteps - 1 && st == 3;
+            const int r = tm_batch_eval_rhs(bt, nullptr, TM_DEVICE, alpha, variant, bt->rk_k[st].p, TM_DEVICE,
+                                            nullptr, TM_DEVICE, fin ? last : nullptr);
+            if (r != TM_OK && r != TM_ERR_MAXITER) return r;
+            if (r == TM_ERR_MAXITER) status = r;
+        }
+        hipLaunchKernelGGL(k_b_rk_final, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS,
+                           (const double*)bt->rk_base.p, (const double*)bt->rk_k[0].p, (const double*)bt->rk_k[1].p,
+                           (const double*)bt->rk_k[2].p, (const double*)bt->rk_k[3].p, dtau, bt->m.p);
+        TM_HIP(hipGetLastError());
+    }
+    TM_HIP(hipStreamSynchronize(s));
+    return status;
 }
 
 int tm_batch_step_euler(tm_batch* bt, const double* dm, int where, double dtau) {

@@ -30,11 +30,13 @@ namespace tmh {
 namespace {
 
 // Slices handled by this block's wavefronts: s = wave_global, += total waves.
-#define CG_SLICE_LOOP(A)                                                      \
-    const int lane = threadIdx.x & (kWave - 1);                               \
-    const int nsl = ((A).n + kWave - 1) / kWave;                              \
-    for (int s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < nsl; \
-         s += gridDim.x * (kBlock / kWave))
+#define CG_SLICE_LOOP(A)                                                                             \
+    const int lane = threadIdx.x & (kWave - 1);                                                       \
+    const int nsl = ((A).n + kWave - 1) / kWave;                                                      \
+    const int spb_ = ((nsl + (int)gridDim.x - 1) / (int)gridDim.x + kBlock / kWave - 1) / (kBlock / kWave) * \
+                     (kBlock / kWave);                                                                \
+    const int s0_ = xcd_block() * spb_, s1_ = min(nsl, s0_ + spb_);                                   \
+    for (int s = s0_ + (int)threadIdx.x / kWave; s < s1_; s += kBlock / kWave)
 
 __device__ inline void block_partials(double* part, int stride, double v0, double v1, int nv) {
     __shared__ double sp[2][kBlock / kWave];
@@ -337,7 +339,7 @@ int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* 
     TM_CHECK(q.alloc(n));
     TM_CHECK(p0.alloc(n));
     TM_CHECK(p1.alloc(n));
-    grid = std::max(1, std::min(kMaxCgBlocks, (pat.nslices + kBlock / kWave - 1) / (kBlock / kWave)));
+    grid = xcd_grid(std::min(kMaxCgBlocks, (pat.nslices + kBlock / kWave - 1) / (kBlock / kWave)));
     TM_CHECK(part.alloc((size_t)4 * grid));
     TM_CHECK(st.alloc(1));
     TM_HIP(hipMemsetAsync(st.p, 0, sizeof(CgState), s));

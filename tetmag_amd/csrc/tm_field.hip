@@ -98,7 +98,7 @@ namespace {
 // kernels
 // --------------------------------------------------------------------------
 #define SELL_ROW_PROLOGUE                                          \
-    const int row = blockIdx.x * kBlock + threadIdx.x;             \
+    const int row = xcd_block() * kBlock + threadIdx.x;             \
     const int s = row / kWave, lane = row & (kWave - 1);           \
     if (s >= (A.n + kWave - 1) / kWave) return;                    \
     const int64_t base = A.off[s] + lane;                          \
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a) {
     __shared__ double smem[kBlock / kWave];
     const SellView A = a.A;
     const int n = A.n;
-    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int row = xcd_block() * kBlock + threadIdx.x;
     const int s = row / kWave, lane = row & (kWave - 1);
     double t = 0.0;
     if (s < (n + kWave - 1) / kWave) {
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(EvalArgs a, const double* __r
     __shared__ double smem[kEnergyTerms * (kBlock / kWave)];
     const SellView A = a.A;
     const int n = A.n;
-    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int row = xcd_block() * kBlock + threadIdx.x;
     const int s = row / kWave, lane = row & (kWave - 1);
     double e[kEnergyTerms];
 #pragma unroll
@@ -648,7 +648,7 @@ int energy_dispatch(const EvalArgs& a, bool e, bool d, bool m, const double* vol
 }  // namespace
 
 namespace tmh {
-int sell_grid(const tm_ctx* c) { return grid_for((int64_t)c->pat.nslices * kWave); }
+int sell_grid(const tm_ctx* c) { return xcd_grid(grid_for((int64_t)c->pat.nslices * kWave)); }
 
 // Stage a caller pointer as a device pointer (copying host data into `tmp`).
 int stage_in(tm_ctx* c, const double* src, int where, size_t count, DevBuf<double>& tmp, const double** out) {

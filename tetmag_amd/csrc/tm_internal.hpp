@@ -1,6 +1,8 @@
 // tm_internal.hpp -- shared internals of the HIP library (not part of the ABI).
 #pragma once
 
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -205,6 +207,20 @@ __device__ inline void atomic_max_nonneg(double* addr, double v) {
 }
 
 inline int grid_for(int64_t n, int block = kBlock) { return (int)((n + block - 1) / block); }
+
+// XCD-aware block order.  The 8 XCDs of the chip each have their own L2, and
+// consecutive blocks are dispatched to the XCDs in turn.  Kernels whose rows
+// gather from shared vectors map physical block b to the logical block
+// (b % 8) * (G / 8) + b / 8, so each XCD walks one contiguous eighth of the
+// rows and neighbouring rows (which share columns) hit the same L2.  The
+// grid must be a multiple of 8 (xcd_grid); otherwise the order is unchanged.
+constexpr int kXcd = 8;
+inline int xcd_grid(int g) { return (std::max(g, 1) + kXcd - 1) / kXcd * kXcd; }
+__device__ inline int xcd_block() {
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G % kXcd) return b;
+    return (b % kXcd) * (G / kXcd) + b / kXcd;
+}
 
 // Copies between a caller pointer (host or device) and a device buffer.
 int copy_in(double* dst_dev, const double* src, int where, size_t count, hipStream_t s);

@@ -103,7 +103,7 @@ int copy_out(double* dst, int where, const double* src_dev, size_t count, hipStr
 // y = A x, one row per lane.
 __global__ __launch_bounds__(kBlock) void k_sell_spmv(SellView A, const double* __restrict__ val,
                                                       const double* __restrict__ x, double* __restrict__ y) {
-    const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int row = xcd_block() * kBlock + threadIdx.x;
     const int s = row / kWave, lane = row & (kWave - 1);
     if (s >= (A.n + kWave - 1) / kWave) return;
     const int64_t base = A.off[s] + lane;
@@ -118,7 +118,8 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv(SellView A, const double* 
 
 int spmv(const SellPattern& P, const double* val, const double* x, double* y, hipStream_t s) {
     if (P.n == 0) return TM_OK;
-    hipLaunchKernelGGL(k_sell_spmv, dim3(grid_for((int64_t)P.nslices * kWave)), dim3(kBlock), 0, s, view(P), val, x, y);
+    hipLaunchKernelGGL(k_sell_spmv, dim3(xcd_grid(grid_for((int64_t)P.nslices * kWave))), dim3(kBlock), 0, s, view(P),
+                       val, x, y);
     TM_HIP(hipGetLastError());
     return TM_OK;
 }
