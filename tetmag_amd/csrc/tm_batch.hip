@@ -81,6 +81,7 @@ struct OState {
 #define B_ROW_LOOP(NROWS)                                                                              \
     const int lane = threadIdx.x & (kWave - 1);                                                        \
     const bool on = lane < S;                                                                          \
+    const int lc = min(lane, S - 1); /* gather lane: in bounds for every lane, so loads need no mask */ \
     const int rpb_ = (((NROWS) + (int)gridDim.x - 1) / (int)gridDim.x + kRowsPerBlock - 1) / kRowsPerBlock * \
                      kRowsPerBlock;                                                                    \
     const int r0_ = xcd_block() * rpb_;                                                                \
@@ -100,21 +101,31 @@ __device__ inline void chunk_cols(const int* __restrict__ ci, int q, int q1, int
     for (int k = 0; k < CH; ++k) c[k] = ci[q + k < q1 ? q + k : q];
 }
 
-// acc = sum_q v[q] x[ci[q]][lane] over the row (x zero-extended for lanes >= S)
+// The chunk's matrix values, loaded together with the columns (past the row
+// end: the row's first value, masked out by the caller's select).
+template <int CH, typename V>
+__device__ inline void chunk_vals(const V* __restrict__ v, int q, int q1, double (&w)[CH]) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) w[k] = (double)v[q + k < q1 ? q + k : q];
+}
+
+// acc = sum_q v[q] x[ci[q]][lc] over the row (lanes >= S read lane S-1's
+// column and their results are discarded)
 template <typename T, typename V>
-__device__ inline T row_dot(const CsrArgs& A, const V* __restrict__ v, int row, int S, int lane, bool on,
+__device__ inline T row_dot(const CsrArgs& A, const V* __restrict__ v, int row, int S, int lc,
                             const T* __restrict__ x) {
     const int q0 = A.rp[row], q1 = A.rp[row + 1];
     T acc = 0;
     for (int q = q0; q < q1; q += kCh) {
         int c[kCh];
+        double w[kCh];
         chunk_cols<kCh>(A.ci, q, q1, c);
+        chunk_vals<kCh>(v, q, q1, w);
         T xv[kCh];
 #pragma unroll
-        for (int k = 0; k < kCh; ++k) xv[k] = on ? x[(size_t)c[k] * S + lane] : (T)0;
+        for (int k = 0; k < kCh; ++k) xv[k] = x[(size_t)c[k] * S + lc];
 #pragma unroll
-        for (int k = 0; k < kCh; ++k)
-            if (q + k < q1) acc += (T)v[q + k] * xv[k];
+        for (int k = 0; k < kCh; ++k) acc = q + k < q1 ? acc + (T)w[k] * xv[k] : acc;
     }
     return acc;
 }
@@ -163,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_b_resid(CsrArgs A, int S, const doub
                                                     T* __restrict__ rT, double* part_bb, double* part_rr) {
     double vb = 0.0, vr = 0.0;
     B_ROW_LOOP(A.n) {
-        const double acc = row_dot<double>(A, A.v, row, S, lane, on, x);
+        const double acc = row_dot<double>(A, A.v, row, S, lc, x);
         if (on) {
             const size_t e = (size_t)row * S + lane;
             const double bi = b[e], ri = bi - acc;
@@ -228,16 +239,17 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_spmv(CsrArgs A, int S, const T*
         const int q0 = A.rp[row], q1 = A.rp[row + 1];
         for (int qq = q0; qq < q1; qq += kCh) {
             int c[kCh];
+            double w[kCh];
             chunk_cols<kCh>(A.ci, qq, q1, c);
+            chunk_vals<kCh>(A.v, qq, q1, w);
             T pc[kCh];
 #pragma unroll
             for (int j = 0; j < kCh; ++j) {
-                const size_t e = (size_t)c[j] * S + lane;
-                pc[j] = on ? (k > 0 ? z[e] + beta * pold[e] : z[e]) : (T)0;
+                const size_t e = (size_t)c[j] * S + lc;
+                pc[j] = k > 0 ? z[e] + beta * pold[e] : z[e];
             }
 #pragma unroll
-            for (int j = 0; j < kCh; ++j)
-                if (qq + j < q1) acc += (T)A.v[qq + j] * pc[j];
+            for (int j = 0; j < kCh; ++j) acc = qq + j < q1 ? acc + (T)w[j] * pc[j] : acc;
         }
         if (on) {
             const size_t e = (size_t)row * S + lane;
@@ -308,13 +320,17 @@ __global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs A, int S, const double
         const int q0 = A.rp[row], q1 = A.rp[row + 1];
         for (int q = q0; q < q1; q += kCh) {
             int c[kCh];
+            double wv[kCh];
             chunk_cols<kCh>(A.ci, q, q1, c);
+            chunk_vals<kCh>(A.v, q, q1, wv);
             T bv[kCh];
 #pragma unroll
-            for (int k = 0; k < kCh; ++k) bv[k] = on ? b[(size_t)c[k] * S + lane] : (T)0;
+            for (int k = 0; k < kCh; ++k) bv[k] = b[(size_t)c[k] * S + lc];
+            T dv[kCh];
 #pragma unroll
-            for (int k = 0; k < kCh; ++k)
-                if (q + k < q1) acc += (T)A.v[q + k] * ((T)(w * dinv[c[k]]) * bv[k]);
+            for (int k = 0; k < kCh; ++k) dv[k] = (T)(w * dinv[c[k]]);
+#pragma unroll
+            for (int k = 0; k < kCh; ++k) acc = q + k < q1 ? acc + (T)wv[k] * (dv[k] * bv[k]) : acc;
         }
         if (on) {
             const size_t e = (size_t)row * S + lane;
@@ -329,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs A, int S, const double
 template <typename T, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_b_spmm(CsrArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
     B_ROW_LOOP(M.n) {
-        const T acc = row_dot<T>(M, M.v, row, S, lane, on, x);
+        const T acc = row_dot<T>(M, M.v, row, S, lc, x);
         if (on) {
             const size_t e = (size_t)row * S + lane;
             if (ACC) y[e] += acc;
@@ -345,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void k_b_post(CsrArgs A, int S, const doubl
                                                    T* __restrict__ xo, double* part) {
     double v = 0.0;
     B_ROW_LOOP(A.n) {
-        const T acc = row_dot<T>(A, A.v, row, S, lane, on, x);
+        const T acc = row_dot<T>(A, A.v, row, S, lc, x);
         if (on) {
             const size_t e = (size_t)row * S + lane;
             const T bi = b[e];
@@ -384,10 +400,10 @@ __global__ __launch_bounds__(kBlock) void k_b_div(CsrArgs tx, const double* __re
             double mv[4][3];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const size_t e = (size_t)c[k] * S + lane;
-                mv[k][0] = on ? m[e] : 0.0;
-                mv[k][1] = on ? m[NS + e] : 0.0;
-                mv[k][2] = on ? m[2 * NS + e] : 0.0;
+                const size_t e = (size_t)c[k] * S + lc;
+                mv[k][0] = m[e];
+                mv[k][1] = m[NS + e];
+                mv[k][2] = m[2 * NS + e];
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k)
@@ -434,11 +450,11 @@ __global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
             double mv[4][3], uv[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const size_t e = (size_t)c[k] * S + lane;
-                mv[k][0] = on ? a.m[e] : 0.0;
-                mv[k][1] = on ? a.m[NS + e] : 0.0;
-                mv[k][2] = on ? a.m[2 * NS + e] : 0.0;
-                uv[k] = on ? a.u1[e] + a.u2[e] : 0.0;
+                const size_t e = (size_t)c[k] * S + lc;
+                mv[k][0] = a.m[e];
+                mv[k][1] = a.m[NS + e];
+                mv[k][2] = a.m[2 * NS + e];
+                uv[k] = a.u1[e] + a.u2[e];
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
