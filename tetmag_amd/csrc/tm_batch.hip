@@ -73,21 +73,110 @@ struct OState {
     int done[kMaxS];
 };
 
-// Rows are split into gridDim.x contiguous ranges.  Consecutive blocks are
-// dispatched to the 8 XCDs in turn, so block b takes range
-// (b % 8) * (G / 8) + b / 8 (G a multiple of 8): each XCD walks one contiguous
-// eighth of the rows, and the gathers of neighbouring rows (which share
-// columns) hit that XCD's L2 instead of every L2 fetching every vector.
+// Wavefront w of block b walks rows b*4 + w, then + 4*G, ...  (A contiguous
+// range per XCD was measured slower for these kernels.)
 #define B_ROW_LOOP(NROWS)                                                                              \
     const int lane = threadIdx.x & (kWave - 1);                                                        \
     const bool on = lane < S;                                                                          \
     const int lc = min(lane, S - 1); /* gather lane: in bounds for every lane, so loads need no mask */ \
-    const int rpb_ = (((NROWS) + (int)gridDim.x - 1) / (int)gridDim.x + kRowsPerBlock - 1) / kRowsPerBlock * \
-                     kRowsPerBlock;                                                                    \
-    const int r0_ = xcd_block() * rpb_;                                                                \
-    const int r1_ = min((NROWS), r0_ + rpb_);                                                          \
-    for (int row = __builtin_amdgcn_readfirstlane(r0_ + (int)(threadIdx.x / kWave)); row < r1_;        \
-         row += kRowsPerBlock)
+    for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
+         row < (NROWS); row += gridDim.x * kRowsPerBlock)
+
+// ---- pipelined row walk for the solver kernels ----------------------------
+// A row's entries sit one per lane (lane k = entry k, one vector load for
+// indices and one for values); gather addresses are read from those lanes
+// with readlane (wave-uniform), so a chunk of kGr gathers issues back to back.
+// The next row's pointers and entries are fetched before the current row's
+// gathers, so a wavefront has both in flight.  Rows longer than 64 entries
+// are walked in 64-entry segments.
+constexpr int kGr = 16;
+
+struct RowEnt {
+    int q0, q1;  // row extent (wave-uniform)
+    int c;       // column of entry q0 + lane (lanes past the end: 0)
+    double v;    // value of entry q0 + lane (lanes past the end: 0)
+};
+
+__device__ inline double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// acc = sum_k v_k xf(c_k) over one segment held in e (len <= 64 entries),
+// in entry order.
+template <typename T, typename XF>
+__device__ inline T seg_sum(const RowEnt& e, int len, XF&& xf) {
+    T acc = 0;
+    for (int k0 = 0; k0 < len; k0 += kGr) {
+        T xv[kGr];
+        double w[kGr];
+#pragma unroll
+        for (int k = 0; k < kGr; ++k) {
+            const int kk = min(k0 + k, len - 1);
+            const int c = __builtin_amdgcn_readlane(e.c, kk);
+            w[k] = readlane_f64(e.v, kk);
+            xv[k] = xf(c);
+        }
+#pragma unroll
+        for (int k = 0; k < kGr; ++k) acc = k0 + k < len ? acc + (T)w[k] * xv[k] : acc;
+    }
+    return acc;
+}
+
+template <typename T, typename V, typename XF>
+__device__ inline T row_sum(const CsrArgs& A, const V* __restrict__ vals, const RowEnt& e, int lane, XF&& xf) {
+    const int len = e.q1 - e.q0;
+    if (len <= kWave) return seg_sum<T>(e, len, xf);
+    T acc = 0;
+    for (int q = e.q0; q < e.q1; q += kWave) {  // long rows: 64-entry segments
+        RowEnt s;
+        s.q0 = q;
+        s.q1 = min(e.q1, q + kWave);
+        const bool in = q + lane < s.q1;
+        s.c = in ? A.ci[q + lane] : 0;
+        s.v = in ? (double)vals[q + lane] : 0.0;
+        acc += seg_sum<T>(s, s.q1 - s.q0, xf);
+    }
+    return acc;
+}
+
+template <typename V>
+__device__ inline void row_entries(const CsrArgs& A, const V* __restrict__ vals, int q0, int q1, int lane,
+                                   RowEnt& e) {
+    e.q0 = q0;
+    e.q1 = q1;
+    const int q = q0 + lane;
+    const bool in = q < q1;
+    e.c = in ? A.ci[q] : 0;
+    e.v = in ? (double)vals[q] : 0.0;
+}
+
+// Row loop with look-ahead: while row i is gathered, the entries of row i+1
+// (whose pointers arrived during row i-1) and the pointers of row i+2 are in
+// flight.  body(row, e) handles one row.
+#define B_ROW_PIPE(A, VALS, NROWS, BODY)                                                                \
+    {                                                                                                   \
+        const int stride_ = gridDim.x * kRowsPerBlock;                                                  \
+        int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
+        RowEnt cur_, nxt_;                                                                              \
+        int p0_ = 0, p1_ = 0;                                                                           \
+        if (row < (NROWS)) row_entries((A), (VALS), (A).rp[row], (A).rp[row + 1], lane, cur_);          \
+        if (row + stride_ < (NROWS)) {                                                                  \
+            p0_ = (A).rp[row + stride_];                                                                \
+            p1_ = (A).rp[row + stride_ + 1];                                                            \
+        }                                                                                               \
+        for (; row < (NROWS); row += stride_) {                                                         \
+            if (row + stride_ < (NROWS)) row_entries((A), (VALS), p0_, p1_, lane, nxt_);                \
+            if (row + 2 * stride_ < (NROWS)) {                                                          \
+                p0_ = (A).rp[row + 2 * stride_];                                                        \
+                p1_ = (A).rp[row + 2 * stride_ + 1];                                                    \
+            }                                                                                           \
+            BODY(row, cur_);                                                                            \
+            cur_ = nxt_;                                                                                \
+        }                                                                                               \
+    }
 
 // Row entries are walked in chunks of kCh: the chunk's column indices first,
 // then kCh independent gathers, then the multiply-adds in the original entry
@@ -172,18 +261,23 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_resid(CsrArgs A, int S, const double* __restrict__ b,
                                                     const double* __restrict__ x, double* __restrict__ r64,
                                                     T* __restrict__ rT, double* part_bb, double* part_rr) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int lc = min(lane, S - 1);
     double vb = 0.0, vr = 0.0;
-    B_ROW_LOOP(A.n) {
-        const double acc = row_dot<double>(A, A.v, row, S, lc, x);
+    auto body = [&](int row, const RowEnt& e) {
+        const double bi = b[(size_t)row * S + lc];
+        const double acc = row_sum<double>(A, A.v, e, lane, [&](int c) { return x[(size_t)c * S + lc]; });
         if (on) {
-            const size_t e = (size_t)row * S + lane;
-            const double bi = b[e], ri = bi - acc;
-            r64[e] = ri;
-            rT[e] = (T)ri;
+            const size_t ee = (size_t)row * S + lane;
+            const double ri = bi - acc;
+            r64[ee] = ri;
+            rT[ee] = (T)ri;
             vb += bi * bi;
             vr += ri * ri;
         }
-    }
+    };
+    B_ROW_PIPE(A, A.v, A.n, body);
     b_partial(part_bb, S, vb);
     b_partial(part_rr, S, vr);
 }
@@ -231,34 +325,28 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_cg_spmv(CsrArgs A, int S, const T* __restrict__ z,
                                                       const T* __restrict__ pold, T* __restrict__ pnew,
                                                       T* __restrict__ q, int k, const BState* bs, double* part) {
-    const int ln = threadIdx.x & (kWave - 1);
-    const T beta = (k > 0 && ln < S) ? (T)bs->beta[ln] : (T)0;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int lc = min(lane, S - 1);
+    const T beta = (k > 0 && on) ? (T)bs->beta[lane] : (T)0;
+    const T betac = k > 0 ? (T)bs->beta[lc] : (T)0;  // the gather lane's beta
     double v = 0.0;
-    B_ROW_LOOP(A.n) {
-        T acc = 0;
-        const int q0 = A.rp[row], q1 = A.rp[row + 1];
-        for (int qq = q0; qq < q1; qq += kCh) {
-            int c[kCh];
-            double w[kCh];
-            chunk_cols<kCh>(A.ci, qq, q1, c);
-            chunk_vals<kCh>(A.v, qq, q1, w);
-            T pc[kCh];
-#pragma unroll
-            for (int j = 0; j < kCh; ++j) {
-                const size_t e = (size_t)c[j] * S + lc;
-                pc[j] = k > 0 ? z[e] + beta * pold[e] : z[e];
-            }
-#pragma unroll
-            for (int j = 0; j < kCh; ++j) acc = qq + j < q1 ? acc + (T)w[j] * pc[j] : acc;
-        }
+    auto body = [&](int row, const RowEnt& e) {
+        const size_t er = (size_t)row * S + lc;
+        const T pi = k > 0 ? z[er] + betac * pold[er] : z[er];
+        const T acc = row_sum<T>(A, A.v, e, lane, [&](int c) {
+            const size_t ec = (size_t)c * S + lc;
+            return k > 0 ? z[ec] + betac * pold[ec] : z[ec];
+        });
         if (on) {
-            const size_t e = (size_t)row * S + lane;
-            const T pi = k > 0 ? z[e] + beta * pold[e] : z[e];
-            pnew[e] = pi;
-            q[e] = acc;
+            const size_t ee = (size_t)row * S + lane;
+            pnew[ee] = pi;
+            q[ee] = acc;
             v += (double)pi * (double)acc;
         }
-    }
+    };
+    B_ROW_PIPE(A, A.v, A.n, body);
+    (void)beta;
     b_partial(part, S, v);
 }
 
@@ -315,43 +403,34 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs A, int S, const double* __restrict__ dinv,
                                                   const T* __restrict__ b, double w, T* __restrict__ x,
                                                   T* __restrict__ r) {
-    B_ROW_LOOP(A.n) {
-        T acc = 0;
-        const int q0 = A.rp[row], q1 = A.rp[row + 1];
-        for (int q = q0; q < q1; q += kCh) {
-            int c[kCh];
-            double wv[kCh];
-            chunk_cols<kCh>(A.ci, q, q1, c);
-            chunk_vals<kCh>(A.v, q, q1, wv);
-            T bv[kCh];
-#pragma unroll
-            for (int k = 0; k < kCh; ++k) bv[k] = b[(size_t)c[k] * S + lc];
-            T dv[kCh];
-#pragma unroll
-            for (int k = 0; k < kCh; ++k) dv[k] = (T)(w * dinv[c[k]]);
-#pragma unroll
-            for (int k = 0; k < kCh; ++k) acc = q + k < q1 ? acc + (T)wv[k] * (dv[k] * bv[k]) : acc;
-        }
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int lc = min(lane, S - 1);
+    auto body = [&](int row, const RowEnt& e) {
+        const T bi = b[(size_t)row * S + lc];
+        const T acc = row_sum<T>(A, A.v, e, lane,
+                                 [&](int c) { return (T)(w * dinv[c]) * b[(size_t)c * S + lc]; });
         if (on) {
-            const size_t e = (size_t)row * S + lane;
-            const T bi = b[e];
-            x[e] = (T)(w * dinv[row]) * bi;
-            r[e] = bi - acc;
+            const size_t ee = (size_t)row * S + lane;
+            x[ee] = (T)(w * dinv[row]) * bi;
+            r[ee] = bi - acc;
         }
-    }
+    };
+    B_ROW_PIPE(A, A.v, A.n, body);
 }
 
 // y = M x (ACC: y += M x)
 template <typename T, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_b_spmm(CsrArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
-    B_ROW_LOOP(M.n) {
-        const T acc = row_dot<T>(M, M.v, row, S, lc, x);
-        if (on) {
-            const size_t e = (size_t)row * S + lane;
-            if (ACC) y[e] += acc;
-            else y[e] = acc;
-        }
-    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int lc = min(lane, S - 1);
+    auto body = [&](int row, const RowEnt& e) {
+        const T yi = ACC ? y[(size_t)row * S + lc] : (T)0;
+        const T acc = row_sum<T>(M, M.v, e, lane, [&](int c) { return x[(size_t)c * S + lc]; });
+        if (on) y[(size_t)row * S + lane] = ACC ? yi + acc : acc;
+    };
+    B_ROW_PIPE(M, M.v, M.n, body);
 }
 
 // xo = x + w dinv (b - A x); optional partial (b, xo)
@@ -359,17 +438,21 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_post(CsrArgs A, int S, const double* __restrict__ dinv,
                                                    const T* __restrict__ b, double w, const T* __restrict__ x,
                                                    T* __restrict__ xo, double* part) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int lc = min(lane, S - 1);
     double v = 0.0;
-    B_ROW_LOOP(A.n) {
-        const T acc = row_dot<T>(A, A.v, row, S, lc, x);
+    auto body = [&](int row, const RowEnt& e) {
+        const size_t er = (size_t)row * S + lc;
+        const T bi = b[er], xi = x[er];
+        const T acc = row_sum<T>(A, A.v, e, lane, [&](int c) { return x[(size_t)c * S + lc]; });
         if (on) {
-            const size_t e = (size_t)row * S + lane;
-            const T bi = b[e];
-            const T zi = x[e] + (T)(w * dinv[row]) * (bi - acc);
-            xo[e] = zi;
+            const T zi = xi + (T)(w * dinv[row]) * (bi - acc);
+            xo[(size_t)row * S + lane] = zi;
             v += (double)bi * (double)zi;
         }
-    }
+    };
+    B_ROW_PIPE(A, A.v, A.n, body);
     if (part) b_partial(part, S, v);
 }
 
