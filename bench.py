@@ -107,7 +107,8 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
     name = {N.TM_PREC_FP64: "fp64", N.TM_PREC_MIXED: "mixed", N.TM_PREC_MIXED_BEM32: "mixed_bem32"}[precision]
     bc = BatchContext(ctx, nsamples, precision)
     mb = to_batch(fields[:nsamples])
-    out = {"form": name, "samples": nsamples}
+    out = {"form": name, "samples": nsamples, "gemm_tuning": bc.gemm_info()}
+    log(f"[{name}] boundary GEMM tuning {out['gemm_tuning']}")
     if check:
         dm, H, st = bc.eval_rhs_host(mb)
         u1, u2 = bc.potentials()

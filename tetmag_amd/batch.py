@@ -86,6 +86,13 @@ class BatchContext:
                 "tm_batch_run_rk4")
         return st.as_dict()
 
+    def gemm_info(self) -> dict:
+        sol, cand = C.c_int(), C.c_int()
+        d, b = C.c_double(), C.c_double()
+        N.check(N.lib().tm_batch_gemm_info(self.h, C.byref(sol), C.byref(cand), C.byref(d), C.byref(b)),
+                "tm_batch_gemm_info")
+        return {"solution": sol.value, "candidates": cand.value, "ms_default": d.value, "ms_tuned": b.value}
+
     def enable_timers(self, on: bool = True):
         N.check(N.lib().tm_batch_enable_timers(self.h, int(on)), "tm_batch_enable_timers")
 

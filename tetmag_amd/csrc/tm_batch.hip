@@ -17,6 +17,7 @@
 // batched form); T = float runs the inner iterations in FP32 to a relative
 // 1e-5 and recovers FP64 accuracy by iterative refinement (the mixed form).
 // Converged samples are frozen (their step lengths are zero).
+#define ROCBLAS_BETA_FEATURES_API  // rocblas_gemm_ex_get_solutions (the GEMM autotuner)
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -321,32 +322,34 @@ __global__ __launch_bounds__(kBlock) void k_b_zero(size_t tot, T* __restrict__ x
 
 // ------------------------------------------------------------- inner CG --
 // p = z + beta p_old (p_old not read for k = 0); q = A p; partial (p, q).
+// p = z + beta p_old (p_old not read for k = 0)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(CsrArgs A, int S, const T* __restrict__ z,
-                                                      const T* __restrict__ pold, T* __restrict__ pnew,
-                                                      T* __restrict__ q, int k, const BState* bs, double* part) {
+__global__ __launch_bounds__(kBlock) void k_b_cg_dir(size_t NS, int S, const T* __restrict__ z,
+                                                     const T* __restrict__ pold, T* __restrict__ pnew, int k,
+                                                     const BState* bs) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock) {
+        const int sm = (int)(e % S);
+        pnew[e] = k > 0 ? z[e] + (T)bs->beta[sm] * pold[e] : z[e];
+    }
+}
+
+// q = A p; partial (p, q)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(CsrArgs A, int S, const T* __restrict__ p, T* __restrict__ q,
+                                                      double* part) {
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
-    const T beta = (k > 0 && on) ? (T)bs->beta[lane] : (T)0;
-    const T betac = k > 0 ? (T)bs->beta[lc] : (T)0;  // the gather lane's beta
     double v = 0.0;
     auto body = [&](int row, const RowEnt& e) {
-        const size_t er = (size_t)row * S + lc;
-        const T pi = k > 0 ? z[er] + betac * pold[er] : z[er];
-        const T acc = row_sum<T>(A, A.v, e, lane, [&](int c) {
-            const size_t ec = (size_t)c * S + lc;
-            return k > 0 ? z[ec] + betac * pold[ec] : z[ec];
-        });
+        const T pi = p[(size_t)row * S + lc];
+        const T acc = row_sum<T>(A, A.v, e, lane, [&](int c) { return p[(size_t)c * S + lc]; });
         if (on) {
-            const size_t ee = (size_t)row * S + lane;
-            pnew[ee] = pi;
-            q[ee] = acc;
+            q[(size_t)row * S + lane] = acc;
             v += (double)pi * (double)acc;
         }
     };
     B_ROW_PIPE(A, A.v, A.n, body);
-    (void)beta;
     b_partial(part, S, v);
 }
 
@@ -831,8 +834,10 @@ struct BatchSolver {
             T* po = p1.p;
             for (;;) {
                 for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
-                    hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, (const T*)z.p,
-                                       (const T*)po, pc, q.p, enq, (const BState*)bst.p, ppq);
+                    hipLaunchKernelGGL((k_b_cg_dir<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S,
+                                       (const T*)z.p, (const T*)po, pc, enq, (const BState*)bst.p);
+                    hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, (const T*)pc, q.p,
+                                       ppq);
                     hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, ppq, bst.p);
                     hipLaunchKernelGGL((k_b_cg_update<T>), dim3(G), dim3(kBlock), 0, s, n, S, (const T*)pc,
                                        (const T*)q.p, d.p, rT.p, (const BState*)bst.p, prr);
@@ -881,6 +886,9 @@ struct tm_batch {
     DevBuf<double> m, dmdt, heff, u1, u2, rhsN, rhsD, U, Gm, hext, torque;
     DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product of the mixed form
     DevBuf<double> rk_k[4], rk_base;   // RK4 stages and the step's start state
+    int gemm_sol = 0;                  // tuned rocBLAS solution index (0: library default)
+    int gemm_candidates = 0;
+    float gemm_ms_default = 0.0f, gemm_ms_best = 0.0f;
     BatchSolver<double> sN64, sD64;
     BatchSolver<float> sN32, sD32;
     bool timers = false;
@@ -910,6 +918,92 @@ int upload_op(tm_batch* bt, int op, DevBuf<double>& dst) {
     TM_CHECK(csr_on_pattern(*bt->cv.pat, *bt->cv.hop_rp[op], *bt->cv.hop_col[op], *bt->cv.hop_val[op], v));
     TM_CHECK(dst.upload(v.data(), v.size(), bt->stream));
     TM_HIP(hipStreamSynchronize(bt->stream));
+    return TM_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// The boundary GEMM G (b x S) = B (b x b) * U^T (S x b) is a tall, thin
+// product; the library's default tile wastes half of a 128-wide tile on
+// S = 64.  At batch creation every solution rocBLAS offers for this exact
+// shape and layout is timed (HIP events, one warm-up and two timed runs) and
+// the fastest is kept.
+int gemm_run(tm_batch* bt, int sol) {
+    const CtxView& v = bt->cv;
+    const int b = v.bnx, S = bt->S;
+    const bool f32 = bt->prec == TM_PREC_MIXED_BEM32;
+    const rocblas_datatype t = f32 ? rocblas_datatype_f32_r : rocblas_datatype_f64_r;
+    const double one = 1.0, zero = 0.0;
+    const float onef = 1.0f, zerof = 0.0f;
+    const void* al = f32 ? (const void*)&onef : (const void*)&one;
+    const void* be = f32 ? (const void*)&zerof : (const void*)&zero;
+    const void* A = f32 ? (const void*)bt->Bf.p : (const void*)v.B;
+    const void* U = f32 ? (const void*)bt->Uf.p : (const void*)bt->U.p;
+    void* G = f32 ? (void*)bt->Gf.p : (void*)bt->Gm.p;
+    const rocblas_status r =
+        rocblas_gemm_ex(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, al, A, t, v.bld, U, t,
+                        S, be, G, t, b, G, t, b, t, sol > 0 ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard,
+                        sol, 0);
+    return r == rocblas_status_success ? TM_OK : TM_ERR_HIP;
+}
+
+int tune_gemm(tm_batch* bt) {
+    const CtxView& v = bt->cv;
+    const int b = v.bnx, S = bt->S;
+    const bool f32 = bt->prec == TM_PREC_MIXED_BEM32;
+    const rocblas_datatype t = f32 ? rocblas_datatype_f32_r : rocblas_datatype_f64_r;
+    const double one = 1.0, zero = 0.0;
+    const float onef = 1.0f, zerof = 0.0f;
+    const void* al = f32 ? (const void*)&onef : (const void*)&one;
+    const void* be = f32 ? (const void*)&zerof : (const void*)&zero;
+    const void* A = f32 ? (const void*)bt->Bf.p : (const void*)v.B;
+    void* U = f32 ? (void*)bt->Uf.p : (void*)bt->U.p;
+    void* G = f32 ? (void*)bt->Gf.p : (void*)bt->Gm.p;
+    TM_HIP(hipMemsetAsync(U, 0, (size_t)b * S * (f32 ? sizeof(float) : sizeof(double)), bt->stream));
+    rocblas_int nsol = 0;
+    if (rocblas_gemm_ex_get_solutions(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, al, A,
+                                      t, v.bld, U, t, S, be, G, t, b, G, t, b, t, rocblas_gemm_algo_solution_index, 0,
+                                      nullptr, &nsol) != rocblas_status_success ||
+        nsol <= 0)
+        return TM_OK;  // keep the default
+    std::vector<rocblas_int> sols((size_t)nsol);
+    if (rocblas_gemm_ex_get_solutions(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, al, A,
+                                      t, v.bld, U, t, S, be, G, t, b, G, t, b, t, rocblas_gemm_algo_solution_index, 0,
+                                      sols.data(), &nsol) != rocblas_status_success)
+        return TM_OK;
+    hipEvent_t e0, e1;
+    TM_HIP(hipEventCreate(&e0));
+    TM_HIP(hipEventCreate(&e1));
+    auto time_sol = [&](int sol, float* ms) -> bool {
+        if (gemm_run(bt, sol) != TM_OK) return false;
+        if (hipEventRecord(e0, bt->stream) != hipSuccess) return false;
+        for (int k = 0; k < 2; ++k)
+            if (gemm_run(bt, sol) != TM_OK) return false;
+        if (hipEventRecord(e1, bt->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return false;
+        float t2 = 0.0f;
+        (void)hipEventElapsedTime(&t2, e0, e1);
+        *ms = 0.5f * t2;
+        return true;
+    };
+    float best = 0.0f;
+    if (!time_sol(0, &best)) best = 1e30f;
+    bt->gemm_ms_default = best;
+    bt->gemm_sol = 0;
+    const int cap = std::min<int>((int)sols.size(), 64);
+    for (int k = 0; k < cap; ++k) {
+        float ms = 0.0f;
+        if (time_sol(sols[k], &ms) && ms < best) {
+            best = ms;
+            bt->gemm_sol = sols[k];
+        }
+    }
+    (void)hipGetLastError();  // a rejected candidate leaves no error behind
+    bt->gemm_candidates = cap;
+    bt->gemm_ms_best = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return TM_OK;
 }
 
@@ -1014,6 +1108,7 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
         if ((st = bt->sD32.setup(*v.cgD, nsamples, bt->stream)) != TM_OK) return fail(st);
     }
     if (hipStreamSynchronize(bt->stream) != hipSuccess) return fail(TM_ERR_HIP);
+    if ((st = tune_gemm(bt)) != TM_OK) return fail(st);
     *out = bt;
     return TM_OK;
 }
@@ -1102,18 +1197,14 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
         hipLaunchKernelGGL(k_b_gather<float>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
                            (const double*)bt->u1.p, bt->Uf.p);
         const float one = 1.0f, zero = 0.0f;
-        if (rocblas_sgemm(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, &one, bt->Bf.p,
-                          v.bld, bt->Uf.p, S, &zero, bt->Gf.p, b) != rocblas_status_success)
-            return TM_ERR_HIP;
+        TM_CHECK(gemm_run(bt, bt->gemm_sol));
         hipLaunchKernelGGL(k_b_scatter<float>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
                            (const float*)bt->Gf.p, bt->rhsD.p);
     } else {
         hipLaunchKernelGGL(k_b_gather<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
                            (const double*)bt->u1.p, bt->U.p);
         const double one = 1.0, zero = 0.0;
-        if (rocblas_dgemm(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, &one, v.B, v.bld,
-                          bt->U.p, S, &zero, bt->Gm.p, b) != rocblas_status_success)
-            return TM_ERR_HIP;
+        TM_CHECK(gemm_run(bt, bt->gemm_sol));
         hipLaunchKernelGGL(k_b_scatter<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
                            (const double*)bt->Gm.p, bt->rhsD.p);
     }
@@ -1212,6 +1303,15 @@ int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int va
     }
     TM_HIP(hipStreamSynchronize(s));
     return status;
+}
+
+int tm_batch_gemm_info(tm_batch* bt, int* solution, int* candidates, double* ms_default, double* ms_tuned) {
+    if (!bt) return TM_ERR_BAD_ARG;
+    if (solution) *solution = bt->gemm_sol;
+    if (candidates) *candidates = bt->gemm_candidates;
+    if (ms_default) *ms_default = bt->gemm_ms_default;
+    if (ms_tuned) *ms_tuned = bt->gemm_ms_best;
+    return TM_OK;
 }
 
 int tm_batch_step_euler(tm_batch* bt, const double* dm, int where, double dtau) {
