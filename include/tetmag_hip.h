@@ -359,9 +359,9 @@ int tm_ctx_get_timers(tm_ctx* c, tm_phase_times* out, int reset);
 #define TM_BATCH_MAX 64
 #define TM_PREC_FP64 0  /* all-FP64 batched solves */
 #define TM_PREC_MIXED 1 /* FP32 inner CG iterations, FP64 iterative refinement to the tolerance */
-#define TM_PREC_MIXED_BEM32 2 /* as MIXED, and the boundary product in FP32 (FP32 B copy, FP32
-                               * accumulation): faster, but ~1e-5 relative field error on a 41k-node
-                               * boundary, outside a 1e-6 bar */
+#define TM_PREC_MIXED_BEM32 2 /* as MIXED, and the boundary product from an FP32 copy of B: FP32
+                               * products accumulated in FP32 over 512-column chunks, chunk sums
+                               * added in FP64 */
 
 typedef struct tm_batch tm_batch;
 typedef struct tm_batch_stats {

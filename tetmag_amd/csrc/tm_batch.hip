@@ -465,8 +465,17 @@ __global__ __launch_bounds__(kBlock) void k_b_dense(int nc, int S, const double*
                                                     const T* __restrict__ b, T* __restrict__ x) {
     B_ROW_LOOP(nc) {
         double acc = 0.0;
-        for (int k = 0; k < nc; ++k)
-            if (on) acc += C[(size_t)row * nc + k] * (double)b[(size_t)k * S + lane];
+        for (int k0 = 0; k0 < nc; k0 += kGr) {
+            double cv[kGr], bv[kGr];
+#pragma unroll
+            for (int k = 0; k < kGr; ++k) {
+                const int kk = min(k0 + k, nc - 1);
+                cv[k] = C[(size_t)row * nc + kk];
+                bv[k] = (double)b[(size_t)kk * S + lc];
+            }
+#pragma unroll
+            for (int k = 0; k < kGr; ++k) acc = k0 + k < nc ? acc + cv[k] * bv[k] : acc;
+        }
         if (on) x[(size_t)row * S + lane] = (T)acc;
     }
 }
@@ -631,6 +640,17 @@ __global__ __launch_bounds__(kBlock) void k_b_scatter(int b, int S, const int* _
     const size_t tot = (size_t)b * S;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
         out[(size_t)bn[e % b] * S + e / b] = (double)G[e];
+}
+
+// out[bn[j]][s] = sum_c P[c][j + s b] in FP64 (the split-K FP32 partials)
+__global__ __launch_bounds__(kBlock) void k_b_chunk_sum(int b, int S, int nch, const int* __restrict__ bn,
+                                                        const float* __restrict__ P, double* __restrict__ out) {
+    const size_t tot = (size_t)b * S;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) {
+        double acc = 0.0;
+        for (int c = 0; c < nch; ++c) acc += (double)P[(size_t)c * tot + e];
+        out[(size_t)bn[e % b] * S + e / b] = acc;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_to_float(size_t tot, const double* __restrict__ a,
@@ -884,7 +904,9 @@ struct tm_batch {
     DevBuf<double> gx, gy, gz, tx, ty, tz, cx, cy, cz;
     bool exch = false, dmi = false;
     DevBuf<double> m, dmdt, heff, u1, u2, rhsN, rhsD, U, Gm, hext, torque;
-    DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product of the mixed form
+    DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product (TM_PREC_MIXED_BEM32)
+    DevBuf<float> Pf;                  // its split-K partial products (nchunk x b x S)
+    int kchunk = 512, nchunk = 0;
     DevBuf<double> rk_k[4], rk_base;   // RK4 stages and the step's start state
     int gemm_sol = 0;                  // tuned rocBLAS solution index (0: library default)
     int gemm_candidates = 0;
@@ -930,10 +952,31 @@ namespace {
 // S = 64.  At batch creation every solution rocBLAS offers for this exact
 // shape and layout is timed (HIP events, one warm-up and two timed runs) and
 // the fastest is kept.
+// FP32 products over K chunks of kchunk columns of B (strided-batched GEMM,
+// FP32 accumulation inside a chunk only), partials left in Pf.
+int gemm_split_f32(tm_batch* bt) {
+    const CtxView& v = bt->cv;
+    const int b = v.bnx, S = bt->S, kc = bt->kchunk;
+    const int nfull = b / kc, tail = b - nfull * kc;
+    const float one = 1.0f, zero = 0.0f;
+    if (nfull > 0 &&
+        rocblas_sgemm_strided_batched(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, kc, &one,
+                                      bt->Bf.p, v.bld, (rocblas_stride)kc * v.bld, bt->Uf.p, S, (rocblas_stride)kc * S,
+                                      &zero, bt->Pf.p, b, (rocblas_stride)b * S, nfull) != rocblas_status_success)
+        return TM_ERR_HIP;
+    if (tail > 0 &&
+        rocblas_sgemm(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, tail, &one,
+                      bt->Bf.p + (size_t)nfull * kc * v.bld, v.bld, bt->Uf.p + (size_t)nfull * kc * S, S, &zero,
+                      bt->Pf.p + (size_t)nfull * b * S, b) != rocblas_status_success)
+        return TM_ERR_HIP;
+    return TM_OK;
+}
+
 int gemm_run(tm_batch* bt, int sol) {
     const CtxView& v = bt->cv;
     const int b = v.bnx, S = bt->S;
     const bool f32 = bt->prec == TM_PREC_MIXED_BEM32;
+    if (f32) return gemm_split_f32(bt);
     const rocblas_datatype t = f32 ? rocblas_datatype_f32_r : rocblas_datatype_f64_r;
     const double one = 1.0, zero = 0.0;
     const float onef = 1.0f, zerof = 0.0f;
@@ -962,6 +1005,7 @@ int tune_gemm(tm_batch* bt) {
     void* U = f32 ? (void*)bt->Uf.p : (void*)bt->U.p;
     void* G = f32 ? (void*)bt->Gf.p : (void*)bt->Gm.p;
     TM_HIP(hipMemsetAsync(U, 0, (size_t)b * S * (f32 ? sizeof(float) : sizeof(double)), bt->stream));
+    if (f32) return TM_OK;  // split-K FP32 product: library defaults
     rocblas_int nsol = 0;
     if (rocblas_gemm_ex_get_solutions(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, al, A,
                                       t, v.bld, U, t, S, be, G, t, b, G, t, b, t, rocblas_gemm_algo_solution_index, 0,
@@ -1077,9 +1121,11 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     a |= bt->rhsN.alloc(NS);
     a |= bt->rhsD.alloc(NS);
     if (precision == TM_PREC_MIXED_BEM32) {
+        bt->nchunk = (v.bnx + bt->kchunk - 1) / bt->kchunk;
         a |= bt->Uf.alloc((size_t)v.bnx * nsamples);
         a |= bt->Gf.alloc((size_t)v.bnx * nsamples);
         a |= bt->Bf.alloc((size_t)v.bld * v.bnx);
+        a |= bt->Pf.alloc((size_t)bt->nchunk * v.bnx * nsamples);
     } else {
         a |= bt->U.alloc((size_t)v.bnx * nsamples);
         a |= bt->Gm.alloc((size_t)v.bnx * nsamples);
@@ -1190,16 +1236,16 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     const int b = v.bnx;
     // column-major G (b x S) = B (b x b, streamed in its storage order) * U^T,
     // U = gathered boundary potentials (S x b).  FP64, except for
-    // TM_PREC_MIXED_BEM32 (an FP32 copy of B and FP32 accumulation: ~1e-5
-    // relative field error at 41k boundary nodes, from cancellation in the
-    // long row sums)
+    // TM_PREC_MIXED_BEM32: an FP32 copy of B, FP32 products accumulated in
+    // FP32 over 512-column chunks of B only, the chunk results summed in FP64
+    // (FP32 accumulation over all 41k columns costs ~1e-5 relative field error)
     if (bt->prec == TM_PREC_MIXED_BEM32) {
         hipLaunchKernelGGL(k_b_gather<float>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
                            (const double*)bt->u1.p, bt->Uf.p);
         const float one = 1.0f, zero = 0.0f;
-        TM_CHECK(gemm_run(bt, bt->gemm_sol));
-        hipLaunchKernelGGL(k_b_scatter<float>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
-                           (const float*)bt->Gf.p, bt->rhsD.p);
+        TM_CHECK(gemm_split_f32(bt));
+        hipLaunchKernelGGL(k_b_chunk_sum, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, bt->nchunk,
+                           v.bnodes, (const float*)bt->Pf.p, bt->rhsD.p);
     } else {
         hipLaunchKernelGGL(k_b_gather<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
                            (const double*)bt->u1.p, bt->U.p);
