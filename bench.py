@@ -159,6 +159,20 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
     return out
 
 
+def measured_traffic(workload, b):
+    """HBM bytes per boundary product from the committed FETCH_SIZE / WRITE_SIZE
+    counter runs (calibrated on a 1 GiB copy), when they cover this workload."""
+    path = os.path.join(ROOT, "profiles", f"bem_gemv_traffic_{workload.rsplit('-h', 1)[0]}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    if d.get("b") != b:
+        return None
+    return d["hbm_bytes_per_product"]["total"]
+
+
 def kernel_only(ctx, prob, reps):
     """Launch the dense boundary product (k_bem_gemv_part + k_bem_gemv_finish)
     and a plain 1 GiB device-to-device copy `reps` times each, timed with HIP
@@ -313,7 +327,7 @@ def main():
         "batched": legs,
         "roofline": {"bound": "hbm", "kernel": "k_bem_gemv_part+finish (dense B u1, FP64, plain form)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": measured_traffic(prob.name, b)},
     }
     if not args.no_cpu_baseline:
         try:

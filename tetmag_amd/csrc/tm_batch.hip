@@ -642,6 +642,78 @@ __global__ __launch_bounds__(kBlock) void k_b_scatter(int b, int S, const int* _
         out[(size_t)bn[e % b] * S + e / b] = (double)G[e];
 }
 
+// ---- FP64 MFMA GEMM for the boundary product ---------------------------
+// P[ks] (b x S, column-major) = B[:, K_ks] (b x |K_ks|, column-major, ld) *
+// U^T[K_ks, :] with U S x b column-major (ld S), S = 64 columns.  Block: 128
+// rows (4 wavefronts x 32 rows), all 64 columns, one K range ks of the
+// split; U^T chunks of kGk rows are staged in LDS.  v_mfma_f64_16x16x4f64
+// operand layout (measured): A lane l = A[l%16][l/16], B lane l =
+// B[l/16][l%16], D lane l register r = C[4r + l/16][l%16].
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int kGm = 128;  // rows per block
+constexpr int kGk = 64;   // K rows of U^T per LDS chunk
+
+__global__ __launch_bounds__(kBlock) void k_b_gemm64(int b, int ld, const double* __restrict__ B,
+                                                     const double* __restrict__ U, int kper,
+                                                     double* __restrict__ P) {
+    __shared__ double us[kGk * 64];  // U^T chunk: us[k * 64 + j]
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int m0 = blockIdx.x * kGm + wv * 32;  // this wavefront's 32 rows
+    const int ks = blockIdx.y;
+    const int kb = ks * kper, ke = min(b, kb + kper);
+    const int li = lane & 15, lk = lane >> 4;
+    dbl4 acc[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[t][u] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int r0 = m0 + li, r1 = m0 + 16 + li;  // the A rows of this lane
+    const bool ok0 = r0 < b, ok1 = r1 < b;
+    for (int k0 = kb; k0 < ke; k0 += kGk) {
+        __syncthreads();  // the previous chunk has been consumed
+        for (int t = threadIdx.x; t < kGk * 64; t += kBlock) {
+            const int kk = k0 + t / 64;
+            us[t] = kk < ke ? U[(size_t)kk * 64 + (t & 63)] : 0.0;
+        }
+        __syncthreads();
+        const int kn = min(kGk, ke - k0);
+        for (int q = 0; q < kn; q += 4) {
+            const int kc = k0 + q + lk;  // this lane's K column
+            const bool kin = kc < ke;
+            const double a0 = (ok0 && kin) ? B[(size_t)kc * ld + r0] : 0.0;
+            const double a1 = (ok1 && kin) ? B[(size_t)kc * ld + r1] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double bu = us[(q + lk) * 64 + u * 16 + li];
+                acc[0][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bu, acc[0][u], 0, 0, 0);
+                acc[1][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bu, acc[1][u], 0, 0, 0);
+            }
+        }
+    }
+    double* out = P + (size_t)ks * b * 64;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + t * 16 + 4 * r + lk;
+                if (row < b) out[(size_t)(u * 16 + li) * b + row] = acc[t][u][r];
+            }
+}
+
+// out[bn[j]][s] = sum_ks P[ks][j + s b] (FP64 split-K partials of k_b_gemm64)
+__global__ __launch_bounds__(kBlock) void k_b_ksum_scatter(int b, int S, int nks, const int* __restrict__ bn,
+                                                           const double* __restrict__ P, double* __restrict__ out) {
+    const size_t tot = (size_t)b * S;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) {
+        double acc = 0.0;
+        for (int c = 0; c < nks; ++c) acc += P[(size_t)c * tot + e];
+        if (bn) out[(size_t)bn[e % b] * S + e / b] = acc;
+        else out[e] = acc;  // G itself (b x S)
+    }
+}
+
 // out[bn[j]][s] = sum_c P[c][j + s b] in FP64 (the split-K FP32 partials)
 __global__ __launch_bounds__(kBlock) void k_b_chunk_sum(int b, int S, int nch, const int* __restrict__ bn,
                                                         const float* __restrict__ P, double* __restrict__ out) {
@@ -909,6 +981,10 @@ struct tm_batch {
     int kchunk = 512, nchunk = 0;
     DevBuf<double> rk_k[4], rk_base;   // RK4 stages and the step's start state
     int gemm_sol = 0;                  // tuned rocBLAS solution index (0: library default)
+    bool gemm_own = false;             // the MFMA kernel k_b_gemm64 won the tuning
+    int gemm_ks = 1;                   // its K split
+    DevBuf<double> Pd;                 // its split-K partials (ks x b x S)
+    double gemm_own_err = -1.0;        // its max relative difference to rocBLAS at tuning
     int gemm_candidates = 0;
     float gemm_ms_default = 0.0f, gemm_ms_best = 0.0f;
     BatchSolver<double> sN64, sD64;
@@ -992,6 +1068,26 @@ int gemm_run(tm_batch* bt, int sol) {
     return r == rocblas_status_success ? TM_OK : TM_ERR_HIP;
 }
 
+__global__ __launch_bounds__(kBlock) void k_b_fill_pattern(size_t n, double* __restrict__ x) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock)
+        x[e] = sin(0.37 * (double)(e % 100003)) + 0.25;
+}
+
+int gemm_own_run(tm_batch* bt, double* G) {
+    const CtxView& v = bt->cv;
+    const int b = v.bnx;
+    const int kper = (b + bt->gemm_ks - 1) / bt->gemm_ks;
+    hipLaunchKernelGGL(k_b_gemm64, dim3((b + kGm - 1) / kGm, bt->gemm_ks), dim3(kBlock), 0, bt->stream, b, v.bld, v.B,
+                       (const double*)bt->U.p, kper, bt->Pd.p);
+    TM_HIP(hipGetLastError());
+    if (G) {  // sum the partials into G (b x S) without scatter: identity map through a stride-1 index
+        hipLaunchKernelGGL(k_b_ksum_scatter, dim3(elem_grid((size_t)b * bt->S)), dim3(kBlock), 0, bt->stream, b,
+                           bt->S, bt->gemm_ks, (const int*)nullptr, (const double*)bt->Pd.p, G);
+        TM_HIP(hipGetLastError());
+    }
+    return TM_OK;
+}
+
 int tune_gemm(tm_batch* bt) {
     const CtxView& v = bt->cv;
     const int b = v.bnx, S = bt->S;
@@ -1045,6 +1141,47 @@ int tune_gemm(tm_batch* bt) {
     }
     (void)hipGetLastError();  // a rejected candidate leaves no error behind
     bt->gemm_candidates = cap;
+    // the MFMA kernel: checked against the library on a non-zero U, then timed
+    if (!f32 && S == 64) {
+        const int grid_m = (b + kGm - 1) / kGm;
+        bt->gemm_ks = std::max(1, std::min(16, (1024 + grid_m - 1) / grid_m));
+        if (bt->Pd.alloc((size_t)bt->gemm_ks * b * S) == TM_OK) {
+            hipLaunchKernelGGL(k_b_fill_pattern, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, bt->stream,
+                               (size_t)b * S, bt->U.p);
+            std::vector<double> ref((size_t)b * S), own((size_t)b * S);
+            DevBuf<double> G2;
+            bool ok = G2.alloc((size_t)b * S) == TM_OK && gemm_run(bt, bt->gemm_sol) == TM_OK &&
+                      hipMemcpyAsync(ref.data(), bt->Gm.p, ref.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                     bt->stream) == hipSuccess &&
+                      gemm_own_run(bt, G2.p) == TM_OK &&
+                      hipMemcpyAsync(own.data(), G2.p, own.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                     bt->stream) == hipSuccess &&
+                      hipStreamSynchronize(bt->stream) == hipSuccess;
+            if (ok) {
+                double dmax = 0.0, rmax = 0.0;
+                for (size_t e = 0; e < ref.size(); ++e) {
+                    dmax = std::max(dmax, std::fabs(own[e] - ref[e]));
+                    rmax = std::max(rmax, std::fabs(ref[e]));
+                }
+                bt->gemm_own_err = rmax > 0 ? dmax / rmax : dmax;
+                float ms = 0.0f;
+                if (bt->gemm_own_err <= 1e-12 && hipEventRecord(e0, bt->stream) == hipSuccess) {
+                    bool run_ok = true;
+                    for (int k = 0; k < 2 && run_ok; ++k) run_ok = gemm_own_run(bt, nullptr) == TM_OK;
+                    if (run_ok && hipEventRecord(e1, bt->stream) == hipSuccess &&
+                        hipEventSynchronize(e1) == hipSuccess) {
+                        (void)hipEventElapsedTime(&ms, e0, e1);
+                        ms *= 0.5f;
+                        if (ms < best) {
+                            best = ms;
+                            bt->gemm_own = true;
+                        }
+                    }
+                }
+            }
+            TM_HIP(hipMemsetAsync(bt->U.p, 0, (size_t)b * S * sizeof(double), bt->stream));
+        }
+    }
     bt->gemm_ms_best = best;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1250,9 +1387,15 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
         hipLaunchKernelGGL(k_b_gather<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
                            (const double*)bt->u1.p, bt->U.p);
         const double one = 1.0, zero = 0.0;
-        TM_CHECK(gemm_run(bt, bt->gemm_sol));
-        hipLaunchKernelGGL(k_b_scatter<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
-                           (const double*)bt->Gm.p, bt->rhsD.p);
+        if (bt->gemm_own) {
+            TM_CHECK(gemm_own_run(bt, nullptr));
+            hipLaunchKernelGGL(k_b_ksum_scatter, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S,
+                               bt->gemm_ks, v.bnodes, (const double*)bt->Pd.p, bt->rhsD.p);
+        } else {
+            TM_CHECK(gemm_run(bt, bt->gemm_sol));
+            hipLaunchKernelGGL(k_b_scatter<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S,
+                               v.bnodes, (const double*)bt->Gm.p, bt->rhsD.p);
+        }
     }
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[3], s));
     int stD = bt->prec == TM_PREC_FP64 ? bt->sD64.solve(bt->rhsD.p, bt->u2.p, v.tol, v.maxit, s, &oD, &iD, &rD)
@@ -1353,7 +1496,7 @@ int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int va
 
 int tm_batch_gemm_info(tm_batch* bt, int* solution, int* candidates, double* ms_default, double* ms_tuned) {
     if (!bt) return TM_ERR_BAD_ARG;
-    if (solution) *solution = bt->gemm_sol;
+    if (solution) *solution = bt->gemm_own ? -1 : bt->gemm_sol;
     if (candidates) *candidates = bt->gemm_candidates;
     if (ms_default) *ms_default = bt->gemm_ms_default;
     if (ms_tuned) *ms_tuned = bt->gemm_ms_best;
