@@ -676,17 +676,23 @@ __global__ __launch_bounds__(kBlock) void k_b_gemm64(int b, int ld, const double
             us[t] = kk < ke ? U[(size_t)kk * 64 + (t & 63)] : 0.0;
         }
         __syncthreads();
-        const int kn = min(kGk, ke - k0);
-        for (int q = 0; q < kn; q += 4) {
-            const int kc = k0 + q + lk;  // this lane's K column
+        // the chunk's B loads first (kGk / 4 k-steps x 2 row tiles per lane),
+        // so they are all in flight before the MFMAs consume them
+        double a0[kGk / 4], a1[kGk / 4];
+#pragma unroll
+        for (int q = 0; q < kGk / 4; ++q) {
+            const int kc = k0 + 4 * q + lk;  // this lane's K column
             const bool kin = kc < ke;
-            const double a0 = (ok0 && kin) ? B[(size_t)kc * ld + r0] : 0.0;
-            const double a1 = (ok1 && kin) ? B[(size_t)kc * ld + r1] : 0.0;
+            a0[q] = (ok0 && kin) ? B[(size_t)kc * ld + r0] : 0.0;
+            a1[q] = (ok1 && kin) ? B[(size_t)kc * ld + r1] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kGk / 4; ++q) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const double bu = us[(q + lk) * 64 + u * 16 + li];
-                acc[0][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bu, acc[0][u], 0, 0, 0);
-                acc[1][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bu, acc[1][u], 0, 0, 0);
+                const double bu = us[(4 * q + lk) * 64 + u * 16 + li];
+                acc[0][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], bu, acc[0][u], 0, 0, 0);
+                acc[1][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], bu, acc[1][u], 0, 0, 0);
             }
         }
     }
