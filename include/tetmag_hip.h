@@ -393,8 +393,10 @@ int tm_batch_step_euler(tm_batch* b, const double* dmdt, int where, double dtau)
  * reference steps one field value at a time through CVODE
  * (Hysteresis.cpp:43-169, TheLLG.cpp:578-711). */
 int tm_batch_run_rk4(tm_batch* b, double dtau, int nsteps, double alpha, int variant, tm_batch_stats* last);
-/* The boundary GEMM's rocBLAS solution chosen by timing every candidate for
- * this shape at tm_batch_create (0 = library default), and the timings. */
+/* The boundary GEMM chosen by timing every candidate for this shape at
+ * tm_batch_create: a rocBLAS solution index (0 = library default) or, when
+ * negative, -1 - k for tile shape k of the library's own FP64 MFMA kernel
+ * (only chosen when it agrees with rocBLAS to 1e-12); and the timings. */
 int tm_batch_gemm_info(tm_batch* b, int* solution, int* candidates, double* ms_default, double* ms_tuned);
 int tm_batch_enable_timers(tm_batch* b, int on);
 int tm_batch_get_timers(tm_batch* b, tm_phase_times* out, int reset);

@@ -15,7 +15,7 @@
 //   x += d, repeat until ||b - A x|| <= tol ||b|| for every sample.
 // T = double solves to the final tolerance in one outer step (the FP64
 // batched form); T = float runs the inner iterations in FP32 to a relative
-// 1e-5 and recovers FP64 accuracy by iterative refinement (the mixed form).
+// 1e-6 and recovers FP64 accuracy by iterative refinement (the mixed form).
 // Converged samples are frozen (their step lengths are zero).
 #define ROCBLAS_BETA_FEATURES_API  // rocblas_gemm_ex_get_solutions (the GEMM autotuner)
 #include <rocblas/rocblas.h>
@@ -650,55 +650,62 @@ __global__ __launch_bounds__(kBlock) void k_b_scatter(int b, int S, const int* _
 // operand layout (measured): A lane l = A[l%16][l/16], B lane l =
 // B[l/16][l%16], D lane l register r = C[4r + l/16][l%16].
 typedef double dbl4 __attribute__((ext_vector_type(4)));
-constexpr int kGm = 128;  // rows per block
-constexpr int kGk = 64;   // K rows of U^T per LDS chunk
 
+// KGK: K rows of U^T per LDS chunk; MT: 16-row tiles per wavefront (block
+// rows = 4 x 16 x MT).  The autotuner picks among the instantiations.
+template <int KGK, int MT>
 __global__ __launch_bounds__(kBlock) void k_b_gemm64(int b, int ld, const double* __restrict__ B,
                                                      const double* __restrict__ U, int kper,
                                                      double* __restrict__ P) {
-    __shared__ double us[kGk * 64];  // U^T chunk: us[k * 64 + j]
+    __shared__ double us[KGK * 64];  // U^T chunk: us[k * 64 + j]
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const int m0 = blockIdx.x * kGm + wv * 32;  // this wavefront's 32 rows
+    const int m0 = (blockIdx.x * (kBlock / kWave) + wv) * 16 * MT;  // this wavefront's rows
     const int ks = blockIdx.y;
     const int kb = ks * kper, ke = min(b, kb + kper);
     const int li = lane & 15, lk = lane >> 4;
-    dbl4 acc[2][4];
+    dbl4 acc[MT][4];
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < MT; ++t)
 #pragma unroll
         for (int u = 0; u < 4; ++u) acc[t][u] = dbl4{0.0, 0.0, 0.0, 0.0};
-    const int r0 = m0 + li, r1 = m0 + 16 + li;  // the A rows of this lane
-    const bool ok0 = r0 < b, ok1 = r1 < b;
-    for (int k0 = kb; k0 < ke; k0 += kGk) {
+    int rr[MT];
+    bool ok[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        rr[t] = m0 + 16 * t + li;  // the A row of this lane in tile t
+        ok[t] = rr[t] < b;
+    }
+    for (int k0 = kb; k0 < ke; k0 += KGK) {
         __syncthreads();  // the previous chunk has been consumed
-        for (int t = threadIdx.x; t < kGk * 64; t += kBlock) {
+        for (int t = threadIdx.x; t < KGK * 64; t += kBlock) {
             const int kk = k0 + t / 64;
             us[t] = kk < ke ? U[(size_t)kk * 64 + (t & 63)] : 0.0;
         }
         __syncthreads();
-        // the chunk's B loads first (kGk / 4 k-steps x 2 row tiles per lane),
-        // so they are all in flight before the MFMAs consume them
-        double a0[kGk / 4], a1[kGk / 4];
+        // the chunk's B loads first, so they are all in flight before the
+        // MFMAs consume them
+        double a[MT][KGK / 4];
 #pragma unroll
-        for (int q = 0; q < kGk / 4; ++q) {
+        for (int q = 0; q < KGK / 4; ++q) {
             const int kc = k0 + 4 * q + lk;  // this lane's K column
             const bool kin = kc < ke;
-            a0[q] = (ok0 && kin) ? B[(size_t)kc * ld + r0] : 0.0;
-            a1[q] = (ok1 && kin) ? B[(size_t)kc * ld + r1] : 0.0;
+#pragma unroll
+            for (int t = 0; t < MT; ++t) a[t][q] = (ok[t] && kin) ? B[(size_t)kc * ld + rr[t]] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < kGk / 4; ++q) {
+        for (int q = 0; q < KGK / 4; ++q) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const double bu = us[(4 * q + lk) * 64 + u * 16 + li];
-                acc[0][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], bu, acc[0][u], 0, 0, 0);
-                acc[1][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], bu, acc[1][u], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < MT; ++t)
+                    acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][q], bu, acc[t][u], 0, 0, 0);
             }
         }
     }
     double* out = P + (size_t)ks * b * 64;
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < MT; ++t)
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -706,6 +713,24 @@ __global__ __launch_bounds__(kBlock) void k_b_gemm64(int b, int ld, const double
                 const int row = m0 + t * 16 + 4 * r + lk;
                 if (row < b) out[(size_t)(u * 16 + li) * b + row] = acc[t][u][r];
             }
+}
+
+struct GemmShape {
+    int kgk, mt;
+};
+constexpr GemmShape kGemmShapes[] = {{64, 2}, {32, 2}, {64, 1}, {32, 1}};
+constexpr int kNumGemmShapes = 4;
+
+inline int gemm_rows_per_block(int shape) { return (kBlock / kWave) * 16 * kGemmShapes[shape].mt; }
+
+void launch_gemm64(int shape, dim3 g, hipStream_t s, int b, int ld, const double* B, const double* U, int kper,
+                   double* P) {
+    switch (shape) {
+        case 0: hipLaunchKernelGGL((k_b_gemm64<64, 2>), g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); break;
+        case 1: hipLaunchKernelGGL((k_b_gemm64<32, 2>), g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); break;
+        case 2: hipLaunchKernelGGL((k_b_gemm64<64, 1>), g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); break;
+        default: hipLaunchKernelGGL((k_b_gemm64<32, 1>), g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); break;
+    }
 }
 
 // out[bn[j]][s] = sum_ks P[ks][j + s b] (FP64 split-K partials of k_b_gemm64)
@@ -904,7 +929,7 @@ struct BatchSolver {
         double* prr = part.p + (size_t)S * G;
         double* prz = part.p + (size_t)2 * S * G;
         double* ppq = part.p + (size_t)3 * S * G;
-        const double itol = sizeof(T) == 4 ? 1e-5 : 0.0;
+        const double itol = sizeof(T) == 4 ? 1e-6 : 0.0;  // FP32 inner target per refinement step
         int outer = 0, inner_total = 0;
         hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
         for (;;) {
@@ -988,9 +1013,9 @@ struct tm_batch {
     DevBuf<double> rk_k[4], rk_base;   // RK4 stages and the step's start state
     int gemm_sol = 0;                  // tuned rocBLAS solution index (0: library default)
     bool gemm_own = false;             // the MFMA kernel k_b_gemm64 won the tuning
-    int gemm_ks = 1;                   // its K split
+    int gemm_shape = 0, gemm_ks = 1;   // its tile shape and K split
     DevBuf<double> Pd;                 // its split-K partials (ks x b x S)
-    double gemm_own_err = -1.0;        // its max relative difference to rocBLAS at tuning
+    double gemm_own_err = 0.0;         // its largest relative difference to rocBLAS at tuning
     int gemm_candidates = 0;
     float gemm_ms_default = 0.0f, gemm_ms_best = 0.0f;
     BatchSolver<double> sN64, sD64;
@@ -1079,12 +1104,19 @@ __global__ __launch_bounds__(kBlock) void k_b_fill_pattern(size_t n, double* __r
         x[e] = sin(0.37 * (double)(e % 100003)) + 0.25;
 }
 
+int gemm_own_ks(int b, int shape) {
+    const int rows = gemm_rows_per_block(shape);
+    const int grid_m = (b + rows - 1) / rows;
+    return std::max(1, std::min(16, (1024 + grid_m - 1) / grid_m));
+}
+
 int gemm_own_run(tm_batch* bt, double* G) {
     const CtxView& v = bt->cv;
     const int b = v.bnx;
+    const int rows = gemm_rows_per_block(bt->gemm_shape);
     const int kper = (b + bt->gemm_ks - 1) / bt->gemm_ks;
-    hipLaunchKernelGGL(k_b_gemm64, dim3((b + kGm - 1) / kGm, bt->gemm_ks), dim3(kBlock), 0, bt->stream, b, v.bld, v.B,
-                       (const double*)bt->U.p, kper, bt->Pd.p);
+    launch_gemm64(bt->gemm_shape, dim3((b + rows - 1) / rows, bt->gemm_ks), bt->stream, b, v.bld, v.B,
+                  (const double*)bt->U.p, kper, bt->Pd.p);
     TM_HIP(hipGetLastError());
     if (G) {  // sum the partials into G (b x S) without scatter: identity map through a stride-1 index
         hipLaunchKernelGGL(k_b_ksum_scatter, dim3(elem_grid((size_t)b * bt->S)), dim3(kBlock), 0, bt->stream, b,
@@ -1147,11 +1179,12 @@ int tune_gemm(tm_batch* bt) {
     }
     (void)hipGetLastError();  // a rejected candidate leaves no error behind
     bt->gemm_candidates = cap;
-    // the MFMA kernel: checked against the library on a non-zero U, then timed
+    // the MFMA kernel, each tile shape: checked against the library on a
+    // non-zero U, then timed
     if (!f32 && S == 64) {
-        const int grid_m = (b + kGm - 1) / kGm;
-        bt->gemm_ks = std::max(1, std::min(16, (1024 + grid_m - 1) / grid_m));
-        if (bt->Pd.alloc((size_t)bt->gemm_ks * b * S) == TM_OK) {
+        int ksmax = 1;
+        for (int sh = 0; sh < kNumGemmShapes; ++sh) ksmax = std::max(ksmax, gemm_own_ks(b, sh));
+        if (bt->Pd.alloc((size_t)ksmax * b * S) == TM_OK) {
             hipLaunchKernelGGL(k_b_fill_pattern, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, bt->stream,
                                (size_t)b * S, bt->U.p);
             std::vector<double> ref((size_t)b * S), own((size_t)b * S);
@@ -1159,31 +1192,44 @@ int tune_gemm(tm_batch* bt) {
             bool ok = G2.alloc((size_t)b * S) == TM_OK && gemm_run(bt, bt->gemm_sol) == TM_OK &&
                       hipMemcpyAsync(ref.data(), bt->Gm.p, ref.size() * sizeof(double), hipMemcpyDeviceToHost,
                                      bt->stream) == hipSuccess &&
-                      gemm_own_run(bt, G2.p) == TM_OK &&
-                      hipMemcpyAsync(own.data(), G2.p, own.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                     bt->stream) == hipSuccess &&
                       hipStreamSynchronize(bt->stream) == hipSuccess;
-            if (ok) {
-                double dmax = 0.0, rmax = 0.0;
-                for (size_t e = 0; e < ref.size(); ++e) {
-                    dmax = std::max(dmax, std::fabs(own[e] - ref[e]));
-                    rmax = std::max(rmax, std::fabs(ref[e]));
-                }
-                bt->gemm_own_err = rmax > 0 ? dmax / rmax : dmax;
+            double rmax = 0.0;
+            for (size_t e = 0; ok && e < ref.size(); ++e) rmax = std::max(rmax, std::fabs(ref[e]));
+            const int keep_shape = bt->gemm_shape, keep_ks = bt->gemm_ks;
+            int best_shape = -1, best_ks = 1;
+            for (int sh = 0; ok && sh < kNumGemmShapes; ++sh) {
+                bt->gemm_shape = sh;
+                bt->gemm_ks = gemm_own_ks(b, sh);
+                if (gemm_own_run(bt, G2.p) != TM_OK ||
+                    hipMemcpyAsync(own.data(), G2.p, own.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   bt->stream) != hipSuccess ||
+                    hipStreamSynchronize(bt->stream) != hipSuccess)
+                    break;
+                double dmax = 0.0;
+                for (size_t e = 0; e < ref.size(); ++e) dmax = std::max(dmax, std::fabs(own[e] - ref[e]));
+                const double err = rmax > 0 ? dmax / rmax : dmax;
+                bt->gemm_own_err = std::max(bt->gemm_own_err, err);
+                if (err > 1e-12) continue;  // never choose a kernel that disagrees with the library
                 float ms = 0.0f;
-                if (bt->gemm_own_err <= 1e-12 && hipEventRecord(e0, bt->stream) == hipSuccess) {
-                    bool run_ok = true;
-                    for (int k = 0; k < 2 && run_ok; ++k) run_ok = gemm_own_run(bt, nullptr) == TM_OK;
-                    if (run_ok && hipEventRecord(e1, bt->stream) == hipSuccess &&
-                        hipEventSynchronize(e1) == hipSuccess) {
-                        (void)hipEventElapsedTime(&ms, e0, e1);
-                        ms *= 0.5f;
-                        if (ms < best) {
-                            best = ms;
-                            bt->gemm_own = true;
-                        }
+                bool run_ok = hipEventRecord(e0, bt->stream) == hipSuccess;
+                for (int k = 0; k < 2 && run_ok; ++k) run_ok = gemm_own_run(bt, nullptr) == TM_OK;
+                if (run_ok && hipEventRecord(e1, bt->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess) {
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    ms *= 0.5f;
+                    if (ms < best) {
+                        best = ms;
+                        best_shape = sh;
+                        best_ks = bt->gemm_ks;
                     }
                 }
+            }
+            if (best_shape >= 0) {
+                bt->gemm_own = true;
+                bt->gemm_shape = best_shape;
+                bt->gemm_ks = best_ks;
+            } else {
+                bt->gemm_shape = keep_shape;
+                bt->gemm_ks = keep_ks;
             }
             TM_HIP(hipMemsetAsync(bt->U.p, 0, (size_t)b * S * sizeof(double), bt->stream));
         }
@@ -1502,7 +1548,7 @@ int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int va
 
 int tm_batch_gemm_info(tm_batch* bt, int* solution, int* candidates, double* ms_default, double* ms_tuned) {
     if (!bt) return TM_ERR_BAD_ARG;
-    if (solution) *solution = bt->gemm_own ? -1 : bt->gemm_sol;
+    if (solution) *solution = bt->gemm_own ? -1 - bt->gemm_shape : bt->gemm_sol;
     if (candidates) *candidates = bt->gemm_candidates;
     if (ms_default) *ms_default = bt->gemm_ms_default;
     if (ms_tuned) *ms_tuned = bt->gemm_ms_best;
