@@ -15,7 +15,7 @@
 //   x += d, repeat until ||b - A x|| <= tol ||b|| for every sample.
 // T = double solves to the final tolerance in one outer step (the FP64
 // batched form); T = float runs the inner iterations in FP32 to a relative
-// 1e-6 and recovers FP64 accuracy by iterative refinement (the mixed form).
+// 1e-5 and recovers FP64 accuracy by iterative refinement (the mixed form).
 // Converged samples are frozen (their step lengths are zero).
 #define ROCBLAS_BETA_FEATURES_API  // rocblas_gemm_ex_get_solutions (the GEMM autotuner)
 #include <rocblas/rocblas.h>
@@ -929,7 +929,7 @@ struct BatchSolver {
         double* prr = part.p + (size_t)S * G;
         double* prz = part.p + (size_t)2 * S * G;
         double* ppq = part.p + (size_t)3 * S * G;
-        const double itol = sizeof(T) == 4 ? 1e-6 : 0.0;  // FP32 inner target per refinement step
+        const double itol = sizeof(T) == 4 ? 1e-5 : 0.0;  // FP32 inner target per refinement step
         int outer = 0, inner_total = 0;
         hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
         for (;;) {
