@@ -370,8 +370,8 @@ def main():
         else:
             ach = flops / (leg["phase_ms"]["bem_ms"] * 1e-3) / 1e12
             gt = leg.get("gemm_tuning", {})
-            kern = ("k_b_gemm64 (FP64 MFMA, tile shape %d)" % (-1 - gt["solution"])) if gt.get("solution", 0) < 0 \
-                else "rocBLAS DGEMM"
+            kern = ("k_b_gemm64, FP64 MFMA, tile shape kGemmShapes[%d] of tm_batch.hip, K split %d"
+                    % (-1 - gt["solution"], gt.get("ksplit", 1))) if gt.get("solution", 0) < 0 else "rocBLAS DGEMM"
             result["roofline"] = {"bound": "mfma", "kernel": f"boundary GEMM B(b x b) U^T(b x {args.batch}): {kern}",
                                   "achieved": ach, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
                                   "frac": ach / FP64_MFMA_PEAK_TFS, "traffic": None,

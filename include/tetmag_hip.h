@@ -396,8 +396,16 @@ int tm_batch_run_rk4(tm_batch* b, double dtau, int nsteps, double alpha, int var
 /* The boundary GEMM chosen by timing every candidate for this shape at
  * tm_batch_create: a rocBLAS solution index (0 = library default) or, when
  * negative, -1 - k for tile shape k of the library's own FP64 MFMA kernel
- * (only chosen when it agrees with rocBLAS to 1e-12); and the timings. */
-int tm_batch_gemm_info(tm_batch* b, int* solution, int* candidates, double* ms_default, double* ms_tuned);
+ * (only chosen when it agrees with rocBLAS to 1e-12), run with `ksplit`
+ * partial products over K; the number of candidates timed; the timings of
+ * the library default and of the choice. */
+int tm_batch_gemm_info(tm_batch* b, int* solution, int* candidates, double* ms_default, double* ms_tuned,
+                       int* ksplit);
+/* y[j][s] = sum_k B[j][k] x[k][s] for every sample s: the batch's boundary
+ * product (the GEMM tm_batch_eval_rhs runs, in the batch's precision) on
+ * caller values; x, y: b x S, boundary-node major, the sample index fastest.
+ * The batched form of DemagField::denseMVP (DemagField.cpp:145-147). */
+int tm_batch_bem_apply(tm_batch* b, const double* x, int xwhere, double* y, int ywhere);
 int tm_batch_enable_timers(tm_batch* b, int on);
 int tm_batch_get_timers(tm_batch* b, tm_phase_times* out, int reset);
 

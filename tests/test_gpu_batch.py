@@ -129,3 +129,32 @@ def test_batch_rk4_matches_single_samples(base):
         fc.run_rk4(dtau, 4, use_graph=False)
         assert rel_l2(from_batch(mb, s), fc.get_mag()) < 1e-9
         fc.close()
+
+
+@pytest.mark.parametrize("prec,S", [("fp64", 64), ("mixed", 64), ("fp64", 3), ("bem32", 64)])
+def test_batch_bem_apply(base, prec, S):
+    """tm_batch_bem_apply against the dense FP64 product B X on the host: the
+    boundary GEMM the batched evaluation runs (tuned MFMA kernel or rocBLAS for
+    S = 64, rocBLAS otherwise), host and device operands."""
+    import torch
+    from tetmag_amd import _native as N
+    from tetmag_amd.batch import BatchContext
+    prob, ctx, _ = base
+    prec_id = {"fp64": N.TM_PREC_FP64, "mixed": N.TM_PREC_MIXED, "bem32": N.TM_PREC_MIXED_BEM32}[prec]
+    bc = BatchContext(ctx, S, prec_id)
+    print(prec, S, bc.gemm_info())
+    B = ctx.bem_matrix()
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((B.shape[0], S))
+    ref = B @ X
+    Y = bc.bem_apply(X)
+    tol = 1e-5 if prec == "bem32" else 1e-13  # FP32 products summed in FP64 per 512-column chunk
+    assert rel_l2(Y, ref) < tol
+    Xd = torch.from_numpy(X).to(ctx.device)
+    torch.cuda.synchronize()
+    Yd = bc.bem_apply(Xd)
+    ctx.synchronize()
+    assert np.array_equal(Yd.cpu().numpy(), Y)  # same kernels, same order
+    Y2 = bc.bem_apply(X[::-1].copy())  # a second call on new values
+    assert rel_l2(Y2, B @ X[::-1]) < tol
+    bc.close()

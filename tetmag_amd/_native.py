@@ -210,7 +210,8 @@ _SIGNATURES = {
     "tm_batch_get_potentials": [_P, _P, _P, _I],
     "tm_batch_step_euler": [_P, _P, _I, _D],
     "tm_batch_run_rk4": [_P, _D, _I, _D, _I, _P],
-    "tm_batch_gemm_info": [_P, _P, _P, _P, _P],
+    "tm_batch_gemm_info": [_P, _P, _P, _P, _P, _P],
+    "tm_batch_bem_apply": [_P, _P, _I, _P, _I],
     "tm_batch_enable_timers": [_P, _I],
     "tm_batch_get_timers": [_P, _P, _I],
     "tm_version": [],

@@ -87,11 +87,31 @@ class BatchContext:
         return st.as_dict()
 
     def gemm_info(self) -> dict:
-        sol, cand = C.c_int(), C.c_int()
+        sol, cand, ks = C.c_int(), C.c_int(), C.c_int()
         d, b = C.c_double(), C.c_double()
-        N.check(N.lib().tm_batch_gemm_info(self.h, C.byref(sol), C.byref(cand), C.byref(d), C.byref(b)),
-                "tm_batch_gemm_info")
-        return {"solution": sol.value, "candidates": cand.value, "ms_default": d.value, "ms_tuned": b.value}
+        N.check(N.lib().tm_batch_gemm_info(self.h, C.byref(sol), C.byref(cand), C.byref(d), C.byref(b),
+                                           C.byref(ks)), "tm_batch_gemm_info")
+        return {"solution": sol.value, "candidates": cand.value, "ms_default": d.value, "ms_tuned": b.value,
+                "ksplit": ks.value}
+
+    def bem_apply(self, x):
+        """Boundary product of every sample, y[j][s] = sum_k B[j][k] x[k][s]
+        (tm_batch_bem_apply), in the batch's precision.  x: (b, S) numpy
+        array (returns numpy) or device tensor (returns a device tensor on the
+        context's stream)."""
+        b = self.ctx.prob.bnd.bnx
+        if isinstance(x, np.ndarray):
+            a = N.f64(np.asarray(x).reshape(b, self.S))
+            y = np.empty((b, self.S))
+            N.check(N.lib().tm_batch_bem_apply(self.h, N.ptr(a), N.TM_HOST, N.ptr(y), N.TM_HOST),
+                    "tm_batch_bem_apply")
+            return y
+        import torch
+        with torch.cuda.stream(self.ctx.stream):
+            y = torch.empty((b, self.S), dtype=torch.float64, device=self.ctx.device)
+        N.check(N.lib().tm_batch_bem_apply(self.h, N.ptr(x), N.TM_DEVICE, N.ptr(y), N.TM_DEVICE),
+                "tm_batch_bem_apply")
+        return y
 
     def enable_timers(self, on: bool = True):
         N.check(N.lib().tm_batch_enable_timers(self.h, int(on)), "tm_batch_enable_timers")

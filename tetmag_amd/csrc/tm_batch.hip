@@ -154,6 +154,98 @@ __device__ inline void row_entries(const CsrArgs& A, const V* __restrict__ vals,
     e.v = in ? (double)vals[q] : 0.0;
 }
 
+// Two rows' sums with all of their gathers in flight together (the walk is
+// bound by gather latency, so a wavefront keeps twice as many loads
+// outstanding).  mid() runs after the gathers are issued and before they are
+// consumed (the next rows' entry loads go there).  Rows of more than kGr
+// entries fall back to one row at a time.  An empty row (len 0) gathers
+// entry 0 of its RowEnt (column 0, a valid row) and adds nothing.
+template <typename T, typename V, typename XF, typename MID>
+__device__ inline void row_sum2(const CsrArgs& A, const V* __restrict__ vals, const RowEnt& ea, const RowEnt& eb,
+                                int lane, XF&& xf, MID&& mid, T& sa, T& sb) {
+    const int la = ea.q1 - ea.q0, lb = eb.q1 - eb.q0;
+    if (la > kGr || lb > kGr) {
+        mid();
+        sa = row_sum<T>(A, vals, ea, lane, xf);
+        sb = row_sum<T>(A, vals, eb, lane, xf);
+        return;
+    }
+    T xa[kGr], xb[kGr];
+#pragma unroll
+    for (int k = 0; k < kGr; ++k) {
+        xa[k] = xf(__builtin_amdgcn_readlane(ea.c, max(0, min(k, la - 1))));
+        xb[k] = xf(__builtin_amdgcn_readlane(eb.c, max(0, min(k, lb - 1))));
+    }
+    mid();
+    T acc_a = 0, acc_b = 0;
+#pragma unroll
+    for (int k = 0; k < kGr; ++k) {
+        const double wa = readlane_f64(ea.v, max(0, min(k, la - 1)));
+        const double wb = readlane_f64(eb.v, max(0, min(k, lb - 1)));
+        acc_a = k < la ? acc_a + (T)wa * xa[k] : acc_a;
+        acc_b = k < lb ? acc_b + (T)wb * xb[k] : acc_b;
+    }
+    sa = acc_a;
+    sb = acc_b;
+}
+
+__device__ inline void row_empty(RowEnt& e) {
+    e.q0 = e.q1 = 0;
+    e.c = 0;
+    e.v = 0.0;
+}
+
+// Row-pair loop: a wavefront takes rows (row, row + stride), then advances by
+// 2 stride.  Per pair: ld(row) loads what the row's epilogue needs, the
+// gathers of both rows are issued, then the entries of the next pair (from
+// row pointers loaded during the previous pair) and the row pointers of the
+// pair after (one vector load: lanes 0-3 hold rp[ra], rp[ra+1], rp[rb],
+// rp[rb+1], read with readlane one pair later, so no wait on them stalls the
+// gathers), then the sums are formed and epi(row, acc, ld(row)) finishes
+// each row.
+#define B_ROW_PIPE2(T_, A, VALS, NROWS, LD, XF, EPI)                                                   \
+    {                                                                                                   \
+        const int stride_ = gridDim.x * kRowsPerBlock;                                                  \
+        const int n_ = (NROWS);                                                                         \
+        int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
+        auto rp_pair_ = [&](int ra) {                                                                   \
+            const int rr = lane < 2 ? ra : ra + stride_;                                                \
+            return (lane < 4 && rr < n_) ? (A).rp[rr + (lane & 1)] : 0;                                 \
+        };                                                                                              \
+        RowEnt ca_, cb_, na_, nb_;                                                                      \
+        int rpv_ = rp_pair_(row);                                                                       \
+        if (row < n_) row_entries((A), (VALS), __builtin_amdgcn_readlane(rpv_, 0),                      \
+                                  __builtin_amdgcn_readlane(rpv_, 1), lane, na_);                       \
+        else row_empty(na_);                                                                            \
+        if (row + stride_ < n_) row_entries((A), (VALS), __builtin_amdgcn_readlane(rpv_, 2),            \
+                                            __builtin_amdgcn_readlane(rpv_, 3), lane, nb_);             \
+        else row_empty(nb_);                                                                            \
+        rpv_ = rp_pair_(row + 2 * stride_);                                                             \
+        /* the first entries and pointers arrive here, so at the loop head   \
+           no wait on them is left for the back edge to inherit */            \
+        asm volatile("" ::"v"(na_.c), "v"(na_.v), "v"(nb_.c), "v"(nb_.v), "v"(rpv_));                  \
+        for (; row < n_; row += 2 * stride_) {                                                          \
+            ca_ = na_;                                                                                  \
+            cb_ = nb_;                                                                                  \
+            const bool hb_ = row + stride_ < n_;                                                        \
+            const auto la_ = LD(row);                                                                   \
+            const auto lb_ = LD(hb_ ? row + stride_ : row);                                             \
+            auto mid_ = [&]() {                                                                         \
+                if (row + 2 * stride_ < n_) row_entries((A), (VALS), __builtin_amdgcn_readlane(rpv_, 0), \
+                                                        __builtin_amdgcn_readlane(rpv_, 1), lane, na_); \
+                else row_empty(na_);                                                                    \
+                if (row + 3 * stride_ < n_) row_entries((A), (VALS), __builtin_amdgcn_readlane(rpv_, 2), \
+                                                        __builtin_amdgcn_readlane(rpv_, 3), lane, nb_); \
+                else row_empty(nb_);                                                                    \
+                rpv_ = rp_pair_(row + 4 * stride_);                                                     \
+            };                                                                                          \
+            T_ sa_, sb_;                                                                                \
+            row_sum2<T_>((A), (VALS), ca_, cb_, lane, XF, mid_, sa_, sb_);                              \
+            EPI(row, sa_, la_);                                                                         \
+            if (hb_) EPI(row + stride_, sb_, lb_);                                                      \
+        }                                                                                               \
+    }
+
 // Row loop with look-ahead: while row i is gathered, the entries of row i+1
 // (whose pointers arrived during row i-1) and the pointers of row i+2 are in
 // flight.  body(row, e) handles one row.
@@ -341,15 +433,15 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_spmv(CsrArgs A, int S, const T*
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
     double v = 0.0;
-    auto body = [&](int row, const RowEnt& e) {
-        const T pi = p[(size_t)row * S + lc];
-        const T acc = row_sum<T>(A, A.v, e, lane, [&](int c) { return p[(size_t)c * S + lc]; });
+    auto ld = [&](int row) { return p[(size_t)row * S + lc]; };
+    auto xf = [&](int c) { return p[(size_t)c * S + lc]; };
+    auto epi = [&](int row, T acc, T pi) {
         if (on) {
             q[(size_t)row * S + lane] = acc;
             v += (double)pi * (double)acc;
         }
     };
-    B_ROW_PIPE(A, A.v, A.n, body);
+    B_ROW_PIPE2(T, A, A.v, A.n, ld, xf, epi);
     b_partial(part, S, v);
 }
 
@@ -401,25 +493,35 @@ __global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* pa
 }
 
 // --------------------------------------------------------- AMG V-cycle --
-// x = w dinv b; r = b - A x
+// aw[k] = v[k] w dinv[ci[k]]: the operator with the Jacobi step folded into
+// its columns, A w D^-1
+__global__ __launch_bounds__(kBlock) void k_b_scale_cols(size_t nnz, const int* __restrict__ ci,
+                                                         const double* __restrict__ v,
+                                                         const double* __restrict__ dinv, double w,
+                                                         double* __restrict__ aw) {
+    for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < nnz; k += (size_t)gridDim.x * kBlock)
+        aw[k] = v[k] * w * dinv[ci[k]];
+}
+
+// x = w dinv b; r = b - A x, with AW = A w D^-1 (k_b_scale_cols) in A.v
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs A, int S, const double* __restrict__ dinv,
+__global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs AW, int S, const double* __restrict__ dinv,
                                                   const T* __restrict__ b, double w, T* __restrict__ x,
                                                   T* __restrict__ r) {
+    const CsrArgs& A = AW;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
-    auto body = [&](int row, const RowEnt& e) {
-        const T bi = b[(size_t)row * S + lc];
-        const T acc = row_sum<T>(A, A.v, e, lane,
-                                 [&](int c) { return (T)(w * dinv[c]) * b[(size_t)c * S + lc]; });
+    auto ld = [&](int row) { return b[(size_t)row * S + lc]; };
+    auto xf = [&](int c) { return b[(size_t)c * S + lc]; };
+    auto epi = [&](int row, T acc, T bi) {
         if (on) {
             const size_t ee = (size_t)row * S + lane;
             x[ee] = (T)(w * dinv[row]) * bi;
             r[ee] = bi - acc;
         }
     };
-    B_ROW_PIPE(A, A.v, A.n, body);
+    B_ROW_PIPE2(T, A, A.v, A.n, ld, xf, epi);
 }
 
 // y = M x (ACC: y += M x)
@@ -428,12 +530,12 @@ __global__ __launch_bounds__(kBlock) void k_b_spmm(CsrArgs M, int S, const T* __
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
-    auto body = [&](int row, const RowEnt& e) {
-        const T yi = ACC ? y[(size_t)row * S + lc] : (T)0;
-        const T acc = row_sum<T>(M, M.v, e, lane, [&](int c) { return x[(size_t)c * S + lc]; });
+    auto ld = [&](int row) { return ACC ? y[(size_t)row * S + lc] : (T)0; };
+    auto xf = [&](int c) { return x[(size_t)c * S + lc]; };
+    auto epi = [&](int row, T acc, T yi) {
         if (on) y[(size_t)row * S + lane] = ACC ? yi + acc : acc;
     };
-    B_ROW_PIPE(M, M.v, M.n, body);
+    B_ROW_PIPE2(T, M, M.v, M.n, ld, xf, epi);
 }
 
 // xo = x + w dinv (b - A x); optional partial (b, xo)
@@ -445,17 +547,22 @@ __global__ __launch_bounds__(kBlock) void k_b_post(CsrArgs A, int S, const doubl
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
     double v = 0.0;
-    auto body = [&](int row, const RowEnt& e) {
+    struct BX {
+        T b, x;
+    };
+    auto ld = [&](int row) {
         const size_t er = (size_t)row * S + lc;
-        const T bi = b[er], xi = x[er];
-        const T acc = row_sum<T>(A, A.v, e, lane, [&](int c) { return x[(size_t)c * S + lc]; });
+        return BX{b[er], x[er]};
+    };
+    auto xf = [&](int c) { return x[(size_t)c * S + lc]; };
+    auto epi = [&](int row, T acc, const BX& bx) {
         if (on) {
-            const T zi = xi + (T)(w * dinv[row]) * (bi - acc);
+            const T zi = bx.x + (T)(w * dinv[row]) * (bx.b - acc);
             xo[(size_t)row * S + lane] = zi;
-            v += (double)bi * (double)zi;
+            v += (double)bx.b * (double)zi;
         }
     };
-    B_ROW_PIPE(A, A.v, A.n, body);
+    B_ROW_PIPE2(T, A, A.v, A.n, ld, xf, epi);
     if (part) b_partial(part, S, v);
 }
 
@@ -630,38 +737,45 @@ __global__ __launch_bounds__(kBlock) void k_b_gather(int b, int S, const int* __
                                                      const double* __restrict__ u, T* __restrict__ U) {
     const size_t tot = (size_t)b * S;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
-        U[e] = (T)u[(size_t)bn[e / S] * S + e % S];
+        U[e] = (T)u[bn ? (size_t)bn[e / S] * S + e % S : e];
 }
 
-// out[bn[j]][s] = G[j + s b]  (G = B U^T, b x S column-major)
+// out[bn[j]][s] = G[j + s b]  (G = B U^T, b x S column-major); bn null: out[j][s]
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_scatter(int b, int S, const int* __restrict__ bn,
                                                       const T* __restrict__ G, double* __restrict__ out) {
     const size_t tot = (size_t)b * S;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
-        out[(size_t)bn[e % b] * S + e / b] = (double)G[e];
+        out[(size_t)(bn ? bn[e % b] : e % b) * S + e / b] = (double)G[e];
 }
 
 // ---- FP64 MFMA GEMM for the boundary product ---------------------------
-// P[ks] (b x S, column-major) = B[:, K_ks] (b x |K_ks|, column-major, ld) *
-// U^T[K_ks, :] with U S x b column-major (ld S), S = 64 columns.  Block: 128
-// rows (4 wavefronts x 32 rows), all 64 columns, one K range ks of the
-// split; U^T chunks of kGk rows are staged in LDS.  v_mfma_f64_16x16x4f64
-// operand layout (measured): A lane l = A[l%16][l/16], B lane l =
-// B[l/16][l%16], D lane l register r = C[4r + l/16][l%16].
+// P[ks] (b x S, row-major: P[ks][row][j]) = B[:, K_ks] (b x |K_ks|,
+// column-major, ld) * U^T[K_ks, :] with U S x b column-major (ld S), S = 64
+// columns.  Block: 4 wavefronts x 16 MT rows, all 64 columns, one K range ks
+// of the split; U^T chunks of KGK rows are staged in LDS.
+// v_mfma_f64_16x16x4f64 operand layout (measured): A lane l = A[l%16][l/16],
+// B lane l = B[l/16][l%16], D lane l register r = C[4r + l/16][l%16].
+// PIPE = false: per chunk, stage U^T, then load the chunk's B values and run
+// its MFMAs.  PIPE = true: software-pipelined; while the MFMAs of chunk c
+// run, the B values of chunk c+1 are in flight into registers and its U^T
+// rows into registers, stored to the other of two LDS buffers afterwards
+// (one barrier per chunk).
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// KGK: K rows of U^T per LDS chunk; MT: 16-row tiles per wavefront (block
-// rows = 4 x 16 x MT).  The autotuner picks among the instantiations.
-template <int KGK, int MT>
+template <int KGK, int MT, bool PIPE>
 __global__ __launch_bounds__(kBlock) void k_b_gemm64(int b, int ld, const double* __restrict__ B,
                                                      const double* __restrict__ U, int kper,
                                                      double* __restrict__ P) {
-    __shared__ double us[KGK * 64];  // U^T chunk: us[k * 64 + j]
+    constexpr int NB = PIPE ? 2 : 1;
+    constexpr int UPT = KGK * 64 / kBlock;  // U^T values per thread per chunk
+    static_assert(KGK % 4 == 0 && (KGK * 64) % kBlock == 0, "chunk shape");
+    __shared__ double us[NB][KGK * 64];  // U^T chunk: us[buf][k * 64 + j]
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int m0 = (blockIdx.x * (kBlock / kWave) + wv) * 16 * MT;  // this wavefront's rows
     const int ks = blockIdx.y;
     const int kb = ks * kper, ke = min(b, kb + kper);
+    const int nc = ke > kb ? (ke - kb + KGK - 1) / KGK : 0;  // chunks, uniform over the block
     const int li = lane & 15, lk = lane >> 4;
     dbl4 acc[MT][4];
 #pragma unroll
@@ -675,32 +789,70 @@ __global__ __launch_bounds__(kBlock) void k_b_gemm64(int b, int ld, const double
         rr[t] = m0 + 16 * t + li;  // the A row of this lane in tile t
         ok[t] = rr[t] < b;
     }
-    for (int k0 = kb; k0 < ke; k0 += KGK) {
-        __syncthreads();  // the previous chunk has been consumed
-        for (int t = threadIdx.x; t < KGK * 64; t += kBlock) {
-            const int kk = k0 + t / 64;
-            us[t] = kk < ke ? U[(size_t)kk * 64 + (t & 63)] : 0.0;
-        }
-        __syncthreads();
-        // the chunk's B loads first, so they are all in flight before the
-        // MFMAs consume them
-        double a[MT][KGK / 4];
+    double a[MT][KGK / 4], an[MT][KGK / 4], ur[UPT];
+    auto load_b = [&](int k0, double (&dst)[MT][KGK / 4]) {
 #pragma unroll
         for (int q = 0; q < KGK / 4; ++q) {
             const int kc = k0 + 4 * q + lk;  // this lane's K column
             const bool kin = kc < ke;
 #pragma unroll
-            for (int t = 0; t < MT; ++t) a[t][q] = (ok[t] && kin) ? B[(size_t)kc * ld + rr[t]] : 0.0;
+            for (int t = 0; t < MT; ++t) dst[t][q] = (ok[t] && kin) ? B[(size_t)kc * ld + rr[t]] : 0.0;
         }
+    };
+    auto load_u = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < UPT; ++i) {
+            const int t = threadIdx.x + i * kBlock;
+            const int kk = k0 + t / 64;
+            ur[i] = kk < ke ? U[(size_t)kk * 64 + (t & 63)] : 0.0;
+        }
+    };
+    auto store_u = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < UPT; ++i) us[buf][threadIdx.x + i * kBlock] = ur[i];
+    };
+    auto mfmas = [&](const double (&av)[MT][KGK / 4], int buf) {
 #pragma unroll
         for (int q = 0; q < KGK / 4; ++q) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const double bu = us[(4 * q + lk) * 64 + u * 16 + li];
+                const double bu = us[buf][(4 * q + lk) * 64 + u * 16 + li];
 #pragma unroll
                 for (int t = 0; t < MT; ++t)
-                    acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][q], bu, acc[t][u], 0, 0, 0);
+                    acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t][q], bu, acc[t][u], 0, 0, 0);
             }
+        }
+    };
+    if constexpr (PIPE) {
+        if (nc > 0) {
+            load_u(kb);
+            load_b(kb, an);
+            store_u(0);
+        }
+        __syncthreads();
+        for (int c = 0; c < nc; ++c) {
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int q = 0; q < KGK / 4; ++q) a[t][q] = an[t][q];
+            const bool more = c + 1 < nc;
+            if (more) {  // next chunk's loads in flight during this chunk's MFMAs
+                load_u(kb + (c + 1) * KGK);
+                load_b(kb + (c + 1) * KGK, an);
+            }
+            mfmas(a, c & 1);
+            if (more) store_u((c + 1) & 1);  // that buffer was last read in chunk c-1
+            __syncthreads();
+        }
+    } else {
+        for (int c = 0; c < nc; ++c) {
+            const int k0 = kb + c * KGK;
+            __syncthreads();  // the previous chunk has been consumed
+            load_u(k0);
+            store_u(0);
+            __syncthreads();
+            load_b(k0, a);  // all of the chunk's B loads in flight before its MFMAs
+            mfmas(a, 0);
         }
     }
     double* out = P + (size_t)ks * b * 64;
@@ -711,48 +863,72 @@ __global__ __launch_bounds__(kBlock) void k_b_gemm64(int b, int ld, const double
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = m0 + t * 16 + 4 * r + lk;
-                if (row < b) out[(size_t)(u * 16 + li) * b + row] = acc[t][u][r];
+                if (row < b) out[(size_t)row * 64 + u * 16 + li] = acc[t][u][r];
             }
 }
 
 struct GemmShape {
     int kgk, mt;
+    bool pipe;
 };
-constexpr GemmShape kGemmShapes[] = {{64, 2}, {32, 2}, {64, 1}, {32, 1}};
-constexpr int kNumGemmShapes = 4;
+constexpr GemmShape kGemmShapes[] = {{32, 2, false}, {64, 2, false}, {32, 2, true}, {16, 2, true},
+                                     {16, 4, true},  {32, 4, true},  {8, 2, true},  {8, 4, true},
+                                     {16, 1, true},  {32, 1, true}};
+constexpr int kNumGemmShapes = sizeof(kGemmShapes) / sizeof(kGemmShapes[0]);
 
 inline int gemm_rows_per_block(int shape) { return (kBlock / kWave) * 16 * kGemmShapes[shape].mt; }
 
-void launch_gemm64(int shape, dim3 g, hipStream_t s, int b, int ld, const double* B, const double* U, int kper,
-                   double* P) {
+template <class F>
+void gemm_dispatch(int shape, F&& f) {
     switch (shape) {
-        case 0: hipLaunchKernelGGL((k_b_gemm64<64, 2>), g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); break;
-        case 1: hipLaunchKernelGGL((k_b_gemm64<32, 2>), g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); break;
-        case 2: hipLaunchKernelGGL((k_b_gemm64<64, 1>), g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); break;
-        default: hipLaunchKernelGGL((k_b_gemm64<32, 1>), g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); break;
+        case 0: f(k_b_gemm64<32, 2, false>); break;
+        case 1: f(k_b_gemm64<64, 2, false>); break;
+        case 2: f(k_b_gemm64<32, 2, true>); break;
+        case 3: f(k_b_gemm64<16, 2, true>); break;
+        case 4: f(k_b_gemm64<16, 4, true>); break;
+        case 5: f(k_b_gemm64<32, 4, true>); break;
+        case 6: f(k_b_gemm64<8, 2, true>); break;
+        case 7: f(k_b_gemm64<8, 4, true>); break;
+        case 8: f(k_b_gemm64<16, 1, true>); break;
+        default: f(k_b_gemm64<32, 1, true>); break;
     }
 }
 
-// out[bn[j]][s] = sum_ks P[ks][j + s b] (FP64 split-K partials of k_b_gemm64)
+void launch_gemm64(int shape, dim3 g, hipStream_t s, int b, int ld, const double* B, const double* U, int kper,
+                   double* P) {
+    gemm_dispatch(shape, [&](auto k) { hipLaunchKernelGGL(k, g, dim3(kBlock), 0, s, b, ld, B, U, kper, P); });
+}
+
+// resident blocks per CU of a tile shape (0 if the runtime cannot say)
+int gemm_blocks_per_cu(int shape) {
+    int nb = 0;
+    gemm_dispatch(shape, [&](auto k) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kBlock, 0) != hipSuccess) nb = 0;
+    });
+    return nb;
+}
+
+// out[bn[j]][s] = sum_ks P[ks][j][s] (FP64 split-K partials of k_b_gemm64,
+// row-major); without bn: out[j][s]
 __global__ __launch_bounds__(kBlock) void k_b_ksum_scatter(int b, int S, int nks, const int* __restrict__ bn,
                                                            const double* __restrict__ P, double* __restrict__ out) {
     const size_t tot = (size_t)b * S;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) {
         double acc = 0.0;
         for (int c = 0; c < nks; ++c) acc += P[(size_t)c * tot + e];
-        if (bn) out[(size_t)bn[e % b] * S + e / b] = acc;
-        else out[e] = acc;  // G itself (b x S)
+        out[bn ? (size_t)bn[e / S] * S + e % S : e] = acc;
     }
 }
 
-// out[bn[j]][s] = sum_c P[c][j + s b] in FP64 (the split-K FP32 partials)
+// out[bn[j]][s] = sum_c P[c][j + s b] in FP64 (the split-K FP32 partials);
+// bn null: out[j][s]
 __global__ __launch_bounds__(kBlock) void k_b_chunk_sum(int b, int S, int nch, const int* __restrict__ bn,
                                                         const float* __restrict__ P, double* __restrict__ out) {
     const size_t tot = (size_t)b * S;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) {
         double acc = 0.0;
         for (int c = 0; c < nch; ++c) acc += (double)P[(size_t)c * tot + e];
-        out[(size_t)bn[e % b] * S + e / b] = acc;
+        out[(size_t)(bn ? bn[e % b] : e % b) * S + e / b] = acc;
     }
 }
 
@@ -829,6 +1005,7 @@ struct BatchSolver {
         DevBuf<T> x, x2, r, b;
     };
     std::vector<Lv> lv;
+    std::vector<DevBuf<double>> aw;  // per level: A w D^-1 values (pre-smoother)
     DevBuf<T> xc;
     DevBuf<double> part;          // 4 slots x S x G
     DevBuf<OState> ost;
@@ -867,6 +1044,21 @@ struct BatchSolver {
             TM_CHECK(lv[l].r.alloc((size_t)L.n * S));
             TM_CHECK(lv[l].b.alloc((size_t)L.R.n * S));
         }
+        aw.clear();
+        aw.resize(solver.amg.levels.size());
+        for (size_t l = 0; l < solver.amg.levels.size(); ++l) {
+            const auto& L = solver.amg.levels[l];
+            const size_t nnz = l == 0 ? (size_t)solver.pat.h_rowptr[n] : (size_t)L.nnz_a;
+            const int* ci = l == 0 ? A.ci.p : L.A.ci.p;
+            const double* v = l == 0 ? A.v.p : L.A.v.p;
+            const double* di = l == 0 ? dinv.p : L.dinv.p;
+            TM_CHECK(aw[l].alloc(std::max<size_t>(nnz, 1)));
+            if (nnz > 0) {
+                hipLaunchKernelGGL(k_b_scale_cols, dim3(elem_grid(nnz)), dim3(kBlock), 0, s, nnz, ci, v, di,
+                                   L.omega, aw[l].p);
+                TM_HIP(hipGetLastError());
+            }
+        }
         TM_CHECK(xc.alloc((size_t)std::max(1, solver.amg.nc) * S));
         TM_CHECK(part.alloc((size_t)4 * S * G));
         TM_CHECK(ost.alloc(1));
@@ -890,11 +1082,12 @@ struct BatchSolver {
         for (int l = 0; l < L; ++l) {
             const auto& V = M.levels[l];
             if (l == 0)
-                hipLaunchKernelGGL((k_b_pre<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, dinv.p, b, V.omega,
-                                   lv[0].x.p, lv[0].r.p);
+                hipLaunchKernelGGL((k_b_pre<T>), dim3(G), dim3(kBlock), 0, s, CsrArgs{A.n, A.rp.p, A.ci.p, aw[0].p},
+                                   S, dinv.p, b, V.omega, lv[0].x.p, lv[0].r.p);
             else
-                hipLaunchKernelGGL((k_b_pre<T>), dim3(row_grid(V.n)), dim3(kBlock), 0, s, args(V.A), S, V.dinv.p, b,
-                                   V.omega, lv[l].x.p, lv[l].r.p);
+                hipLaunchKernelGGL((k_b_pre<T>), dim3(row_grid(V.n)), dim3(kBlock), 0, s,
+                                   CsrArgs{V.A.n, V.A.rp.p, V.A.ci.p, aw[l].p}, S, V.dinv.p, b, V.omega, lv[l].x.p,
+                                   lv[l].r.p);
             hipLaunchKernelGGL((k_b_spmm<T, false>), dim3(row_grid(V.R.n)), dim3(kBlock), 0, s, args(V.R), S,
                                (const T*)lv[l].r.p, lv[l].b.p);
             b = lv[l].b.p;
@@ -1014,6 +1207,7 @@ struct tm_batch {
     int gemm_sol = 0;                  // tuned rocBLAS solution index (0: library default)
     bool gemm_own = false;             // the MFMA kernel k_b_gemm64 won the tuning
     int gemm_shape = 0, gemm_ks = 1;   // its tile shape and K split
+    int gemm_own_candidates = 0;       // (shape, K split) pairs timed
     DevBuf<double> Pd;                 // its split-K partials (ks x b x S)
     double gemm_own_err = 0.0;         // its largest relative difference to rocBLAS at tuning
     int gemm_candidates = 0;
@@ -1104,10 +1298,20 @@ __global__ __launch_bounds__(kBlock) void k_b_fill_pattern(size_t n, double* __r
         x[e] = sin(0.37 * (double)(e % 100003)) + 0.25;
 }
 
-int gemm_own_ks(int b, int shape) {
+// K splits worth timing for a tile shape: the one that gives ~1024 blocks,
+// and those that fill j = 1..6 rounds of the resident block slots (all
+// blocks carry the same work, so a last round that is only partly filled
+// leaves CUs idle).  At most 16 partial products.
+std::vector<int> gemm_ks_candidates(int b, int shape, int ncu) {
     const int rows = gemm_rows_per_block(shape);
     const int grid_m = (b + rows - 1) / rows;
-    return std::max(1, std::min(16, (1024 + grid_m - 1) / grid_m));
+    std::vector<int> ks = {std::max(1, std::min(16, (1024 + grid_m - 1) / grid_m))};
+    const int slots = ncu * gemm_blocks_per_cu(shape);
+    for (int j = 1; j <= 6 && slots > 0; ++j) {
+        const int k = std::max(1, std::min(16, j * slots / grid_m));
+        if (std::find(ks.begin(), ks.end(), k) == ks.end()) ks.push_back(k);
+    }
+    return ks;
 }
 
 int gemm_own_run(tm_batch* bt, double* G) {
@@ -1118,7 +1322,7 @@ int gemm_own_run(tm_batch* bt, double* G) {
     launch_gemm64(bt->gemm_shape, dim3((b + rows - 1) / rows, bt->gemm_ks), bt->stream, b, v.bld, v.B,
                   (const double*)bt->U.p, kper, bt->Pd.p);
     TM_HIP(hipGetLastError());
-    if (G) {  // sum the partials into G (b x S) without scatter: identity map through a stride-1 index
+    if (G) {  // sum the partials into G (b x S, row-major) without scatter
         hipLaunchKernelGGL(k_b_ksum_scatter, dim3(elem_grid((size_t)b * bt->S)), dim3(kBlock), 0, bt->stream, b,
                            bt->S, bt->gemm_ks, (const int*)nullptr, (const double*)bt->Pd.p, G);
         TM_HIP(hipGetLastError());
@@ -1179,11 +1383,20 @@ int tune_gemm(tm_batch* bt) {
     }
     (void)hipGetLastError();  // a rejected candidate leaves no error behind
     bt->gemm_candidates = cap;
-    // the MFMA kernel, each tile shape: checked against the library on a
-    // non-zero U, then timed
+    // the MFMA kernel, each tile shape and K split: checked against the
+    // library on a non-zero U, then timed
     if (!f32 && S == 64) {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 0;
+        std::vector<std::pair<int, int>> cand;  // (shape, ks)
         int ksmax = 1;
-        for (int sh = 0; sh < kNumGemmShapes; ++sh) ksmax = std::max(ksmax, gemm_own_ks(b, sh));
+        for (int sh = 0; sh < kNumGemmShapes; ++sh)
+            for (int k : gemm_ks_candidates(b, sh, ncu)) {
+                cand.push_back({sh, k});
+                ksmax = std::max(ksmax, k);
+            }
         if (bt->Pd.alloc((size_t)ksmax * b * S) == TM_OK) {
             hipLaunchKernelGGL(k_b_fill_pattern, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, bt->stream,
                                (size_t)b * S, bt->U.p);
@@ -1197,16 +1410,17 @@ int tune_gemm(tm_batch* bt) {
             for (size_t e = 0; ok && e < ref.size(); ++e) rmax = std::max(rmax, std::fabs(ref[e]));
             const int keep_shape = bt->gemm_shape, keep_ks = bt->gemm_ks;
             int best_shape = -1, best_ks = 1;
-            for (int sh = 0; ok && sh < kNumGemmShapes; ++sh) {
-                bt->gemm_shape = sh;
-                bt->gemm_ks = gemm_own_ks(b, sh);
+            for (size_t ci = 0; ok && ci < cand.size(); ++ci) {
+                bt->gemm_shape = cand[ci].first;
+                bt->gemm_ks = cand[ci].second;
                 if (gemm_own_run(bt, G2.p) != TM_OK ||
                     hipMemcpyAsync(own.data(), G2.p, own.size() * sizeof(double), hipMemcpyDeviceToHost,
                                    bt->stream) != hipSuccess ||
                     hipStreamSynchronize(bt->stream) != hipSuccess)
                     break;
                 double dmax = 0.0;
-                for (size_t e = 0; e < ref.size(); ++e) dmax = std::max(dmax, std::fabs(own[e] - ref[e]));
+                for (size_t e = 0; e < ref.size(); ++e)  // own: b x S row-major; ref: column-major
+                    dmax = std::max(dmax, std::fabs(own[e] - ref[(e % S) * b + e / S]));
                 const double err = rmax > 0 ? dmax / rmax : dmax;
                 bt->gemm_own_err = std::max(bt->gemm_own_err, err);
                 if (err > 1e-12) continue;  // never choose a kernel that disagrees with the library
@@ -1218,11 +1432,12 @@ int tune_gemm(tm_batch* bt) {
                     ms *= 0.5f;
                     if (ms < best) {
                         best = ms;
-                        best_shape = sh;
+                        best_shape = bt->gemm_shape;
                         best_ks = bt->gemm_ks;
                     }
                 }
             }
+            bt->gemm_own_candidates = (int)cand.size();
             if (best_shape >= 0) {
                 bt->gemm_own = true;
                 bt->gemm_shape = best_shape;
@@ -1237,6 +1452,38 @@ int tune_gemm(tm_batch* bt) {
     bt->gemm_ms_best = best;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    return TM_OK;
+}
+
+// out[bn[j]][s] = sum_k B[j][k] u[bn[k]][s] for every sample (bn null: the
+// boundary values themselves, out[j][s] and u[k][s]), in the batch's
+// precision (DemagField.cpp:145-147 for one vector).  Column-major G (b x S)
+// = B (b x b, streamed in its storage order) * U^T, U = gathered boundary
+// values (S x b).  FP64, except for TM_PREC_MIXED_BEM32: an FP32 copy of B,
+// FP32 products accumulated in FP32 over 512-column chunks of B only, the
+// chunk results summed in FP64 (FP32 accumulation over all 41k columns costs
+// ~1e-5 relative field error).
+int boundary_product(tm_batch* bt, const int* bn, const double* u, double* out) {
+    const CtxView& v = bt->cv;
+    const int b = v.bnx, S = bt->S;
+    hipStream_t s = bt->stream;
+    const dim3 g(elem_grid((size_t)b * S));
+    if (bt->prec == TM_PREC_MIXED_BEM32) {
+        hipLaunchKernelGGL(k_b_gather<float>, g, dim3(kBlock), 0, s, b, S, bn, u, bt->Uf.p);
+        TM_CHECK(gemm_split_f32(bt));
+        hipLaunchKernelGGL(k_b_chunk_sum, g, dim3(kBlock), 0, s, b, S, bt->nchunk, bn, (const float*)bt->Pf.p, out);
+    } else {
+        hipLaunchKernelGGL(k_b_gather<double>, g, dim3(kBlock), 0, s, b, S, bn, u, bt->U.p);
+        if (bt->gemm_own) {
+            TM_CHECK(gemm_own_run(bt, nullptr));
+            hipLaunchKernelGGL(k_b_ksum_scatter, g, dim3(kBlock), 0, s, b, S, bt->gemm_ks, bn,
+                               (const double*)bt->Pd.p, out);
+        } else {
+            TM_CHECK(gemm_run(bt, bt->gemm_sol));
+            hipLaunchKernelGGL(k_b_scatter<double>, g, dim3(kBlock), 0, s, b, S, bn, (const double*)bt->Gm.p, out);
+        }
+    }
+    TM_HIP(hipGetLastError());
     return TM_OK;
 }
 
@@ -1421,34 +1668,9 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
                                        : bt->sN32.solve(bt->rhsN.p, bt->u1.p, v.tol, v.maxit, s, &oN, &iN, &rN);
     if (stN != TM_OK && stN != TM_ERR_MAXITER) return stN;
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[2], s));
-    // boundary integral (DemagField.cpp:184-201) as a GEMM: G = B U
-    const int b = v.bnx;
-    // column-major G (b x S) = B (b x b, streamed in its storage order) * U^T,
-    // U = gathered boundary potentials (S x b).  FP64, except for
-    // TM_PREC_MIXED_BEM32: an FP32 copy of B, FP32 products accumulated in
-    // FP32 over 512-column chunks of B only, the chunk results summed in FP64
-    // (FP32 accumulation over all 41k columns costs ~1e-5 relative field error)
-    if (bt->prec == TM_PREC_MIXED_BEM32) {
-        hipLaunchKernelGGL(k_b_gather<float>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
-                           (const double*)bt->u1.p, bt->Uf.p);
-        const float one = 1.0f, zero = 0.0f;
-        TM_CHECK(gemm_split_f32(bt));
-        hipLaunchKernelGGL(k_b_chunk_sum, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, bt->nchunk,
-                           v.bnodes, (const float*)bt->Pf.p, bt->rhsD.p);
-    } else {
-        hipLaunchKernelGGL(k_b_gather<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S, v.bnodes,
-                           (const double*)bt->u1.p, bt->U.p);
-        const double one = 1.0, zero = 0.0;
-        if (bt->gemm_own) {
-            TM_CHECK(gemm_own_run(bt, nullptr));
-            hipLaunchKernelGGL(k_b_ksum_scatter, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S,
-                               bt->gemm_ks, v.bnodes, (const double*)bt->Pd.p, bt->rhsD.p);
-        } else {
-            TM_CHECK(gemm_run(bt, bt->gemm_sol));
-            hipLaunchKernelGGL(k_b_scatter<double>, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, s, b, S,
-                               v.bnodes, (const double*)bt->Gm.p, bt->rhsD.p);
-        }
-    }
+    // boundary integral (DemagField.cpp:184-201) as a GEMM: G = B U^T, U the
+    // gathered boundary potentials of every sample
+    TM_CHECK(boundary_product(bt, v.bnodes, bt->u1.p, bt->rhsD.p));
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[3], s));
     int stD = bt->prec == TM_PREC_FP64 ? bt->sD64.solve(bt->rhsD.p, bt->u2.p, v.tol, v.maxit, s, &oD, &iD, &rD)
                                        : bt->sD32.solve(bt->rhsD.p, bt->u2.p, v.tol, v.maxit, s, &oD, &iD, &rD);
@@ -1546,12 +1768,29 @@ int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int va
     return status;
 }
 
-int tm_batch_gemm_info(tm_batch* bt, int* solution, int* candidates, double* ms_default, double* ms_tuned) {
+int tm_batch_gemm_info(tm_batch* bt, int* solution, int* candidates, double* ms_default, double* ms_tuned,
+                       int* ksplit) {
     if (!bt) return TM_ERR_BAD_ARG;
     if (solution) *solution = bt->gemm_own ? -1 - bt->gemm_shape : bt->gemm_sol;
-    if (candidates) *candidates = bt->gemm_candidates;
+    if (candidates) *candidates = bt->gemm_candidates + bt->gemm_own_candidates;
+    if (ksplit) *ksplit = bt->gemm_own ? bt->gemm_ks : 1;
     if (ms_default) *ms_default = bt->gemm_ms_default;
     if (ms_tuned) *ms_tuned = bt->gemm_ms_best;
+    return TM_OK;
+}
+
+int tm_batch_bem_apply(tm_batch* bt, const double* x, int xwhere, double* y, int ywhere) {
+    if (!bt || !x || !y || (xwhere != TM_HOST && xwhere != TM_DEVICE) || (ywhere != TM_HOST && ywhere != TM_DEVICE))
+        return TM_ERR_BAD_ARG;
+    const size_t bs = (size_t)bt->cv.bnx * bt->S;
+    const double* xd = x;
+    if (xwhere == TM_HOST) {  // staged in rhsN (n x S >= b x S)
+        TM_CHECK(copy_in(bt->rhsN.p, x, TM_HOST, bs, bt->stream));
+        xd = bt->rhsN.p;
+    }
+    double* yd = ywhere == TM_HOST ? bt->rhsD.p : y;
+    TM_CHECK(boundary_product(bt, nullptr, xd, yd));
+    if (ywhere == TM_HOST) TM_CHECK(copy_out(y, TM_HOST, yd, bs, bt->stream));
     return TM_OK;
 }
 
