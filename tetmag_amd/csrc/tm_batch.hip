@@ -83,193 +83,133 @@ struct OState {
     for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
          row < (NROWS); row += gridDim.x * kRowsPerBlock)
 
-// ---- pipelined row walk for the solver kernels ----------------------------
-// A row's entries sit one per lane (lane k = entry k, one vector load for
-// indices and one for values); gather addresses are read from those lanes
-// with readlane (wave-uniform), so a chunk of kGr gathers issues back to back.
-// The next row's pointers and entries are fetched before the current row's
-// gathers, so a wavefront has both in flight.  Rows longer than 64 entries
-// are walked in 64-entry segments.
-constexpr int kGr = 16;
+// ---- slot layout for the solver kernels ----------------------------------
+// Every operator the batched solves apply is stored in row slots of kSlot
+// entries: row i owns slots sp[i] .. sp[i+1]-1 (a multiple of kSlot, padded
+// with offset 0 and weight 0), each slot holding the byte offset of the
+// gathered row (column * S * sizeof(element)) and the weight.  A wavefront
+// reads a chunk's offsets and weights with scalar loads (16 consecutive
+// dwords each: wavefront-uniform data goes to SGPRs, no lane broadcasts),
+// gathers with buffer loads whose row offset is the SGPR operand (no address
+// arithmetic per entry) and accumulates with the weight as the SGPR operand
+// of the FMA.  Per entry that is one load and one vector instruction; the
+// earlier form (indices and values in lanes, read with readlane, 64-bit
+// address adds) was bound by vector-instruction issue.
+constexpr int kSlot = 16;
 
-struct RowEnt {
-    int q0, q1;  // row extent (wave-uniform)
-    int c;       // column of entry q0 + lane (lanes past the end: 0)
-    double v;    // value of entry q0 + lane (lanes past the end: 0)
+struct SlotArgs {
+    int n;            // rows
+    int xbytes;       // bytes of the gathered vector (ncols * S * element)
+    const int* sp;    // n + 1 slot starts
+    const int* off;   // per slot: byte offset of the gathered row
+    const void* w;    // per slot: weight (float or double)
 };
 
-__device__ inline double readlane_f64(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+typedef const int __attribute__((address_space(4)))* cint_p;
+template <typename W>
+using cw_p = const W __attribute__((address_space(4)))*;
+
+template <typename T>
+__device__ inline T buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ inline float buf_ld<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ inline double buf_ld<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
-// acc = sum_k v_k xf(c_k) over one segment held in e (len <= 64 entries),
-// in entry order.
-template <typename T, typename XF>
-__device__ inline T seg_sum(const RowEnt& e, int len, XF&& xf) {
-    T acc = 0;
-    for (int k0 = 0; k0 < len; k0 += kGr) {
-        T xv[kGr];
-        double w[kGr];
+// raw buffer over the gathered vector (word 3: 32-bit data format, gfx9)
+__device__ inline __amdgpu_buffer_rsrc_t gather_rsrc(const void* x, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(x), (short)0, bytes, 0x00020000);
+}
+
+// acc += sum over the kSlot slots at q of w * x[row]
+template <typename T, typename W>
+__device__ inline T slot_chunk(cint_p off, cw_p<W> w, int q, __amdgpu_buffer_rsrc_t xr, int voff, T acc) {
+    int o[kSlot];
+    T xv[kSlot];
 #pragma unroll
-        for (int k = 0; k < kGr; ++k) {
-            const int kk = min(k0 + k, len - 1);
-            const int c = __builtin_amdgcn_readlane(e.c, kk);
-            w[k] = readlane_f64(e.v, kk);
-            xv[k] = xf(c);
+    for (int k = 0; k < kSlot; ++k) o[k] = off[q + k];
+#pragma unroll
+    for (int k = 0; k < kSlot; ++k) xv[k] = buf_ld<T>(xr, voff, o[k]);
+#pragma unroll
+    for (int k = 0; k < kSlot; ++k) acc += (T)w[q + k] * xv[k];
+    return acc;
+}
+
+// rows a wavefront walks at a time: two in FP32 (2 x 32 SGPRs of offsets and
+// weights), one in FP64 (the weights take twice the SGPRs)
+template <typename T>
+constexpr int kRowsPerStep = sizeof(T) == 4 ? 2 : 1;
+
+// Row walk: wavefront w of block b takes rows r0 + (b/8)*4 + w, then
+// + 4*G/8, ... within its XCD's eighth [r0, r1); with RPI = 2 it takes two
+// rows per step and issues both rows' first-chunk gathers before consuming
+// any (twice the loads in flight).
+// ld(row) loads what the row's epilogue needs (issued with the gathers);
+// epi(row, acc, ld(row)) finishes the row.  Lanes >= S gather lane S-1's
+// element (lc) and their results are discarded by epi.
+template <typename T, typename W, int RPI, typename LDF, typename EPIF>
+__device__ inline void slot_walk(const SlotArgs& M, const void* x, int lc, LDF&& ld, EPIF&& epi) {
+    const int n = M.n;
+    const __amdgpu_buffer_rsrc_t xr = gather_rsrc(x, M.xbytes);
+    const cint_p sp = (cint_p)M.sp;
+    const cint_p of = (cint_p)M.off;
+    const cw_p<W> w = (cw_p<W>)M.w;
+    const int vo = lc * (int)sizeof(T);
+    // each XCD (blocks b with equal b % 8) walks its own contiguous eighth of
+    // the rows, so neighbouring rows share that XCD's L2
+    int r0 = 0, r1 = n, lb = blockIdx.x, gx = gridDim.x;
+    if (gridDim.x % kXcd == 0) {
+        const int x = blockIdx.x % kXcd;
+        gx = gridDim.x / kXcd;
+        lb = blockIdx.x / kXcd;
+        r0 = (int)((int64_t)n * x / kXcd);
+        r1 = (int)((int64_t)n * (x + 1) / kXcd);
+    }
+    const int stride = gx * kRowsPerBlock;
+    for (int row = __builtin_amdgcn_readfirstlane(r0 + lb * kRowsPerBlock + (int)(threadIdx.x / kWave)); row < r1;
+         row += RPI * stride) {
+        if constexpr (RPI == 2) {
+            const int rb = row + stride;
+            const bool hb = rb < r1;
+            const int a0 = sp[row], a1 = sp[row + 1];
+            const int b0 = hb ? sp[rb] : a0, b1 = hb ? sp[rb + 1] : a0 + kSlot;
+            const auto la = ld(row);
+            const auto lb = ld(hb ? rb : row);
+            int oa[kSlot], ob[kSlot];
+            T xa[kSlot], xb[kSlot];
+#pragma unroll
+            for (int k = 0; k < kSlot; ++k) {
+                oa[k] = of[a0 + k];
+                ob[k] = of[b0 + k];
+            }
+#pragma unroll
+            for (int k = 0; k < kSlot; ++k) {
+                xa[k] = buf_ld<T>(xr, vo, oa[k]);
+                xb[k] = buf_ld<T>(xr, vo, ob[k]);
+            }
+            T sa = 0, sb = 0;
+#pragma unroll
+            for (int k = 0; k < kSlot; ++k) {
+                sa += (T)w[a0 + k] * xa[k];
+                sb += (T)w[b0 + k] * xb[k];
+            }
+            for (int q = a0 + kSlot; q < a1; q += kSlot) sa = slot_chunk<T, W>(of, w, q, xr, vo, sa);
+            for (int q = b0 + kSlot; q < b1; q += kSlot) sb = slot_chunk<T, W>(of, w, q, xr, vo, sb);
+            epi(row, sa, la);
+            if (hb) epi(rb, sb, lb);
+        } else {
+            const int a0 = sp[row], a1 = sp[row + 1];
+            const auto la = ld(row);
+            T sa = 0;
+            for (int q = a0; q < a1; q += kSlot) sa = slot_chunk<T, W>(of, w, q, xr, vo, sa);
+            epi(row, sa, la);
         }
-#pragma unroll
-        for (int k = 0; k < kGr; ++k) acc = k0 + k < len ? acc + (T)w[k] * xv[k] : acc;
     }
-    return acc;
 }
-
-template <typename T, typename V, typename XF>
-__device__ inline T row_sum(const CsrArgs& A, const V* __restrict__ vals, const RowEnt& e, int lane, XF&& xf) {
-    const int len = e.q1 - e.q0;
-    if (len <= kWave) return seg_sum<T>(e, len, xf);
-    T acc = 0;
-    for (int q = e.q0; q < e.q1; q += kWave) {  // long rows: 64-entry segments
-        RowEnt s;
-        s.q0 = q;
-        s.q1 = min(e.q1, q + kWave);
-        const bool in = q + lane < s.q1;
-        s.c = in ? A.ci[q + lane] : 0;
-        s.v = in ? (double)vals[q + lane] : 0.0;
-        acc += seg_sum<T>(s, s.q1 - s.q0, xf);
-    }
-    return acc;
-}
-
-template <typename V>
-__device__ inline void row_entries(const CsrArgs& A, const V* __restrict__ vals, int q0, int q1, int lane,
-                                   RowEnt& e) {
-    e.q0 = q0;
-    e.q1 = q1;
-    const int q = q0 + lane;
-    const bool in = q < q1;
-    e.c = in ? A.ci[q] : 0;
-    e.v = in ? (double)vals[q] : 0.0;
-}
-
-// Two rows' sums with all of their gathers in flight together (the walk is
-// bound by gather latency, so a wavefront keeps twice as many loads
-// outstanding).  mid() runs after the gathers are issued and before they are
-// consumed (the next rows' entry loads go there).  Rows of more than kGr
-// entries fall back to one row at a time.  An empty row (len 0) gathers
-// entry 0 of its RowEnt (column 0, a valid row) and adds nothing.
-template <typename T, typename V, typename XF, typename MID>
-__device__ inline void row_sum2(const CsrArgs& A, const V* __restrict__ vals, const RowEnt& ea, const RowEnt& eb,
-                                int lane, XF&& xf, MID&& mid, T& sa, T& sb) {
-    const int la = ea.q1 - ea.q0, lb = eb.q1 - eb.q0;
-    if (la > kGr || lb > kGr) {
-        mid();
-        sa = row_sum<T>(A, vals, ea, lane, xf);
-        sb = row_sum<T>(A, vals, eb, lane, xf);
-        return;
-    }
-    T xa[kGr], xb[kGr];
-#pragma unroll
-    for (int k = 0; k < kGr; ++k) {
-        xa[k] = xf(__builtin_amdgcn_readlane(ea.c, max(0, min(k, la - 1))));
-        xb[k] = xf(__builtin_amdgcn_readlane(eb.c, max(0, min(k, lb - 1))));
-    }
-    mid();
-    T acc_a = 0, acc_b = 0;
-#pragma unroll
-    for (int k = 0; k < kGr; ++k) {
-        const double wa = readlane_f64(ea.v, max(0, min(k, la - 1)));
-        const double wb = readlane_f64(eb.v, max(0, min(k, lb - 1)));
-        acc_a = k < la ? acc_a + (T)wa * xa[k] : acc_a;
-        acc_b = k < lb ? acc_b + (T)wb * xb[k] : acc_b;
-    }
-    sa = acc_a;
-    sb = acc_b;
-}
-
-__device__ inline void row_empty(RowEnt& e) {
-    e.q0 = e.q1 = 0;
-    e.c = 0;
-    e.v = 0.0;
-}
-
-// Row-pair loop: a wavefront takes rows (row, row + stride), then advances by
-// 2 stride.  Per pair: ld(row) loads what the row's epilogue needs, the
-// gathers of both rows are issued, then the entries of the next pair (from
-// row pointers loaded during the previous pair) and the row pointers of the
-// pair after (one vector load: lanes 0-3 hold rp[ra], rp[ra+1], rp[rb],
-// rp[rb+1], read with readlane one pair later, so no wait on them stalls the
-// gathers), then the sums are formed and epi(row, acc, ld(row)) finishes
-// each row.
-#define B_ROW_PIPE2(T_, A, VALS, NROWS, LD, XF, EPI)                                                   \
-    {                                                                                                   \
-        const int stride_ = gridDim.x * kRowsPerBlock;                                                  \
-        const int n_ = (NROWS);                                                                         \
-        int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
-        auto rp_pair_ = [&](int ra) {                                                                   \
-            const int rr = lane < 2 ? ra : ra + stride_;                                                \
-            return (lane < 4 && rr < n_) ? (A).rp[rr + (lane & 1)] : 0;                                 \
-        };                                                                                              \
-        RowEnt ca_, cb_, na_, nb_;                                                                      \
-        int rpv_ = rp_pair_(row);                                                                       \
-        if (row < n_) row_entries((A), (VALS), __builtin_amdgcn_readlane(rpv_, 0),                      \
-                                  __builtin_amdgcn_readlane(rpv_, 1), lane, na_);                       \
-        else row_empty(na_);                                                                            \
-        if (row + stride_ < n_) row_entries((A), (VALS), __builtin_amdgcn_readlane(rpv_, 2),            \
-                                            __builtin_amdgcn_readlane(rpv_, 3), lane, nb_);             \
-        else row_empty(nb_);                                                                            \
-        rpv_ = rp_pair_(row + 2 * stride_);                                                             \
-        /* the first entries and pointers arrive here, so at the loop head   \
-           no wait on them is left for the back edge to inherit */            \
-        asm volatile("" ::"v"(na_.c), "v"(na_.v), "v"(nb_.c), "v"(nb_.v), "v"(rpv_));                  \
-        for (; row < n_; row += 2 * stride_) {                                                          \
-            ca_ = na_;                                                                                  \
-            cb_ = nb_;                                                                                  \
-            const bool hb_ = row + stride_ < n_;                                                        \
-            const auto la_ = LD(row);                                                                   \
-            const auto lb_ = LD(hb_ ? row + stride_ : row);                                             \
-            auto mid_ = [&]() {                                                                         \
-                if (row + 2 * stride_ < n_) row_entries((A), (VALS), __builtin_amdgcn_readlane(rpv_, 0), \
-                                                        __builtin_amdgcn_readlane(rpv_, 1), lane, na_); \
-                else row_empty(na_);                                                                    \
-                if (row + 3 * stride_ < n_) row_entries((A), (VALS), __builtin_amdgcn_readlane(rpv_, 2), \
-                                                        __builtin_amdgcn_readlane(rpv_, 3), lane, nb_); \
-                else row_empty(nb_);                                                                    \
-                rpv_ = rp_pair_(row + 4 * stride_);                                                     \
-            };                                                                                          \
-            T_ sa_, sb_;                                                                                \
-            row_sum2<T_>((A), (VALS), ca_, cb_, lane, XF, mid_, sa_, sb_);                              \
-            EPI(row, sa_, la_);                                                                         \
-            if (hb_) EPI(row + stride_, sb_, lb_);                                                      \
-        }                                                                                               \
-    }
-
-// Row loop with look-ahead: while row i is gathered, the entries of row i+1
-// (whose pointers arrived during row i-1) and the pointers of row i+2 are in
-// flight.  body(row, e) handles one row.
-#define B_ROW_PIPE(A, VALS, NROWS, BODY)                                                                \
-    {                                                                                                   \
-        const int stride_ = gridDim.x * kRowsPerBlock;                                                  \
-        int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
-        RowEnt cur_, nxt_;                                                                              \
-        int p0_ = 0, p1_ = 0;                                                                           \
-        if (row < (NROWS)) row_entries((A), (VALS), (A).rp[row], (A).rp[row + 1], lane, cur_);          \
-        if (row + stride_ < (NROWS)) {                                                                  \
-            p0_ = (A).rp[row + stride_];                                                                \
-            p1_ = (A).rp[row + stride_ + 1];                                                            \
-        }                                                                                               \
-        for (; row < (NROWS); row += stride_) {                                                         \
-            if (row + stride_ < (NROWS)) row_entries((A), (VALS), p0_, p1_, lane, nxt_);                \
-            if (row + 2 * stride_ < (NROWS)) {                                                          \
-                p0_ = (A).rp[row + 2 * stride_];                                                        \
-                p1_ = (A).rp[row + 2 * stride_ + 1];                                                    \
-            }                                                                                           \
-            BODY(row, cur_);                                                                            \
-            cur_ = nxt_;                                                                                \
-        }                                                                                               \
-    }
 
 // Row entries are walked in chunks of kCh: the chunk's column indices first,
 // then kCh independent gathers, then the multiply-adds in the original entry
@@ -291,25 +231,135 @@ __device__ inline void chunk_vals(const V* __restrict__ v, int q, int q1, double
     for (int k = 0; k < CH; ++k) w[k] = (double)v[q + k < q1 ? q + k : q];
 }
 
-// acc = sum_q v[q] x[ci[q]][lc] over the row (lanes >= S read lane S-1's
-// column and their results are discarded)
-template <typename T, typename V>
-__device__ inline T row_dot(const CsrArgs& A, const V* __restrict__ v, int row, int S, int lc,
-                            const T* __restrict__ x) {
-    const int q0 = A.rp[row], q1 = A.rp[row + 1];
-    T acc = 0;
-    for (int q = q0; q < q1; q += kCh) {
-        int c[kCh];
-        double w[kCh];
-        chunk_cols<kCh>(A.ci, q, q1, c);
-        chunk_vals<kCh>(v, q, q1, w);
-        T xv[kCh];
-#pragma unroll
-        for (int k = 0; k < kCh; ++k) xv[k] = x[(size_t)c[k] * S + lc];
-#pragma unroll
-        for (int k = 0; k < kCh; ++k) acc = q + k < q1 ? acc + (T)w[k] * xv[k] : acc;
+// ---- wide gathers for 64 FP32 samples -------------------------------------
+// With S = 64 in FP32 a node's row of samples is 256 B.  Lanes gather 16 B
+// each (4 samples), so 16 lanes cover a row and one load instruction gathers
+// four different rows: lane group g = lane / 16 takes slots 4g .. 4g+3 of
+// each 16-slot chunk (their offsets and weights arrive as one 16-byte load
+// each), lane j = lane % 16 holds samples 4j .. 4j+3.  The groups' partial
+// sums are added across lanes at the end of the row.  Against one dword
+// gather per slot this issues a quarter of the vector-memory instructions
+// (the per-instruction address work of the texture path bounds the 64-lane
+// dword form).
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f4 buf_ld4(__amdgpu_buffer_rsrc_t r, int voff) {
+    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+}
+
+__device__ inline f4 shfl_xor4(f4 v, int m) {
+    return f4{__shfl_xor(v.x, m), __shfl_xor(v.y, m), __shfl_xor(v.z, m), __shfl_xor(v.w, m)};
+}
+
+// Row walk as slot_walk (XCD-contiguous eighths, one row per wavefront per
+// step).  PIPE (for operators of long rows) software-pipelines the chunks:
+// while a chunk's gathers are in flight, the
+// offsets and weights of the next chunk (of this row, or the first of the
+// next row, whose slot range is read one row ahead) are loaded, and at a row's
+// last chunk the next row's epilogue operands.  ld(row, j) loads the
+// epilogue's operands for samples 4j .. 4j+3; epi(row, j, acc, ld) runs on
+// lanes 0-15.
+template <bool PIPE, typename LDF, typename EPIF>
+__device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPIF&& epi) {
+    const int n = M.n;
+    const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, j = lane & 15;
+    const __amdgpu_buffer_rsrc_t xr = gather_rsrc(x, M.xbytes);
+    const i4* __restrict__ of = (const i4*)M.off;
+    const f4* __restrict__ w = (const f4*)M.w;
+    const cint_p sp = (cint_p)M.sp;
+    const int vo = j * 16;
+    int r0 = 0, r1 = n, lb = blockIdx.x, gx = gridDim.x;
+    if (gridDim.x % kXcd == 0) {
+        const int xi = blockIdx.x % kXcd;
+        gx = gridDim.x / kXcd;
+        lb = blockIdx.x / kXcd;
+        r0 = (int)((int64_t)n * xi / kXcd);
+        r1 = (int)((int64_t)n * (xi + 1) / kXcd);
     }
-    return acc;
+    const int stride = gx * kRowsPerBlock;
+    int row = __builtin_amdgcn_readfirstlane(r0 + lb * kRowsPerBlock + (int)(threadIdx.x / kWave));
+    if constexpr (!PIPE) {
+        for (; row < r1; row += stride) {
+            const int a0 = sp[row], a1 = sp[row + 1];
+            const auto lv = ld(row, j);
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int q = a0; q < a1; q += kSlot) {
+                const i4 o = of[(q >> 2) + g];
+                const f4 ww = w[(q >> 2) + g];
+                const f4 x0 = buf_ld4(xr, vo + o.x), x1 = buf_ld4(xr, vo + o.y);
+                const f4 x2 = buf_ld4(xr, vo + o.z), x3 = buf_ld4(xr, vo + o.w);
+                acc += ww.x * x0;
+                acc += ww.y * x1;
+                acc += ww.z * x2;
+                acc += ww.w * x3;
+            }
+            acc += shfl_xor4(acc, 16);
+            acc += shfl_xor4(acc, 32);
+            if (g == 0) epi(row, j, acc, lv);
+        }
+        return;
+    }
+    if (row >= r1) return;
+    int a1 = sp[row + 1];
+    int q = sp[row];
+    int nr = row + stride;  // next row and its slot range
+    int n0 = nr < r1 ? sp[nr] : 0, n1 = nr < r1 ? sp[nr + 1] : 0;
+    i4 o = of[(q >> 2) + g];
+    f4 ww = w[(q >> 2) + g];
+    auto lv = ld(row, j);
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (;;) {
+        const f4 x0 = buf_ld4(xr, vo + o.x), x1 = buf_ld4(xr, vo + o.y);
+        const f4 x2 = buf_ld4(xr, vo + o.z), x3 = buf_ld4(xr, vo + o.w);
+        const bool last = q + kSlot >= a1;   // last chunk of this row
+        const bool more = !last || nr < r1;  // a chunk follows (here or in the next row)
+        const int qn = last ? n0 : q + kSlot;
+        i4 on = o;
+        f4 wn = ww;
+        if (more) {
+            on = of[(qn >> 2) + g];
+            wn = w[(qn >> 2) + g];
+        }
+        decltype(lv) lvn = lv;
+        if (last && more) lvn = ld(nr, j);
+        acc += ww.x * x0;
+        acc += ww.y * x1;
+        acc += ww.z * x2;
+        acc += ww.w * x3;
+        if (last) {
+            acc += shfl_xor4(acc, 16);
+            acc += shfl_xor4(acc, 32);
+            if (g == 0) epi(row, j, acc, lv);
+            if (!more) break;
+            row = nr;
+            a1 = n1;
+            nr = row + stride;
+            n0 = nr < r1 ? sp[nr] : 0;
+            n1 = nr < r1 ? sp[nr + 1] : 0;
+            lv = lvn;
+            acc = f4{0.f, 0.f, 0.f, 0.f};
+        }
+        q = qn;
+        o = on;
+        ww = wn;
+    }
+}
+
+// part[s*G + block] for samples 4j + c held by lanes j < 16 (v[c])
+__device__ inline void b_partial4(double* part, const double (&v)[4]) {
+    __shared__ double sp[kRowsPerBlock][kWave];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    if (lane < 16)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sp[wv][4 * lane + c] = v[c];
+    __syncthreads();
+    if (wv == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kRowsPerBlock; ++k) t += sp[k][lane];
+        part[(size_t)lane * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
 }
 
 // Writes the block's per-sample partial sum v (lane = sample) to part[s*G + block].
@@ -349,18 +399,18 @@ __global__ __launch_bounds__(kBlock) void k_b_fix(int n, int S, const unsigned c
         if (fixed[e / S]) x[e] = b[e];
 }
 
-// r = b - A x (FP64), rT = (T) r, partials of (b,b) and (r,r).
+// r = b - A x (FP64), rT = (T) r, partials of (b,b) and (r,r).  A: FP64
+// weights, offsets of FP64 rows.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_resid(CsrArgs A, int S, const double* __restrict__ b,
+__global__ __launch_bounds__(kBlock) void k_b_resid(SlotArgs A, int S, const double* __restrict__ b,
                                                     const double* __restrict__ x, double* __restrict__ r64,
                                                     T* __restrict__ rT, double* part_bb, double* part_rr) {
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
     double vb = 0.0, vr = 0.0;
-    auto body = [&](int row, const RowEnt& e) {
-        const double bi = b[(size_t)row * S + lc];
-        const double acc = row_sum<double>(A, A.v, e, lane, [&](int c) { return x[(size_t)c * S + lc]; });
+    auto ld = [&](int row) { return b[(size_t)row * S + lc]; };
+    auto epi = [&](int row, double acc, double bi) {
         if (on) {
             const size_t ee = (size_t)row * S + lane;
             const double ri = bi - acc;
@@ -370,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_b_resid(CsrArgs A, int S, const doub
             vr += ri * ri;
         }
     };
-    B_ROW_PIPE(A, A.v, A.n, body);
+    slot_walk<double, double, 1>(A, x, lc, ld, epi);
     b_partial(part_bb, S, vb);
     b_partial(part_rr, S, vr);
 }
@@ -427,22 +477,37 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_dir(size_t NS, int S, const T* 
 
 // q = A p; partial (p, q)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(CsrArgs A, int S, const T* __restrict__ p, T* __restrict__ q,
+__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T* __restrict__ p, T* __restrict__ q,
                                                       double* part) {
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
     double v = 0.0;
     auto ld = [&](int row) { return p[(size_t)row * S + lc]; };
-    auto xf = [&](int c) { return p[(size_t)c * S + lc]; };
     auto epi = [&](int row, T acc, T pi) {
         if (on) {
             q[(size_t)row * S + lane] = acc;
             v += (double)pi * (double)acc;
         }
     };
-    B_ROW_PIPE2(T, A, A.v, A.n, ld, xf, epi);
+    slot_walk<T, T, kRowsPerStep<T>>(A, p, lc, ld, epi);
     b_partial(part, S, v);
+}
+
+// q = A p; partial (p, q) -- FP32, S = 64, wide gathers
+__global__ __launch_bounds__(kBlock) void k_b_cg_spmv_w(SlotArgs A, const float* __restrict__ p,
+                                                        float* __restrict__ q, double* part) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    auto ld = [&](int row, int j) { return *(const f4*)(p + (size_t)row * 64 + 4 * j); };
+    auto epi = [&](int row, int j, f4 acc, f4 pi) {
+        *(f4*)(q + (size_t)row * 64 + 4 * j) = acc;
+        v[0] += (double)pi.x * (double)acc.x;
+        v[1] += (double)pi.y * (double)acc.y;
+        v[2] += (double)pi.z * (double)acc.z;
+        v[3] += (double)pi.w * (double)acc.w;
+    };
+    wide_walk<false>(A, p, ld, epi);
+    b_partial4(part, v);
 }
 
 __global__ void k_b_red_pq(int G, int S, const double* part, BState* bs) {
@@ -493,27 +558,15 @@ __global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* pa
 }
 
 // --------------------------------------------------------- AMG V-cycle --
-// aw[k] = v[k] w dinv[ci[k]]: the operator with the Jacobi step folded into
-// its columns, A w D^-1
-__global__ __launch_bounds__(kBlock) void k_b_scale_cols(size_t nnz, const int* __restrict__ ci,
-                                                         const double* __restrict__ v,
-                                                         const double* __restrict__ dinv, double w,
-                                                         double* __restrict__ aw) {
-    for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < nnz; k += (size_t)gridDim.x * kBlock)
-        aw[k] = v[k] * w * dinv[ci[k]];
-}
-
-// x = w dinv b; r = b - A x, with AW = A w D^-1 (k_b_scale_cols) in A.v
+// x = w dinv b; r = b - A x, with AW = A w D^-1 as the operator
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs AW, int S, const double* __restrict__ dinv,
+__global__ __launch_bounds__(kBlock) void k_b_pre(SlotArgs AW, int S, const double* __restrict__ dinv,
                                                   const T* __restrict__ b, double w, T* __restrict__ x,
                                                   T* __restrict__ r) {
-    const CsrArgs& A = AW;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
     auto ld = [&](int row) { return b[(size_t)row * S + lc]; };
-    auto xf = [&](int c) { return b[(size_t)c * S + lc]; };
     auto epi = [&](int row, T acc, T bi) {
         if (on) {
             const size_t ee = (size_t)row * S + lane;
@@ -521,26 +574,25 @@ __global__ __launch_bounds__(kBlock) void k_b_pre(CsrArgs AW, int S, const doubl
             r[ee] = bi - acc;
         }
     };
-    B_ROW_PIPE2(T, A, A.v, A.n, ld, xf, epi);
+    slot_walk<T, T, kRowsPerStep<T>>(AW, b, lc, ld, epi);
 }
 
 // y = M x (ACC: y += M x)
 template <typename T, bool ACC>
-__global__ __launch_bounds__(kBlock) void k_b_spmm(CsrArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
+__global__ __launch_bounds__(kBlock) void k_b_spmm(SlotArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
     auto ld = [&](int row) { return ACC ? y[(size_t)row * S + lc] : (T)0; };
-    auto xf = [&](int c) { return x[(size_t)c * S + lc]; };
     auto epi = [&](int row, T acc, T yi) {
         if (on) y[(size_t)row * S + lane] = ACC ? yi + acc : acc;
     };
-    B_ROW_PIPE2(T, M, M.v, M.n, ld, xf, epi);
+    slot_walk<T, T, kRowsPerStep<T>>(M, x, lc, ld, epi);
 }
 
 // xo = x + w dinv (b - A x); optional partial (b, xo)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_post(CsrArgs A, int S, const double* __restrict__ dinv,
+__global__ __launch_bounds__(kBlock) void k_b_post(SlotArgs A, int S, const double* __restrict__ dinv,
                                                    const T* __restrict__ b, double w, const T* __restrict__ x,
                                                    T* __restrict__ xo, double* part) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -554,7 +606,6 @@ __global__ __launch_bounds__(kBlock) void k_b_post(CsrArgs A, int S, const doubl
         const size_t er = (size_t)row * S + lc;
         return BX{b[er], x[er]};
     };
-    auto xf = [&](int c) { return x[(size_t)c * S + lc]; };
     auto epi = [&](int row, T acc, const BX& bx) {
         if (on) {
             const T zi = bx.x + (T)(w * dinv[row]) * (bx.b - acc);
@@ -562,26 +613,76 @@ __global__ __launch_bounds__(kBlock) void k_b_post(CsrArgs A, int S, const doubl
             v += (double)bx.b * (double)zi;
         }
     };
-    B_ROW_PIPE2(T, A, A.v, A.n, ld, xf, epi);
+    slot_walk<T, T, kRowsPerStep<T>>(A, x, lc, ld, epi);
     if (part) b_partial(part, S, v);
 }
 
+// Wide-gather forms (FP32, S = 64) of the pre-smoother, the transfer
+// products and the post-smoother.
+template <bool PIPE>
+__global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const double* __restrict__ dinv,
+                                                    const float* __restrict__ b, double w, float* __restrict__ x,
+                                                    float* __restrict__ r) {
+    auto ld = [&](int row, int j) { return *(const f4*)(b + (size_t)row * 64 + 4 * j); };
+    auto epi = [&](int row, int j, f4 acc, f4 bi) {
+        const size_t e = (size_t)row * 64 + 4 * j;
+        *(f4*)(x + e) = (float)(w * dinv[row]) * bi;
+        *(f4*)(r + e) = bi - acc;
+    };
+    wide_walk<PIPE>(AW, b, ld, epi);
+}
+
+template <bool ACC, bool PIPE>
+__global__ __launch_bounds__(kBlock) void k_b_spmm_w(SlotArgs M, const float* __restrict__ x, float* __restrict__ y) {
+    auto ld = [&](int row, int j) {
+        return ACC ? *(const f4*)(y + (size_t)row * 64 + 4 * j) : f4{0.f, 0.f, 0.f, 0.f};
+    };
+    auto epi = [&](int row, int j, f4 acc, f4 yi) { *(f4*)(y + (size_t)row * 64 + 4 * j) = ACC ? yi + acc : acc; };
+    wide_walk<PIPE>(M, x, ld, epi);
+}
+
+template <bool PIPE>
+__global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const double* __restrict__ dinv,
+                                                     const float* __restrict__ b, double w,
+                                                     const float* __restrict__ x, float* __restrict__ xo,
+                                                     double* part) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    struct BX {
+        f4 b, x;
+    };
+    auto ld = [&](int row, int j) {
+        const size_t e = (size_t)row * 64 + 4 * j;
+        return BX{*(const f4*)(b + e), *(const f4*)(x + e)};
+    };
+    auto epi = [&](int row, int j, f4 acc, const BX& bx) {
+        const f4 zi = bx.x + (float)(w * dinv[row]) * (bx.b - acc);
+        *(f4*)(xo + (size_t)row * 64 + 4 * j) = zi;
+        v[0] += (double)bx.b.x * (double)zi.x;
+        v[1] += (double)bx.b.y * (double)zi.y;
+        v[2] += (double)bx.b.z * (double)zi.z;
+        v[3] += (double)bx.b.w * (double)zi.w;
+    };
+    wide_walk<PIPE>(A, x, ld, epi);
+    if (part) b_partial4(part, v);
+}
+
 // x = C b for the dense coarse inverse (row-major nc x nc)
+constexpr int kDenseCh = 16;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_dense(int nc, int S, const double* __restrict__ C,
                                                     const T* __restrict__ b, T* __restrict__ x) {
     B_ROW_LOOP(nc) {
         double acc = 0.0;
-        for (int k0 = 0; k0 < nc; k0 += kGr) {
-            double cv[kGr], bv[kGr];
+        for (int k0 = 0; k0 < nc; k0 += kDenseCh) {
+            double cv[kDenseCh], bv[kDenseCh];
 #pragma unroll
-            for (int k = 0; k < kGr; ++k) {
+            for (int k = 0; k < kDenseCh; ++k) {
                 const int kk = min(k0 + k, nc - 1);
                 cv[k] = C[(size_t)row * nc + kk];
                 bv[k] = (double)b[(size_t)kk * S + lc];
             }
 #pragma unroll
-            for (int k = 0; k < kGr; ++k) acc = k0 + k < nc ? acc + cv[k] * bv[k] : acc;
+            for (int k = 0; k < kDenseCh; ++k) acc = k0 + k < nc ? acc + cv[k] * bv[k] : acc;
         }
         if (on) x[(size_t)row * S + lane] = (T)acc;
     }
@@ -992,11 +1093,94 @@ inline int row_grid(int rows) {
 // ---------------------------------------------------------------------------
 // Batched solver for one demag system, sharing the context solver's operator
 // and AMG hierarchy.
+// Host copy of a device CSR matrix (slot layouts are built on the host).
+struct HostCsr {
+    int n = 0;
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+};
+
+int fetch_csr(const AmgPrecond::CsrDev& M, HostCsr& h, hipStream_t s) {
+    h.n = M.n;
+    h.rp.assign((size_t)M.n + 1, 0);
+    TM_HIP(hipMemcpyAsync(h.rp.data(), M.rp.p, h.rp.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    TM_HIP(hipStreamSynchronize(s));
+    const size_t nnz = (size_t)h.rp[M.n];
+    h.ci.assign(nnz, 0);
+    h.v.assign(nnz, 0.0);
+    if (nnz) {
+        TM_HIP(hipMemcpyAsync(h.ci.data(), M.ci.p, nnz * sizeof(int), hipMemcpyDeviceToHost, s));
+        TM_HIP(hipMemcpyAsync(h.v.data(), M.v.p, nnz * sizeof(double), hipMemcpyDeviceToHost, s));
+        TM_HIP(hipStreamSynchronize(s));
+    }
+    return TM_OK;
+}
+
+// A CSR pattern in row slots (see SlotArgs): every row gets a multiple of
+// kSlot slots, at least one chunk, padded with offset 0 / weight 0; byte
+// offsets for gathered elements of 4 and of 8 bytes.
+struct SlotOp {
+    int n = 0, ncols = 0, S = 0;
+    std::vector<int> h_rp, h_sp;
+    DevBuf<int> sp, off4, off8;
+
+    int build(int rows, int cols, const int* rp, const int* ci, int S_, hipStream_t s) {
+        n = rows;
+        ncols = cols;
+        S = S_;
+        if ((int64_t)cols * S * 8 > (int64_t)INT32_MAX) return TM_ERR_UNSUPPORTED;  // 32-bit byte offsets
+        h_rp.assign(rp, rp + rows + 1);
+        h_sp.assign((size_t)rows + 1, 0);
+        for (int i = 0; i < rows; ++i) {
+            const int len = rp[i + 1] - rp[i];
+            h_sp[i + 1] = h_sp[i] + std::max(1, (len + kSlot - 1) / kSlot) * kSlot;
+        }
+        const size_t ns = (size_t)h_sp[rows];
+        std::vector<int> o4(ns, 0), o8(ns, 0);
+        for (int i = 0; i < rows; ++i)
+            for (int q = rp[i]; q < rp[i + 1]; ++q) {
+                const size_t k = (size_t)h_sp[i] + (q - rp[i]);
+                o4[k] = ci[q] * S * 4;
+                o8[k] = ci[q] * S * 8;
+            }
+        TM_CHECK(sp.upload(h_sp.data(), h_sp.size(), s));
+        TM_CHECK(off4.upload(o4.data(), ns, s));
+        TM_CHECK(off8.upload(o8.data(), ns, s));
+        TM_HIP(hipStreamSynchronize(s));
+        return TM_OK;
+    }
+
+    // per-slot weights from CSR values v (optionally scaled by colscale[col] * w0)
+    template <typename W>
+    int values(const std::vector<double>& v, DevBuf<W>& out, hipStream_t s, const int* ci = nullptr,
+               const double* colscale = nullptr, double w0 = 1.0) const {
+        std::vector<W> h((size_t)h_sp[n], (W)0);
+        for (int i = 0; i < n; ++i)
+            for (int q = h_rp[i]; q < h_rp[i + 1]; ++q)
+                h[(size_t)h_sp[i] + (q - h_rp[i])] = (W)(colscale ? v[q] * w0 * colscale[ci[q]] : v[q]);
+        TM_CHECK(out.upload(h.data(), h.size(), s));
+        TM_HIP(hipStreamSynchronize(s));
+        return TM_OK;
+    }
+
+    bool long_rows() const { return n > 0 && (int64_t)h_sp[n] > (int64_t)kSlot * n * 3 / 2; }
+    SlotArgs args(int ebytes, const void* w) const {
+        return SlotArgs{n, ncols * S * ebytes, sp.p, ebytes == 4 ? off4.p : off8.p, w};
+    }
+};
+
 template <typename T>
 struct BatchSolver {
     const CgSolver* cg = nullptr;
     int n = 0, S = 0, G = 0;
-    DCsr A;                       // level-0 operator (FP64 values)
+    SlotOp sl0;                   // level-0 operator in slots
+    DevBuf<T> wA0, wAW0;          // its weights: A, and A w D^-1 (pre-smoother)
+    DevBuf<double> wA64;          // A in FP64 (outer residual)
+    struct LvOps {
+        SlotOp A, R, P;           // level l >= 1: A; every level: restriction, prolongation
+        DevBuf<T> wA, wAW, wR, wP;
+    };
+    std::vector<LvOps> lo;
     DevBuf<double> dinv;
     DevBuf<unsigned char> fixed;
     DevBuf<double> r64;
@@ -1005,7 +1189,6 @@ struct BatchSolver {
         DevBuf<T> x, x2, r, b;
     };
     std::vector<Lv> lv;
-    std::vector<DevBuf<double>> aw;  // per level: A w D^-1 values (pre-smoother)
     DevBuf<T> xc;
     DevBuf<double> part;          // 4 slots x S x G
     DevBuf<OState> ost;
@@ -1024,7 +1207,36 @@ struct BatchSolver {
         n = solver.n;
         S = S_;
         G = row_grid(n);
-        TM_CHECK(A.upload(n, solver.pat.h_rowptr.data(), solver.pat.h_csrcol.data(), solver.h_val.data(), s));
+        const AmgPrecond& M = solver.amg;
+        const int L = (int)M.levels.size();
+        TM_CHECK(sl0.build(n, n, solver.pat.h_rowptr.data(), solver.pat.h_csrcol.data(), S, s));
+        TM_CHECK(sl0.values(solver.h_val, wA0, s));
+        TM_CHECK(sl0.values(solver.h_val, wA64, s));
+        if (L > 0)
+            TM_CHECK(sl0.values(solver.h_val, wAW0, s, solver.pat.h_csrcol.data(), solver.h_dinv.data(),
+                                M.levels[0].omega));
+        lo.clear();
+        lo.resize((size_t)L);
+        for (int l = 0; l < L; ++l) {
+            const auto& V = M.levels[l];
+            const int nnext = l + 1 < L ? M.levels[l + 1].n : M.nc;
+            HostCsr h;
+            if (l > 0) {
+                TM_CHECK(fetch_csr(V.A, h, s));
+                std::vector<double> dv((size_t)V.n);
+                TM_HIP(hipMemcpyAsync(dv.data(), V.dinv.p, dv.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+                TM_HIP(hipStreamSynchronize(s));
+                TM_CHECK(lo[l].A.build(V.n, V.n, h.rp.data(), h.ci.data(), S, s));
+                TM_CHECK(lo[l].A.values(h.v, lo[l].wA, s));
+                TM_CHECK(lo[l].A.values(h.v, lo[l].wAW, s, h.ci.data(), dv.data(), V.omega));
+            }
+            TM_CHECK(fetch_csr(V.R, h, s));
+            TM_CHECK(lo[l].R.build(V.R.n, V.n, h.rp.data(), h.ci.data(), S, s));
+            TM_CHECK(lo[l].R.values(h.v, lo[l].wR, s));
+            TM_CHECK(fetch_csr(V.P, h, s));
+            TM_CHECK(lo[l].P.build(V.P.n, nnext, h.rp.data(), h.ci.data(), S, s));
+            TM_CHECK(lo[l].P.values(h.v, lo[l].wP, s));
+        }
         TM_CHECK(dinv.upload(solver.h_dinv.data(), n, s));
         TM_CHECK(fixed.upload(solver.h_fixed.data(), n, s));
         const size_t NS = (size_t)n * S;
@@ -1044,21 +1256,6 @@ struct BatchSolver {
             TM_CHECK(lv[l].r.alloc((size_t)L.n * S));
             TM_CHECK(lv[l].b.alloc((size_t)L.R.n * S));
         }
-        aw.clear();
-        aw.resize(solver.amg.levels.size());
-        for (size_t l = 0; l < solver.amg.levels.size(); ++l) {
-            const auto& L = solver.amg.levels[l];
-            const size_t nnz = l == 0 ? (size_t)solver.pat.h_rowptr[n] : (size_t)L.nnz_a;
-            const int* ci = l == 0 ? A.ci.p : L.A.ci.p;
-            const double* v = l == 0 ? A.v.p : L.A.v.p;
-            const double* di = l == 0 ? dinv.p : L.dinv.p;
-            TM_CHECK(aw[l].alloc(std::max<size_t>(nnz, 1)));
-            if (nnz > 0) {
-                hipLaunchKernelGGL(k_b_scale_cols, dim3(elem_grid(nnz)), dim3(kBlock), 0, s, nnz, ci, v, di,
-                                   L.omega, aw[l].p);
-                TM_HIP(hipGetLastError());
-            }
-        }
         TM_CHECK(xc.alloc((size_t)std::max(1, solver.amg.nc) * S));
         TM_CHECK(part.alloc((size_t)4 * S * G));
         TM_CHECK(ost.alloc(1));
@@ -1073,44 +1270,95 @@ struct BatchSolver {
     int precond(const T* r, T* zout, double* prz, hipStream_t s) {
         const AmgPrecond& M = cg->amg;
         const int L = (int)M.levels.size();
-        if (L == 0) {
-            hipLaunchKernelGGL((k_b_dense<T>), dim3(row_grid(M.nc)), dim3(kBlock), 0, s, M.nc, S, M.Cinv.p, r, zout);
-            // (r, z) partial via a post-sweep with w = 0 on the identity: reuse k_b_cg_update-free path
-            return TM_ERR_UNSUPPORTED;
-        }
+        if (L == 0) return TM_ERR_UNSUPPORTED;  // the batched solve needs at least one AMG level
+        const int e = (int)sizeof(T);
         const T* b = r;
         for (int l = 0; l < L; ++l) {
             const auto& V = M.levels[l];
+            const LvOps& O = lo[l];
             if (l == 0)
-                hipLaunchKernelGGL((k_b_pre<T>), dim3(G), dim3(kBlock), 0, s, CsrArgs{A.n, A.rp.p, A.ci.p, aw[0].p},
-                                   S, dinv.p, b, V.omega, lv[0].x.p, lv[0].r.p);
+                pre(sl0.long_rows(), dim3(G), sl0.args(e, wAW0.p), dinv.p, b, V.omega, lv[0].x.p, lv[0].r.p, s);
             else
-                hipLaunchKernelGGL((k_b_pre<T>), dim3(row_grid(V.n)), dim3(kBlock), 0, s,
-                                   CsrArgs{V.A.n, V.A.rp.p, V.A.ci.p, aw[l].p}, S, V.dinv.p, b, V.omega, lv[l].x.p,
-                                   lv[l].r.p);
-            hipLaunchKernelGGL((k_b_spmm<T, false>), dim3(row_grid(V.R.n)), dim3(kBlock), 0, s, args(V.R), S,
-                               (const T*)lv[l].r.p, lv[l].b.p);
+                pre(O.A.long_rows(), dim3(row_grid(V.n)), O.A.args(e, O.wAW.p), V.dinv.p, b, V.omega, lv[l].x.p,
+                    lv[l].r.p, s);
+            spmm(false, O.R.long_rows(), dim3(row_grid(V.R.n)), O.R.args(e, O.wR.p), lv[l].r.p, lv[l].b.p, s);
             b = lv[l].b.p;
         }
         hipLaunchKernelGGL((k_b_dense<T>), dim3(row_grid(M.nc)), dim3(kBlock), 0, s, M.nc, S, M.Cinv.p, b, xc.p);
         const T* xcoarse = xc.p;
         for (int l = L - 1; l >= 0; --l) {
             const auto& V = M.levels[l];
+            const LvOps& O = lo[l];
             const T* bl = l == 0 ? r : lv[l - 1].b.p;
-            hipLaunchKernelGGL((k_b_spmm<T, true>), dim3(row_grid(V.P.n)), dim3(kBlock), 0, s, args(V.P), S, xcoarse,
-                               lv[l].x.p);
+            spmm(true, O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), xcoarse, lv[l].x.p, s);
             if (l == 0) {
-                hipLaunchKernelGGL((k_b_post<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, dinv.p, bl, V.omega,
-                                   (const T*)lv[0].x.p, zout, prz);
+                post(sl0.long_rows(), dim3(G), sl0.args(e, wA0.p), dinv.p, bl, V.omega, lv[0].x.p, zout, prz, s);
                 xcoarse = zout;
             } else {
-                hipLaunchKernelGGL((k_b_post<T>), dim3(row_grid(V.n)), dim3(kBlock), 0, s, args(V.A), S, V.dinv.p, bl,
-                                   V.omega, (const T*)lv[l].x.p, lv[l].x2.p, (double*)nullptr);
+                post(O.A.long_rows(), dim3(row_grid(V.n)), O.A.args(e, O.wA.p), V.dinv.p, bl, V.omega, lv[l].x.p,
+                     lv[l].x2.p, nullptr, s);
                 xcoarse = lv[l].x2.p;
             }
         }
         TM_HIP(hipGetLastError());
         return TM_OK;
+    }
+
+    // V-cycle kernels; the wide-gather forms for 64 FP32 samples
+    static constexpr bool kWideT = sizeof(T) == 4;
+    bool wide() const { return kWideT && S == kMaxS; }
+    void pre(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, T* x, T* rr, hipStream_t s) {
+        if constexpr (kWideT)
+            if (wide()) {
+                if (lg)
+                    hipLaunchKernelGGL(k_b_pre_w<true>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w, (float*)x,
+                                       (float*)rr);
+                else
+                    hipLaunchKernelGGL(k_b_pre_w<false>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w, (float*)x,
+                                       (float*)rr);
+                return;
+            }
+        hipLaunchKernelGGL((k_b_pre<T>), g, dim3(kBlock), 0, s, a, S, di, bb, w, x, rr);
+    }
+    void spmm(bool acc, bool lg, dim3 g, SlotArgs a, const T* x, T* y, hipStream_t s) {
+        if constexpr (kWideT)
+            if (wide()) {
+                const float* xf = (const float*)x;
+                float* yf = (float*)y;
+                if (acc && lg) hipLaunchKernelGGL((k_b_spmm_w<true, true>), g, dim3(kBlock), 0, s, a, xf, yf);
+                else if (acc) hipLaunchKernelGGL((k_b_spmm_w<true, false>), g, dim3(kBlock), 0, s, a, xf, yf);
+                else if (lg) hipLaunchKernelGGL((k_b_spmm_w<false, true>), g, dim3(kBlock), 0, s, a, xf, yf);
+                else hipLaunchKernelGGL((k_b_spmm_w<false, false>), g, dim3(kBlock), 0, s, a, xf, yf);
+                return;
+            }
+        if (acc) hipLaunchKernelGGL((k_b_spmm<T, true>), g, dim3(kBlock), 0, s, a, S, x, y);
+        else hipLaunchKernelGGL((k_b_spmm<T, false>), g, dim3(kBlock), 0, s, a, S, x, y);
+    }
+    void post(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* x, T* xo, double* part,
+              hipStream_t s) {
+        if constexpr (kWideT)
+            if (wide()) {
+                if (lg)
+                    hipLaunchKernelGGL(k_b_post_w<true>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w,
+                                       (const float*)x, (float*)xo, part);
+                else
+                    hipLaunchKernelGGL(k_b_post_w<false>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w,
+                                       (const float*)x, (float*)xo, part);
+                return;
+            }
+        hipLaunchKernelGGL((k_b_post<T>), g, dim3(kBlock), 0, s, a, S, di, bb, w, x, xo, part);
+    }
+
+    // q = A p, partial (p, q): the wide-gather kernel for 64 FP32 samples
+    void cg_spmv(const T* pc, double* ppq, hipStream_t s) {
+        if constexpr (kWideT)
+            if (wide()) {
+                hipLaunchKernelGGL(k_b_cg_spmv_w, dim3(G), dim3(kBlock), 0, s, sl0.args(4, wA0.p), (const float*)pc,
+                                   (float*)q.p, ppq);
+                return;
+            }
+        hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, sl0.args((int)sizeof(T), wA0.p), S, pc, q.p,
+                           ppq);
     }
 
     // Solves A x = b for all samples ([n][S] FP64); x is the warm start.
@@ -1126,7 +1374,7 @@ struct BatchSolver {
         int outer = 0, inner_total = 0;
         hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
         for (;;) {
-            hipLaunchKernelGGL((k_b_resid<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, b, (const double*)x, r64.p,
+            hipLaunchKernelGGL((k_b_resid<T>), dim3(G), dim3(kBlock), 0, s, sl0.args(8, wA64.p), S, b, (const double*)x, r64.p,
                                rT.p, pbb, prr);
             hipLaunchKernelGGL(k_b_outer_red, dim3(S), dim3(kBlock), 0, s, G, S, pbb, prr, tol, itol, ost.p, bst.p);
             TM_HIP(hipMemcpyAsync(h_ost, ost.p, sizeof(OState), hipMemcpyDeviceToHost, s));
@@ -1152,8 +1400,7 @@ struct BatchSolver {
                 for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
                     hipLaunchKernelGGL((k_b_cg_dir<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S,
                                        (const T*)z.p, (const T*)po, pc, enq, (const BState*)bst.p);
-                    hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, args(A), S, (const T*)pc, q.p,
-                                       ppq);
+                    cg_spmv(pc, ppq, s);
                     hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, ppq, bst.p);
                     hipLaunchKernelGGL((k_b_cg_update<T>), dim3(G), dim3(kBlock), 0, s, n, S, (const T*)pc,
                                        (const T*)q.p, d.p, rT.p, (const BState*)bst.p, prr);
