@@ -175,6 +175,21 @@ def measured_traffic(workload, b):
     return d["hbm_bytes_per_product"]["total"]
 
 
+def measured_gemm_traffic(workload, b):
+    """HBM bytes per batched boundary product (GEMM kernel + split-K sum) from
+    the committed FETCH_SIZE / WRITE_SIZE counter runs, when they cover this
+    workload."""
+    path = os.path.join(ROOT, "profiles", f"boundary_gemm_traffic_{workload.rsplit('-h', 1)[0]}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    if d.get("b") != b:
+        return None
+    return d["hbm_bytes_per_product"]["total"]
+
+
 def kernel_only(ctx, prob, reps):
     """Launch the dense boundary product (k_bem_gemv_part + k_bem_gemv_finish),
     a plain 1 GiB device-to-device copy and the batched boundary product of 64
@@ -374,7 +389,8 @@ def main():
                     % (-1 - gt["solution"], gt.get("ksplit", 1))) if gt.get("solution", 0) < 0 else "rocBLAS DGEMM"
             result["roofline"] = {"bound": "mfma", "kernel": f"boundary GEMM B(b x b) U^T(b x {args.batch}): {kern}",
                                   "achieved": ach, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
-                                  "frac": ach / FP64_MFMA_PEAK_TFS, "traffic": None,
+                                  "frac": ach / FP64_MFMA_PEAK_TFS,
+                                  "traffic": measured_gemm_traffic(prob.name, b) if kern != "rocBLAS DGEMM" else None,
                                   "algorithmic_flops": flops, "b_bytes_read": 8.0 * b * b}
     if not args.no_cpu_baseline:
         try:
