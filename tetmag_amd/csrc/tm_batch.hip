@@ -248,22 +248,31 @@ __device__ inline f4 buf_ld4(__amdgpu_buffer_rsrc_t r, int voff) {
     return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
 }
 
+// the gather of a slot; where its weight is zero (padding slots) the offset
+// is moved past the buffer's end, so the load returns zeros without a
+// memory access (raw-buffer range check; gathered vectors are < 1 GiB, see
+// SlotOp::build)
+constexpr int kOutOfRange = 0x7ffffff0;
+__device__ inline f4 ld4_if(__amdgpu_buffer_rsrc_t r, int voff, float w) {
+    return buf_ld4(r, w != 0.f ? voff : kOutOfRange);
+}
+
 __device__ inline f4 shfl_xor4(f4 v, int m) {
     return f4{__shfl_xor(v.x, m), __shfl_xor(v.y, m), __shfl_xor(v.z, m), __shfl_xor(v.w, m)};
 }
 
-// Row walk as slot_walk (XCD-contiguous eighths, one row per wavefront per
-// step).  PIPE (for operators of long rows) software-pipelines the chunks:
-// while a chunk's gathers are in flight, the
-// offsets and weights of the next chunk (of this row, or the first of the
-// next row, whose slot range is read one row ahead) are loaded, and at a row's
-// last chunk the next row's epilogue operands.  ld(row, j) loads the
-// epilogue's operands for samples 4j .. 4j+3; epi(row, j, acc, ld) runs on
-// lanes 0-15.
-template <bool PIPE, typename LDF, typename EPIF>
+// Row walk as slot_walk (XCD-contiguous eighths).  BLK = false: one row per
+// wavefront per step.  BLK = true (operators of long rows: the coarse
+// levels' Galerkin products, the restrictions): one row per block, its
+// 16-slot chunks dealt round-robin to the block's 4 wavefronts, the
+// wavefronts' sums added in LDS.  ld(row, j) loads the epilogue's operands
+// for samples 4j .. 4j+3; epi(row, j, acc, ld) runs on lanes 0-15 (of
+// wavefront 0 with BLK).
+template <bool BLK, typename LDF, typename EPIF>
 __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPIF&& epi) {
     const int n = M.n;
     const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, j = lane & 15;
+    const int wv = threadIdx.x / kWave;
     const __amdgpu_buffer_rsrc_t xr = gather_rsrc(x, M.xbytes);
     const i4* __restrict__ of = (const i4*)M.off;
     const f4* __restrict__ w = (const f4*)M.w;
@@ -277,72 +286,47 @@ __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPI
         r0 = (int)((int64_t)n * xi / kXcd);
         r1 = (int)((int64_t)n * (xi + 1) / kXcd);
     }
-    const int stride = gx * kRowsPerBlock;
-    int row = __builtin_amdgcn_readfirstlane(r0 + lb * kRowsPerBlock + (int)(threadIdx.x / kWave));
-    if constexpr (!PIPE) {
-        for (; row < r1; row += stride) {
-            const int a0 = sp[row], a1 = sp[row + 1];
-            const auto lv = ld(row, j);
-            f4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int q = a0; q < a1; q += kSlot) {
-                const i4 o = of[(q >> 2) + g];
-                const f4 ww = w[(q >> 2) + g];
-                const f4 x0 = buf_ld4(xr, vo + o.x), x1 = buf_ld4(xr, vo + o.y);
-                const f4 x2 = buf_ld4(xr, vo + o.z), x3 = buf_ld4(xr, vo + o.w);
-                acc += ww.x * x0;
-                acc += ww.y * x1;
-                acc += ww.z * x2;
-                acc += ww.w * x3;
-            }
-            acc += shfl_xor4(acc, 16);
-            acc += shfl_xor4(acc, 32);
-            if (g == 0) epi(row, j, acc, lv);
-        }
-        return;
-    }
-    if (row >= r1) return;
-    int a1 = sp[row + 1];
-    int q = sp[row];
-    int nr = row + stride;  // next row and its slot range
-    int n0 = nr < r1 ? sp[nr] : 0, n1 = nr < r1 ? sp[nr + 1] : 0;
-    i4 o = of[(q >> 2) + g];
-    f4 ww = w[(q >> 2) + g];
-    auto lv = ld(row, j);
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (;;) {
-        const f4 x0 = buf_ld4(xr, vo + o.x), x1 = buf_ld4(xr, vo + o.y);
-        const f4 x2 = buf_ld4(xr, vo + o.z), x3 = buf_ld4(xr, vo + o.w);
-        const bool last = q + kSlot >= a1;   // last chunk of this row
-        const bool more = !last || nr < r1;  // a chunk follows (here or in the next row)
-        const int qn = last ? n0 : q + kSlot;
-        i4 on = o;
-        f4 wn = ww;
-        if (more) {
-            on = of[(qn >> 2) + g];
-            wn = w[(qn >> 2) + g];
-        }
-        decltype(lv) lvn = lv;
-        if (last && more) lvn = ld(nr, j);
+    auto chunk = [&](int q, f4 acc) {
+        const i4 o = of[(q >> 2) + g];
+        const f4 ww = w[(q >> 2) + g];
+        const f4 x0 = ld4_if(xr, vo + o.x, ww.x), x1 = ld4_if(xr, vo + o.y, ww.y);
+        const f4 x2 = ld4_if(xr, vo + o.z, ww.z), x3 = ld4_if(xr, vo + o.w, ww.w);
         acc += ww.x * x0;
         acc += ww.y * x1;
         acc += ww.z * x2;
         acc += ww.w * x3;
-        if (last) {
+        return acc;
+    };
+    if constexpr (!BLK) {
+        const int stride = gx * kRowsPerBlock;
+        for (int row = __builtin_amdgcn_readfirstlane(r0 + lb * kRowsPerBlock + wv); row < r1; row += stride) {
+            const int a0 = sp[row], a1 = sp[row + 1];
+            const auto lv = ld(row, j);
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int q = a0; q < a1; q += kSlot) acc = chunk(q, acc);
             acc += shfl_xor4(acc, 16);
             acc += shfl_xor4(acc, 32);
             if (g == 0) epi(row, j, acc, lv);
-            if (!more) break;
-            row = nr;
-            a1 = n1;
-            nr = row + stride;
-            n0 = nr < r1 ? sp[nr] : 0;
-            n1 = nr < r1 ? sp[nr + 1] : 0;
-            lv = lvn;
-            acc = f4{0.f, 0.f, 0.f, 0.f};
         }
-        q = qn;
-        o = on;
-        ww = wn;
+    } else {
+        __shared__ f4 red[kRowsPerBlock][16];
+        for (int row = r0 + lb; row < r1; row += gx) {  // block-uniform
+            const int a0 = sp[row], a1 = sp[row + 1];
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int q = a0 + kSlot * wv; q < a1; q += kSlot * kRowsPerBlock) acc = chunk(q, acc);
+            acc += shfl_xor4(acc, 16);
+            acc += shfl_xor4(acc, 32);
+            if (g == 0) red[wv][j] = acc;
+            __syncthreads();
+            if (wv == 0 && g == 0) {
+                const auto lv = ld(row, j);
+                f4 t = red[0][j];
+#pragma unroll
+                for (int k = 1; k < kRowsPerBlock; ++k) t += red[k][j];
+                epi(row, j, t, lv);
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -619,7 +603,7 @@ __global__ __launch_bounds__(kBlock) void k_b_post(SlotArgs A, int S, const doub
 
 // Wide-gather forms (FP32, S = 64) of the pre-smoother, the transfer
 // products and the post-smoother.
-template <bool PIPE>
+template <bool BLK>
 __global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const double* __restrict__ dinv,
                                                     const float* __restrict__ b, double w, float* __restrict__ x,
                                                     float* __restrict__ r) {
@@ -629,19 +613,19 @@ __global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const double* _
         *(f4*)(x + e) = (float)(w * dinv[row]) * bi;
         *(f4*)(r + e) = bi - acc;
     };
-    wide_walk<PIPE>(AW, b, ld, epi);
+    wide_walk<BLK>(AW, b, ld, epi);
 }
 
-template <bool ACC, bool PIPE>
+template <bool ACC, bool BLK>
 __global__ __launch_bounds__(kBlock) void k_b_spmm_w(SlotArgs M, const float* __restrict__ x, float* __restrict__ y) {
     auto ld = [&](int row, int j) {
         return ACC ? *(const f4*)(y + (size_t)row * 64 + 4 * j) : f4{0.f, 0.f, 0.f, 0.f};
     };
     auto epi = [&](int row, int j, f4 acc, f4 yi) { *(f4*)(y + (size_t)row * 64 + 4 * j) = ACC ? yi + acc : acc; };
-    wide_walk<PIPE>(M, x, ld, epi);
+    wide_walk<BLK>(M, x, ld, epi);
 }
 
-template <bool PIPE>
+template <bool BLK>
 __global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const double* __restrict__ dinv,
                                                      const float* __restrict__ b, double w,
                                                      const float* __restrict__ x, float* __restrict__ xo,
@@ -662,29 +646,42 @@ __global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const double* _
         v[2] += (double)bx.b.z * (double)zi.z;
         v[3] += (double)bx.b.w * (double)zi.w;
     };
-    wide_walk<PIPE>(A, x, ld, epi);
+    wide_walk<BLK>(A, x, ld, epi);
     if (part) b_partial4(part, v);
 }
 
-// x = C b for the dense coarse inverse (row-major nc x nc)
+// x = C b for the dense coarse inverse (row-major nc x nc): one row per block,
+// its columns dealt to the 4 wavefronts in 16-column chunks (C's entries are
+// wavefront-uniform scalar loads), the wavefronts' sums added in LDS.
 constexpr int kDenseCh = 16;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_dense(int nc, int S, const double* __restrict__ C,
                                                     const T* __restrict__ b, T* __restrict__ x) {
-    B_ROW_LOOP(nc) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int lc = min(lane, S - 1);
+    const cw_p<double> Cc = (cw_p<double>)C;
+    __shared__ double red[kRowsPerBlock][kWave];
+    for (int row = blockIdx.x; row < nc; row += gridDim.x) {  // block-uniform
         double acc = 0.0;
-        for (int k0 = 0; k0 < nc; k0 += kDenseCh) {
+        for (int k0 = kDenseCh * wv; k0 < nc; k0 += kDenseCh * kRowsPerBlock) {
             double cv[kDenseCh], bv[kDenseCh];
 #pragma unroll
             for (int k = 0; k < kDenseCh; ++k) {
                 const int kk = min(k0 + k, nc - 1);
-                cv[k] = C[(size_t)row * nc + kk];
+                cv[k] = Cc[(size_t)row * nc + kk];
                 bv[k] = (double)b[(size_t)kk * S + lc];
             }
 #pragma unroll
             for (int k = 0; k < kDenseCh; ++k) acc = k0 + k < nc ? acc + cv[k] * bv[k] : acc;
         }
-        if (on) x[(size_t)row * S + lane] = (T)acc;
+        red[wv][lane] = acc;
+        __syncthreads();
+        if (wv == 0 && lane < S) {
+            double t = red[0][lane];
+            for (int k = 1; k < kRowsPerBlock; ++k) t += red[k][lane];
+            x[(size_t)row * S + lane] = (T)t;
+        }
+        __syncthreads();
     }
 }
 
@@ -1163,7 +1160,8 @@ struct SlotOp {
         return TM_OK;
     }
 
-    bool long_rows() const { return n > 0 && (int64_t)h_sp[n] > (int64_t)kSlot * n * 3 / 2; }
+    // long rows: more than 6 chunks on average (one block per row pays off there)
+    bool long_rows() const { return n > 0 && (int64_t)h_sp[n] > (int64_t)kSlot * n * 6; }
     SlotArgs args(int ebytes, const void* w) const {
         return SlotArgs{n, ncols * S * ebytes, sp.p, ebytes == 4 ? off4.p : off8.p, w};
     }
@@ -1284,7 +1282,8 @@ struct BatchSolver {
             spmm(false, O.R.long_rows(), dim3(row_grid(V.R.n)), O.R.args(e, O.wR.p), lv[l].r.p, lv[l].b.p, s);
             b = lv[l].b.p;
         }
-        hipLaunchKernelGGL((k_b_dense<T>), dim3(row_grid(M.nc)), dim3(kBlock), 0, s, M.nc, S, M.Cinv.p, b, xc.p);
+        hipLaunchKernelGGL((k_b_dense<T>), dim3(std::max(1, std::min(M.nc, 4096))), dim3(kBlock), 0, s, M.nc, S,
+                           M.Cinv.p, b, xc.p);
         const T* xcoarse = xc.p;
         for (int l = L - 1; l >= 0; --l) {
             const auto& V = M.levels[l];
@@ -1307,12 +1306,14 @@ struct BatchSolver {
     // V-cycle kernels; the wide-gather forms for 64 FP32 samples
     static constexpr bool kWideT = sizeof(T) == 4;
     bool wide() const { return kWideT && S == kMaxS; }
+    // one block per row for long-row operators (grid: a multiple of the XCD count)
+    static dim3 blk_grid(int rows) { return dim3(xcd_grid(std::min(4096, std::max(rows, 1)))); }
     void pre(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, T* x, T* rr, hipStream_t s) {
         if constexpr (kWideT)
             if (wide()) {
                 if (lg)
-                    hipLaunchKernelGGL(k_b_pre_w<true>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w, (float*)x,
-                                       (float*)rr);
+                    hipLaunchKernelGGL(k_b_pre_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, di, (const float*)bb, w,
+                                       (float*)x, (float*)rr);
                 else
                     hipLaunchKernelGGL(k_b_pre_w<false>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w, (float*)x,
                                        (float*)rr);
@@ -1325,9 +1326,10 @@ struct BatchSolver {
             if (wide()) {
                 const float* xf = (const float*)x;
                 float* yf = (float*)y;
-                if (acc && lg) hipLaunchKernelGGL((k_b_spmm_w<true, true>), g, dim3(kBlock), 0, s, a, xf, yf);
+                const dim3 gb = blk_grid(a.n);
+                if (acc && lg) hipLaunchKernelGGL((k_b_spmm_w<true, true>), gb, dim3(kBlock), 0, s, a, xf, yf);
                 else if (acc) hipLaunchKernelGGL((k_b_spmm_w<true, false>), g, dim3(kBlock), 0, s, a, xf, yf);
-                else if (lg) hipLaunchKernelGGL((k_b_spmm_w<false, true>), g, dim3(kBlock), 0, s, a, xf, yf);
+                else if (lg) hipLaunchKernelGGL((k_b_spmm_w<false, true>), gb, dim3(kBlock), 0, s, a, xf, yf);
                 else hipLaunchKernelGGL((k_b_spmm_w<false, false>), g, dim3(kBlock), 0, s, a, xf, yf);
                 return;
             }
@@ -1338,9 +1340,9 @@ struct BatchSolver {
               hipStream_t s) {
         if constexpr (kWideT)
             if (wide()) {
-                if (lg)
-                    hipLaunchKernelGGL(k_b_post_w<true>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w,
-                                       (const float*)x, (float*)xo, part);
+                if (lg && !part)  // partial sums are laid out per block of the row grid g
+                    hipLaunchKernelGGL(k_b_post_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, di, (const float*)bb,
+                                       w, (const float*)x, (float*)xo, part);
                 else
                     hipLaunchKernelGGL(k_b_post_w<false>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w,
                                        (const float*)x, (float*)xo, part);
