@@ -33,28 +33,6 @@ namespace {
 constexpr int kMaxS = 64;
 constexpr int kRowsPerBlock = kBlock / kWave;  // one row per wavefront
 
-struct DCsr {
-    int n = 0;
-    DevBuf<int> rp, ci;
-    DevBuf<double> v;
-    int upload(int rows, const int* rowptr, const int* col, const double* val, hipStream_t s) {
-        n = rows;
-        TM_CHECK(rp.upload(rowptr, (size_t)rows + 1, s));
-        TM_CHECK(ci.upload(col, (size_t)rowptr[rows], s));
-        TM_CHECK(v.upload(val, (size_t)rowptr[rows], s));
-        TM_HIP(hipStreamSynchronize(s));
-        return TM_OK;
-    }
-};
-
-struct CsrArgs {
-    int n;
-    const int* __restrict__ rp;
-    const int* __restrict__ ci;
-    const double* __restrict__ v;
-};
-inline CsrArgs args(const DCsr& M) { return CsrArgs{M.n, M.rp.p, M.ci.p, M.v.p}; }
-inline CsrArgs args(const AmgPrecond::CsrDev& M) { return CsrArgs{M.n, M.rp.p, M.ci.p, M.v.p}; }
 
 // Per-sample CG state of one inner solve.
 struct BState {
@@ -82,6 +60,16 @@ struct OState {
     const int lc = min(lane, S - 1); /* gather lane: in bounds for every lane, so loads need no mask */ \
     for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
          row < (NROWS); row += gridDim.x * kRowsPerBlock)
+
+// As B_ROW_LOOP, over XCD-contiguous eighths of the rows (blocks with equal
+// blockIdx % 8 share one XCD); needs lane, on, lc declared by the caller.
+#define B_ROW_LOOP_XCD(NROWS)                                                                          \
+    const int xr0_ = gridDim.x % kXcd ? 0 : (int)((int64_t)(NROWS) * (blockIdx.x % kXcd) / kXcd);      \
+    const int xr1_ = gridDim.x % kXcd ? (NROWS) : (int)((int64_t)(NROWS) * (blockIdx.x % kXcd + 1) / kXcd); \
+    const int xlb_ = gridDim.x % kXcd ? blockIdx.x : blockIdx.x / kXcd;                                \
+    const int xst_ = (gridDim.x % kXcd ? gridDim.x : gridDim.x / kXcd) * kRowsPerBlock;                \
+    for (int row = __builtin_amdgcn_readfirstlane(xr0_ + xlb_ * kRowsPerBlock + (int)(threadIdx.x / kWave)); \
+         row < xr1_; row += xst_)
 
 // ---- slot layout for the solver kernels ----------------------------------
 // Every operator the batched solves apply is stored in row slots of kSlot
@@ -211,25 +199,6 @@ __device__ inline void slot_walk(const SlotArgs& M, const void* x, int lc, LDF&&
     }
 }
 
-// Row entries are walked in chunks of kCh: the chunk's column indices first,
-// then kCh independent gathers, then the multiply-adds in the original entry
-// order (so sums are unchanged).  Slots past the row end repeat the row's
-// first column (an in-bounds address) and are skipped in the sums.
-constexpr int kCh = 8;
-
-template <int CH>
-__device__ inline void chunk_cols(const int* __restrict__ ci, int q, int q1, int (&c)[CH]) {
-#pragma unroll
-    for (int k = 0; k < CH; ++k) c[k] = ci[q + k < q1 ? q + k : q];
-}
-
-// The chunk's matrix values, loaded together with the columns (past the row
-// end: the row's first value, masked out by the caller's select).
-template <int CH, typename V>
-__device__ inline void chunk_vals(const V* __restrict__ v, int q, int q1, double (&w)[CH]) {
-#pragma unroll
-    for (int k = 0; k < CH; ++k) w[k] = (double)v[q + k < q1 ? q + k : q];
-}
 
 // ---- wide gathers for 64 FP32 samples -------------------------------------
 // With S = 64 in FP32 a node's row of samples is 256 B.  Lanes gather 16 B
@@ -686,40 +655,62 @@ __global__ __launch_bounds__(kBlock) void k_b_dense(int nc, int S, const double*
 }
 
 // ------------------------------------------------------ fields and LLG --
-__global__ __launch_bounds__(kBlock) void k_b_div(CsrArgs tx, const double* __restrict__ ty,
-                                                  const double* __restrict__ tz, int S, int n,
-                                                  const double* __restrict__ js, const double* __restrict__ m,
+// The per-node operators (exchange, grad, tGrad, curl) share the union
+// pattern, stored in row slots (SlotArgs) with FP64-row offsets; the slot
+// weights of each operator and the slot columns are separate arrays.  A row
+// is walked in halves of 8 slots: offsets, columns and weights by scalar
+// loads, gathers by buffer loads with the row offset (plus the component's
+// byte offset) as the SGPR operand.  Sums run in entry order (padding slots
+// have weight 0).
+constexpr int kHalf = 8;
+
+__device__ inline double buf_ld8(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// b = sum_d tGrad_d (Js m_d) scale for every sample (DemagField.cpp:124-136)
+__global__ __launch_bounds__(kBlock) void k_b_div(SlotArgs P, const double* __restrict__ wty,
+                                                  const double* __restrict__ wtz, const int* __restrict__ cidx,
+                                                  int S, const double* __restrict__ js, const double* __restrict__ m,
                                                   double scale, double* __restrict__ out) {
-    const size_t NS = (size_t)n * S;
-    B_ROW_LOOP(n) {
+    const int n = P.n;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int lc = min(lane, S - 1);
+    const int c8 = n * S * 8;  // bytes of one component
+    const __amdgpu_buffer_rsrc_t mr = gather_rsrc(m, 3 * c8);
+    const cint_p sp = (cint_p)P.sp, of = (cint_p)P.off, ci = (cint_p)cidx;
+    const cw_p<double> wx = (cw_p<double>)P.w, wy = (cw_p<double>)wty, wz = (cw_p<double>)wtz;
+    const cw_p<double> jc = (cw_p<double>)js;
+    const int vo = lc * 8;
+    B_ROW_LOOP_XCD(n) {
         double sx = 0, sy = 0, sz = 0;
-        const int q0 = tx.rp[row], q1 = tx.rp[row + 1];
-        for (int q = q0; q < q1; q += 4) {
-            int c[4];
-            chunk_cols<4>(tx.ci, q, q1, c);
-            double mv[4][3];
+        const int a0 = sp[row], a1 = sp[row + 1];
+        for (int q = a0; q < a1; q += kHalf) {
+            int o[kHalf];
+            double mx[kHalf], my[kHalf], mz[kHalf];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const size_t e = (size_t)c[k] * S + lc;
-                mv[k][0] = m[e];
-                mv[k][1] = m[NS + e];
-                mv[k][2] = m[2 * NS + e];
+            for (int k = 0; k < kHalf; ++k) o[k] = of[q + k];
+#pragma unroll
+            for (int k = 0; k < kHalf; ++k) {
+                mx[k] = buf_ld8(mr, vo, o[k]);
+                my[k] = buf_ld8(mr, vo, o[k] + c8);
+                mz[k] = buf_ld8(mr, vo, o[k] + 2 * c8);
             }
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (q + k < q1) {
-                    const double j = js[c[k]];
-                    sx += tx.v[q + k] * (mv[k][0] * j);
-                    sy += ty[q + k] * (mv[k][1] * j);
-                    sz += tz[q + k] * (mv[k][2] * j);
-                }
+            for (int k = 0; k < kHalf; ++k) {
+                const double j = jc[ci[q + k]];
+                sx += wx[q + k] * (mx[k] * j);
+                sy += wy[q + k] * (my[k] * j);
+                sz += wz[q + k] * (mz[k] * j);
+            }
         }
         if (on) out[(size_t)row * S + lane] = (sx + sy + sz) * scale;
     }
 }
 
 struct BEvalArgs {
-    CsrArgs A;                         // pattern + exchange values (v may be null)
+    SlotArgs P;                        // union pattern in slots, w = exchange weights (may be null)
     const double* gx; const double* gy; const double* gz;
     const double* cx; const double* cy; const double* cz;
     int S, n;
@@ -739,38 +730,49 @@ template <bool DMI>
 __global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
     const int S = a.S;
     const size_t NS = (size_t)a.n * S;
-    const CsrArgs A = a.A;
+    const int n = a.n;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int lc = min(lane, S - 1);
+    const int c8 = n * S * 8;
+    const __amdgpu_buffer_rsrc_t mr = gather_rsrc(a.m, 3 * c8);
+    const __amdgpu_buffer_rsrc_t u1r = gather_rsrc(a.u1, c8), u2r = gather_rsrc(a.u2, c8);
+    const cint_p sp = (cint_p)a.P.sp, of = (cint_p)a.P.off;
+    const cw_p<double> wx = (cw_p<double>)a.P.w;
+    const cw_p<double> gx = (cw_p<double>)a.gx, gy = (cw_p<double>)a.gy, gz = (cw_p<double>)a.gz;
+    const cw_p<double> cx = (cw_p<double>)a.cx, cy = (cw_p<double>)a.cy, cz = (cw_p<double>)a.cz;
+    const bool exch = a.P.w != nullptr;
+    const int vo = lc * 8;
     double tmax = 0.0;
-    B_ROW_LOOP(a.n) {
+    B_ROW_LOOP_XCD(n) {
         double ex = 0, ey = 0, ez = 0, dx = 0, dy = 0, dz = 0, cxs = 0, cys = 0, czs = 0;
-        const int q0 = A.rp[row], q1 = A.rp[row + 1];
-        for (int q = q0; q < q1; q += 4) {
-            int c[4];
-            chunk_cols<4>(A.ci, q, q1, c);
-            double mv[4][3], uv[4];
+        const int a0 = sp[row], a1 = sp[row + 1];
+        for (int q = a0; q < a1; q += kHalf) {
+            int o[kHalf];
+            double mv[kHalf][3], uv[kHalf];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const size_t e = (size_t)c[k] * S + lc;
-                mv[k][0] = a.m[e];
-                mv[k][1] = a.m[NS + e];
-                mv[k][2] = a.m[2 * NS + e];
-                uv[k] = a.u1[e] + a.u2[e];
+            for (int k = 0; k < kHalf; ++k) o[k] = of[q + k];
+#pragma unroll
+            for (int k = 0; k < kHalf; ++k) {
+                mv[k][0] = buf_ld8(mr, vo, o[k]);
+                mv[k][1] = buf_ld8(mr, vo, o[k] + c8);
+                mv[k][2] = buf_ld8(mr, vo, o[k] + 2 * c8);
+                uv[k] = buf_ld8(u1r, vo, o[k]) + buf_ld8(u2r, vo, o[k]);
             }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (q + k >= q1) continue;
+            for (int k = 0; k < kHalf; ++k) {
                 const double mx = mv[k][0], my = mv[k][1], mz = mv[k][2];
-                if (A.v) {
-                    const double xc = A.v[q + k];
+                if (exch) {
+                    const double xc = wx[q + k];
                     ex += xc * mx;
                     ey += xc * my;
                     ez += xc * mz;
                 }
-                dx += a.gx[q + k] * uv[k];
-                dy += a.gy[q + k] * uv[k];
-                dz += a.gz[q + k] * uv[k];
+                dx += gx[q + k] * uv[k];
+                dy += gy[q + k] * uv[k];
+                dz += gz[q + k] * uv[k];
                 if (DMI) {
-                    const double X = a.cx[q + k], Y = a.cy[q + k], Z = a.cz[q + k];
+                    const double X = cx[q + k], Y = cy[q + k], Z = cz[q + k];
                     cxs += Y * mz - Z * my;
                     cys += Z * mx - X * mz;
                     czs += X * my - Y * mx;
@@ -1445,8 +1447,9 @@ struct tm_batch {
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
     CtxView cv{};
-    DCsr ops;                          // union pattern with exchange values
-    DevBuf<double> gx, gy, gz, tx, ty, tz, cx, cy, cz;
+    SlotOp eo;                         // union pattern of the per-node operators, in row slots
+    DevBuf<int> eci;                   // its slot columns
+    DevBuf<double> xw, gx, gy, gz, tx, ty, tz, cx, cy, cz;  // slot weights: exchange, grad, tGrad, curl
     bool exch = false, dmi = false;
     DevBuf<double> m, dmdt, heff, u1, u2, rhsN, rhsD, U, Gm, hext, torque;
     DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product (TM_PREC_MIXED_BEM32)
@@ -1488,9 +1491,7 @@ int csr_on_pattern(const SellPattern& P, const std::vector<int>& rp, const std::
 int upload_op(tm_batch* bt, int op, DevBuf<double>& dst) {
     std::vector<double> v;
     TM_CHECK(csr_on_pattern(*bt->cv.pat, *bt->cv.hop_rp[op], *bt->cv.hop_col[op], *bt->cv.hop_val[op], v));
-    TM_CHECK(dst.upload(v.data(), v.size(), bt->stream));
-    TM_HIP(hipStreamSynchronize(bt->stream));
-    return TM_OK;
+    return bt->eo.values(v, dst, bt->stream);
 }
 
 }  // namespace
@@ -1773,16 +1774,22 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     if (rocblas_create_handle(&bt->blas) != rocblas_status_success) return fail(TM_ERR_HIP);
     rocblas_set_stream(bt->blas, bt->stream);
     const SellPattern& P = *v.pat;
-    std::vector<double> xc(P.h_csrcol.size(), 0.0);
-    bt->exch = v.hop_set[TM_OP_EXCHANGE];
-    if (bt->exch) {
-        if ((st = csr_on_pattern(P, *v.hop_rp[TM_OP_EXCHANGE], *v.hop_col[TM_OP_EXCHANGE], *v.hop_val[TM_OP_EXCHANGE],
-                                 xc)) != TM_OK)
-            return fail(st);
-    }
-    if ((st = bt->ops.upload(P.n, P.h_rowptr.data(), P.h_csrcol.data(), xc.data(), bt->stream)) != TM_OK)
+    // the field kernels address the three components of m and the potentials
+    // with 32-bit byte offsets
+    if ((int64_t)3 * P.n * nsamples * 8 > (int64_t)INT32_MAX) return fail(TM_ERR_UNSUPPORTED);
+    if ((st = bt->eo.build(P.n, P.n, P.h_rowptr.data(), P.h_csrcol.data(), nsamples, bt->stream)) != TM_OK)
         return fail(st);
+    {
+        std::vector<int> hc((size_t)bt->eo.h_sp[P.n], 0);
+        for (int i = 0; i < P.n; ++i)
+            for (int q = P.h_rowptr[i]; q < P.h_rowptr[i + 1]; ++q)
+                hc[(size_t)bt->eo.h_sp[i] + (q - P.h_rowptr[i])] = P.h_csrcol[q];
+        if ((st = bt->eci.upload(hc.data(), hc.size(), bt->stream)) != TM_OK) return fail(st);
+        if (hipStreamSynchronize(bt->stream) != hipSuccess) return fail(TM_ERR_HIP);
+    }
+    bt->exch = v.hop_set[TM_OP_EXCHANGE];
     int ops_ok = TM_OK;
+    if (bt->exch) ops_ok |= upload_op(bt, TM_OP_EXCHANGE, bt->xw);
     ops_ok |= upload_op(bt, TM_OP_GRAD_X, bt->gx);
     ops_ok |= upload_op(bt, TM_OP_GRAD_Y, bt->gy);
     ops_ok |= upload_op(bt, TM_OP_GRAD_Z, bt->gz);
@@ -1906,9 +1913,9 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     if (m) TM_CHECK(copy_in(bt->m.p, m, mwhere, 3 * NS, s));
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[0], s));
     // Poisson right-hand side (DemagField.cpp:124-136), all samples
-    hipLaunchKernelGGL(k_b_div, dim3(row_grid(n)), dim3(kBlock), 0, s,
-                       CsrArgs{n, bt->ops.rp.p, bt->ops.ci.p, bt->tx.p}, (const double*)bt->ty.p,
-                       (const double*)bt->tz.p, S, n, v.js, (const double*)bt->m.p, 1.0 / kMu0, bt->rhsN.p);
+    hipLaunchKernelGGL(k_b_div, dim3(row_grid(n)), dim3(kBlock), 0, s, bt->eo.args(8, bt->tx.p),
+                       (const double*)bt->ty.p, (const double*)bt->tz.p, (const int*)bt->eci.p, S, v.js,
+                       (const double*)bt->m.p, 1.0 / kMu0, bt->rhsN.p);
     TM_HIP(hipGetLastError());
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[1], s));
     int oN = 0, iN = 0, oD = 0, iD = 0;
@@ -1927,7 +1934,7 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[4], s));
     // fused fields + LLG
     BEvalArgs a = {};
-    a.A = CsrArgs{n, bt->ops.rp.p, bt->ops.ci.p, bt->exch ? bt->ops.v.p : nullptr};
+    a.P = bt->eo.args(8, bt->exch ? bt->xw.p : nullptr);
     a.gx = bt->gx.p;
     a.gy = bt->gy.p;
     a.gz = bt->gz.p;
