@@ -404,6 +404,12 @@ __global__ void k_b_outer_red(int G, int S, const double* part_bb, const double*
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_b_sum2(size_t tot, const double* __restrict__ a,
+                                                   const double* __restrict__ b, double* __restrict__ c) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
+        c[e] = a[e] + b[e];
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_add(size_t tot, const T* __restrict__ d, double* __restrict__ x) {
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
@@ -579,10 +585,23 @@ __global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const double* _
     auto ld = [&](int row, int j) { return *(const f4*)(b + (size_t)row * 64 + 4 * j); };
     auto epi = [&](int row, int j, f4 acc, f4 bi) {
         const size_t e = (size_t)row * 64 + 4 * j;
-        *(f4*)(x + e) = (float)(w * dinv[row]) * bi;
+        if (x) *(f4*)(x + e) = (float)(w * dinv[row]) * bi;
         *(f4*)(r + e) = bi - acc;
     };
     wide_walk<BLK>(AW, b, ld, epi);
+}
+
+// x = w dinv b + P xc: the pre-smoothed iterate (not stored by k_b_pre_w)
+// plus the coarse correction, in one pass
+template <bool BLK>
+__global__ __launch_bounds__(kBlock) void k_b_prolong_w(SlotArgs P, const double* __restrict__ dinv,
+                                                        const float* __restrict__ b, double w,
+                                                        const float* __restrict__ xc, float* __restrict__ x) {
+    auto ld = [&](int row, int j) { return *(const f4*)(b + (size_t)row * 64 + 4 * j); };
+    auto epi = [&](int row, int j, f4 acc, f4 bi) {
+        *(f4*)(x + (size_t)row * 64 + 4 * j) = (float)(w * dinv[row]) * bi + acc;
+    };
+    wide_walk<BLK>(P, xc, ld, epi);
 }
 
 template <bool ACC, bool BLK>
@@ -714,7 +733,7 @@ struct BEvalArgs {
     const double* gx; const double* gy; const double* gz;
     const double* cx; const double* cy; const double* cz;
     int S, n;
-    const double* m; const double* u1; const double* u2;
+    const double* m; const double* u;   // u = u1 + u2
     const double* invjs; const double* ku; const double* kaxis;
     const double* dmi_fac; const double* nvs;
     const double* hext;   // [3][S] per-sample uniform applied field
@@ -736,7 +755,7 @@ __global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
     const int lc = min(lane, S - 1);
     const int c8 = n * S * 8;
     const __amdgpu_buffer_rsrc_t mr = gather_rsrc(a.m, 3 * c8);
-    const __amdgpu_buffer_rsrc_t u1r = gather_rsrc(a.u1, c8), u2r = gather_rsrc(a.u2, c8);
+    const __amdgpu_buffer_rsrc_t ur = gather_rsrc(a.u, c8);
     const cint_p sp = (cint_p)a.P.sp, of = (cint_p)a.P.off;
     const cw_p<double> wx = (cw_p<double>)a.P.w;
     const cw_p<double> gx = (cw_p<double>)a.gx, gy = (cw_p<double>)a.gy, gz = (cw_p<double>)a.gz;
@@ -757,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
                 mv[k][0] = buf_ld8(mr, vo, o[k]);
                 mv[k][1] = buf_ld8(mr, vo, o[k] + c8);
                 mv[k][2] = buf_ld8(mr, vo, o[k] + 2 * c8);
-                uv[k] = buf_ld8(u1r, vo, o[k]) + buf_ld8(u2r, vo, o[k]);
+                uv[k] = buf_ld8(ur, vo, o[k]);
             }
 #pragma unroll
             for (int k = 0; k < kHalf; ++k) {
@@ -1276,10 +1295,12 @@ struct BatchSolver {
         for (int l = 0; l < L; ++l) {
             const auto& V = M.levels[l];
             const LvOps& O = lo[l];
+            // the wide form leaves x = w dinv b to the prolongation pass
+            T* xpre = wide() ? nullptr : lv[l].x.p;
             if (l == 0)
-                pre(sl0.long_rows(), dim3(G), sl0.args(e, wAW0.p), dinv.p, b, V.omega, lv[0].x.p, lv[0].r.p, s);
+                pre(sl0.long_rows(), dim3(G), sl0.args(e, wAW0.p), dinv.p, b, V.omega, xpre, lv[0].r.p, s);
             else
-                pre(O.A.long_rows(), dim3(row_grid(V.n)), O.A.args(e, O.wAW.p), V.dinv.p, b, V.omega, lv[l].x.p,
+                pre(O.A.long_rows(), dim3(row_grid(V.n)), O.A.args(e, O.wAW.p), V.dinv.p, b, V.omega, xpre,
                     lv[l].r.p, s);
             spmm(false, O.R.long_rows(), dim3(row_grid(V.R.n)), O.R.args(e, O.wR.p), lv[l].r.p, lv[l].b.p, s);
             b = lv[l].b.p;
@@ -1291,7 +1312,11 @@ struct BatchSolver {
             const auto& V = M.levels[l];
             const LvOps& O = lo[l];
             const T* bl = l == 0 ? r : lv[l - 1].b.p;
-            spmm(true, O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), xcoarse, lv[l].x.p, s);
+            if (wide())
+                prolong(O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), l == 0 ? dinv.p : V.dinv.p, bl,
+                        V.omega, xcoarse, lv[l].x.p, s);
+            else
+                spmm(true, O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), xcoarse, lv[l].x.p, s);
             if (l == 0) {
                 post(sl0.long_rows(), dim3(G), sl0.args(e, wA0.p), dinv.p, bl, V.omega, lv[0].x.p, zout, prz, s);
                 xcoarse = zout;
@@ -1337,6 +1362,18 @@ struct BatchSolver {
             }
         if (acc) hipLaunchKernelGGL((k_b_spmm<T, true>), g, dim3(kBlock), 0, s, a, S, x, y);
         else hipLaunchKernelGGL((k_b_spmm<T, false>), g, dim3(kBlock), 0, s, a, S, x, y);
+    }
+    // x = w dinv b + P xc (wide form only)
+    void prolong(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* xc, T* x,
+                 hipStream_t s) {
+        if constexpr (kWideT) {
+            if (lg)
+                hipLaunchKernelGGL(k_b_prolong_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, di, (const float*)bb, w,
+                                   (const float*)xc, (float*)x);
+            else
+                hipLaunchKernelGGL(k_b_prolong_w<false>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w,
+                                   (const float*)xc, (float*)x);
+        }
     }
     void post(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* x, T* xo, double* part,
               hipStream_t s) {
@@ -1451,7 +1488,7 @@ struct tm_batch {
     DevBuf<int> eci;                   // its slot columns
     DevBuf<double> xw, gx, gy, gz, tx, ty, tz, cx, cy, cz;  // slot weights: exchange, grad, tGrad, curl
     bool exch = false, dmi = false;
-    DevBuf<double> m, dmdt, heff, u1, u2, rhsN, rhsD, U, Gm, hext, torque;
+    DevBuf<double> m, dmdt, heff, u1, u2, usum, rhsN, rhsD, U, Gm, hext, torque;
     DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product (TM_PREC_MIXED_BEM32)
     DevBuf<float> Pf;                  // its split-K partial products (nchunk x b x S)
     int kchunk = 512, nchunk = 0;
@@ -1810,6 +1847,7 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     a |= bt->heff.alloc(3 * NS);
     a |= bt->u1.alloc(NS);
     a |= bt->u2.alloc(NS);
+    a |= bt->usum.alloc(NS);
     a |= bt->rhsN.alloc(NS);
     a |= bt->rhsD.alloc(NS);
     if (precision == TM_PREC_MIXED_BEM32) {
@@ -1932,6 +1970,9 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
                                        : bt->sD32.solve(bt->rhsD.p, bt->u2.p, v.tol, v.maxit, s, &oD, &iD, &rD);
     if (stD != TM_OK && stD != TM_ERR_MAXITER) return stD;
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[4], s));
+    // total potential, gathered once per entry by the field kernel
+    hipLaunchKernelGGL(k_b_sum2, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const double*)bt->u1.p,
+                       (const double*)bt->u2.p, bt->usum.p);
     // fused fields + LLG
     BEvalArgs a = {};
     a.P = bt->eo.args(8, bt->exch ? bt->xw.p : nullptr);
@@ -1948,8 +1989,7 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     a.S = S;
     a.n = n;
     a.m = bt->m.p;
-    a.u1 = bt->u1.p;
-    a.u2 = bt->u2.p;
+    a.u = bt->usum.p;
     a.invjs = v.invjs;
     a.ku = v.has_ku ? v.ku : nullptr;
     a.kaxis = v.has_ku ? v.kaxis : nullptr;
