@@ -454,11 +454,13 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T
 }
 
 // q = A p; partial (p, q) -- FP32, S = 64, wide gathers
-__global__ __launch_bounds__(kBlock) void k_b_cg_spmv_w(SlotArgs A, const float* __restrict__ p,
-                                                        float* __restrict__ q, double* part) {
+__global__ __launch_bounds__(kBlock) void k_b_cg_spmv_w(SlotArgs A, const float* __restrict__ dg,
+                                                        const float* __restrict__ p, float* __restrict__ q,
+                                                        double* part) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     auto ld = [&](int row, int j) { return *(const f4*)(p + (size_t)row * 64 + 4 * j); };
     auto epi = [&](int row, int j, f4 acc, f4 pi) {
+        if (dg) acc += dg[row] * pi;  // diagonal kept out of the slots
         *(f4*)(q + (size_t)row * 64 + 4 * j) = acc;
         v[0] += (double)pi.x * (double)acc.x;
         v[1] += (double)pi.y * (double)acc.y;
@@ -579,11 +581,12 @@ __global__ __launch_bounds__(kBlock) void k_b_post(SlotArgs A, int S, const doub
 // Wide-gather forms (FP32, S = 64) of the pre-smoother, the transfer
 // products and the post-smoother.
 template <bool BLK>
-__global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const double* __restrict__ dinv,
-                                                    const float* __restrict__ b, double w, float* __restrict__ x,
-                                                    float* __restrict__ r) {
+__global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const float* __restrict__ dg,
+                                                    const double* __restrict__ dinv, const float* __restrict__ b,
+                                                    double w, float* __restrict__ x, float* __restrict__ r) {
     auto ld = [&](int row, int j) { return *(const f4*)(b + (size_t)row * 64 + 4 * j); };
     auto epi = [&](int row, int j, f4 acc, f4 bi) {
+        if (dg) acc += dg[row] * bi;  // diagonal kept out of the slots
         const size_t e = (size_t)row * 64 + 4 * j;
         if (x) *(f4*)(x + e) = (float)(w * dinv[row]) * bi;
         *(f4*)(r + e) = bi - acc;
@@ -614,9 +617,9 @@ __global__ __launch_bounds__(kBlock) void k_b_spmm_w(SlotArgs M, const float* __
 }
 
 template <bool BLK>
-__global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const double* __restrict__ dinv,
-                                                     const float* __restrict__ b, double w,
-                                                     const float* __restrict__ x, float* __restrict__ xo,
+__global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const float* __restrict__ dg,
+                                                     const double* __restrict__ dinv, const float* __restrict__ b,
+                                                     double w, const float* __restrict__ x, float* __restrict__ xo,
                                                      double* part) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     struct BX {
@@ -627,6 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const double* _
         return BX{*(const f4*)(b + e), *(const f4*)(x + e)};
     };
     auto epi = [&](int row, int j, f4 acc, const BX& bx) {
+        if (dg) acc += dg[row] * bx.x;  // diagonal kept out of the slots
         const f4 zi = bx.x + (float)(w * dinv[row]) * (bx.b - acc);
         *(f4*)(xo + (size_t)row * 64 + 4 * j) = zi;
         v[0] += (double)bx.b.x * (double)zi.x;
@@ -1139,28 +1143,37 @@ int fetch_csr(const AmgPrecond::CsrDev& M, HostCsr& h, hipStream_t s) {
 // offsets for gathered elements of 4 and of 8 bytes.
 struct SlotOp {
     int n = 0, ncols = 0, S = 0;
-    std::vector<int> h_rp, h_sp;
+    bool nodiag = false;          // diagonal entries left out (applied by the kernel's epilogue)
+    std::vector<int> h_rp, h_ci, h_sp;
     DevBuf<int> sp, off4, off8;
 
-    int build(int rows, int cols, const int* rp, const int* ci, int S_, hipStream_t s) {
+    bool kept(int i, int q) const { return !(nodiag && h_ci[q] == i); }
+
+    int build(int rows, int cols, const int* rp, const int* ci, int S_, hipStream_t s, bool drop_diag = false) {
         n = rows;
         ncols = cols;
         S = S_;
+        nodiag = drop_diag;
         if ((int64_t)cols * S * 8 > (int64_t)INT32_MAX) return TM_ERR_UNSUPPORTED;  // 32-bit byte offsets
         h_rp.assign(rp, rp + rows + 1);
+        h_ci.assign(ci, ci + rp[rows]);
         h_sp.assign((size_t)rows + 1, 0);
         for (int i = 0; i < rows; ++i) {
-            const int len = rp[i + 1] - rp[i];
+            int len = 0;
+            for (int q = rp[i]; q < rp[i + 1]; ++q) len += kept(i, q);
             h_sp[i + 1] = h_sp[i] + std::max(1, (len + kSlot - 1) / kSlot) * kSlot;
         }
         const size_t ns = (size_t)h_sp[rows];
         std::vector<int> o4(ns, 0), o8(ns, 0);
-        for (int i = 0; i < rows; ++i)
-            for (int q = rp[i]; q < rp[i + 1]; ++q) {
-                const size_t k = (size_t)h_sp[i] + (q - rp[i]);
-                o4[k] = ci[q] * S * 4;
-                o8[k] = ci[q] * S * 8;
-            }
+        for (int i = 0; i < rows; ++i) {
+            size_t k = (size_t)h_sp[i];
+            for (int q = rp[i]; q < rp[i + 1]; ++q)
+                if (kept(i, q)) {
+                    o4[k] = ci[q] * S * 4;
+                    o8[k] = ci[q] * S * 8;
+                    ++k;
+                }
+        }
         TM_CHECK(sp.upload(h_sp.data(), h_sp.size(), s));
         TM_CHECK(off4.upload(o4.data(), ns, s));
         TM_CHECK(off8.upload(o8.data(), ns, s));
@@ -1173,9 +1186,27 @@ struct SlotOp {
     int values(const std::vector<double>& v, DevBuf<W>& out, hipStream_t s, const int* ci = nullptr,
                const double* colscale = nullptr, double w0 = 1.0) const {
         std::vector<W> h((size_t)h_sp[n], (W)0);
-        for (int i = 0; i < n; ++i)
+        for (int i = 0; i < n; ++i) {
+            size_t k = (size_t)h_sp[i];
             for (int q = h_rp[i]; q < h_rp[i + 1]; ++q)
-                h[(size_t)h_sp[i] + (q - h_rp[i])] = (W)(colscale ? v[q] * w0 * colscale[ci[q]] : v[q]);
+                if (kept(i, q)) h[k++] = (W)(colscale ? v[q] * w0 * colscale[ci[q]] : v[q]);
+        }
+        TM_CHECK(out.upload(h.data(), h.size(), s));
+        TM_HIP(hipStreamSynchronize(s));
+        return TM_OK;
+    }
+
+    // the diagonal (summed if repeated) of CSR values v, optionally scaled as in values()
+    template <typename W>
+    int diagonal(const std::vector<double>& v, DevBuf<W>& out, hipStream_t s, const double* colscale = nullptr,
+                 double w0 = 1.0) const {
+        std::vector<W> h((size_t)n, (W)0);
+        for (int i = 0; i < n; ++i) {
+            double d = 0.0;
+            for (int q = h_rp[i]; q < h_rp[i + 1]; ++q)
+                if (h_ci[q] == i) d += colscale ? v[q] * w0 * colscale[i] : v[q];
+            h[i] = (W)d;
+        }
         TM_CHECK(out.upload(h.data(), h.size(), s));
         TM_HIP(hipStreamSynchronize(s));
         return TM_OK;
@@ -1194,6 +1225,8 @@ struct BatchSolver {
     int n = 0, S = 0, G = 0;
     SlotOp sl0;                   // level-0 operator in slots
     DevBuf<T> wA0, wAW0;          // its weights: A, and A w D^-1 (pre-smoother)
+    SlotOp sl0w;                  // the same without the diagonal, for the wide kernels
+    DevBuf<T> wA0w, wAW0w, dA0, dAW0;  // its weights and the diagonals
     DevBuf<double> wA64;          // A in FP64 (outer residual)
     struct LvOps {
         SlotOp A, R, P;           // level l >= 1: A; every level: restriction, prolongation
@@ -1234,6 +1267,14 @@ struct BatchSolver {
         if (L > 0)
             TM_CHECK(sl0.values(solver.h_val, wAW0, s, solver.pat.h_csrcol.data(), solver.h_dinv.data(),
                                 M.levels[0].omega));
+        if (wide() && L > 0) {
+            const int* cc = solver.pat.h_csrcol.data();
+            TM_CHECK(sl0w.build(n, n, solver.pat.h_rowptr.data(), cc, S, s, true));
+            TM_CHECK(sl0w.values(solver.h_val, wA0w, s));
+            TM_CHECK(sl0w.values(solver.h_val, wAW0w, s, cc, solver.h_dinv.data(), M.levels[0].omega));
+            TM_CHECK(sl0w.diagonal(solver.h_val, dA0, s));
+            TM_CHECK(sl0w.diagonal(solver.h_val, dAW0, s, solver.h_dinv.data(), M.levels[0].omega));
+        }
         lo.clear();
         lo.resize((size_t)L);
         for (int l = 0; l < L; ++l) {
@@ -1297,7 +1338,9 @@ struct BatchSolver {
             const LvOps& O = lo[l];
             // the wide form leaves x = w dinv b to the prolongation pass
             T* xpre = wide() ? nullptr : lv[l].x.p;
-            if (l == 0)
+            if (l == 0 && wide())
+                pre(false, dim3(G), sl0w.args(e, wAW0w.p), dinv.p, b, V.omega, xpre, lv[0].r.p, s, dAW0.p);
+            else if (l == 0)
                 pre(sl0.long_rows(), dim3(G), sl0.args(e, wAW0.p), dinv.p, b, V.omega, xpre, lv[0].r.p, s);
             else
                 pre(O.A.long_rows(), dim3(row_grid(V.n)), O.A.args(e, O.wAW.p), V.dinv.p, b, V.omega, xpre,
@@ -1318,7 +1361,10 @@ struct BatchSolver {
             else
                 spmm(true, O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), xcoarse, lv[l].x.p, s);
             if (l == 0) {
-                post(sl0.long_rows(), dim3(G), sl0.args(e, wA0.p), dinv.p, bl, V.omega, lv[0].x.p, zout, prz, s);
+                if (wide())
+                    post(false, dim3(G), sl0w.args(e, wA0w.p), dinv.p, bl, V.omega, lv[0].x.p, zout, prz, s, dA0.p);
+                else
+                    post(sl0.long_rows(), dim3(G), sl0.args(e, wA0.p), dinv.p, bl, V.omega, lv[0].x.p, zout, prz, s);
                 xcoarse = zout;
             } else {
                 post(O.A.long_rows(), dim3(row_grid(V.n)), O.A.args(e, O.wA.p), V.dinv.p, bl, V.omega, lv[l].x.p,
@@ -1335,15 +1381,17 @@ struct BatchSolver {
     bool wide() const { return kWideT && S == kMaxS; }
     // one block per row for long-row operators (grid: a multiple of the XCD count)
     static dim3 blk_grid(int rows) { return dim3(xcd_grid(std::min(4096, std::max(rows, 1)))); }
-    void pre(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, T* x, T* rr, hipStream_t s) {
+    void pre(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, T* x, T* rr, hipStream_t s,
+             const T* dg = nullptr) {
         if constexpr (kWideT)
             if (wide()) {
+                const float* df = (const float*)dg;
                 if (lg)
-                    hipLaunchKernelGGL(k_b_pre_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, di, (const float*)bb, w,
-                                       (float*)x, (float*)rr);
+                    hipLaunchKernelGGL(k_b_pre_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, df, di,
+                                       (const float*)bb, w, (float*)x, (float*)rr);
                 else
-                    hipLaunchKernelGGL(k_b_pre_w<false>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w, (float*)x,
-                                       (float*)rr);
+                    hipLaunchKernelGGL(k_b_pre_w<false>, g, dim3(kBlock), 0, s, a, df, di, (const float*)bb, w,
+                                       (float*)x, (float*)rr);
                 return;
             }
         hipLaunchKernelGGL((k_b_pre<T>), g, dim3(kBlock), 0, s, a, S, di, bb, w, x, rr);
@@ -1376,14 +1424,15 @@ struct BatchSolver {
         }
     }
     void post(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* x, T* xo, double* part,
-              hipStream_t s) {
+              hipStream_t s, const T* dg = nullptr) {
         if constexpr (kWideT)
             if (wide()) {
+                const float* df = (const float*)dg;
                 if (lg && !part)  // partial sums are laid out per block of the row grid g
-                    hipLaunchKernelGGL(k_b_post_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, di, (const float*)bb,
-                                       w, (const float*)x, (float*)xo, part);
+                    hipLaunchKernelGGL(k_b_post_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, df, di,
+                                       (const float*)bb, w, (const float*)x, (float*)xo, part);
                 else
-                    hipLaunchKernelGGL(k_b_post_w<false>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w,
+                    hipLaunchKernelGGL(k_b_post_w<false>, g, dim3(kBlock), 0, s, a, df, di, (const float*)bb, w,
                                        (const float*)x, (float*)xo, part);
                 return;
             }
@@ -1394,8 +1443,8 @@ struct BatchSolver {
     void cg_spmv(const T* pc, double* ppq, hipStream_t s) {
         if constexpr (kWideT)
             if (wide()) {
-                hipLaunchKernelGGL(k_b_cg_spmv_w, dim3(G), dim3(kBlock), 0, s, sl0.args(4, wA0.p), (const float*)pc,
-                                   (float*)q.p, ppq);
+                hipLaunchKernelGGL(k_b_cg_spmv_w, dim3(G), dim3(kBlock), 0, s, sl0w.args(4, wA0w.p),
+                                   (const float*)dA0.p, (const float*)pc, (float*)q.p, ppq);
                 return;
             }
         hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, sl0.args((int)sizeof(T), wA0.p), S, pc, q.p,
