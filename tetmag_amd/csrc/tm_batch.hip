@@ -21,6 +21,7 @@
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <vector>
 
@@ -30,7 +31,7 @@ namespace tmh {
 
 namespace {
 
-constexpr int kMaxS = 64;
+constexpr int kMaxS = 64;  // any_active() reads 64 flags
 constexpr int kRowsPerBlock = kBlock / kWave;  // one row per wavefront
 
 
@@ -91,9 +92,25 @@ struct SlotArgs {
     const int* sp;    // n + 1 slot starts
     const int* off;   // per slot: byte offset of the gathered row
     const void* w;    // per slot: weight (float or double)
+    const int* gate = nullptr;  // per-sample active flags: the kernel does nothing when none is set
+    int gate_n = 0;
 };
 
 typedef const int __attribute__((address_space(4)))* cint_p;
+
+// true when any of the flags is set (or there are no flags).  The batched
+// CG enqueues iterations in chunks ahead of its convergence check; once every
+// sample has converged, the iterations still queued return at once.  All
+// kMaxS flags are read (four 16-dword scalar loads; flags past the batch size
+// stay 0 from the solver's set-up).
+__device__ inline bool any_active(const int* act, int) {
+    if (!act) return true;
+    const cint_p a = (cint_p)act;
+    int any = 0;
+#pragma unroll
+    for (int s = 0; s < 64; ++s) any |= a[s];
+    return any != 0;
+}
 template <typename W>
 using cw_p = const W __attribute__((address_space(4)))*;
 
@@ -428,6 +445,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_cg_dir(size_t NS, int S, const T* __restrict__ z,
                                                      const T* __restrict__ pold, T* __restrict__ pnew, int k,
                                                      const BState* bs) {
+    if (!any_active(bs->active, S)) return;
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock) {
         const int sm = (int)(e % S);
         pnew[e] = k > 0 ? z[e] + (T)bs->beta[sm] * pold[e] : z[e];
@@ -438,6 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_dir(size_t NS, int S, const T* 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T* __restrict__ p, T* __restrict__ q,
                                                       double* part) {
+    if (!any_active(A.gate, A.gate_n)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
@@ -457,6 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T
 __global__ __launch_bounds__(kBlock) void k_b_cg_spmv_w(SlotArgs A, const float* __restrict__ dg,
                                                         const float* __restrict__ p, float* __restrict__ q,
                                                         double* part) {
+    if (!any_active(A.gate, A.gate_n)) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     auto ld = [&](int row, int j) { return *(const f4*)(p + (size_t)row * 64 + 4 * j); };
     auto epi = [&](int row, int j, f4 acc, f4 pi) {
@@ -486,6 +506,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_cg_update(int n, int S, const T* __restrict__ p,
                                                         const T* __restrict__ q, T* __restrict__ d,
                                                         T* __restrict__ r, const BState* bs, double* part) {
+    if (!any_active(bs->active, S)) return;
     const int ln = threadIdx.x & (kWave - 1);
     const T alpha = ln < S ? (T)bs->alpha[ln] : (T)0;
     double v = 0.0;
@@ -524,6 +545,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_pre(SlotArgs AW, int S, const double* __restrict__ dinv,
                                                   const T* __restrict__ b, double w, T* __restrict__ x,
                                                   T* __restrict__ r) {
+    if (!any_active(AW.gate, AW.gate_n)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
@@ -541,6 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_b_pre(SlotArgs AW, int S, const doub
 // y = M x (ACC: y += M x)
 template <typename T, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_b_spmm(SlotArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
+    if (!any_active(M.gate, M.gate_n)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
@@ -556,6 +579,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_post(SlotArgs A, int S, const double* __restrict__ dinv,
                                                    const T* __restrict__ b, double w, const T* __restrict__ x,
                                                    T* __restrict__ xo, double* part) {
+    if (!any_active(A.gate, A.gate_n)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
@@ -584,6 +608,7 @@ template <bool BLK>
 __global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const float* __restrict__ dg,
                                                     const double* __restrict__ dinv, const float* __restrict__ b,
                                                     double w, float* __restrict__ x, float* __restrict__ r) {
+    if (!any_active(AW.gate, AW.gate_n)) return;
     auto ld = [&](int row, int j) { return *(const f4*)(b + (size_t)row * 64 + 4 * j); };
     auto epi = [&](int row, int j, f4 acc, f4 bi) {
         if (dg) acc += dg[row] * bi;  // diagonal kept out of the slots
@@ -600,6 +625,7 @@ template <bool BLK>
 __global__ __launch_bounds__(kBlock) void k_b_prolong_w(SlotArgs P, const double* __restrict__ dinv,
                                                         const float* __restrict__ b, double w,
                                                         const float* __restrict__ xc, float* __restrict__ x) {
+    if (!any_active(P.gate, P.gate_n)) return;
     auto ld = [&](int row, int j) { return *(const f4*)(b + (size_t)row * 64 + 4 * j); };
     auto epi = [&](int row, int j, f4 acc, f4 bi) {
         *(f4*)(x + (size_t)row * 64 + 4 * j) = (float)(w * dinv[row]) * bi + acc;
@@ -609,6 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_b_prolong_w(SlotArgs P, const double
 
 template <bool ACC, bool BLK>
 __global__ __launch_bounds__(kBlock) void k_b_spmm_w(SlotArgs M, const float* __restrict__ x, float* __restrict__ y) {
+    if (!any_active(M.gate, M.gate_n)) return;
     auto ld = [&](int row, int j) {
         return ACC ? *(const f4*)(y + (size_t)row * 64 + 4 * j) : f4{0.f, 0.f, 0.f, 0.f};
     };
@@ -621,6 +648,7 @@ __global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const float* __
                                                      const double* __restrict__ dinv, const float* __restrict__ b,
                                                      double w, const float* __restrict__ x, float* __restrict__ xo,
                                                      double* part) {
+    if (!any_active(A.gate, A.gate_n)) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     struct BX {
         f4 b, x;
@@ -648,7 +676,9 @@ __global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const float* __
 constexpr int kDenseCh = 16;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_dense(int nc, int S, const double* __restrict__ C,
-                                                    const T* __restrict__ b, T* __restrict__ x) {
+                                                    const T* __restrict__ b, T* __restrict__ x,
+                                                    const int* gate) {
+    if (!any_active(gate, S)) return;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int lc = min(lane, S - 1);
     const cw_p<double> Cc = (cw_p<double>)C;
@@ -1247,7 +1277,8 @@ struct BatchSolver {
     DevBuf<BState> bst;
     OState* h_ost = nullptr;
     BState* h_bst = nullptr;
-    int last_inner = 8;
+    static constexpr int kMaxOuter = 8;
+    int last_inner[kMaxOuter] = {8, 8, 8, 8, 8, 8, 8, 8};  // per refinement step
 
     ~BatchSolver() {
         if (h_ost) (void)hipHostFree(h_ost);
@@ -1320,6 +1351,7 @@ struct BatchSolver {
         TM_CHECK(part.alloc((size_t)4 * S * G));
         TM_CHECK(ost.alloc(1));
         TM_CHECK(bst.alloc(1));
+        TM_HIP(hipMemsetAsync(bst.p, 0, sizeof(BState), s));  // flags past the batch size stay 0
         if (!h_ost) TM_HIP(hipHostMalloc((void**)&h_ost, sizeof(OState), hipHostMallocDefault));
         if (!h_bst) TM_HIP(hipHostMalloc((void**)&h_bst, sizeof(BState), hipHostMallocDefault));
         TM_HIP(hipStreamSynchronize(s));
@@ -1349,7 +1381,7 @@ struct BatchSolver {
             b = lv[l].b.p;
         }
         hipLaunchKernelGGL((k_b_dense<T>), dim3(std::max(1, std::min(M.nc, 4096))), dim3(kBlock), 0, s, M.nc, S,
-                           M.Cinv.p, b, xc.p);
+                           M.Cinv.p, b, xc.p, gate());
         const T* xcoarse = xc.p;
         for (int l = L - 1; l >= 0; --l) {
             const auto& V = M.levels[l];
@@ -1376,6 +1408,16 @@ struct BatchSolver {
         return TM_OK;
     }
 
+    // the per-sample active flags of the inner solve (device address)
+    const int* gate() const {
+        return bst.p ? (const int*)((const char*)bst.p + offsetof(BState, active)) : nullptr;
+    }
+    SlotArgs gated(SlotArgs a) const {
+        a.gate = gate();
+        a.gate_n = S;
+        return a;
+    }
+
     // V-cycle kernels; the wide-gather forms for 64 FP32 samples
     static constexpr bool kWideT = sizeof(T) == 4;
     bool wide() const { return kWideT && S == kMaxS; }
@@ -1383,6 +1425,7 @@ struct BatchSolver {
     static dim3 blk_grid(int rows) { return dim3(xcd_grid(std::min(4096, std::max(rows, 1)))); }
     void pre(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, T* x, T* rr, hipStream_t s,
              const T* dg = nullptr) {
+        a = gated(a);
         if constexpr (kWideT)
             if (wide()) {
                 const float* df = (const float*)dg;
@@ -1397,6 +1440,7 @@ struct BatchSolver {
         hipLaunchKernelGGL((k_b_pre<T>), g, dim3(kBlock), 0, s, a, S, di, bb, w, x, rr);
     }
     void spmm(bool acc, bool lg, dim3 g, SlotArgs a, const T* x, T* y, hipStream_t s) {
+        a = gated(a);
         if constexpr (kWideT)
             if (wide()) {
                 const float* xf = (const float*)x;
@@ -1414,6 +1458,7 @@ struct BatchSolver {
     // x = w dinv b + P xc (wide form only)
     void prolong(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* xc, T* x,
                  hipStream_t s) {
+        a = gated(a);
         if constexpr (kWideT) {
             if (lg)
                 hipLaunchKernelGGL(k_b_prolong_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, di, (const float*)bb, w,
@@ -1425,6 +1470,7 @@ struct BatchSolver {
     }
     void post(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* x, T* xo, double* part,
               hipStream_t s, const T* dg = nullptr) {
+        a = gated(a);
         if constexpr (kWideT)
             if (wide()) {
                 const float* df = (const float*)dg;
@@ -1443,12 +1489,12 @@ struct BatchSolver {
     void cg_spmv(const T* pc, double* ppq, hipStream_t s) {
         if constexpr (kWideT)
             if (wide()) {
-                hipLaunchKernelGGL(k_b_cg_spmv_w, dim3(G), dim3(kBlock), 0, s, sl0w.args(4, wA0w.p),
+                hipLaunchKernelGGL(k_b_cg_spmv_w, dim3(G), dim3(kBlock), 0, s, gated(sl0w.args(4, wA0w.p)),
                                    (const float*)dA0.p, (const float*)pc, (float*)q.p, ppq);
                 return;
             }
-        hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, sl0.args((int)sizeof(T), wA0.p), S, pc, q.p,
-                           ppq);
+        hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, gated(sl0.args((int)sizeof(T), wA0.p)), S,
+                           pc, q.p, ppq);
     }
 
     // Solves A x = b for all samples ([n][S] FP64); x is the warm start.
@@ -1483,7 +1529,11 @@ struct BatchSolver {
             TM_CHECK(precond(rT.p, z.p, prz, s));
             hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, -1, bst.p);
             int enq = 0;
-            int chunk = std::max(4, last_inner);
+            // iterations enqueued before the first check: what this refinement
+            // step needed last time, plus one (queued iterations past
+            // convergence return at once)
+            const int step = std::min(outer, kMaxOuter) - 1;
+            int chunk = std::max(2, last_inner[step] + 1);
             T* pc = p0.p;
             T* po = p1.p;
             for (;;) {
@@ -1506,8 +1556,10 @@ struct BatchSolver {
                 if (!any || enq >= maxit) break;
                 chunk = 2;
             }
-            last_inner = std::max(4, enq - 1);
-            inner_total += enq;
+            int used = 0;  // iterations the slowest sample ran
+            for (int k = 0; k < S; ++k) used = std::max(used, h_bst->iters[k]);
+            last_inner[step] = used;
+            inner_total += used;
             hipLaunchKernelGGL((k_b_add<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const T*)d.p, x);
         }
         if (outer_out) *outer_out = outer;
