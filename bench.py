@@ -252,7 +252,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--dt-fs", type=float, default=20.0, help="Euler step between evaluations (fs)")
-    ap.add_argument("--cpu-evals", type=int, default=2)
+    ap.add_argument("--cpu-evals", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="20x20x4 cuboid instead (quick check)")
     ap.add_argument("--batch", type=int, default=64, help="samples per batched evaluation")
