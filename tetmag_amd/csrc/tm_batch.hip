@@ -4,10 +4,12 @@
 // and its hysteresis driver runs field values one after another
 // (Hysteresis.cpp:43-169).  Here up to 64 samples share every operator read:
 // vectors are stored node-major, sample-fastest ([node][sample]); each sparse
-// product runs one wavefront per row with one lane per sample, so the row's
-// column indices and values are wavefront-uniform loads read once for all
-// samples and every gather is one contiguous access.  The boundary-element
-// product becomes a GEMM (B times the b x S boundary potentials).
+// product runs one wavefront per row with the samples across its lanes, so
+// the row's offsets and weights (row-slot layout, below) are wavefront-uniform
+// scalar loads read once for all samples and every gather is one contiguous
+// access (64 FP32 samples: 16 lanes x 16 bytes per gathered row, four rows per
+// load instruction).  The boundary-element product becomes a GEMM (B times the
+// b x S boundary potentials) on the FP64 matrix cores.
 //
 // Both demag solves use the same scheme per sample:
 //   outer (FP64):  x_fixed = b_fixed;  r = b - A x  (FP64 operator and vectors)
@@ -16,7 +18,8 @@
 // T = double solves to the final tolerance in one outer step (the FP64
 // batched form); T = float runs the inner iterations in FP32 to a relative
 // 1e-5 and recovers FP64 accuracy by iterative refinement (the mixed form).
-// Converged samples are frozen (their step lengths are zero).
+// Converged samples are frozen (their step lengths are zero); once all are,
+// the iterations still queued return at once.
 #define ROCBLAS_BETA_FEATURES_API  // rocblas_gemm_ex_get_solutions (the GEMM autotuner)
 #include <rocblas/rocblas.h>
 
