@@ -5,7 +5,7 @@
 // AMGCLSolver.cpp:39-52).  AMGCL is not vendored, so this is a fresh
 // implementation of the published method (Vanek, Mandel, Brezina, Computing
 // 56 (1996) 179-196):
-//   * strength of connection |a_ij| >= theta sqrt(a_ii a_jj), theta = 0.08;
+//   * strength of connection |a_ij| >= theta sqrt(a_ii a_jj), theta = 0.04;
 //   * greedy aggregation (root nodes with unaggregated strong neighbourhoods,
 //     then attachment of the remaining nodes, then leftover aggregates);
 //   * tentative prolongator = aggregate indicator (constant near-null space),
@@ -439,7 +439,7 @@ int AmgPrecond::setup(int n, const int* rowptr, const int* colidx, const double*
         const double rho = spectral_radius(cur, d);
         const double w = 4.0 / (3.0 * rho);
         std::vector<int> agg;
-        const int na = aggregate(cur, d, excl, 0.08, agg);
+        const int na = aggregate(cur, d, excl, 0.04, agg);
         if (na == 0 || na >= cur.n) break;
         // tentative prolongator
         Csr T;
