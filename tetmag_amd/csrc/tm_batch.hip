@@ -232,6 +232,8 @@ __device__ inline void slot_walk(const SlotArgs& M, const void* x, int lc, LDF&&
 // dword form).
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef int i4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef int i2 __attribute__((ext_vector_type(2)));
 
 __device__ inline f4 buf_ld4(__amdgpu_buffer_rsrc_t r, int voff) {
     return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
@@ -257,8 +259,9 @@ __device__ inline f4 shfl_xor4(f4 v, int m) {
 // wavefronts' sums added in LDS.  ld(row, j) loads the epilogue's operands
 // for samples 4j .. 4j+3; epi(row, j, acc, ld) runs on lanes 0-15 (of
 // wavefront 0 with BLK).
-template <bool BLK, typename LDF, typename EPIF>
+template <bool BLK, int CH = kSlot, typename LDF, typename EPIF>
 __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPIF&& epi) {
+    static_assert(CH == 16 || CH == 8, "chunks of 16 or 8 slots");
     const int n = M.n;
     const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, j = lane & 15;
     const int wv = threadIdx.x / kWave;
@@ -276,14 +279,22 @@ __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPI
         r1 = (int)((int64_t)n * (xi + 1) / kXcd);
     }
     auto chunk = [&](int q, f4 acc) {
-        const i4 o = of[(q >> 2) + g];
-        const f4 ww = w[(q >> 2) + g];
-        const f4 x0 = ld4_if(xr, vo + o.x, ww.x), x1 = ld4_if(xr, vo + o.y, ww.y);
-        const f4 x2 = ld4_if(xr, vo + o.z, ww.z), x3 = ld4_if(xr, vo + o.w, ww.w);
-        acc += ww.x * x0;
-        acc += ww.y * x1;
-        acc += ww.z * x2;
-        acc += ww.w * x3;
+        if constexpr (CH == 16) {  // lane group g: slots q+4g .. q+4g+3
+            const i4 o = of[(q >> 2) + g];
+            const f4 ww = w[(q >> 2) + g];
+            const f4 x0 = ld4_if(xr, vo + o.x, ww.x), x1 = ld4_if(xr, vo + o.y, ww.y);
+            const f4 x2 = ld4_if(xr, vo + o.z, ww.z), x3 = ld4_if(xr, vo + o.w, ww.w);
+            acc += ww.x * x0;
+            acc += ww.y * x1;
+            acc += ww.z * x2;
+            acc += ww.w * x3;
+        } else {  // lane group g: slots q+2g, q+2g+1
+            const i2 o = ((const i2*)M.off)[(q >> 1) + g];
+            const f2 ww = ((const f2*)M.w)[(q >> 1) + g];
+            const f4 x0 = ld4_if(xr, vo + o.x, ww.x), x1 = ld4_if(xr, vo + o.y, ww.y);
+            acc += ww.x * x0;
+            acc += ww.y * x1;
+        }
         return acc;
     };
     if constexpr (!BLK) {
@@ -292,7 +303,7 @@ __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPI
             const int a0 = sp[row], a1 = sp[row + 1];
             const auto lv = ld(row, j);
             f4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int q = a0; q < a1; q += kSlot) acc = chunk(q, acc);
+            for (int q = a0; q < a1; q += CH) acc = chunk(q, acc);
             acc += shfl_xor4(acc, 16);
             acc += shfl_xor4(acc, 32);
             if (g == 0) epi(row, j, acc, lv);
@@ -302,7 +313,7 @@ __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPI
         for (int row = r0 + lb; row < r1; row += gx) {  // block-uniform
             const int a0 = sp[row], a1 = sp[row + 1];
             f4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int q = a0 + kSlot * wv; q < a1; q += kSlot * kRowsPerBlock) acc = chunk(q, acc);
+            for (int q = a0 + CH * wv; q < a1; q += CH * kRowsPerBlock) acc = chunk(q, acc);
             acc += shfl_xor4(acc, 16);
             acc += shfl_xor4(acc, 32);
             if (g == 0) red[wv][j] = acc;
@@ -623,8 +634,8 @@ __global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const float* __
 }
 
 // x = w dinv b + P xc: the pre-smoothed iterate (not stored by k_b_pre_w)
-// plus the coarse correction, in one pass
-template <bool BLK>
+// plus the coarse correction, in one pass (P in chunks of CH slots)
+template <bool BLK, int CH>
 __global__ __launch_bounds__(kBlock) void k_b_prolong_w(SlotArgs P, const double* __restrict__ dinv,
                                                         const float* __restrict__ b, double w,
                                                         const float* __restrict__ xc, float* __restrict__ x) {
@@ -633,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_b_prolong_w(SlotArgs P, const double
     auto epi = [&](int row, int j, f4 acc, f4 bi) {
         *(f4*)(x + (size_t)row * 64 + 4 * j) = (float)(w * dinv[row]) * bi + acc;
     };
-    wide_walk<BLK>(P, xc, ld, epi);
+    wide_walk<BLK, CH>(P, xc, ld, epi);
 }
 
 template <bool ACC, bool BLK>
@@ -1176,17 +1187,20 @@ int fetch_csr(const AmgPrecond::CsrDev& M, HostCsr& h, hipStream_t s) {
 // offsets for gathered elements of 4 and of 8 bytes.
 struct SlotOp {
     int n = 0, ncols = 0, S = 0;
+    int chunk = kSlot;            // rows padded to a multiple of this (16; 8 for the wide kernels' short rows)
     bool nodiag = false;          // diagonal entries left out (applied by the kernel's epilogue)
     std::vector<int> h_rp, h_ci, h_sp;
     DevBuf<int> sp, off4, off8;
 
     bool kept(int i, int q) const { return !(nodiag && h_ci[q] == i); }
 
-    int build(int rows, int cols, const int* rp, const int* ci, int S_, hipStream_t s, bool drop_diag = false) {
+    int build(int rows, int cols, const int* rp, const int* ci, int S_, hipStream_t s, bool drop_diag = false,
+              int chunk_ = kSlot) {
         n = rows;
         ncols = cols;
         S = S_;
         nodiag = drop_diag;
+        chunk = chunk_;
         if ((int64_t)cols * S * 8 > (int64_t)INT32_MAX) return TM_ERR_UNSUPPORTED;  // 32-bit byte offsets
         h_rp.assign(rp, rp + rows + 1);
         h_ci.assign(ci, ci + rp[rows]);
@@ -1194,7 +1208,7 @@ struct SlotOp {
         for (int i = 0; i < rows; ++i) {
             int len = 0;
             for (int q = rp[i]; q < rp[i + 1]; ++q) len += kept(i, q);
-            h_sp[i + 1] = h_sp[i] + std::max(1, (len + kSlot - 1) / kSlot) * kSlot;
+            h_sp[i + 1] = h_sp[i] + std::max(1, (len + chunk - 1) / chunk) * chunk;
         }
         const size_t ns = (size_t)h_sp[rows];
         std::vector<int> o4(ns, 0), o8(ns, 0);
@@ -1245,7 +1259,20 @@ struct SlotOp {
         return TM_OK;
     }
 
-    // long rows: more than 6 chunks on average (one block per row pays off there)
+    // Chunk size for the wide kernels: 8 where it cuts the padded slots by a
+    // quarter or more (the prolongators' rows, a few coarse columns each).
+    // A padding slot costs a gather instruction's lanes like a real entry
+    // (its load is dropped by the range check, not skipped).
+    static int wide_chunk(int rows, const int* rp) {
+        int64_t s16 = 0, s8 = 0;
+        for (int i = 0; i < rows; ++i) {
+            const int len = rp[i + 1] - rp[i];
+            s16 += std::max(1, (len + 15) / 16) * 16;
+            s8 += std::max(1, (len + 7) / 8) * 8;
+        }
+        return 4 * s8 <= 3 * s16 ? 8 : kSlot;
+    }
+    // long rows: more than 6 chunks of 16 on average (one block per row pays off there)
     bool long_rows() const { return n > 0 && (int64_t)h_sp[n] > (int64_t)kSlot * n * 6; }
     SlotArgs args(int ebytes, const void* w) const {
         return SlotArgs{n, ncols * S * ebytes, sp.p, ebytes == 4 ? off4.p : off8.p, w};
@@ -1328,7 +1355,8 @@ struct BatchSolver {
             TM_CHECK(lo[l].R.build(V.R.n, V.n, h.rp.data(), h.ci.data(), S, s));
             TM_CHECK(lo[l].R.values(h.v, lo[l].wR, s));
             TM_CHECK(fetch_csr(V.P, h, s));
-            TM_CHECK(lo[l].P.build(V.P.n, nnext, h.rp.data(), h.ci.data(), S, s));
+            TM_CHECK(lo[l].P.build(V.P.n, nnext, h.rp.data(), h.ci.data(), S, s, false,
+                                   wide() ? SlotOp::wide_chunk(V.P.n, h.rp.data()) : kSlot));
             TM_CHECK(lo[l].P.values(h.v, lo[l].wP, s));
         }
         TM_CHECK(dinv.upload(solver.h_dinv.data(), n, s));
@@ -1391,8 +1419,8 @@ struct BatchSolver {
             const LvOps& O = lo[l];
             const T* bl = l == 0 ? r : lv[l - 1].b.p;
             if (wide())
-                prolong(O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), l == 0 ? dinv.p : V.dinv.p, bl,
-                        V.omega, xcoarse, lv[l].x.p, s);
+                prolong(O.P.long_rows(), O.P.chunk, dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p),
+                        l == 0 ? dinv.p : V.dinv.p, bl, V.omega, xcoarse, lv[l].x.p, s);
             else
                 spmm(true, O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), xcoarse, lv[l].x.p, s);
             if (l == 0) {
@@ -1458,17 +1486,20 @@ struct BatchSolver {
         if (acc) hipLaunchKernelGGL((k_b_spmm<T, true>), g, dim3(kBlock), 0, s, a, S, x, y);
         else hipLaunchKernelGGL((k_b_spmm<T, false>), g, dim3(kBlock), 0, s, a, S, x, y);
     }
-    // x = w dinv b + P xc (wide form only)
-    void prolong(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* xc, T* x,
+    // x = w dinv b + P xc (wide form only); ch: P's chunk size
+    void prolong(bool lg, int ch, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* xc, T* x,
                  hipStream_t s) {
         a = gated(a);
         if constexpr (kWideT) {
-            if (lg)
-                hipLaunchKernelGGL(k_b_prolong_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, di, (const float*)bb, w,
-                                   (const float*)xc, (float*)x);
+            const float* bf = (const float*)bb;
+            const float* xf = (const float*)xc;
+            if (ch == 8)
+                hipLaunchKernelGGL((k_b_prolong_w<false, 8>), g, dim3(kBlock), 0, s, a, di, bf, w, xf, (float*)x);
+            else if (lg)
+                hipLaunchKernelGGL((k_b_prolong_w<true, kSlot>), blk_grid(a.n), dim3(kBlock), 0, s, a, di, bf, w, xf,
+                                   (float*)x);
             else
-                hipLaunchKernelGGL(k_b_prolong_w<false>, g, dim3(kBlock), 0, s, a, di, (const float*)bb, w,
-                                   (const float*)xc, (float*)x);
+                hipLaunchKernelGGL((k_b_prolong_w<false, kSlot>), g, dim3(kBlock), 0, s, a, di, bf, w, xf, (float*)x);
         }
     }
     void post(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* x, T* xo, double* part,
