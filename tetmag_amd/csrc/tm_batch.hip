@@ -441,10 +441,66 @@ __global__ __launch_bounds__(kBlock) void k_b_sum2(size_t tot, const double* __r
         c[e] = a[e] + b[e];
 }
 
+// x += sum_k alpha_k p_k over the K directions of an inner solve (ring of
+// [n][S] vectors, alpha_k per sample in ah[k * kMaxS + s]); directions of
+// iterations a sample did not take have alpha 0 and are selected out of the
+// sum (their contents may be stale).  Four directions per step, so four
+// loads are in flight.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_add(size_t tot, const T* __restrict__ d, double* __restrict__ x) {
-    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
-        x[e] += (double)d[e];
+__global__ __launch_bounds__(kBlock) void k_b_combine(size_t NS, int S, int K, const T* __restrict__ ring,
+                                                      const double* __restrict__ ah, double* __restrict__ x) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock) {
+        const int sm = (int)(e % S);
+        double acc = 0.0;
+        int k = 0;
+        for (; k + 4 <= K; k += 4) {
+            T v[4];
+            double a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                v[u] = ring[(size_t)(k + u) * NS + e];
+                a[u] = ah[(k + u) * kMaxS + sm];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += a[u] != 0.0 ? a[u] * (double)v[u] : 0.0;
+        }
+        for (; k < K; ++k) {
+            const double a = ah[k * kMaxS + sm];
+            const T v = ring[(size_t)k * NS + e];
+            acc += a != 0.0 ? a * (double)v : 0.0;
+        }
+        x[e] += acc;
+    }
+}
+
+// k_b_combine for FP32 directions and S a multiple of 4: four samples per
+// thread (16-byte loads of each direction)
+__global__ __launch_bounds__(kBlock) void k_b_combine4(size_t NS, int S, int K, const float* __restrict__ ring,
+                                                       const double* __restrict__ ah, double* __restrict__ x) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const size_t N4 = NS / 4;
+    for (size_t e4 = (size_t)blockIdx.x * kBlock + threadIdx.x; e4 < N4; e4 += (size_t)gridDim.x * kBlock) {
+        const int sm = (int)((4 * e4) % S);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        int k = 0;
+        auto add = [&](const f4& v, const double* a) {
+            acc[0] += a[0] != 0.0 ? a[0] * (double)v.x : 0.0;
+            acc[1] += a[1] != 0.0 ? a[1] * (double)v.y : 0.0;
+            acc[2] += a[2] != 0.0 ? a[2] * (double)v.z : 0.0;
+            acc[3] += a[3] != 0.0 ? a[3] * (double)v.w : 0.0;
+        };
+        for (; k + 2 <= K; k += 2) {
+            const f4 v0 = ((const f4*)(ring + (size_t)k * NS))[e4];
+            const f4 v1 = ((const f4*)(ring + (size_t)(k + 1) * NS))[e4];
+            add(v0, ah + k * kMaxS + sm);
+            add(v1, ah + (k + 1) * kMaxS + sm);
+        }
+        if (k < K) add(((const f4*)(ring + (size_t)k * NS))[e4], ah + k * kMaxS + sm);
+        d2* xp = (d2*)(x + 4 * e4);
+        const d2 x0 = xp[0], x1 = xp[1];
+        xp[0] = d2{x0.x + acc[0], x0.y + acc[1]};
+        xp[1] = d2{x1.x + acc[2], x1.y + acc[3]};
+    }
 }
 
 template <typename T>
@@ -453,73 +509,95 @@ __global__ __launch_bounds__(kBlock) void k_b_zero(size_t tot, T* __restrict__ x
 }
 
 // ------------------------------------------------------------- inner CG --
-// p = z + beta p_old (p_old not read for k = 0); q = A p; partial (p, q).
-// p = z + beta p_old (p_old not read for k = 0)
+// Iteration k: p_k = z + beta p_{k-1} and q_k = A p_k = A z + beta q_{k-1}
+// in one pass (the gathers read z; p_{k-1} and q_{k-1} are the row's own
+// entries), partial (p_k, q_k).  The directions p_k are kept (a ring of
+// vectors) and the inner solution sum_k alpha_k p_k is formed once at the
+// end (k_b_combine), so no iteration reads or writes the iterate.
+// k = 0: p = z, q = A z (p_{-1}, q_{-1} not read).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_cg_dir(size_t NS, int S, const T* __restrict__ z,
-                                                     const T* __restrict__ pold, T* __restrict__ pnew, int k,
-                                                     const BState* bs) {
-    if (!any_active(bs->active, S)) return;
-    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock) {
-        const int sm = (int)(e % S);
-        pnew[e] = k > 0 ? z[e] + (T)bs->beta[sm] * pold[e] : z[e];
-    }
-}
-
-// q = A p; partial (p, q)
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T* __restrict__ p, T* __restrict__ q,
-                                                      double* part) {
+__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T* __restrict__ z,
+                                                      const T* __restrict__ pold, T* __restrict__ pnew,
+                                                      T* __restrict__ q, int k, const BState* bs, double* part) {
     if (!any_active(A.gate, A.gate_n)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
+    const T beta = k > 0 ? (T)bs->beta[lc] : (T)0;
     double v = 0.0;
-    auto ld = [&](int row) { return p[(size_t)row * S + lc]; };
-    auto epi = [&](int row, T acc, T pi) {
+    struct ZPQ {
+        T z, p, q;
+    };
+    auto ld = [&](int row) {
+        const size_t e = (size_t)row * S + lc;
+        return ZPQ{z[e], k > 0 ? pold[e] : (T)0, k > 0 ? q[e] : (T)0};
+    };
+    auto epi = [&](int row, T acc, const ZPQ& o) {
         if (on) {
-            q[(size_t)row * S + lane] = acc;
-            v += (double)pi * (double)acc;
+            const size_t e = (size_t)row * S + lane;
+            const T pi = o.z + beta * o.p;
+            const T qi = acc + beta * o.q;
+            pnew[e] = pi;
+            q[e] = qi;
+            v += (double)pi * (double)qi;
         }
     };
-    slot_walk<T, T, kRowsPerStep<T>>(A, p, lc, ld, epi);
+    slot_walk<T, T, kRowsPerStep<T>>(A, z, lc, ld, epi);
     b_partial(part, S, v);
 }
 
-// q = A p; partial (p, q) -- FP32, S = 64, wide gathers
+// k_b_cg_spmv for 64 FP32 samples, wide gathers
 __global__ __launch_bounds__(kBlock) void k_b_cg_spmv_w(SlotArgs A, const float* __restrict__ dg,
-                                                        const float* __restrict__ p, float* __restrict__ q,
-                                                        double* part) {
+                                                        const float* __restrict__ z, const float* __restrict__ pold,
+                                                        float* __restrict__ pnew, float* __restrict__ q, int k,
+                                                        const BState* bs, double* part) {
     if (!any_active(A.gate, A.gate_n)) return;
+    const int j = threadIdx.x & 15;
+    const f4 beta = k > 0 ? f4{(float)bs->beta[4 * j], (float)bs->beta[4 * j + 1], (float)bs->beta[4 * j + 2],
+                               (float)bs->beta[4 * j + 3]}
+                          : f4{0.f, 0.f, 0.f, 0.f};
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    auto ld = [&](int row, int j) { return *(const f4*)(p + (size_t)row * 64 + 4 * j); };
-    auto epi = [&](int row, int j, f4 acc, f4 pi) {
-        if (dg) acc += dg[row] * pi;  // diagonal kept out of the slots
-        *(f4*)(q + (size_t)row * 64 + 4 * j) = acc;
-        v[0] += (double)pi.x * (double)acc.x;
-        v[1] += (double)pi.y * (double)acc.y;
-        v[2] += (double)pi.z * (double)acc.z;
-        v[3] += (double)pi.w * (double)acc.w;
+    struct ZPQ {
+        f4 z, p, q;
     };
-    wide_walk<false>(A, p, ld, epi);
+    auto ld = [&](int row, int jj) {
+        const size_t e = (size_t)row * 64 + 4 * jj;
+        const f4 zero = {0.f, 0.f, 0.f, 0.f};
+        return ZPQ{*(const f4*)(z + e), k > 0 ? *(const f4*)(pold + e) : zero, k > 0 ? *(const f4*)(q + e) : zero};
+    };
+    auto epi = [&](int row, int jj, f4 acc, const ZPQ& o) {
+        const size_t e = (size_t)row * 64 + 4 * jj;
+        acc += dg[row] * o.z;  // diagonal kept out of the slots
+        const f4 pi = o.z + beta * o.p;
+        const f4 qi = acc + beta * o.q;
+        *(f4*)(pnew + e) = pi;
+        *(f4*)(q + e) = qi;
+        v[0] += (double)pi.x * (double)qi.x;
+        v[1] += (double)pi.y * (double)qi.y;
+        v[2] += (double)pi.z * (double)qi.z;
+        v[3] += (double)pi.w * (double)qi.w;
+    };
+    wide_walk<false>(A, z, ld, epi);
     b_partial4(part, v);
 }
 
-__global__ void k_b_red_pq(int G, int S, const double* part, BState* bs) {
+// alpha = (r, z) / (p, q); also kept in ah[s] (this iteration's slot of the history)
+__global__ void k_b_red_pq(int G, int S, const double* part, BState* bs, double* ah) {
     const int s = blockIdx.x;
     const double pq = b_total(part, G);
     if (threadIdx.x == 0 && s < S) {
         const bool ok = bs->active[s] && pq > 0.0 && std::isfinite(pq);
         if (bs->active[s] && !ok) bs->active[s] = 0;  // breakdown: freeze the sample
-        bs->alpha[s] = ok ? bs->gam[s] / pq : 0.0;
+        const double a = ok ? bs->gam[s] / pq : 0.0;
+        bs->alpha[s] = a;
+        ah[s] = a;
     }
 }
 
-// d += alpha p; r -= alpha q; partial (r, r)
+// r -= alpha q; partial (r, r)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_cg_update(int n, int S, const T* __restrict__ p,
-                                                        const T* __restrict__ q, T* __restrict__ d,
-                                                        T* __restrict__ r, const BState* bs, double* part) {
+__global__ __launch_bounds__(kBlock) void k_b_cg_update(int n, int S, const T* __restrict__ q, T* __restrict__ r,
+                                                        const BState* bs, double* part) {
     if (!any_active(bs->active, S)) return;
     const int ln = threadIdx.x & (kWave - 1);
     const T alpha = ln < S ? (T)bs->alpha[ln] : (T)0;
@@ -527,7 +605,6 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_update(int n, int S, const T* _
     B_ROW_LOOP(n) {
         if (on) {
             const size_t e = (size_t)row * S + lane;
-            d[e] += alpha * p[e];
             const T ri = r[e] - alpha * q[e];
             r[e] = ri;
             v += (double)ri * (double)ri;
@@ -1296,7 +1373,10 @@ struct BatchSolver {
     DevBuf<double> dinv;
     DevBuf<unsigned char> fixed;
     DevBuf<double> r64;
-    DevBuf<T> rT, d, z, p0, p1, q;
+    DevBuf<T> rT, z, q;
+    static constexpr int kRing = 40;  // directions kept per inner solve (a longer one is cut there; the outer loop goes on)
+    DevBuf<T> ring;                   // kRing x [n][S]: the directions p_k
+    DevBuf<double> ahist;             // kRing x kMaxS: their step lengths alpha_k
     struct Lv {
         DevBuf<T> x, x2, r, b;
     };
@@ -1364,10 +1444,9 @@ struct BatchSolver {
         const size_t NS = (size_t)n * S;
         TM_CHECK(r64.alloc(NS));
         TM_CHECK(rT.alloc(NS));
-        TM_CHECK(d.alloc(NS));
+        TM_CHECK(ring.alloc((size_t)kRing * NS));
+        TM_CHECK(ahist.alloc((size_t)kRing * kMaxS));
         TM_CHECK(z.alloc(NS));
-        TM_CHECK(p0.alloc(NS));
-        TM_CHECK(p1.alloc(NS));
         TM_CHECK(q.alloc(NS));
         lv.clear();
         lv.resize(solver.amg.levels.size());
@@ -1519,16 +1598,17 @@ struct BatchSolver {
         hipLaunchKernelGGL((k_b_post<T>), g, dim3(kBlock), 0, s, a, S, di, bb, w, x, xo, part);
     }
 
-    // q = A p, partial (p, q): the wide-gather kernel for 64 FP32 samples
-    void cg_spmv(const T* pc, double* ppq, hipStream_t s) {
+    // iteration k: p_k = z + beta p_{k-1}, q = A p_k, partial (p_k, q)
+    void cg_spmv(const T* zc, const T* pold, T* pnew, int k, double* ppq, hipStream_t s) {
         if constexpr (kWideT)
             if (wide()) {
                 hipLaunchKernelGGL(k_b_cg_spmv_w, dim3(G), dim3(kBlock), 0, s, gated(sl0w.args(4, wA0w.p)),
-                                   (const float*)dA0.p, (const float*)pc, (float*)q.p, ppq);
+                                   (const float*)dA0.p, (const float*)zc, (const float*)pold, (float*)pnew,
+                                   (float*)q.p, k, (const BState*)bst.p, ppq);
                 return;
             }
         hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, gated(sl0.args((int)sizeof(T), wA0.p)), S,
-                           pc, q.p, ppq);
+                           zc, pold, pnew, q.p, k, (const BState*)bst.p, ppq);
     }
 
     // Solves A x = b for all samples ([n][S] FP64); x is the warm start.
@@ -1558,43 +1638,45 @@ struct BatchSolver {
             if (maxrel) *maxrel = worst;
             if (all || outer >= 8 || inner_total >= maxit) break;
             ++outer;
-            // inner solve A d = rT from d = 0
-            hipLaunchKernelGGL((k_b_zero<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, d.p);
+            // inner solve A d = rT from d = 0; x += d at the end
             TM_CHECK(precond(rT.p, z.p, prz, s));
             hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, -1, bst.p);
             int enq = 0;
+            const int kmax = std::min(kRing, maxit);
             // iterations enqueued before the first check: what this refinement
             // step needed last time, plus one (queued iterations past
             // convergence return at once)
             const int step = std::min(outer, kMaxOuter) - 1;
             int chunk = std::max(2, last_inner[step] + 1);
-            T* pc = p0.p;
-            T* po = p1.p;
             for (;;) {
-                for (int c = 0; c < chunk && enq < maxit; ++c, ++enq) {
-                    hipLaunchKernelGGL((k_b_cg_dir<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S,
-                                       (const T*)z.p, (const T*)po, pc, enq, (const BState*)bst.p);
-                    cg_spmv(pc, ppq, s);
-                    hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, ppq, bst.p);
-                    hipLaunchKernelGGL((k_b_cg_update<T>), dim3(G), dim3(kBlock), 0, s, n, S, (const T*)pc,
-                                       (const T*)q.p, d.p, rT.p, (const BState*)bst.p, prr);
+                for (int c = 0; c < chunk && enq < kmax; ++c, ++enq) {
+                    T* pk = ring.p + (size_t)enq * NS;
+                    cg_spmv(z.p, enq > 0 ? pk - NS : pk, pk, enq, ppq, s);  // p_{-1} is not read
+                    hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, ppq, bst.p,
+                                       ahist.p + (size_t)enq * kMaxS);
+                    hipLaunchKernelGGL((k_b_cg_update<T>), dim3(G), dim3(kBlock), 0, s, n, S, (const T*)q.p, rT.p,
+                                       (const BState*)bst.p, prr);
                     TM_CHECK(precond(rT.p, z.p, prz, s));
                     hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, enq, bst.p);
-                    std::swap(pc, po);
                 }
                 TM_HIP(hipGetLastError());
                 TM_HIP(hipMemcpyAsync(h_bst, bst.p, sizeof(BState), hipMemcpyDeviceToHost, s));
                 TM_HIP(hipStreamSynchronize(s));
                 bool any = false;
                 for (int k = 0; k < S; ++k) any = any || h_bst->active[k];
-                if (!any || enq >= maxit) break;
+                if (!any || enq >= kmax) break;
                 chunk = 2;
             }
             int used = 0;  // iterations the slowest sample ran
             for (int k = 0; k < S; ++k) used = std::max(used, h_bst->iters[k]);
             last_inner[step] = used;
             inner_total += used;
-            hipLaunchKernelGGL((k_b_add<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const T*)d.p, x);
+            if (sizeof(T) == 4 && S % 4 == 0)
+                hipLaunchKernelGGL(k_b_combine4, dim3(elem_grid(NS / 4)), dim3(kBlock), 0, s, NS, S, enq,
+                                   (const float*)ring.p, (const double*)ahist.p, x);
+            else
+                hipLaunchKernelGGL((k_b_combine<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S, enq,
+                                   (const T*)ring.p, (const double*)ahist.p, x);
         }
         if (outer_out) *outer_out = outer;
         if (inner_out) *inner_out = inner_total;
