@@ -441,61 +441,48 @@ __global__ __launch_bounds__(kBlock) void k_b_sum2(size_t tot, const double* __r
         c[e] = a[e] + b[e];
 }
 
-// x += sum_k alpha_k p_k over the K directions of an inner solve (ring of
-// [n][S] vectors, alpha_k per sample in ah[k * kMaxS + s]); directions of
-// iterations a sample did not take have alpha 0 and are selected out of the
-// sum (their contents may be stale).  Four directions per step, so four
-// loads are in flight.
+// x += sum_k alpha_k p_k of an inner solve of K iterations, from the kept
+// preconditioned residuals z_i (ring of [n][S] vectors): p_k = z_k +
+// beta_k p_{k-1} gives sum_k alpha_k p_k = sum_i c_i z_i with c_{K-1} =
+// alpha_{K-1}, c_i = alpha_i + beta_{i+1} c_{i+1} (alpha_i in ah[i * kMaxS +
+// s], beta_i in bh[i * kMaxS + s]).  Iterations a sample did not take have
+// alpha 0 (and beta 0 after them), so c_i = 0 selects their z_i (whose
+// contents may be stale) out of the sum.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_combine(size_t NS, int S, int K, const T* __restrict__ ring,
-                                                      const double* __restrict__ ah, double* __restrict__ x) {
+                                                      const double* __restrict__ ah, const double* __restrict__ bh,
+                                                      double* __restrict__ x) {
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock) {
         const int sm = (int)(e % S);
-        double acc = 0.0;
-        int k = 0;
-        for (; k + 4 <= K; k += 4) {
-            T v[4];
-            double a[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                v[u] = ring[(size_t)(k + u) * NS + e];
-                a[u] = ah[(k + u) * kMaxS + sm];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc += a[u] != 0.0 ? a[u] * (double)v[u] : 0.0;
-        }
-        for (; k < K; ++k) {
-            const double a = ah[k * kMaxS + sm];
-            const T v = ring[(size_t)k * NS + e];
-            acc += a != 0.0 ? a * (double)v : 0.0;
+        double acc = 0.0, c = 0.0;
+        for (int i = K - 1; i >= 0; --i) {
+            c = ah[i * kMaxS + sm] + bh[(i + 1) * kMaxS + sm] * c;
+            const T v = ring[(size_t)i * NS + e];
+            acc += c != 0.0 ? c * (double)v : 0.0;
         }
         x[e] += acc;
     }
 }
 
-// k_b_combine for FP32 directions and S a multiple of 4: four samples per
-// thread (16-byte loads of each direction)
+// k_b_combine for FP32 and S a multiple of 4: four samples per thread
+// (16-byte loads of each z_i)
 __global__ __launch_bounds__(kBlock) void k_b_combine4(size_t NS, int S, int K, const float* __restrict__ ring,
-                                                       const double* __restrict__ ah, double* __restrict__ x) {
+                                                       const double* __restrict__ ah, const double* __restrict__ bh,
+                                                       double* __restrict__ x) {
     typedef double d2 __attribute__((ext_vector_type(2)));
     const size_t N4 = NS / 4;
     for (size_t e4 = (size_t)blockIdx.x * kBlock + threadIdx.x; e4 < N4; e4 += (size_t)gridDim.x * kBlock) {
         const int sm = (int)((4 * e4) % S);
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        int k = 0;
-        auto add = [&](const f4& v, const double* a) {
-            acc[0] += a[0] != 0.0 ? a[0] * (double)v.x : 0.0;
-            acc[1] += a[1] != 0.0 ? a[1] * (double)v.y : 0.0;
-            acc[2] += a[2] != 0.0 ? a[2] * (double)v.z : 0.0;
-            acc[3] += a[3] != 0.0 ? a[3] * (double)v.w : 0.0;
-        };
-        for (; k + 2 <= K; k += 2) {
-            const f4 v0 = ((const f4*)(ring + (size_t)k * NS))[e4];
-            const f4 v1 = ((const f4*)(ring + (size_t)(k + 1) * NS))[e4];
-            add(v0, ah + k * kMaxS + sm);
-            add(v1, ah + (k + 1) * kMaxS + sm);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0}, c[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = K - 1; i >= 0; --i) {
+            const f4 v = ((const f4*)(ring + (size_t)i * NS))[e4];
+            const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                c[u] = ah[i * kMaxS + sm + u] + bh[(i + 1) * kMaxS + sm + u] * c[u];
+                acc[u] += c[u] != 0.0 ? c[u] * (double)vv[u] : 0.0;
+            }
         }
-        if (k < K) add(((const f4*)(ring + (size_t)k * NS))[e4], ah + k * kMaxS + sm);
         d2* xp = (d2*)(x + 4 * e4);
         const d2 x0 = xp[0], x1 = xp[1];
         xp[0] = d2{x0.x + acc[0], x0.y + acc[1]};
@@ -509,37 +496,35 @@ __global__ __launch_bounds__(kBlock) void k_b_zero(size_t tot, T* __restrict__ x
 }
 
 // ------------------------------------------------------------- inner CG --
-// Iteration k: p_k = z + beta p_{k-1} and q_k = A p_k = A z + beta q_{k-1}
-// in one pass (the gathers read z; p_{k-1} and q_{k-1} are the row's own
-// entries), partial (p_k, q_k).  The directions p_k are kept (a ring of
-// vectors) and the inner solution sum_k alpha_k p_k is formed once at the
-// end (k_b_combine), so no iteration reads or writes the iterate.
-// k = 0: p = z, q = A z (p_{-1}, q_{-1} not read).
+// Iteration k: q_k = A p_k = A z_k + beta_k q_{k-1} in one pass (the gathers
+// read z_k; q_{k-1} is the row's own entry), with the partial (z_k, q_k):
+// (p_k, A p_k) = (z_k, A p_k) as p_{k-1} is A-conjugate to p_k.  The
+// directions are never formed: the preconditioned residuals z_k are kept (a
+// ring of vectors) with alpha_k and beta_k, and the inner solution
+// sum_k alpha_k p_k = sum_i c_i z_i is added once at the end (k_b_combine),
+// so no iteration reads or writes a direction or the iterate.
+// k = 0: q = A z (q_{-1} not read).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T* __restrict__ z,
-                                                      const T* __restrict__ pold, T* __restrict__ pnew,
-                                                      T* __restrict__ q, int k, const BState* bs, double* part) {
+__global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T* __restrict__ z, T* __restrict__ q,
+                                                      int k, const BState* bs, double* part) {
     if (!any_active(A.gate, A.gate_n)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
     const T beta = k > 0 ? (T)bs->beta[lc] : (T)0;
     double v = 0.0;
-    struct ZPQ {
-        T z, p, q;
+    struct ZQ {
+        T z, q;
     };
     auto ld = [&](int row) {
         const size_t e = (size_t)row * S + lc;
-        return ZPQ{z[e], k > 0 ? pold[e] : (T)0, k > 0 ? q[e] : (T)0};
+        return ZQ{z[e], k > 0 ? q[e] : (T)0};
     };
-    auto epi = [&](int row, T acc, const ZPQ& o) {
+    auto epi = [&](int row, T acc, const ZQ& o) {
         if (on) {
-            const size_t e = (size_t)row * S + lane;
-            const T pi = o.z + beta * o.p;
             const T qi = acc + beta * o.q;
-            pnew[e] = pi;
-            q[e] = qi;
-            v += (double)pi * (double)qi;
+            q[(size_t)row * S + lane] = qi;
+            v += (double)o.z * (double)qi;
         }
     };
     slot_walk<T, T, kRowsPerStep<T>>(A, z, lc, ld, epi);
@@ -548,8 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_spmv(SlotArgs A, int S, const T
 
 // k_b_cg_spmv for 64 FP32 samples, wide gathers
 __global__ __launch_bounds__(kBlock) void k_b_cg_spmv_w(SlotArgs A, const float* __restrict__ dg,
-                                                        const float* __restrict__ z, const float* __restrict__ pold,
-                                                        float* __restrict__ pnew, float* __restrict__ q, int k,
+                                                        const float* __restrict__ z, float* __restrict__ q, int k,
                                                         const BState* bs, double* part) {
     if (!any_active(A.gate, A.gate_n)) return;
     const int j = threadIdx.x & 15;
@@ -557,31 +541,26 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_spmv_w(SlotArgs A, const float*
                                (float)bs->beta[4 * j + 3]}
                           : f4{0.f, 0.f, 0.f, 0.f};
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    struct ZPQ {
-        f4 z, p, q;
+    struct ZQ {
+        f4 z, q;
     };
     auto ld = [&](int row, int jj) {
         const size_t e = (size_t)row * 64 + 4 * jj;
-        const f4 zero = {0.f, 0.f, 0.f, 0.f};
-        return ZPQ{*(const f4*)(z + e), k > 0 ? *(const f4*)(pold + e) : zero, k > 0 ? *(const f4*)(q + e) : zero};
+        return ZQ{*(const f4*)(z + e), k > 0 ? *(const f4*)(q + e) : f4{0.f, 0.f, 0.f, 0.f}};
     };
-    auto epi = [&](int row, int jj, f4 acc, const ZPQ& o) {
-        const size_t e = (size_t)row * 64 + 4 * jj;
+    auto epi = [&](int row, int jj, f4 acc, const ZQ& o) {
         acc += dg[row] * o.z;  // diagonal kept out of the slots
-        const f4 pi = o.z + beta * o.p;
         const f4 qi = acc + beta * o.q;
-        *(f4*)(pnew + e) = pi;
-        *(f4*)(q + e) = qi;
-        v[0] += (double)pi.x * (double)qi.x;
-        v[1] += (double)pi.y * (double)qi.y;
-        v[2] += (double)pi.z * (double)qi.z;
-        v[3] += (double)pi.w * (double)qi.w;
+        *(f4*)(q + (size_t)row * 64 + 4 * jj) = qi;
+        v[0] += (double)o.z.x * (double)qi.x;
+        v[1] += (double)o.z.y * (double)qi.y;
+        v[2] += (double)o.z.z * (double)qi.z;
+        v[3] += (double)o.z.w * (double)qi.w;
     };
     wide_walk<false>(A, z, ld, epi);
     b_partial4(part, v);
 }
 
-// alpha = (r, z) / (p, q); also kept in ah[s] (this iteration's slot of the history)
 __global__ void k_b_red_pq(int G, int S, const double* part, BState* bs, double* ah) {
     const int s = blockIdx.x;
     const double pq = b_total(part, G);
@@ -614,7 +593,9 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_update(int n, int S, const T* _
 }
 
 // after z = M r: gamma, convergence, beta
-__global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* part_rr, int k, BState* bs) {
+// (bh[s]: the next iteration's slot of the beta history)
+__global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* part_rr, int k, BState* bs,
+                           double* bh) {
     const int s = blockIdx.x;
     const double rz = b_total(part_rz, G);
     __syncthreads();
@@ -625,6 +606,7 @@ __global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* pa
         bool act = bs->active[s] && r2 > bs->itol2[s] && std::isfinite(rz) && std::isfinite(r2);
         if (k >= 0 && bs->active[s]) bs->iters[s] += 1;
         bs->beta[s] = (act && k >= 0 && bs->gam[s] != 0.0) ? rz / bs->gam[s] : 0.0;
+        bh[s] = bs->beta[s];
         bs->gam[s] = rz;
         bs->active[s] = act ? 1 : 0;
     }
@@ -1373,10 +1355,10 @@ struct BatchSolver {
     DevBuf<double> dinv;
     DevBuf<unsigned char> fixed;
     DevBuf<double> r64;
-    DevBuf<T> rT, z, q;
-    static constexpr int kRing = 40;  // directions kept per inner solve (a longer one is cut there; the outer loop goes on)
-    DevBuf<T> ring;                   // kRing x [n][S]: the directions p_k
-    DevBuf<double> ahist;             // kRing x kMaxS: their step lengths alpha_k
+    DevBuf<T> rT, q;
+    static constexpr int kRing = 40;  // iterations per inner solve (a longer one is cut there; the outer loop goes on)
+    DevBuf<T> ring;                   // (kRing + 1) x [n][S]: the preconditioned residuals z_k
+    DevBuf<double> ahist, bhist;      // (kRing + 1) x kMaxS: alpha_k, beta_k
     struct Lv {
         DevBuf<T> x, x2, r, b;
     };
@@ -1444,9 +1426,9 @@ struct BatchSolver {
         const size_t NS = (size_t)n * S;
         TM_CHECK(r64.alloc(NS));
         TM_CHECK(rT.alloc(NS));
-        TM_CHECK(ring.alloc((size_t)kRing * NS));
-        TM_CHECK(ahist.alloc((size_t)kRing * kMaxS));
-        TM_CHECK(z.alloc(NS));
+        TM_CHECK(ring.alloc((size_t)(kRing + 1) * NS));
+        TM_CHECK(ahist.alloc((size_t)(kRing + 1) * kMaxS));
+        TM_CHECK(bhist.alloc((size_t)(kRing + 1) * kMaxS));
         TM_CHECK(q.alloc(NS));
         lv.clear();
         lv.resize(solver.amg.levels.size());
@@ -1598,17 +1580,16 @@ struct BatchSolver {
         hipLaunchKernelGGL((k_b_post<T>), g, dim3(kBlock), 0, s, a, S, di, bb, w, x, xo, part);
     }
 
-    // iteration k: p_k = z + beta p_{k-1}, q = A p_k, partial (p_k, q)
-    void cg_spmv(const T* zc, const T* pold, T* pnew, int k, double* ppq, hipStream_t s) {
+    // iteration k: q = A p_k = A z_k + beta q, partial (z_k, q)
+    void cg_spmv(const T* zk, int k, double* pzq, hipStream_t s) {
         if constexpr (kWideT)
             if (wide()) {
                 hipLaunchKernelGGL(k_b_cg_spmv_w, dim3(G), dim3(kBlock), 0, s, gated(sl0w.args(4, wA0w.p)),
-                                   (const float*)dA0.p, (const float*)zc, (const float*)pold, (float*)pnew,
-                                   (float*)q.p, k, (const BState*)bst.p, ppq);
+                                   (const float*)dA0.p, (const float*)zk, (float*)q.p, k, (const BState*)bst.p, pzq);
                 return;
             }
         hipLaunchKernelGGL((k_b_cg_spmv<T>), dim3(G), dim3(kBlock), 0, s, gated(sl0.args((int)sizeof(T), wA0.p)), S,
-                           zc, pold, pnew, q.p, k, (const BState*)bst.p, ppq);
+                           zk, q.p, k, (const BState*)bst.p, pzq);
     }
 
     // Solves A x = b for all samples ([n][S] FP64); x is the warm start.
@@ -1619,7 +1600,7 @@ struct BatchSolver {
         double* pbb = part.p;
         double* prr = part.p + (size_t)S * G;
         double* prz = part.p + (size_t)2 * S * G;
-        double* ppq = part.p + (size_t)3 * S * G;
+        double* pzq = part.p + (size_t)3 * S * G;
         const double itol = sizeof(T) == 4 ? 1e-5 : 0.0;  // FP32 inner target per refinement step
         int outer = 0, inner_total = 0;
         hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
@@ -1639,8 +1620,8 @@ struct BatchSolver {
             if (all || outer >= 8 || inner_total >= maxit) break;
             ++outer;
             // inner solve A d = rT from d = 0; x += d at the end
-            TM_CHECK(precond(rT.p, z.p, prz, s));
-            hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, -1, bst.p);
+            TM_CHECK(precond(rT.p, ring.p, prz, s));
+            hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, -1, bst.p, bhist.p);
             int enq = 0;
             const int kmax = std::min(kRing, maxit);
             // iterations enqueued before the first check: what this refinement
@@ -1650,14 +1631,15 @@ struct BatchSolver {
             int chunk = std::max(2, last_inner[step] + 1);
             for (;;) {
                 for (int c = 0; c < chunk && enq < kmax; ++c, ++enq) {
-                    T* pk = ring.p + (size_t)enq * NS;
-                    cg_spmv(z.p, enq > 0 ? pk - NS : pk, pk, enq, ppq, s);  // p_{-1} is not read
-                    hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, ppq, bst.p,
+                    T* zk = ring.p + (size_t)enq * NS;
+                    cg_spmv(zk, enq, pzq, s);
+                    hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, pzq, bst.p,
                                        ahist.p + (size_t)enq * kMaxS);
                     hipLaunchKernelGGL((k_b_cg_update<T>), dim3(G), dim3(kBlock), 0, s, n, S, (const T*)q.p, rT.p,
                                        (const BState*)bst.p, prr);
-                    TM_CHECK(precond(rT.p, z.p, prz, s));
-                    hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, enq, bst.p);
+                    TM_CHECK(precond(rT.p, zk + NS, prz, s));
+                    hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, enq, bst.p,
+                                       bhist.p + (size_t)(enq + 1) * kMaxS);
                 }
                 TM_HIP(hipGetLastError());
                 TM_HIP(hipMemcpyAsync(h_bst, bst.p, sizeof(BState), hipMemcpyDeviceToHost, s));
@@ -1673,10 +1655,10 @@ struct BatchSolver {
             inner_total += used;
             if (sizeof(T) == 4 && S % 4 == 0)
                 hipLaunchKernelGGL(k_b_combine4, dim3(elem_grid(NS / 4)), dim3(kBlock), 0, s, NS, S, enq,
-                                   (const float*)ring.p, (const double*)ahist.p, x);
+                                   (const float*)ring.p, (const double*)ahist.p, (const double*)bhist.p, x);
             else
                 hipLaunchKernelGGL((k_b_combine<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S, enq,
-                                   (const T*)ring.p, (const double*)ahist.p, x);
+                                   (const T*)ring.p, (const double*)ahist.p, (const double*)bhist.p, x);
         }
         if (outer_out) *outer_out = outer;
         if (inner_out) *inner_out = inner_total;
