@@ -383,12 +383,12 @@ __global__ __launch_bounds__(kBlock) void k_b_fix(int n, int S, const unsigned c
         if (fixed[e / S]) x[e] = b[e];
 }
 
-// r = b - A x (FP64), rT = (T) r, partials of (b,b) and (r,r).  A: FP64
-// weights, offsets of FP64 rows.
+// r = b - A x in FP64, stored as rT = (T) r (the inner solve's right-hand
+// side), partials of (b,b) and (r,r).  A: FP64 weights, offsets of FP64 rows.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_resid(SlotArgs A, int S, const double* __restrict__ b,
-                                                    const double* __restrict__ x, double* __restrict__ r64,
-                                                    T* __restrict__ rT, double* part_bb, double* part_rr) {
+                                                    const double* __restrict__ x, T* __restrict__ rT,
+                                                    double* part_bb, double* part_rr) {
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
@@ -398,7 +398,6 @@ __global__ __launch_bounds__(kBlock) void k_b_resid(SlotArgs A, int S, const dou
         if (on) {
             const size_t ee = (size_t)row * S + lane;
             const double ri = bi - acc;
-            r64[ee] = ri;
             rT[ee] = (T)ri;
             vb += bi * bi;
             vr += ri * ri;
@@ -1354,7 +1353,6 @@ struct BatchSolver {
     std::vector<LvOps> lo;
     DevBuf<double> dinv;
     DevBuf<unsigned char> fixed;
-    DevBuf<double> r64;
     DevBuf<T> rT, q;
     static constexpr int kRing = 40;  // iterations per inner solve (a longer one is cut there; the outer loop goes on)
     DevBuf<T> ring;                   // (kRing + 1) x [n][S]: the preconditioned residuals z_k
@@ -1424,7 +1422,6 @@ struct BatchSolver {
         TM_CHECK(dinv.upload(solver.h_dinv.data(), n, s));
         TM_CHECK(fixed.upload(solver.h_fixed.data(), n, s));
         const size_t NS = (size_t)n * S;
-        TM_CHECK(r64.alloc(NS));
         TM_CHECK(rT.alloc(NS));
         TM_CHECK(ring.alloc((size_t)(kRing + 1) * NS));
         TM_CHECK(ahist.alloc((size_t)(kRing + 1) * kMaxS));
@@ -1605,7 +1602,7 @@ struct BatchSolver {
         int outer = 0, inner_total = 0;
         hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
         for (;;) {
-            hipLaunchKernelGGL((k_b_resid<T>), dim3(G), dim3(kBlock), 0, s, sl0.args(8, wA64.p), S, b, (const double*)x, r64.p,
+            hipLaunchKernelGGL((k_b_resid<T>), dim3(G), dim3(kBlock), 0, s, sl0.args(8, wA64.p), S, b, (const double*)x,
                                rT.p, pbb, prr);
             hipLaunchKernelGGL(k_b_outer_red, dim3(S), dim3(kBlock), 0, s, G, S, pbb, prr, tol, itol, ost.p, bst.p);
             TM_HIP(hipMemcpyAsync(h_ost, ost.p, sizeof(OState), hipMemcpyDeviceToHost, s));
