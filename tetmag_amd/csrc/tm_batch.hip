@@ -742,6 +742,27 @@ __global__ __launch_bounds__(kBlock) void k_b_post_w(SlotArgs A, const float* __
     if (part) b_partial4(part, v);
 }
 
+// The level-0 post-smoother with the diagonal in the slots: z = M x + w dinv b
+// with M = I - w D^-1 A (only b is read per row); partial (b, z)
+__global__ __launch_bounds__(kBlock) void k_b_post1_w(SlotArgs M, const double* __restrict__ dinv,
+                                                      const float* __restrict__ b, double w,
+                                                      const float* __restrict__ x, float* __restrict__ z,
+                                                      double* part) {
+    if (!any_active(M.gate, M.gate_n)) return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    auto ld = [&](int row, int j) { return *(const f4*)(b + (size_t)row * 64 + 4 * j); };
+    auto epi = [&](int row, int j, f4 acc, f4 bi) {
+        const f4 zi = acc + (float)(w * dinv[row]) * bi;
+        *(f4*)(z + (size_t)row * 64 + 4 * j) = zi;
+        v[0] += (double)bi.x * (double)zi.x;
+        v[1] += (double)bi.y * (double)zi.y;
+        v[2] += (double)bi.z * (double)zi.z;
+        v[3] += (double)bi.w * (double)zi.w;
+    };
+    wide_walk<false>(M, x, ld, epi);
+    b_partial4(part, v);
+}
+
 // x = C b for the dense coarse inverse (row-major nc x nc): one row per block,
 // its columns dealt to the 4 wavefronts in 16-column chunks (C's entries are
 // wavefront-uniform scalar loads), the wavefronts' sums added in LDS.
@@ -1332,6 +1353,8 @@ struct SlotOp {
     }
     // long rows: more than 6 chunks of 16 on average (one block per row pays off there)
     bool long_rows() const { return n > 0 && (int64_t)h_sp[n] > (int64_t)kSlot * n * 6; }
+    // every row one chunk (rows have at least one)
+    bool one_chunk() const { return n > 0 && (int64_t)h_sp[n] == (int64_t)chunk * n; }
     SlotArgs args(int ebytes, const void* w) const {
         return SlotArgs{n, ncols * S * ebytes, sp.p, ebytes == 4 ? off4.p : off8.p, w};
     }
@@ -1345,6 +1368,8 @@ struct BatchSolver {
     DevBuf<T> wA0, wAW0;          // its weights: A, and A w D^-1 (pre-smoother)
     SlotOp sl0w;                  // the same without the diagonal, for the wide kernels
     DevBuf<T> wA0w, wAW0w, dA0, dAW0;  // its weights and the diagonals
+    DevBuf<T> wIAW0, wIDA0;       // on sl0: I - A w D^-1 and I - w D^-1 A (level-0 smoothers, wide form)
+    bool pre1 = false;            // those are set (every level-0 row fits one chunk)
     DevBuf<double> wA64;          // A in FP64 (outer residual)
     struct LvOps {
         SlotOp A, R, P;           // level l >= 1: A; every level: restriction, prolongation
@@ -1395,6 +1420,23 @@ struct BatchSolver {
             TM_CHECK(sl0w.values(solver.h_val, wAW0w, s, cc, solver.h_dinv.data(), M.levels[0].omega));
             TM_CHECK(sl0w.diagonal(solver.h_val, dA0, s));
             TM_CHECK(sl0w.diagonal(solver.h_val, dAW0, s, solver.h_dinv.data(), M.levels[0].omega));
+            // the level-0 smoothers' operators with the diagonal in the slots
+            // (rows of up to 16 entries: one chunk): I - A w D^-1 and I - w D^-1 A
+            if (sl0.one_chunk()) {
+                const int* rp = solver.pat.h_rowptr.data();
+                const double w = M.levels[0].omega;
+                const std::vector<double>& dv = solver.h_dinv;
+                std::vector<double> v1(solver.h_val.size()), v2(solver.h_val.size());
+                for (int i = 0; i < n; ++i)
+                    for (int q = rp[i]; q < rp[i + 1]; ++q) {
+                        const double id = cc[q] == i ? 1.0 : 0.0;
+                        v1[q] = id - solver.h_val[q] * w * dv[cc[q]];
+                        v2[q] = id - solver.h_val[q] * w * dv[i];
+                    }
+                TM_CHECK(sl0.values(v1, wIAW0, s));
+                TM_CHECK(sl0.values(v2, wIDA0, s));
+                pre1 = true;
+            }
         }
         lo.clear();
         lo.resize((size_t)L);
@@ -1459,7 +1501,9 @@ struct BatchSolver {
             const LvOps& O = lo[l];
             // the wide form leaves x = w dinv b to the prolongation pass
             T* xpre = wide() ? nullptr : lv[l].x.p;
-            if (l == 0 && wide())
+            if (l == 0 && wide() && pre1)  // r = (I - A w D^-1) b: gathers only
+                spmm(false, false, dim3(G), sl0.args(e, wIAW0.p), b, lv[0].r.p, s);
+            else if (l == 0 && wide())
                 pre(false, dim3(G), sl0w.args(e, wAW0w.p), dinv.p, b, V.omega, xpre, lv[0].r.p, s, dAW0.p);
             else if (l == 0)
                 pre(sl0.long_rows(), dim3(G), sl0.args(e, wAW0.p), dinv.p, b, V.omega, xpre, lv[0].r.p, s);
@@ -1482,7 +1526,12 @@ struct BatchSolver {
             else
                 spmm(true, O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), xcoarse, lv[l].x.p, s);
             if (l == 0) {
-                if (wide())
+                if (wide() && pre1) {  // z = (I - w D^-1 A) x + w D^-1 b
+                    if constexpr (kWideT)
+                        hipLaunchKernelGGL(k_b_post1_w, dim3(G), dim3(kBlock), 0, s, gated(sl0.args(e, wIDA0.p)),
+                                           dinv.p, (const float*)bl, V.omega, (const float*)lv[0].x.p, (float*)zout,
+                                           prz);
+                } else if (wide())
                     post(false, dim3(G), sl0w.args(e, wA0w.p), dinv.p, bl, V.omega, lv[0].x.p, zout, prz, s, dA0.p);
                 else
                     post(sl0.long_rows(), dim3(G), sl0.args(e, wA0.p), dinv.p, bl, V.omega, lv[0].x.p, zout, prz, s);
