@@ -463,24 +463,39 @@ __global__ __launch_bounds__(kBlock) void k_b_combine(size_t NS, int S, int K, c
     }
 }
 
-// k_b_combine for FP32 and S a multiple of 4: four samples per thread
-// (16-byte loads of each z_i)
-__global__ __launch_bounds__(kBlock) void k_b_combine4(size_t NS, int S, int K, const float* __restrict__ ring,
-                                                       const double* __restrict__ ah, const double* __restrict__ bh,
-                                                       double* __restrict__ x) {
+// c_i of k_b_combine for every sample (thread s): ch[i * kMaxS + s], i < K
+__global__ void k_b_coef(int S, int K, const double* __restrict__ ah, const double* __restrict__ bh,
+                         double* __restrict__ ch) {
+    const int sm = threadIdx.x;
+    if (sm >= S) return;
+    double c = 0.0;
+    for (int i = K - 1; i >= 0; --i) {
+        c = ah[i * kMaxS + sm] + bh[(i + 1) * kMaxS + sm] * c;
+        ch[i * kMaxS + sm] = c;
+    }
+}
+
+// k_b_combine for FP32 and S = 64 with the c_i from k_b_coef (staged in LDS,
+// K <= kCombineK): four samples per thread (16-byte loads of each z_i)
+constexpr int kCombineK = 48;
+__global__ __launch_bounds__(kBlock) void k_b_combine4(size_t NS, int K, const float* __restrict__ ring,
+                                                       const double* __restrict__ ch, double* __restrict__ x) {
     typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    __shared__ d4 cs[kCombineK * kMaxS / 4];
+    for (int t = threadIdx.x; t < K * kMaxS / 4; t += kBlock) cs[t] = ((const d4*)ch)[t];
+    __syncthreads();
     const size_t N4 = NS / 4;
     for (size_t e4 = (size_t)blockIdx.x * kBlock + threadIdx.x; e4 < N4; e4 += (size_t)gridDim.x * kBlock) {
-        const int sm = (int)((4 * e4) % S);
-        double acc[4] = {0.0, 0.0, 0.0, 0.0}, c[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i = K - 1; i >= 0; --i) {
+        const int q = (int)(e4 % (kMaxS / 4));  // samples 4q .. 4q+3
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < K; ++i) {
             const f4 v = ((const f4*)(ring + (size_t)i * NS))[e4];
-            const float vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                c[u] = ah[i * kMaxS + sm + u] + bh[(i + 1) * kMaxS + sm + u] * c[u];
-                acc[u] += c[u] != 0.0 ? c[u] * (double)vv[u] : 0.0;
-            }
+            const d4 c = cs[i * (kMaxS / 4) + q];
+            acc[0] += c.x != 0.0 ? c.x * (double)v.x : 0.0;
+            acc[1] += c.y != 0.0 ? c.y * (double)v.y : 0.0;
+            acc[2] += c.z != 0.0 ? c.z * (double)v.z : 0.0;
+            acc[3] += c.w != 0.0 ? c.w * (double)v.w : 0.0;
         }
         d2* xp = (d2*)(x + 4 * e4);
         const d2 x0 = xp[0], x1 = xp[1];
@@ -1382,6 +1397,7 @@ struct BatchSolver {
     static constexpr int kRing = 40;  // iterations per inner solve (a longer one is cut there; the outer loop goes on)
     DevBuf<T> ring;                   // (kRing + 1) x [n][S]: the preconditioned residuals z_k
     DevBuf<double> ahist, bhist;      // (kRing + 1) x kMaxS: alpha_k, beta_k
+    DevBuf<double> chist;             // kRing x kMaxS: the combination coefficients c_i
     struct Lv {
         DevBuf<T> x, x2, r, b;
     };
@@ -1468,6 +1484,7 @@ struct BatchSolver {
         TM_CHECK(ring.alloc((size_t)(kRing + 1) * NS));
         TM_CHECK(ahist.alloc((size_t)(kRing + 1) * kMaxS));
         TM_CHECK(bhist.alloc((size_t)(kRing + 1) * kMaxS));
+        TM_CHECK(chist.alloc((size_t)(kRing + 1) * kMaxS));
         TM_CHECK(q.alloc(NS));
         lv.clear();
         lv.resize(solver.amg.levels.size());
@@ -1699,10 +1716,12 @@ struct BatchSolver {
             for (int k = 0; k < S; ++k) used = std::max(used, h_bst->iters[k]);
             last_inner[step] = used;
             inner_total += used;
-            if (sizeof(T) == 4 && S % 4 == 0)
-                hipLaunchKernelGGL(k_b_combine4, dim3(elem_grid(NS / 4)), dim3(kBlock), 0, s, NS, S, enq,
-                                   (const float*)ring.p, (const double*)ahist.p, (const double*)bhist.p, x);
-            else
+            if (sizeof(T) == 4 && S == kMaxS && enq <= kCombineK) {
+                hipLaunchKernelGGL(k_b_coef, dim3(1), dim3(kMaxS), 0, s, S, enq, (const double*)ahist.p,
+                                   (const double*)bhist.p, chist.p);
+                hipLaunchKernelGGL(k_b_combine4, dim3(elem_grid(NS / 4)), dim3(kBlock), 0, s, NS, enq,
+                                   (const float*)ring.p, (const double*)chist.p, x);
+            } else
                 hipLaunchKernelGGL((k_b_combine<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S, enq,
                                    (const T*)ring.p, (const double*)ahist.p, (const double*)bhist.p, x);
         }
