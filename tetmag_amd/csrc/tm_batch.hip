@@ -689,24 +689,10 @@ __global__ __launch_bounds__(kBlock) void k_b_post(SlotArgs A, int S, const doub
     if (part) b_partial(part, S, v);
 }
 
-// Wide-gather forms (FP32, S = 64) of the pre-smoother, the transfer
-// products and the post-smoother.
-template <bool BLK>
-__global__ __launch_bounds__(kBlock) void k_b_pre_w(SlotArgs AW, const float* __restrict__ dg,
-                                                    const double* __restrict__ dinv, const float* __restrict__ b,
-                                                    double w, float* __restrict__ x, float* __restrict__ r) {
-    if (!any_active(AW.gate, AW.gate_n)) return;
-    auto ld = [&](int row, int j) { return *(const f4*)(b + (size_t)row * 64 + 4 * j); };
-    auto epi = [&](int row, int j, f4 acc, f4 bi) {
-        if (dg) acc += dg[row] * bi;  // diagonal kept out of the slots
-        const size_t e = (size_t)row * 64 + 4 * j;
-        if (x) *(f4*)(x + e) = (float)(w * dinv[row]) * bi;
-        *(f4*)(r + e) = bi - acc;
-    };
-    wide_walk<BLK>(AW, b, ld, epi);
-}
-
-// x = w dinv b + P xc: the pre-smoothed iterate (not stored by k_b_pre_w)
+// Wide-gather forms (FP32, S = 64) of the transfer products and the
+// post-smoothers (the pre-smoother from zero is folded into the restriction
+// operator R (I - A w D^-1)).
+// x = w dinv b + P xc: the pre-smoothed iterate (not stored by the restriction pass)
 // plus the coarse correction, in one pass (P in chunks of CH slots)
 template <bool BLK, int CH>
 __global__ __launch_bounds__(kBlock) void k_b_prolong_w(SlotArgs P, const double* __restrict__ dinv,
@@ -1375,6 +1361,42 @@ struct SlotOp {
     }
 };
 
+// C = R (I - A w D^-1): the level-0 pre-smoother from zero followed by the
+// restriction, as one operator (rows of R, columns of A).  R: nr x n CSR;
+// A: n x n CSR (rp, ci, v); dinv: A's inverse diagonal.
+void restrict_presmoothed(const HostCsr& R, int n, const int* arp, const int* aci, const std::vector<double>& av,
+                          const double* dinv, double w, HostCsr& C) {
+    const int nr = (int)R.rp.size() - 1;
+    C.n = nr;
+    C.rp.assign((size_t)nr + 1, 0);
+    C.ci.clear();
+    C.v.clear();
+    std::vector<double> acc((size_t)n, 0.0);
+    std::vector<int> mark((size_t)n, -1), touched;
+    for (int c = 0; c < nr; ++c) {
+        touched.clear();
+        for (int q = R.rp[c]; q < R.rp[c + 1]; ++q) {
+            const int i = R.ci[q];
+            const double r = R.v[q];
+            for (int t = arp[i]; t < arp[i + 1]; ++t) {
+                const int j = aci[t];
+                if (mark[j] != c) {
+                    mark[j] = c;
+                    acc[j] = 0.0;
+                    touched.push_back(j);
+                }
+                acc[j] += r * ((j == i ? 1.0 : 0.0) - av[t] * w * dinv[j]);
+            }
+        }
+        std::sort(touched.begin(), touched.end());
+        for (int j : touched) {
+            C.ci.push_back(j);
+            C.v.push_back(acc[j]);
+        }
+        C.rp[c + 1] = (int)C.ci.size();
+    }
+}
+
 template <typename T>
 struct BatchSolver {
     const CgSolver* cg = nullptr;
@@ -1382,13 +1404,15 @@ struct BatchSolver {
     SlotOp sl0;                   // level-0 operator in slots
     DevBuf<T> wA0, wAW0;          // its weights: A, and A w D^-1 (pre-smoother)
     SlotOp sl0w;                  // the same without the diagonal, for the wide kernels
-    DevBuf<T> wA0w, wAW0w, dA0, dAW0;  // its weights and the diagonals
-    DevBuf<T> wIAW0, wIDA0;       // on sl0: I - A w D^-1 and I - w D^-1 A (level-0 smoothers, wide form)
-    bool pre1 = false;            // those are set (every level-0 row fits one chunk)
+    DevBuf<T> wA0w, dA0;          // its weights and the diagonal
+    DevBuf<T> wIDA0;              // on sl0: I - w D^-1 A (level-0 post-smoother, wide form)
+    bool post1 = false;           // wIDA0 is set (every level-0 row fits one chunk with its diagonal)
     DevBuf<double> wA64;          // A in FP64 (outer residual)
     struct LvOps {
         SlotOp A, R, P;           // level l >= 1: A; every level: restriction, prolongation
         DevBuf<T> wA, wAW, wR, wP;
+        SlotOp RA;                // wide form: R (I - A w D^-1), pre-smoothing from zero + restriction
+        DevBuf<T> wRA;
     };
     std::vector<LvOps> lo;
     DevBuf<double> dinv;
@@ -1433,25 +1457,19 @@ struct BatchSolver {
             const int* cc = solver.pat.h_csrcol.data();
             TM_CHECK(sl0w.build(n, n, solver.pat.h_rowptr.data(), cc, S, s, true));
             TM_CHECK(sl0w.values(solver.h_val, wA0w, s));
-            TM_CHECK(sl0w.values(solver.h_val, wAW0w, s, cc, solver.h_dinv.data(), M.levels[0].omega));
             TM_CHECK(sl0w.diagonal(solver.h_val, dA0, s));
-            TM_CHECK(sl0w.diagonal(solver.h_val, dAW0, s, solver.h_dinv.data(), M.levels[0].omega));
-            // the level-0 smoothers' operators with the diagonal in the slots
-            // (rows of up to 16 entries: one chunk): I - A w D^-1 and I - w D^-1 A
+            // the level-0 post-smoother's operator I - w D^-1 A with the
+            // diagonal in the slots (rows of up to 16 entries: one chunk)
             if (sl0.one_chunk()) {
                 const int* rp = solver.pat.h_rowptr.data();
                 const double w = M.levels[0].omega;
                 const std::vector<double>& dv = solver.h_dinv;
-                std::vector<double> v1(solver.h_val.size()), v2(solver.h_val.size());
+                std::vector<double> v2(solver.h_val.size());
                 for (int i = 0; i < n; ++i)
-                    for (int q = rp[i]; q < rp[i + 1]; ++q) {
-                        const double id = cc[q] == i ? 1.0 : 0.0;
-                        v1[q] = id - solver.h_val[q] * w * dv[cc[q]];
-                        v2[q] = id - solver.h_val[q] * w * dv[i];
-                    }
-                TM_CHECK(sl0.values(v1, wIAW0, s));
+                    for (int q = rp[i]; q < rp[i + 1]; ++q)
+                        v2[q] = (cc[q] == i ? 1.0 : 0.0) - solver.h_val[q] * w * dv[i];
                 TM_CHECK(sl0.values(v2, wIDA0, s));
-                pre1 = true;
+                post1 = true;
             }
         }
         lo.clear();
@@ -1459,19 +1477,30 @@ struct BatchSolver {
         for (int l = 0; l < L; ++l) {
             const auto& V = M.levels[l];
             const int nnext = l + 1 < L ? M.levels[l + 1].n : M.nc;
-            HostCsr h;
+            HostCsr h, ha;
+            std::vector<double> dv;
             if (l > 0) {
-                TM_CHECK(fetch_csr(V.A, h, s));
-                std::vector<double> dv((size_t)V.n);
+                TM_CHECK(fetch_csr(V.A, ha, s));
+                dv.resize((size_t)V.n);
                 TM_HIP(hipMemcpyAsync(dv.data(), V.dinv.p, dv.size() * sizeof(double), hipMemcpyDeviceToHost, s));
                 TM_HIP(hipStreamSynchronize(s));
-                TM_CHECK(lo[l].A.build(V.n, V.n, h.rp.data(), h.ci.data(), S, s));
-                TM_CHECK(lo[l].A.values(h.v, lo[l].wA, s));
-                TM_CHECK(lo[l].A.values(h.v, lo[l].wAW, s, h.ci.data(), dv.data(), V.omega));
+                TM_CHECK(lo[l].A.build(V.n, V.n, ha.rp.data(), ha.ci.data(), S, s));
+                TM_CHECK(lo[l].A.values(ha.v, lo[l].wA, s));
+                TM_CHECK(lo[l].A.values(ha.v, lo[l].wAW, s, ha.ci.data(), dv.data(), V.omega));
             }
             TM_CHECK(fetch_csr(V.R, h, s));
             TM_CHECK(lo[l].R.build(V.R.n, V.n, h.rp.data(), h.ci.data(), S, s));
             TM_CHECK(lo[l].R.values(h.v, lo[l].wR, s));
+            if (wide()) {  // pre-smoothing from zero and restriction in one pass: R (I - A w D^-1)
+                HostCsr c;
+                if (l == 0)
+                    restrict_presmoothed(h, n, solver.pat.h_rowptr.data(), solver.pat.h_csrcol.data(), solver.h_val,
+                                         solver.h_dinv.data(), V.omega, c);
+                else
+                    restrict_presmoothed(h, V.n, ha.rp.data(), ha.ci.data(), ha.v, dv.data(), V.omega, c);
+                TM_CHECK(lo[l].RA.build(V.R.n, V.n, c.rp.data(), c.ci.data(), S, s));
+                TM_CHECK(lo[l].RA.values(c.v, lo[l].wRA, s));
+            }
             TM_CHECK(fetch_csr(V.P, h, s));
             TM_CHECK(lo[l].P.build(V.P.n, nnext, h.rp.data(), h.ci.data(), S, s, false,
                                    wide() ? SlotOp::wide_chunk(V.P.n, h.rp.data()) : kSlot));
@@ -1516,18 +1545,18 @@ struct BatchSolver {
         for (int l = 0; l < L; ++l) {
             const auto& V = M.levels[l];
             const LvOps& O = lo[l];
-            // the wide form leaves x = w dinv b to the prolongation pass
-            T* xpre = wide() ? nullptr : lv[l].x.p;
-            if (l == 0 && wide() && pre1)  // r = (I - A w D^-1) b: gathers only
-                spmm(false, false, dim3(G), sl0.args(e, wIAW0.p), b, lv[0].r.p, s);
-            else if (l == 0 && wide())
-                pre(false, dim3(G), sl0w.args(e, wAW0w.p), dinv.p, b, V.omega, xpre, lv[0].r.p, s, dAW0.p);
-            else if (l == 0)
-                pre(sl0.long_rows(), dim3(G), sl0.args(e, wAW0.p), dinv.p, b, V.omega, xpre, lv[0].r.p, s);
-            else
-                pre(O.A.long_rows(), dim3(row_grid(V.n)), O.A.args(e, O.wAW.p), V.dinv.p, b, V.omega, xpre,
-                    lv[l].r.p, s);
-            spmm(false, O.R.long_rows(), dim3(row_grid(V.R.n)), O.R.args(e, O.wR.p), lv[l].r.p, lv[l].b.p, s);
+            if (wide()) {
+                // b_{l+1} = R (I - A w D^-1) b: pre-smoothing from zero and
+                // restriction in one pass over the coarse rows (the pre-smoothed
+                // iterate w D^-1 b is formed by the prolongation pass)
+                spmm(false, O.RA.long_rows(), dim3(row_grid(V.R.n)), O.RA.args(e, O.wRA.p), b, lv[l].b.p, s);
+            } else {
+                if (l == 0)
+                    pre(dim3(G), sl0.args(e, wAW0.p), dinv.p, b, V.omega, lv[0].x.p, lv[0].r.p, s);
+                else
+                    pre(dim3(row_grid(V.n)), O.A.args(e, O.wAW.p), V.dinv.p, b, V.omega, lv[l].x.p, lv[l].r.p, s);
+                spmm(false, O.R.long_rows(), dim3(row_grid(V.R.n)), O.R.args(e, O.wR.p), lv[l].r.p, lv[l].b.p, s);
+            }
             b = lv[l].b.p;
         }
         hipLaunchKernelGGL((k_b_dense<T>), dim3(std::max(1, std::min(M.nc, 4096))), dim3(kBlock), 0, s, M.nc, S,
@@ -1543,7 +1572,7 @@ struct BatchSolver {
             else
                 spmm(true, O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), xcoarse, lv[l].x.p, s);
             if (l == 0) {
-                if (wide() && pre1) {  // z = (I - w D^-1 A) x + w D^-1 b
+                if (wide() && post1) {  // z = (I - w D^-1 A) x + w D^-1 b
                     if constexpr (kWideT)
                         hipLaunchKernelGGL(k_b_post1_w, dim3(G), dim3(kBlock), 0, s, gated(sl0.args(e, wIDA0.p)),
                                            dinv.p, (const float*)bl, V.omega, (const float*)lv[0].x.p, (float*)zout,
@@ -1578,21 +1607,9 @@ struct BatchSolver {
     bool wide() const { return kWideT && S == kMaxS; }
     // one block per row for long-row operators (grid: a multiple of the XCD count)
     static dim3 blk_grid(int rows) { return dim3(xcd_grid(std::min(4096, std::max(rows, 1)))); }
-    void pre(bool lg, dim3 g, SlotArgs a, const double* di, const T* bb, double w, T* x, T* rr, hipStream_t s,
-             const T* dg = nullptr) {
-        a = gated(a);
-        if constexpr (kWideT)
-            if (wide()) {
-                const float* df = (const float*)dg;
-                if (lg)
-                    hipLaunchKernelGGL(k_b_pre_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, df, di,
-                                       (const float*)bb, w, (float*)x, (float*)rr);
-                else
-                    hipLaunchKernelGGL(k_b_pre_w<false>, g, dim3(kBlock), 0, s, a, df, di, (const float*)bb, w,
-                                       (float*)x, (float*)rr);
-                return;
-            }
-        hipLaunchKernelGGL((k_b_pre<T>), g, dim3(kBlock), 0, s, a, S, di, bb, w, x, rr);
+    // pre-smoothing from zero (generic form; the wide form folds it into the restriction)
+    void pre(dim3 g, SlotArgs a, const double* di, const T* bb, double w, T* x, T* rr, hipStream_t s) {
+        hipLaunchKernelGGL((k_b_pre<T>), g, dim3(kBlock), 0, s, gated(a), S, di, bb, w, x, rr);
     }
     void spmm(bool acc, bool lg, dim3 g, SlotArgs a, const T* x, T* y, hipStream_t s) {
         a = gated(a);
