@@ -274,7 +274,7 @@ def main():
     from tetmag_amd.context import FieldContext
 
     t0 = time.time()
-    prob = P.config0() if args.small else P.config1()
+    prob = P.cuboid_20x20x4() if args.small else P.cuboid_128x128x16()
     s = prob.summary()
     log(f"[setup] {prob.name} {s} host setup {time.time() - t0:.1f}s")
     t0 = time.time()

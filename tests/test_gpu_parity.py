@@ -22,7 +22,7 @@ ORACLE_TOL = 1e-14
 def _case(name):
     from tetmag_amd import problem as P
     if name == "cfg0":
-        return P.config0()
+        return P.cuboid_20x20x4()
     if name == "small_ku_dmi_zeeman":
         return P.small_problem(nx=7, ny=6, nz=3, h=4.0, jitter=0.1, seed=11, Ku=4e4, D=3e-3,
                                B_ext=(0.01, -0.02, 0.015))

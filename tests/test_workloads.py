@@ -34,7 +34,7 @@ def test_disk_delaunay_mesh():
 
 
 def test_hysteresis_sweep_batch():
-    prob, H, m0 = P.config3(h=6.0, nsamples=16)
+    prob, H, m0 = P.disk_hysteresis(h=6.0, nsamples=16)
     Bk = np.linalg.norm(H, axis=0) * MU0 * np.sign(H[0])
     assert np.allclose(Bk, np.linspace(-0.08, 0.08, 16))
     ang = np.arctan2(np.abs(H[1]), np.abs(H[0]))
@@ -44,7 +44,7 @@ def test_hysteresis_sweep_batch():
 
 
 def test_thin_film_skyrmion():
-    prob = P.config4(h=2.0)
+    prob = P.disk_skyrmion(h=2.0)
     n = prob.n
     m = prob.m0.reshape(3, n)
     xyz = prob.mesh.xyz

@@ -163,15 +163,15 @@ def cuboid_problem(name: str, nx: int, ny: int, nz: int, h: float, jitter: float
     return mesh_problem(name, M.kuhn_cuboid(nx, ny, nz, h, jitter, seed), mat, init, seed)
 
 
-def config0(seed: int = 2024) -> Problem:
+def cuboid_20x20x4(seed: int = 2024) -> Problem:
     """Kuhn cuboid 20x20x4 cells at 5 nm: 2,205 nodes, 9,600 tets, 1,122 boundary
     nodes; Ms 8e5 A/m, A 1.3e-11 J/m, K = D = 0, alpha 0.5; seeded random m."""
     return cuboid_problem("cuboid-20x20x4-h5", 20, 20, 4, 5.0, 0.0, seed, Material(), "random")
 
 
-def config1(seed: int = 2025) -> Problem:
+def cuboid_128x128x16(seed: int = 2025) -> Problem:
     """Kuhn cuboid 128x128x16 cells at 3 nm with +-0.1h interior jitter:
-    282,897 nodes, 1,572,864 tets, 40,962 boundary nodes; config-0 material plus
+    282,897 nodes, 1,572,864 tets, 40,962 boundary nodes; the 20x20x4 cuboid's material plus
     Ku = 5e4 J/m^3 along a seeded unit axis; vortex initial m."""
     mat = Material(Ku=5e4, ku_axis=_unit_axis(seed + 7))
     return cuboid_problem("cuboid-128x128x16-h3", 128, 128, 16, 3.0, 0.1, seed, mat, "vortex")
@@ -221,9 +221,9 @@ def neel_skyrmion(mesh: M.Mesh, radius: float, chirality: float = 1.0) -> np.nda
     return np.ascontiguousarray(m.T.reshape(-1))
 
 
-def config3(seed: int = 2027, h: float = 2.5, nsamples: int = 64):
-    """Disk 200 nm x 10 nm, seeded jittered Delaunay at target edge h; config-0
-    material; `nsamples` states, each under B_ext = mu0 H_k (cos t_i, sin t_i, 0)
+def disk_hysteresis(seed: int = 2027, h: float = 2.5, nsamples: int = 64):
+    """Disk 200 nm x 10 nm, seeded jittered Delaunay at target edge h; the 20x20x4
+    cuboid's material; `nsamples` states, each under B_ext = mu0 H_k (cos t_i, sin t_i, 0)
     with mu0 H_k spaced from -80 to +80 mT and t_i seeded in [0, 5 deg],
     starting from a slightly perturbed saturated state along x.
     Returns (problem, fields (3, S) in A/m, m0 (3, n, S))."""
@@ -242,7 +242,7 @@ def config3(seed: int = 2027, h: float = 2.5, nsamples: int = 64):
     return prob, H, m0
 
 
-def config4(seed: int = 2028, h: float = 0.6) -> Problem:
+def disk_skyrmion(seed: int = 2028, h: float = 0.6) -> Problem:
     """Disk 120 nm x 1 nm (seeded jittered Delaunay, edge h) with interfacial
     DMI 3e-3 J/m^2, Ku 8e5 J/m^3 along z, Ms 1.1e6 A/m, A 1.6e-11 J/m,
     alpha 0.3; Neel skyrmion of radius 15 nm."""
