@@ -475,10 +475,29 @@ __global__ void k_b_coef(int S, int K, const double* __restrict__ ah, const doub
     }
 }
 
-// k_b_combine for FP32 and S = 64 with the c_i from k_b_coef (staged in LDS,
-// K <= kCombineK): four samples per thread (16-byte loads of each z_i)
+// k_b_combine for S = 64 with the c_i from k_b_coef (staged in LDS, K <=
+// kCombineK): four samples per thread (vector loads of each z_i)
 constexpr int kCombineK = 48;
-__global__ __launch_bounds__(kBlock) void k_b_combine4(size_t NS, int K, const float* __restrict__ ring,
+// four consecutive elements as FP64 (one 16-byte load for float, two for double)
+template <typename T>
+__device__ inline void load4d(const T* p, double (&v)[4]) {
+    if constexpr (sizeof(T) == 4) {
+        const f4 a = *(const f4*)p;
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = a.z;
+        v[3] = a.w;
+    } else {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 a = ((const d2*)p)[0], b = ((const d2*)p)[1];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = b.x;
+        v[3] = b.y;
+    }
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_combine4(size_t NS, int K, const T* __restrict__ ring,
                                                        const double* __restrict__ ch, double* __restrict__ x) {
     typedef double d2 __attribute__((ext_vector_type(2)));
     typedef double d4 __attribute__((ext_vector_type(4)));
@@ -490,12 +509,13 @@ __global__ __launch_bounds__(kBlock) void k_b_combine4(size_t NS, int K, const f
         const int q = (int)(e4 % (kMaxS / 4));  // samples 4q .. 4q+3
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i = 0; i < K; ++i) {
-            const f4 v = ((const f4*)(ring + (size_t)i * NS))[e4];
+            double v[4];
+            load4d(ring + (size_t)i * NS + 4 * e4, v);
             const d4 c = cs[i * (kMaxS / 4) + q];
-            acc[0] += c.x != 0.0 ? c.x * (double)v.x : 0.0;
-            acc[1] += c.y != 0.0 ? c.y * (double)v.y : 0.0;
-            acc[2] += c.z != 0.0 ? c.z * (double)v.z : 0.0;
-            acc[3] += c.w != 0.0 ? c.w * (double)v.w : 0.0;
+            acc[0] += c.x != 0.0 ? c.x * v[0] : 0.0;
+            acc[1] += c.y != 0.0 ? c.y * v[1] : 0.0;
+            acc[2] += c.z != 0.0 ? c.z * v[2] : 0.0;
+            acc[3] += c.w != 0.0 ? c.w * v[3] : 0.0;
         }
         d2* xp = (d2*)(x + 4 * e4);
         const d2 x0 = xp[0], x1 = xp[1];
@@ -1733,11 +1753,11 @@ struct BatchSolver {
             for (int k = 0; k < S; ++k) used = std::max(used, h_bst->iters[k]);
             last_inner[step] = used;
             inner_total += used;
-            if (sizeof(T) == 4 && S == kMaxS && enq <= kCombineK) {
+            if (S == kMaxS && enq <= kCombineK) {
                 hipLaunchKernelGGL(k_b_coef, dim3(1), dim3(kMaxS), 0, s, S, enq, (const double*)ahist.p,
                                    (const double*)bhist.p, chist.p);
-                hipLaunchKernelGGL(k_b_combine4, dim3(elem_grid(NS / 4)), dim3(kBlock), 0, s, NS, enq,
-                                   (const float*)ring.p, (const double*)chist.p, x);
+                hipLaunchKernelGGL((k_b_combine4<T>), dim3(elem_grid(NS / 4)), dim3(kBlock), 0, s, NS, enq,
+                                   (const T*)ring.p, (const double*)chist.p, x);
             } else
                 hipLaunchKernelGGL((k_b_combine<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S, enq,
                                    (const T*)ring.p, (const double*)ahist.p, (const double*)bhist.p, x);
