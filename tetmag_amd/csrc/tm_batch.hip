@@ -647,26 +647,6 @@ __global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* pa
 }
 
 // --------------------------------------------------------- AMG V-cycle --
-// x = w dinv b; r = b - A x, with AW = A w D^-1 as the operator
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_b_pre(SlotArgs AW, int S, const double* __restrict__ dinv,
-                                                  const T* __restrict__ b, double w, T* __restrict__ x,
-                                                  T* __restrict__ r) {
-    if (!any_active(AW.gate, AW.gate_n)) return;
-    const int lane = threadIdx.x & (kWave - 1);
-    const bool on = lane < S;
-    const int lc = min(lane, S - 1);
-    auto ld = [&](int row) { return b[(size_t)row * S + lc]; };
-    auto epi = [&](int row, T acc, T bi) {
-        if (on) {
-            const size_t ee = (size_t)row * S + lane;
-            x[ee] = (T)(w * dinv[row]) * bi;
-            r[ee] = bi - acc;
-        }
-    };
-    slot_walk<T, T, kRowsPerStep<T>>(AW, b, lc, ld, epi);
-}
-
 // y = M x (ACC: y += M x)
 template <typename T, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_b_spmm(SlotArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
@@ -679,6 +659,23 @@ __global__ __launch_bounds__(kBlock) void k_b_spmm(SlotArgs M, int S, const T* _
         if (on) y[(size_t)row * S + lane] = ACC ? yi + acc : acc;
     };
     slot_walk<T, T, kRowsPerStep<T>>(M, x, lc, ld, epi);
+}
+
+// x = w dinv b + P xc: the pre-smoothed iterate plus the coarse correction
+// (the pre-smoother from zero is folded into the restriction operator)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_prolong(SlotArgs P, int S, const double* __restrict__ dinv,
+                                                      const T* __restrict__ b, double w, const T* __restrict__ xc,
+                                                      T* __restrict__ x) {
+    if (!any_active(P.gate, P.gate_n)) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int lc = min(lane, S - 1);
+    auto ld = [&](int row) { return b[(size_t)row * S + lc]; };
+    auto epi = [&](int row, T acc, T bi) {
+        if (on) x[(size_t)row * S + lane] = (T)(w * dinv[row]) * bi + acc;
+    };
+    slot_walk<T, T, kRowsPerStep<T>>(P, xc, lc, ld, epi);
 }
 
 // xo = x + w dinv (b - A x); optional partial (b, xo)
@@ -1422,7 +1419,7 @@ struct BatchSolver {
     const CgSolver* cg = nullptr;
     int n = 0, S = 0, G = 0;
     SlotOp sl0;                   // level-0 operator in slots
-    DevBuf<T> wA0, wAW0;          // its weights: A, and A w D^-1 (pre-smoother)
+    DevBuf<T> wA0;                // its weights
     SlotOp sl0w;                  // the same without the diagonal, for the wide kernels
     DevBuf<T> wA0w, dA0;          // its weights and the diagonal
     DevBuf<T> wIDA0;              // on sl0: I - w D^-1 A (level-0 post-smoother, wide form)
@@ -1430,8 +1427,8 @@ struct BatchSolver {
     DevBuf<double> wA64;          // A in FP64 (outer residual)
     struct LvOps {
         SlotOp A, R, P;           // level l >= 1: A; every level: restriction, prolongation
-        DevBuf<T> wA, wAW, wR, wP;
-        SlotOp RA;                // wide form: R (I - A w D^-1), pre-smoothing from zero + restriction
+        DevBuf<T> wA, wR, wP;
+        SlotOp RA;                // R (I - A w D^-1): pre-smoothing from zero + restriction
         DevBuf<T> wRA;
     };
     std::vector<LvOps> lo;
@@ -1443,7 +1440,7 @@ struct BatchSolver {
     DevBuf<double> ahist, bhist;      // (kRing + 1) x kMaxS: alpha_k, beta_k
     DevBuf<double> chist;             // kRing x kMaxS: the combination coefficients c_i
     struct Lv {
-        DevBuf<T> x, x2, r, b;
+        DevBuf<T> x, x2, b;
     };
     std::vector<Lv> lv;
     DevBuf<T> xc;
@@ -1470,9 +1467,6 @@ struct BatchSolver {
         TM_CHECK(sl0.build(n, n, solver.pat.h_rowptr.data(), solver.pat.h_csrcol.data(), S, s));
         TM_CHECK(sl0.values(solver.h_val, wA0, s));
         TM_CHECK(sl0.values(solver.h_val, wA64, s));
-        if (L > 0)
-            TM_CHECK(sl0.values(solver.h_val, wAW0, s, solver.pat.h_csrcol.data(), solver.h_dinv.data(),
-                                M.levels[0].omega));
         if (wide() && L > 0) {
             const int* cc = solver.pat.h_csrcol.data();
             TM_CHECK(sl0w.build(n, n, solver.pat.h_rowptr.data(), cc, S, s, true));
@@ -1506,12 +1500,11 @@ struct BatchSolver {
                 TM_HIP(hipStreamSynchronize(s));
                 TM_CHECK(lo[l].A.build(V.n, V.n, ha.rp.data(), ha.ci.data(), S, s));
                 TM_CHECK(lo[l].A.values(ha.v, lo[l].wA, s));
-                TM_CHECK(lo[l].A.values(ha.v, lo[l].wAW, s, ha.ci.data(), dv.data(), V.omega));
             }
             TM_CHECK(fetch_csr(V.R, h, s));
             TM_CHECK(lo[l].R.build(V.R.n, V.n, h.rp.data(), h.ci.data(), S, s));
             TM_CHECK(lo[l].R.values(h.v, lo[l].wR, s));
-            if (wide()) {  // pre-smoothing from zero and restriction in one pass: R (I - A w D^-1)
+            {  // pre-smoothing from zero and restriction in one pass: R (I - A w D^-1)
                 HostCsr c;
                 if (l == 0)
                     restrict_presmoothed(h, n, solver.pat.h_rowptr.data(), solver.pat.h_csrcol.data(), solver.h_val,
@@ -1541,7 +1534,6 @@ struct BatchSolver {
             const auto& L = solver.amg.levels[l];
             TM_CHECK(lv[l].x.alloc((size_t)L.n * S));
             TM_CHECK(lv[l].x2.alloc((size_t)L.n * S));
-            TM_CHECK(lv[l].r.alloc((size_t)L.n * S));
             TM_CHECK(lv[l].b.alloc((size_t)L.R.n * S));
         }
         TM_CHECK(xc.alloc((size_t)std::max(1, solver.amg.nc) * S));
@@ -1565,18 +1557,10 @@ struct BatchSolver {
         for (int l = 0; l < L; ++l) {
             const auto& V = M.levels[l];
             const LvOps& O = lo[l];
-            if (wide()) {
-                // b_{l+1} = R (I - A w D^-1) b: pre-smoothing from zero and
-                // restriction in one pass over the coarse rows (the pre-smoothed
-                // iterate w D^-1 b is formed by the prolongation pass)
-                spmm(false, O.RA.long_rows(), dim3(row_grid(V.R.n)), O.RA.args(e, O.wRA.p), b, lv[l].b.p, s);
-            } else {
-                if (l == 0)
-                    pre(dim3(G), sl0.args(e, wAW0.p), dinv.p, b, V.omega, lv[0].x.p, lv[0].r.p, s);
-                else
-                    pre(dim3(row_grid(V.n)), O.A.args(e, O.wAW.p), V.dinv.p, b, V.omega, lv[l].x.p, lv[l].r.p, s);
-                spmm(false, O.R.long_rows(), dim3(row_grid(V.R.n)), O.R.args(e, O.wR.p), lv[l].r.p, lv[l].b.p, s);
-            }
+            // b_{l+1} = R (I - A w D^-1) b: pre-smoothing from zero and
+            // restriction in one pass over the coarse rows (the pre-smoothed
+            // iterate w D^-1 b is formed by the prolongation pass)
+            spmm(false, O.RA.long_rows(), dim3(row_grid(V.R.n)), O.RA.args(e, O.wRA.p), b, lv[l].b.p, s);
             b = lv[l].b.p;
         }
         hipLaunchKernelGGL((k_b_dense<T>), dim3(std::max(1, std::min(M.nc, 4096))), dim3(kBlock), 0, s, M.nc, S,
@@ -1590,7 +1574,9 @@ struct BatchSolver {
                 prolong(O.P.long_rows(), O.P.chunk, dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p),
                         l == 0 ? dinv.p : V.dinv.p, bl, V.omega, xcoarse, lv[l].x.p, s);
             else
-                spmm(true, O.P.long_rows(), dim3(row_grid(V.P.n)), O.P.args(e, O.wP.p), xcoarse, lv[l].x.p, s);
+                hipLaunchKernelGGL((k_b_prolong<T>), dim3(row_grid(V.P.n)), dim3(kBlock), 0, s,
+                                   gated(O.P.args(e, O.wP.p)), S, l == 0 ? dinv.p : V.dinv.p, bl, V.omega, xcoarse,
+                                   lv[l].x.p);
             if (l == 0) {
                 if (wide() && post1) {  // z = (I - w D^-1 A) x + w D^-1 b
                     if constexpr (kWideT)
@@ -1627,10 +1613,6 @@ struct BatchSolver {
     bool wide() const { return kWideT && S == kMaxS; }
     // one block per row for long-row operators (grid: a multiple of the XCD count)
     static dim3 blk_grid(int rows) { return dim3(xcd_grid(std::min(4096, std::max(rows, 1)))); }
-    // pre-smoothing from zero (generic form; the wide form folds it into the restriction)
-    void pre(dim3 g, SlotArgs a, const double* di, const T* bb, double w, T* x, T* rr, hipStream_t s) {
-        hipLaunchKernelGGL((k_b_pre<T>), g, dim3(kBlock), 0, s, gated(a), S, di, bb, w, x, rr);
-    }
     void spmm(bool acc, bool lg, dim3 g, SlotArgs a, const T* x, T* y, hipStream_t s) {
         a = gated(a);
         if constexpr (kWideT)
