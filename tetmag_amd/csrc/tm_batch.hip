@@ -647,16 +647,16 @@ __global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* pa
 }
 
 // --------------------------------------------------------- AMG V-cycle --
-// y = M x (ACC: y += M x)
-template <typename T, bool ACC>
+// y = M x
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_b_spmm(SlotArgs M, int S, const T* __restrict__ x, T* __restrict__ y) {
     if (!any_active(M.gate, M.gate_n)) return;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
-    auto ld = [&](int row) { return ACC ? y[(size_t)row * S + lc] : (T)0; };
-    auto epi = [&](int row, T acc, T yi) {
-        if (on) y[(size_t)row * S + lane] = ACC ? yi + acc : acc;
+    auto ld = [&](int) { return (T)0; };
+    auto epi = [&](int row, T acc, T) {
+        if (on) y[(size_t)row * S + lane] = acc;
     };
     slot_walk<T, T, kRowsPerStep<T>>(M, x, lc, ld, epi);
 }
@@ -723,13 +723,12 @@ __global__ __launch_bounds__(kBlock) void k_b_prolong_w(SlotArgs P, const double
     wide_walk<BLK, CH>(P, xc, ld, epi);
 }
 
-template <bool ACC, bool BLK>
+// y = M x (the restriction operators)
+template <bool BLK>
 __global__ __launch_bounds__(kBlock) void k_b_spmm_w(SlotArgs M, const float* __restrict__ x, float* __restrict__ y) {
     if (!any_active(M.gate, M.gate_n)) return;
-    auto ld = [&](int row, int j) {
-        return ACC ? *(const f4*)(y + (size_t)row * 64 + 4 * j) : f4{0.f, 0.f, 0.f, 0.f};
-    };
-    auto epi = [&](int row, int j, f4 acc, f4 yi) { *(f4*)(y + (size_t)row * 64 + 4 * j) = ACC ? yi + acc : acc; };
+    auto ld = [&](int, int) { return 0; };
+    auto epi = [&](int row, int j, f4 acc, int) { *(f4*)(y + (size_t)row * 64 + 4 * j) = acc; };
     wide_walk<BLK>(M, x, ld, epi);
 }
 
@@ -1560,7 +1559,7 @@ struct BatchSolver {
             // b_{l+1} = R (I - A w D^-1) b: pre-smoothing from zero and
             // restriction in one pass over the coarse rows (the pre-smoothed
             // iterate w D^-1 b is formed by the prolongation pass)
-            spmm(false, O.RA.long_rows(), dim3(row_grid(V.R.n)), O.RA.args(e, O.wRA.p), b, lv[l].b.p, s);
+            spmm(O.RA.long_rows(), dim3(row_grid(V.R.n)), O.RA.args(e, O.wRA.p), b, lv[l].b.p, s);
             b = lv[l].b.p;
         }
         hipLaunchKernelGGL((k_b_dense<T>), dim3(std::max(1, std::min(M.nc, 4096))), dim3(kBlock), 0, s, M.nc, S,
@@ -1613,21 +1612,18 @@ struct BatchSolver {
     bool wide() const { return kWideT && S == kMaxS; }
     // one block per row for long-row operators (grid: a multiple of the XCD count)
     static dim3 blk_grid(int rows) { return dim3(xcd_grid(std::min(4096, std::max(rows, 1)))); }
-    void spmm(bool acc, bool lg, dim3 g, SlotArgs a, const T* x, T* y, hipStream_t s) {
+    // y = M x; lg: one block per row (long rows, wide form)
+    void spmm(bool lg, dim3 g, SlotArgs a, const T* x, T* y, hipStream_t s) {
         a = gated(a);
         if constexpr (kWideT)
             if (wide()) {
                 const float* xf = (const float*)x;
                 float* yf = (float*)y;
-                const dim3 gb = blk_grid(a.n);
-                if (acc && lg) hipLaunchKernelGGL((k_b_spmm_w<true, true>), gb, dim3(kBlock), 0, s, a, xf, yf);
-                else if (acc) hipLaunchKernelGGL((k_b_spmm_w<true, false>), g, dim3(kBlock), 0, s, a, xf, yf);
-                else if (lg) hipLaunchKernelGGL((k_b_spmm_w<false, true>), gb, dim3(kBlock), 0, s, a, xf, yf);
-                else hipLaunchKernelGGL((k_b_spmm_w<false, false>), g, dim3(kBlock), 0, s, a, xf, yf);
+                if (lg) hipLaunchKernelGGL(k_b_spmm_w<true>, blk_grid(a.n), dim3(kBlock), 0, s, a, xf, yf);
+                else hipLaunchKernelGGL(k_b_spmm_w<false>, g, dim3(kBlock), 0, s, a, xf, yf);
                 return;
             }
-        if (acc) hipLaunchKernelGGL((k_b_spmm<T, true>), g, dim3(kBlock), 0, s, a, S, x, y);
-        else hipLaunchKernelGGL((k_b_spmm<T, false>), g, dim3(kBlock), 0, s, a, S, x, y);
+        hipLaunchKernelGGL((k_b_spmm<T>), g, dim3(kBlock), 0, s, a, S, x, y);
     }
     // x = w dinv b + P xc (wide form only); ch: P's chunk size
     void prolong(bool lg, int ch, dim3 g, SlotArgs a, const double* di, const T* bb, double w, const T* xc, T* x,
