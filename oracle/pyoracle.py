@@ -51,6 +51,11 @@ def lib():
             "ref_dense_gemv": ([I, P, P, P], None),
             "ref_replace_rows_identity": ([I, P, P, P, P, P, P, P], I),
             "ref_solver_create": ([I, P, P, P], P),
+            "ref_solver_create_amg": ([I, P, P, P, D, D, I], P),
+            "ref_solver_levels": ([P, P, I], I),
+            "ref_bem_rows": ([I, I, P, P, P, P, P, P, P, I, P, P], None),
+            "ref_stt_uterm": ([I, P, P, P, P, P, P, P, P, P, P], None),
+            "ref_stt_llg": ([I, D, D, P, P, P], None),
             "ref_solver_destroy": ([P], None),
             "ref_solver_solve": ([P, P, P, D, I, I, P], I),
             "ref_exchange_field": ([I, P, P, P, P, P, P, P], None),
@@ -95,19 +100,37 @@ def i32(a):
 
 
 class Solver:
-    """SolverFactory-like handle: matrix set once, warm-started solves."""
+    """SolverFactory-like handle: matrix set once, warm-started solves.
 
-    def __init__(self, rp, col, val):
+    precond "amg": smoothed-aggregation AMG with ILU(0) smoothing inside
+    GMRES(30), the reference's configuration (AMGCLSolver.h:89-101,
+    AMGCLSolver.cpp:39-44); "ilu0": ILU(0)-preconditioned GMRES(60)."""
+
+    def __init__(self, rp, col, val, precond="amg"):
         self.rp, self.col, self.val = i32(rp), i32(col), f64(val)
         self.n = self.rp.shape[0] - 1
-        self.h = lib().ref_solver_create(self.n, p(self.rp), p(self.col), p(self.val))
+        self.precond = precond
+        if precond == "amg":
+            self.h = lib().ref_solver_create_amg(self.n, p(self.rp), p(self.col), p(self.val), 0.0, 0.0, 0)
+            self.restart = 30
+        elif precond == "ilu0":
+            self.h = lib().ref_solver_create(self.n, p(self.rp), p(self.col), p(self.val))
+            self.restart = 60
+        else:
+            raise ValueError(precond)
         if not self.h:
-            raise RuntimeError("ILU(0) factorisation failed")
+            raise RuntimeError(f"{precond} preconditioner set-up failed")
         self.x = np.zeros(self.n)
         self.last_iters = 0
         self.last_relres = 0.0
 
-    def solve(self, b, tol=1e-14, maxit=20000, restart=60, warm=True):
+    def levels(self):
+        sizes = np.zeros(32, np.int32)
+        k = lib().ref_solver_levels(self.h, p(sizes), 32)
+        return [int(v) for v in sizes[:k]]
+
+    def solve(self, b, tol=1e-14, maxit=20000, restart=None, warm=True):
+        restart = self.restart if restart is None else restart
         b = f64(b)
         if not warm:
             self.x[:] = 0.0
@@ -189,6 +212,15 @@ class Oracle:
     def set_B(self, B):
         self._B = np.asfortranarray(B, dtype=np.float64)
 
+    def bem_rows(self, rows):
+        """Rows `rows` of B (BEMprocessing.cpp:375-411) without forming B:
+        (len(rows), bnx), row k = B[rows[k], :]."""
+        rows = i32(rows)
+        out = np.empty((rows.shape[0], self.bnx))
+        lib().ref_bem_rows(self.bnx, self.nbel, p(self.xyz), p(self.bnodes), p(self.bindex), p(self.faces),
+                           p(self.face_area), p(self.face_normal), p(self.sub), rows.shape[0], p(rows), p(out))
+        return out
+
     # -- demag solvers (DemagField.cpp:204-220, FEMprocessing.cpp:329-369) ----
     def replaced(self, fixed_mask):
         L = lib()
@@ -211,9 +243,14 @@ class Oracle:
         fx[self.bnodes] = True
         return self.replaced(fx)
 
-    def solvers(self):
+    def solvers(self, precond=None):
+        """(Neumann, Dirichlet) solvers; precond "amg" (default, the
+        reference's SA-AMG) or "ilu0", fixed by the first call."""
         if self._solvers is None:
-            self._solvers = (Solver(*self.neumann_matrix()), Solver(*self.dirichlet_matrix()))
+            pc = precond or "amg"
+            self._solvers = (Solver(*self.neumann_matrix(), precond=pc), Solver(*self.dirichlet_matrix(), precond=pc))
+        elif precond is not None and precond != self._solvers[0].precond:
+            raise ValueError(f"solvers already built with {self._solvers[0].precond}")
         return self._solvers
 
     # -- fields ---------------------------------------------------------------
@@ -240,6 +277,21 @@ class Oracle:
         lib().ref_idmi_field(self.n, p(self.rowptr), p(self.col), p(self.gX), p(self.gY), p(self.tX), p(self.tY),
                              p(f64(D)), p(self.nodevol), p(f64(m)), p(H))
         return H
+
+    def stt_uterm(self, m, eta):
+        """Zhang-Li u term U_c = sum_d eta_d .* (grad_d m_c) (TheLLG.cpp:402-416);
+        eta: (3, n) or (3n,) per-node current factors."""
+        e = f64(eta).reshape(3, self.n)
+        U = np.empty(3 * self.n)
+        lib().ref_stt_uterm(self.n, p(self.rowptr), p(self.col), p(self.gX), p(self.gY), p(self.gZ), p(f64(e[0])),
+                            p(f64(e[1])), p(f64(e[2])), p(f64(m)), p(U))
+        return U
+
+    def stt_llg(self, m, U, alpha, beta):
+        """-(beta - alpha) m x U - (1 + alpha beta) m x (m x U) (TheLLG.cpp:209-212)."""
+        out = np.empty(3 * self.n)
+        lib().ref_stt_llg(self.n, float(alpha), float(beta), p(f64(m)), p(f64(U)), p(out))
+        return out
 
     def div(self, m, Js):
         b = np.empty(self.n)
@@ -285,8 +337,9 @@ class Oracle:
     def eval_rhs(self, m, *, Js, A, Ku=None, axis=None, D=None, Hext=None, alpha=0.5, variant=1, demag=True,
                  tol=1e-14, warm=True, Di=None):
         """TheLLG::classicVersion with every term of the composition
-        (TheLLG.cpp:150-178, 238-254); Hext is a uniform (3,) field in A/m.
-        Material arrays are per node and already scaled (A / scale^2, D / scale)."""
+        (TheLLG.cpp:150-178, 238-254); Hext is a uniform (3,) field or a
+        per-node (3n,) field in A/m.  Material arrays are per node and
+        already scaled (A / scale^2, D / scale)."""
         n = self.n
         m = f64(m)
         out = {"Hexc": self.exchange(m, A)}
@@ -298,7 +351,11 @@ class Oracle:
             out["Hidmi"] = self.idmi(m, Di)
             out["Hdmi"] = out["Hidmi"] + out.get("Hdmi", 0.0)
         if Hext is not None and np.any(np.asarray(Hext) != 0):
-            out["Hext"] = np.concatenate([np.full(n, Hext[0]), np.full(n, Hext[1]), np.full(n, Hext[2])])
+            He = f64(Hext).ravel()
+            if He.shape[0] == 3 * n:
+                out["Hext"] = He
+            else:
+                out["Hext"] = np.concatenate([np.full(n, He[0]), np.full(n, He[1]), np.full(n, He[2])])
         if demag:
             out["Hdem"], out["u1"], out["u2"] = self.demag(m, Js, tol=tol, warm=warm)
         out["Heff"] = self.total(Js, out.get("Hexc"), out.get("Hku"), out.get("Hdmi"), out.get("Hext"),

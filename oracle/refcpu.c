@@ -445,25 +445,42 @@ void ref_solid_angles(int ntet, const double* xyz, const int* fel, const int* bi
 /* BEMprocessing.cpp:375-411 -- dense double-layer matrix, column-major
  * bnx x bnx: B(j,j) = sub_j/(4 pi) - 1; B(n, m) += Lindholm weight of vertex
  * m of every face not containing node n. */
+static void bem_row(int nrow, int bnx, int nbel, const double* xyz, const int* bnodes, const int* bindex,
+                    const int* faces, const double* face_area, const double* face_normal, const double* sub,
+                    double* row, size_t stride) {
+    for (int m = 0; m < bnx; ++m) row[(size_t)m * stride] = 0.0;
+    row[(size_t)nrow * stride] = sub[nrow] / (4. * REF_PI) - 1.;
+    const double* x0 = xyz + 3 * bnodes[nrow];
+    for (int f = 0; f < nbel; ++f) {
+        const int* v = faces + 3 * f;
+        int b0 = bindex[v[0]], b1 = bindex[v[1]], b2 = bindex[v[2]];
+        if (nrow == b0 || nrow == b1 || nrow == b2) continue;
+        double L[3];
+        ref_lindholm_weights(x0, xyz + 3 * v[0], xyz + 3 * v[1], xyz + 3 * v[2], face_area[f], face_normal + 3 * f,
+                             L);
+        row[(size_t)b0 * stride] += L[0];
+        row[(size_t)b1 * stride] += L[1];
+        row[(size_t)b2 * stride] += L[2];
+    }
+}
+
 void ref_bem_matrix(int bnx, int nbel, const double* xyz, const int* bnodes, const int* bindex, const int* faces,
                     const double* face_area, const double* face_normal, const double* sub, double* B) {
 #pragma omp parallel for schedule(dynamic, 8)
-    for (int nrow = 0; nrow < bnx; ++nrow) {
-        for (int m = 0; m < bnx; ++m) B[(size_t)m * bnx + nrow] = 0.0;
-        B[(size_t)nrow * bnx + nrow] = sub[nrow] / (4. * REF_PI) - 1.;
-        const double* x0 = xyz + 3 * bnodes[nrow];
-        for (int f = 0; f < nbel; ++f) {
-            const int* v = faces + 3 * f;
-            int b0 = bindex[v[0]], b1 = bindex[v[1]], b2 = bindex[v[2]];
-            if (nrow == b0 || nrow == b1 || nrow == b2) continue;
-            double L[3];
-            ref_lindholm_weights(x0, xyz + 3 * v[0], xyz + 3 * v[1], xyz + 3 * v[2], face_area[f],
-                                 face_normal + 3 * f, L);
-            B[(size_t)b0 * bnx + nrow] += L[0];
-            B[(size_t)b1 * bnx + nrow] += L[1];
-            B[(size_t)b2 * bnx + nrow] += L[2];
-        }
-    }
+    for (int nrow = 0; nrow < bnx; ++nrow)
+        bem_row(nrow, bnx, nbel, xyz, bnodes, bindex, faces, face_area, face_normal, sub, B + nrow, (size_t)bnx);
+}
+
+/* The same matrix, rows rows[0..nrows) only (BEMprocessing.cpp:375-411 visit
+ * every face for each row, so a row costs nbel Lindholm evaluations): out is
+ * nrows x bnx row-major, out[k][m] = B(rows[k], m). */
+void ref_bem_rows(int bnx, int nbel, const double* xyz, const int* bnodes, const int* bindex, const int* faces,
+                  const double* face_area, const double* face_normal, const double* sub, int nrows, const int* rows,
+                  double* out) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < nrows; ++k)
+        bem_row(rows[k], bnx, nbel, xyz, bnodes, bindex, faces, face_area, face_normal, sub, out + (size_t)k * bnx,
+                1);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -599,12 +616,342 @@ static double dot_n(int n, const double* a, const double* b) {
     return s;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Smoothed-aggregation AMG with ILU(0) smoothing, as the reference          */
+/* configures AMGCL (ddemidov/amgcl, fetched at configure time with          */
+/* GIT_TAG master, i.e. unpinned, external/CMakeLists.txt; not vendored):     */
+/* amg<smoothed_aggregation, ilu0> inside gmres (AMGCLSolver.h:89-101), with  */
+/* coarsening.relax = 0.75 (AMGCLSolver.cpp:40, GPUSolver.cu:45),             */
+/* coarse_enough = 100 and direct_coarse = false (AMGCLSolver.cpp:43-44), and */
+/* AMGCL's defaults otherwise: strength eps_strong = 0.08, plain greedy       */
+/* aggregation, Gershgorin bound for rho(D^-1 A_F), one pre- and one          */
+/* post-smoothing sweep, V-cycle, GMRES(30).  Restated from the published     */
+/* method (Vanek, Mandel, Brezina, Computing 56 (1996) 179) and AMGCL's       */
+/* documented parameters; iterates are not claimed to match AMGCL's, only     */
+/* the converged solution (||b - A x|| <= tol ||b||) is relied upon.          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int n, m; /* rows, columns */
+    int* rp;
+    int* col;
+    double* val;
+} csr_t;
+
+static void csr_free(csr_t* a) {
+    free(a->rp); free(a->col); free(a->val);
+    a->rp = NULL; a->col = NULL; a->val = NULL;
+}
+
+static int csr_alloc(csr_t* a, int n, int m, size_t nnz) {
+    a->n = n; a->m = m;
+    a->rp = (int*)calloc((size_t)n + 1, sizeof(int));
+    a->col = (int*)malloc(sizeof(int) * (nnz ? nnz : 1));
+    a->val = (double*)malloc(sizeof(double) * (nnz ? nnz : 1));
+    return a->rp && a->col && a->val ? 0 : -1;
+}
+
+static void csr_spmv(const csr_t* a, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < a->n; ++i) {
+        double s = 0.0;
+        for (int p = a->rp[i]; p < a->rp[i + 1]; ++p) s += a->val[p] * x[a->col[p]];
+        y[i] = s;
+    }
+}
+
+/* C = A^T (columns sorted because rows are visited in order) */
+static int csr_transpose(const csr_t* a, csr_t* t) {
+    size_t nnz = (size_t)a->rp[a->n];
+    if (csr_alloc(t, a->m, a->n, nnz)) return -1;
+    for (size_t p = 0; p < nnz; ++p) t->rp[a->col[p] + 1]++;
+    for (int i = 0; i < a->m; ++i) t->rp[i + 1] += t->rp[i];
+    int* pos = (int*)malloc(sizeof(int) * ((size_t)a->m + 1));
+    memcpy(pos, t->rp, sizeof(int) * ((size_t)a->m + 1));
+    for (int i = 0; i < a->n; ++i)
+        for (int p = a->rp[i]; p < a->rp[i + 1]; ++p) {
+            int q = pos[a->col[p]]++;
+            t->col[q] = i;
+            t->val[q] = a->val[p];
+        }
+    free(pos);
+    return 0;
+}
+
+static int cmp_int_asc(const void* x, const void* y) {
+    int a = *(const int*)x, b = *(const int*)y;
+    return (a > b) - (a < b);
+}
+
+/* C = A B (Gustavson row-by-row product, sorted columns) */
+static int csr_spgemm(const csr_t* A, const csr_t* B, csr_t* C) {
+    const int n = A->n, m = B->m;
+    int* mark = (int*)malloc(sizeof(int) * (size_t)(m > 0 ? m : 1));
+    double* acc = (double*)calloc((size_t)(m > 0 ? m : 1), sizeof(double));
+    int* rp = (int*)calloc((size_t)n + 1, sizeof(int));
+    for (int j = 0; j < m; ++j) mark[j] = -1;
+    size_t nnz = 0;
+    for (int i = 0; i < n; ++i) { /* symbolic */
+        for (int p = A->rp[i]; p < A->rp[i + 1]; ++p) {
+            int k = A->col[p];
+            for (int q = B->rp[k]; q < B->rp[k + 1]; ++q)
+                if (mark[B->col[q]] != i) { mark[B->col[q]] = i; ++nnz; }
+        }
+        rp[i + 1] = (int)nnz;
+    }
+    if (nnz > 0x7fffffffULL) { free(mark); free(acc); free(rp); return -1; }
+    C->n = n; C->m = m; C->rp = rp;
+    C->col = (int*)malloc(sizeof(int) * (nnz ? nnz : 1));
+    C->val = (double*)malloc(sizeof(double) * (nnz ? nnz : 1));
+    for (int j = 0; j < m; ++j) mark[j] = -1;
+    for (int i = 0; i < n; ++i) { /* numeric */
+        int q0 = rp[i], cnt = 0;
+        for (int p = A->rp[i]; p < A->rp[i + 1]; ++p) {
+            int k = A->col[p];
+            double a = A->val[p];
+            for (int q = B->rp[k]; q < B->rp[k + 1]; ++q) {
+                int j = B->col[q];
+                if (mark[j] != i) { mark[j] = i; C->col[q0 + cnt++] = j; acc[j] = 0.0; }
+                acc[j] += a * B->val[q];
+            }
+        }
+        qsort(C->col + q0, (size_t)cnt, sizeof(int), cmp_int_asc);
+        for (int c = 0; c < cnt; ++c) C->val[q0 + c] = acc[C->col[q0 + c]];
+    }
+    free(mark); free(acc);
+    return 0;
+}
+
+/* Plain greedy aggregation on the strength graph (AMGCL plain_aggregates):
+ * j != i is a strong neighbour of i when a_ij^2 > eps^2 |a_ii a_jj|.  Nodes
+ * without strong neighbours are left out (zero rows of P).  Nodes are
+ * visited in order; an unassigned node starts an aggregate, takes its strong
+ * neighbours (also ones tentatively given to an earlier aggregate), and
+ * tentatively claims the unassigned strong neighbours of those; aggregates
+ * left empty are dropped and the rest renumbered.  strong[p] flags entry p.
+ * Returns the number of aggregates. */
+static int sa_aggregates(const csr_t* A, double eps, const double* dia, char* strong, int* id) {
+    const int n = A->n;
+    const int UNDEF = -1, REMOVED = -2;
+    for (int i = 0; i < n; ++i) {
+        int any = 0;
+        for (int p = A->rp[i]; p < A->rp[i + 1]; ++p) {
+            int j = A->col[p];
+            double v = A->val[p];
+            strong[p] = (char)(j != i && v * v > eps * eps * fabs(dia[i] * dia[j]));
+            any |= strong[p];
+        }
+        id[i] = any ? UNDEF : REMOVED;
+    }
+    int count = 0;
+    int* neib = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; ++i) {
+        if (id[i] != UNDEF) continue;
+        int cur = count++;
+        id[i] = cur;
+        int nn = 0;
+        for (int p = A->rp[i]; p < A->rp[i + 1]; ++p) {
+            int c = A->col[p];
+            if (strong[p] && id[c] != REMOVED) { id[c] = cur; neib[nn++] = c; }
+        }
+        for (int k = 0; k < nn; ++k) {
+            int c = neib[k];
+            for (int p = A->rp[c]; p < A->rp[c + 1]; ++p) {
+                int cc = A->col[p];
+                if (strong[p] && id[cc] == UNDEF) id[cc] = cur;
+            }
+        }
+    }
+    free(neib);
+    int* renum = (int*)malloc(sizeof(int) * (size_t)(count > 0 ? count : 1));
+    for (int k = 0; k < count; ++k) renum[k] = 0;
+    for (int i = 0; i < n; ++i) if (id[i] >= 0) renum[id[i]] = 1;
+    int nagg = 0;
+    for (int k = 0; k < count; ++k) renum[k] = renum[k] ? nagg++ : -1;
+    for (int i = 0; i < n; ++i) if (id[i] >= 0) id[i] = renum[id[i]];
+    free(renum);
+    return nagg;
+}
+
+/* Smoothed prolongator P = (I - omega D_F^-1 A_F) P_tent with the filtered
+ * matrix A_F (strong off-diagonal entries kept, weak ones lumped into the
+ * diagonal), omega = relax (4/3) / rho, rho = max_i sum_j |A_F,ij| / |D_F,i|
+ * (Gershgorin), P_tent(i, id[i]) = 1. */
+static int sa_prolongator(const csr_t* A, const char* strong, const int* id, int nagg, double relax, csr_t* P) {
+    const int n = A->n;
+    double* dF = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double rho = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double d = 0.0, s = 0.0;
+        for (int p = A->rp[i]; p < A->rp[i + 1]; ++p)
+            if (A->col[p] == i || !strong[p]) d += A->val[p];
+        for (int p = A->rp[i]; p < A->rp[i + 1]; ++p)
+            if (A->col[p] != i && strong[p]) s += fabs(A->val[p]);
+        dF[i] = d;
+        double r = d != 0.0 ? (s + fabs(d)) / fabs(d) : 0.0;
+        if (r > rho) rho = r;
+    }
+    const double omega = rho > 0 ? relax * (4.0 / 3.0) / rho : 0.0;
+    /* S = I - omega D_F^-1 A_F on A's pattern, then P = S P_tent */
+    csr_t S, T;
+    csr_alloc(&S, n, n, (size_t)A->rp[n]);
+    size_t k = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int p = A->rp[i]; p < A->rp[i + 1]; ++p) {
+            int j = A->col[p];
+            double v;
+            if (j == i) v = 1.0 - omega * (dF[i] != 0.0 ? dF[i] / dF[i] : 0.0);
+            else if (strong[p]) v = -omega * A->val[p] / dF[i];
+            else continue;
+            S.col[k] = j;
+            S.val[k] = v;
+            ++k;
+        }
+        S.rp[i + 1] = (int)k;
+    }
+    csr_alloc(&T, n, nagg, (size_t)n);
+    k = 0;
+    for (int i = 0; i < n; ++i) {
+        if (id[i] >= 0) { T.col[k] = id[i]; T.val[k] = 1.0; ++k; }
+        T.rp[i + 1] = (int)k;
+    }
+    int r = csr_spgemm(&S, &T, P);
+    csr_free(&S); csr_free(&T); free(dF);
+    return r;
+}
+
+typedef struct {
+    csr_t A, P, R; /* P, R: to and from the next coarser level */
+    ilu0_t f;
+    double *x, *b, *r;
+} amg_level_t;
+
+typedef struct {
+    int nlev;
+    amg_level_t lev[32];
+} amg_t;
+
+static void amg_free(amg_t* a) {
+    if (!a) return;
+    for (int l = 0; l < a->nlev; ++l) {
+        amg_level_t* L = &a->lev[l];
+        csr_free(&L->A); csr_free(&L->P); csr_free(&L->R);
+        free(L->f.lu); free(L->f.diag); free(L->x); free(L->b); free(L->r);
+    }
+    free(a);
+}
+
+static amg_t* amg_setup(int n, const int* rowptr, const int* col, const double* val, double relax, double eps,
+                        int coarse_enough) {
+    amg_t* a = (amg_t*)calloc(1, sizeof(amg_t));
+    csr_t* A0 = &a->lev[0].A;
+    csr_alloc(A0, n, n, (size_t)rowptr[n]);
+    memcpy(A0->rp, rowptr, sizeof(int) * ((size_t)n + 1));
+    memcpy(A0->col, col, sizeof(int) * (size_t)rowptr[n]);
+    memcpy(A0->val, val, sizeof(double) * (size_t)rowptr[n]);
+    a->nlev = 1;
+    while (a->nlev < 32 && a->lev[a->nlev - 1].A.n > coarse_enough) {
+        amg_level_t* L = &a->lev[a->nlev - 1];
+        const csr_t* A = &L->A;
+        double* dia = (double*)calloc((size_t)A->n, sizeof(double));
+        for (int i = 0; i < A->n; ++i)
+            for (int p = A->rp[i]; p < A->rp[i + 1]; ++p) if (A->col[p] == i) dia[i] = A->val[p];
+        char* strong = (char*)malloc((size_t)A->rp[A->n] + 1);
+        int* id = (int*)malloc(sizeof(int) * (size_t)A->n);
+        int nagg = sa_aggregates(A, eps, dia, strong, id);
+        int ok = nagg > 0 && nagg < A->n && sa_prolongator(A, strong, id, nagg, relax, &L->P) == 0;
+        free(dia); free(strong); free(id);
+        if (!ok) break;
+        csr_transpose(&L->P, &L->R);
+        csr_t AP;
+        if (csr_spgemm(A, &L->P, &AP)) break;
+        csr_t* Ac = &a->lev[a->nlev].A;
+        int r = csr_spgemm(&L->R, &AP, Ac);
+        csr_free(&AP);
+        if (r) break;
+        a->nlev++;
+    }
+    for (int l = 0; l < a->nlev; ++l) {
+        amg_level_t* L = &a->lev[l];
+        L->f.n = L->A.n;
+        L->f.rowptr = L->A.rp;
+        L->f.col = L->A.col;
+        L->f.val = L->A.val;
+        if (ilu0_factor(&L->f) != 0) { a->nlev = l; amg_free(a); return NULL; }
+        L->x = (double*)calloc((size_t)L->A.n, sizeof(double));
+        L->b = (double*)calloc((size_t)L->A.n, sizeof(double));
+        L->r = (double*)calloc((size_t)L->A.n, sizeof(double));
+    }
+    return a;
+}
+
+/* x += ILU0^-1 (b - A x) (AMGCL relaxation::ilu0 with damping 1) */
+static void amg_smooth(amg_level_t* L, const double* b, double* x) {
+    csr_spmv(&L->A, x, L->r);
+    for (int i = 0; i < L->A.n; ++i) L->r[i] = b[i] - L->r[i];
+    double* z = L->r; /* ilu0_apply reads r[i] before writing z[i] */
+    ilu0_apply(&L->f, L->r, z);
+    for (int i = 0; i < L->A.n; ++i) x[i] += z[i];
+}
+
+/* One V-cycle from x = 0 on level l (AMGCL amg::cycle, npre = npost = 1; the
+ * coarsest level, direct_coarse = false, is only smoothed). */
+static void amg_cycle(amg_t* a, int l, const double* b, double* x) {
+    amg_level_t* L = &a->lev[l];
+    const int n = L->A.n;
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+    amg_smooth(L, b, x);
+    if (l + 1 < a->nlev) {
+        amg_level_t* C = &a->lev[l + 1];
+        csr_spmv(&L->A, x, L->r);
+        for (int i = 0; i < n; ++i) L->r[i] = b[i] - L->r[i];
+        csr_spmv(&L->R, L->r, C->b);
+        amg_cycle(a, l + 1, C->b, C->x);
+        csr_spmv(&L->P, C->x, L->r);
+        for (int i = 0; i < n; ++i) x[i] += L->r[i];
+    }
+    amg_smooth(L, b, x);
+}
+
 typedef struct {
     ilu0_t f;
     int* rowptr;
     int* col;
     double* val;
+    amg_t* amg; /* non-null: AMG V-cycle preconditioner instead of ILU(0) */
 } ref_solver_t;
+
+static void precond_apply(ref_solver_t* s, const double* r, double* z) {
+    if (s->amg) amg_cycle(s->amg, 0, r, z);
+    else ilu0_apply(&s->f, r, z);
+}
+
+/* Number of levels and level sizes of an AMG solver (0 for ILU(0)). */
+int ref_solver_levels(void* h, int* sizes, int cap) {
+    ref_solver_t* s = (ref_solver_t*)h;
+    if (!s || !s->amg) return 0;
+    for (int l = 0; l < s->amg->nlev && l < cap; ++l) sizes[l] = s->amg->lev[l].A.n;
+    return s->amg->nlev;
+}
+
+/* SolverFactory::setMatrix + compute (SolverFactory.cpp:45-48;
+ * AMGCLSolver.cpp:39-95) with the AMG preconditioner described above.
+ * relax <= 0 selects the reference values (0.75, 0.08, 100). */
+void* ref_solver_create_amg(int n, const int* rowptr, const int* col, const double* val, double relax, double eps,
+                            int coarse_enough) {
+    if (relax <= 0) { relax = 0.75; eps = 0.08; coarse_enough = 100; }
+    ref_solver_t* s = (ref_solver_t*)calloc(1, sizeof(ref_solver_t));
+    int nnz = rowptr[n];
+    s->rowptr = (int*)malloc(sizeof(int) * (size_t)(n + 1));
+    s->col = (int*)malloc(sizeof(int) * (size_t)nnz);
+    s->val = (double*)malloc(sizeof(double) * (size_t)nnz);
+    memcpy(s->rowptr, rowptr, sizeof(int) * (size_t)(n + 1));
+    memcpy(s->col, col, sizeof(int) * (size_t)nnz);
+    memcpy(s->val, val, sizeof(double) * (size_t)nnz);
+    s->f.n = n;
+    s->amg = amg_setup(n, rowptr, col, val, relax, eps, coarse_enough);
+    if (!s->amg) { free(s->rowptr); free(s->col); free(s->val); free(s); return NULL; }
+    return s;
+}
 
 /* SolverFactory::setMatrix + compute (SolverFactory.cpp:45-48;
  * AMGCLSolver.cpp:59-95): copy the matrix and build the preconditioner once. */
@@ -631,6 +978,7 @@ void* ref_solver_create(int n, const int* rowptr, const int* col, const double* 
 void ref_solver_destroy(void* h) {
     ref_solver_t* s = (ref_solver_t*)h;
     if (!s) return;
+    amg_free(s->amg);
     free(s->f.lu); free(s->f.diag); free(s->rowptr); free(s->col); free(s->val); free(s);
 }
 
@@ -641,8 +989,7 @@ void ref_solver_destroy(void* h) {
  * final true relative residual. */
 int ref_solver_solve(void* h, const double* b, double* x, double tol, int maxit, int restart, double* relres) {
     ref_solver_t* s = (ref_solver_t*)h;
-    const ilu0_t* f = &s->f;
-    int n = f->n;
+    int n = s->f.n;
     const int *rowptr = s->rowptr, *col = s->col;
     const double* val = s->val;
     int m = restart;
@@ -670,7 +1017,7 @@ int ref_solver_solve(void* h, const double* b, double* x, double tol, int maxit,
         g[0] = beta;
         int kdone = 0;
         for (int j = 0; j < m && it < maxit; ++j) {
-            ilu0_apply(f, V + (size_t)j * n, Z);
+            precond_apply(s, V + (size_t)j * n, Z);
             ref_spmv(n, rowptr, col, val, Z, w);
             for (int i = 0; i <= j; ++i) { /* modified Gram-Schmidt */
                 const double* vi = V + (size_t)i * n;
@@ -712,7 +1059,7 @@ int ref_solver_solve(void* h, const double* b, double* x, double tol, int maxit,
             const double* vi = V + (size_t)i * n;
             for (int q = 0; q < n; ++q) w[q] += yv[i] * vi[q];
         }
-        ilu0_apply(f, w, Z); /* right preconditioning: x += M^-1 V y */
+        precond_apply(s, w, Z); /* right preconditioning: x += M^-1 V y */
         for (int q = 0; q < n; ++q) x[q] += Z[q];
     }
     if (relres) *relres = res;
@@ -872,6 +1219,43 @@ double ref_max_torque(int n, const double* m, const double* H) {
         if (t > mx) mx = t;
     }
     return mx;
+}
+
+/* Zhang-Li spin-transfer torque, u term: TheLLG.cpp:402-416 (calcUtermSTT,
+ * CPU branch; GPU twin EffFieldGPU.cu:107-148):
+ *   U_c = eta_x .* (gradX m_c) + eta_y .* (gradY m_c) + eta_z .* (gradZ m_c)
+ * for c = x, y, z, with the grad operators of the mesh (TheSimulation.cpp:170)
+ * and eta_d = jc_d * btimesMs * invJs (ProgramSpecs.cpp:393-421). */
+void ref_stt_uterm(int n, const int* rowptr, const int* col, const double* gX, const double* gY, const double* gZ,
+                   const double* eta_x, const double* eta_y, const double* eta_z, const double* m, double* U) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            const double* mc = m + (size_t)c * n;
+            double sx = 0, sy = 0, sz = 0;
+            for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+                double v = mc[col[p]];
+                sx += gX[p] * v;
+                sy += gY[p] * v;
+                sz += gZ[p] * v;
+            }
+            U[(size_t)c * n + i] = eta_x[i] * sx + eta_y[i] * sy + eta_z[i] * sz;
+        }
+    }
+}
+
+/* TheLLG.cpp:209-212 (sttDynamics, CPU branch; GPU twin EffFieldGPU.cu:434-475):
+ *   tau = -(beta - alpha) m x U - (1 + alpha beta) m x (m x U). */
+void ref_stt_llg(int n, double alpha, double beta, const double* m, const double* U, double* out) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double mi[3] = {m[i], m[n + i], m[2 * n + i]};
+        double u[3] = {U[i], U[n + i], U[2 * n + i]};
+        double mxu[3], mxmxu[3];
+        v3cross(mi, u, mxu);
+        v3cross(mi, mxu, mxmxu);
+        for (int d = 0; d < 3; ++d) out[(size_t)d * n + i] = -(beta - alpha) * mxu[d] - (1. + alpha * beta) * mxmxu[d];
+    }
 }
 
 /* EffFieldGPU.cu:625-663 -- m_i <- m_i / |m_i| */

@@ -8,7 +8,9 @@ import pytest
 from tetmag_amd import mesh as M
 from tetmag_amd import meshio as IO
 
-SP4 = "/root/reference/examples/example2/sp4_.vtk"
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SP4 = os.path.join(GOLDEN, "sp4_.vtk.xz")  # examples/example2/sp4_.vtk
+SP_STT = os.path.join(GOLDEN, "sp_stt.vtk.xz")  # examples/example3/sp_stt.vtk
 
 
 def _write_legacy(path, mesh, layout, mat=None):
@@ -68,7 +70,6 @@ def test_dense_matrix_file(tmp_path):
     assert np.array_equal(IO.read_dense(str(p)), B)
 
 
-@pytest.mark.skipif(not os.path.exists(SP4), reason="reference example mesh not present")
 def test_reference_example_mesh():
     """The muMAG SP4 mesh of the reference's example 2 (legacy VTK 5.1)."""
     mesh, _ = IO.read_vtk(SP4)
@@ -81,3 +82,12 @@ def test_reference_example_mesh():
     assert abs(np.abs(vol).sum() / 6 - 500 * 125 * 3) < 1e-6 * 500 * 125 * 3
     bnd = M.boundary(mesh)
     assert bnd.bnx > 0 and bnd.nbel > 0
+
+
+def test_reference_stt_example_mesh():
+    """The spin-transfer-torque mesh of the reference's example 3 (legacy VTK 5.1)."""
+    mesh, _ = IO.read_vtk(SP_STT)
+    assert mesh.n == 10269 and mesh.ntet == 39246
+    x, f = mesh.xyz, mesh.fel
+    vol = np.einsum("ij,ij->i", x[f[:, 1]] - x[f[:, 0]], np.cross(x[f[:, 2]] - x[f[:, 0]], x[f[:, 3]] - x[f[:, 0]]))
+    assert np.all(np.abs(vol) > 0)

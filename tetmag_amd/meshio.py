@@ -35,9 +35,15 @@ def _tokens(lines, start, count, dtype):
 
 
 def read_vtk(path: str, scale: float = 1e-9):
-    """Returns (Mesh, element_material or None)."""
-    with open(path) as f:
-        lines = [ln.strip() for ln in f]
+    """Returns (Mesh, element_material or None).  A path ending in ``.xz`` is
+    read through lzma."""
+    if path.endswith(".xz"):
+        import lzma
+        with lzma.open(path, "rt") as f:
+            lines = [ln.strip() for ln in f]
+    else:
+        with open(path) as f:
+            lines = [ln.strip() for ln in f]
     if not lines or not lines[0].startswith("# vtk DataFile"):
         raise ValueError(f"{path}: not a legacy VTK file")
     if lines[2].upper() != "ASCII":
