@@ -41,20 +41,34 @@ def algorithmic_bytes(n, nnz, b, it_n, it_d):
             + (3 * C + 32 * n) + 152 * n)
 
 
-def cpu_baseline(prob, ctx, dtau, sample_evals, tol):
-    """The FP64 CPU oracle (C/OpenMP restatement of the reference CPU path) on
-    the same mesh and trajectory; timed on `sample_evals` evaluations after one
-    untimed warm-up evaluation.  Setup (operators, ILU(0)) is not timed; the
-    dense B comes from the GPU assembly (reference form, identical to the
-    oracle's own to 1e-13)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prob, dtau, sample_evals, tol):
+    """The FP64 CPU oracle (C/OpenMP restatement of the reference CPU path:
+    its own operators, its own dense B in the reference form, SA-AMG + ILU(0)
+    smoothing inside GMRES(30) as the reference configures AMGCL) on the same
+    mesh and trajectory; the median of `sample_evals` warm-started evaluations
+    after one untimed warm-up evaluation.  Setup (operators, B, AMG) is not
+    timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     threads = O.num_threads()
     t0 = time.time()
     orc = O.Oracle(prob.mesh.xyz, prob.mesh.fel)
-    orc.set_B(ctx.bem_matrix())
-    orc.solvers()
-    log(f"[cpu] oracle setup {time.time() - t0:.1f}s threads={threads}")
+    sN, sD = orc.solvers("amg")
+    log(f"[cpu] oracle setup {time.time() - t0:.1f}s threads={threads} AMG levels {sN.levels()} / {sD.levels()}")
+    t0 = time.time()
+    orc.B  # noqa: B018  (assembled here, outside the timed evaluations)
+    log(f"[cpu] oracle B ({orc.bnx} x {orc.bnx}, reference form) assembled in {time.time() - t0:.1f}s")
     m = prob.m0.copy()
     kw = dict(Js=prob.Js, A=prob.A, Ku=prob.Ku, axis=prob.kaxis, D=prob.D, Hext=prob.hext, alpha=prob.alpha, tol=tol)
 
@@ -68,17 +82,26 @@ def cpu_baseline(prob, ctx, dtau, sample_evals, tol):
     t0 = time.perf_counter()
     m = step(m)  # warm-up (cold solves)
     log(f"[cpu] warm-up eval {time.perf_counter() - t0:.2f}s (GMRES iters {orc.last_iters})")
-    dt = 0.0
+    ts = []
     for k in range(sample_evals):
         t0 = time.perf_counter()
         m = step(m)
-        dt += time.perf_counter() - t0
-        log(f"[cpu] eval {k}: {time.perf_counter() - t0:.2f}s (GMRES iters {orc.last_iters})")
-    log(f"[cpu] {sample_evals} evals in {dt:.2f}s (GMRES iters last {orc.last_iters})")
-    return {"value": sample_evals / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"{sample_evals} warm-started FP64 evaluations of the same workload and trajectory "
-                      f"(oracle: GMRES(60)+ILU(0) to tol {tol:g}, OpenMP {threads} threads; "
-                      f"setup and B assembly untimed)"}
+        ts.append(time.perf_counter() - t0)
+        log(f"[cpu] eval {k}: {ts[-1]:.2f}s (GMRES iters {orc.last_iters})")
+    med = float(np.median(ts))
+    log(f"[cpu] median of {sample_evals} evals {med:.2f}s")
+    return {"value": 1.0 / med, "unit": "evals/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "eval_s": {"median": med, "min": float(min(ts)), "max": float(max(ts))},
+            "sample": f"median of {sample_evals} warm-started FP64 evaluations of the same workload and trajectory "
+                      f"after one warm-up (oracle: own operators and dense B, SA-AMG(relax 0.75, ILU0 smoothing) "
+                      f"+ GMRES(30) to tol {tol:g}, OpenMP {threads} threads; setup, B and AMG set-up untimed)"}
+
+
+def step_quantiles(events):
+    """p10 / median / p90 of the per-step durations between consecutive events (ms)."""
+    d = np.array([a.elapsed_time(b) for a, b in zip(events[:-1], events[1:])])
+    return {"p10": float(np.percentile(d, 10)), "median": float(np.median(d)), "p90": float(np.percentile(d, 90))}
 
 
 def grad_csr(prob):
@@ -112,25 +135,31 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
     out = {"form": name, "samples": nsamples, "gemm_tuning": bc.gemm_info()}
     log(f"[{name}] boundary GEMM tuning {out['gemm_tuning']}")
     if check:
-        dm, H, st = bc.eval_rhs_host(mb)
-        u1, u2 = bc.potentials()
         G = grad_csr(prob)
         eH, eD, eU, eE = [], [], [], []
-        for s in range(nsamples):
-            ms = fields[s]
-            d1, h1, _ = ctx.eval_rhs_host(ms)
-            p1, p2 = ctx.potentials()
-            e64 = ctx.energies(ms, recompute_demag=False)["demag"]
-            eH.append(float(np.linalg.norm(from_batch(H, s) - h1) / np.linalg.norm(h1)))
-            eD.append(float(np.linalg.norm(from_batch(dm, s) - d1) / np.linalg.norm(d1)))
-            u64 = p1 + p2
-            eU.append(float(np.linalg.norm(u1[:, s] + u2[:, s] - u64) / np.linalg.norm(u64)))
-            emx = demag_energy_from_u(prob, G, u1[:, s] + u2[:, s], ms)
-            eE.append(float(abs(emx - e64) / abs(e64)))
-        out["err_vs_fp64"] = {"heff_rel_l2_max": max(eH), "dmdt_rel_l2_max": max(eD), "u_rel_l2_max": max(eU),
-                              "demag_energy_rel_max": max(eE), "heff_rel_l2_median": float(np.median(eH))}
-        out["solve_stats"] = {k: v for k, v in st.items() if k != "max_torque"}
-        log(f"[{name}] errors vs FP64: {out['err_vs_fp64']} stats {out['solve_stats']}")
+        nchk = check if isinstance(check, int) and not isinstance(check, bool) else nsamples
+        for c0 in range(0, nchk, nsamples):
+            chunk = list(range(c0, min(c0 + nsamples, nchk)))
+            idx = chunk + list(range(nsamples - len(chunk)))  # a short last batch is padded with the first fields
+            dm, H, st = bc.eval_rhs_host(to_batch([fields[i] for i in idx]))
+            u1, u2 = bc.potentials()
+            for s, i in enumerate(chunk):
+                ms = fields[i]
+                d1, h1, _ = ctx.eval_rhs_host(ms)
+                p1, p2 = ctx.potentials()
+                e64 = ctx.energies(ms, recompute_demag=False)["demag"]
+                eH.append(float(np.linalg.norm(from_batch(H, s) - h1) / np.linalg.norm(h1)))
+                eD.append(float(np.linalg.norm(from_batch(dm, s) - d1) / np.linalg.norm(d1)))
+                u64 = p1 + p2
+                eU.append(float(np.linalg.norm(u1[:, s] + u2[:, s] - u64) / np.linalg.norm(u64)))
+                emx = demag_energy_from_u(prob, G, u1[:, s] + u2[:, s], ms)
+                eE.append(float(abs(emx - e64) / abs(e64)))
+            if c0 == 0:
+                out["solve_stats"] = {k: v for k, v in st.items() if k != "max_torque"}
+        out["err_vs_fp64"] = {"fields": nchk, "heff_rel_l2_max": max(eH), "dmdt_rel_l2_max": max(eD),
+                              "u_rel_l2_max": max(eU), "demag_energy_rel_max": max(eE),
+                              "heff_rel_l2_median": float(np.median(eH))}
+        log(f"[{name}] errors vs FP64 over {nchk} fields: {out['err_vs_fp64']} stats {out['solve_stats']}")
     bc.set_mag(mb)
 
     def step():
@@ -142,17 +171,17 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
     ctx.synchronize()
     bc.enable_timers(True)
     bc.timers(reset=True)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(ctx.stream)
-    for _ in range(steps):
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    evs[0].record(ctx.stream)
+    for k in range(steps):
         step()
-    e1.record(ctx.stream)
+        evs[k + 1].record(ctx.stream)
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
+    ms = evs[0].elapsed_time(evs[-1]) / steps
     ph = bc.timers()
     cnt = max(ph["count"], 1)
     out.update({"ms_per_batch": ms, "evals_per_s": nsamples * 1e3 / ms, "steps": steps,
+                "ms_per_batch_quantiles": step_quantiles(evs),
                 "phase_ms": {k: ph[k] / cnt for k in ("div_ms", "neumann_ms", "bem_ms", "dirichlet_ms", "fields_ms")},
                 "cg_iters_per_batch": [ph["iters_neumann"] / cnt, ph["iters_dirichlet"] / cnt]})
     log(f"[{name}] {steps} batch steps of {nsamples}: {ms:.3f} ms/batch = {out['evals_per_s']:.1f} evals/s; "
@@ -252,7 +281,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--dt-fs", type=float, default=20.0, help="Euler step between evaluations (fs)")
-    ap.add_argument("--cpu-evals", type=int, default=1)
+    ap.add_argument("--cpu-evals", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--small", action="store_true", help="20x20x4 cuboid instead (quick check)")
     ap.add_argument("--batch", type=int, default=64, help="samples per batched evaluation")
@@ -260,6 +289,10 @@ def main():
                     help="timed batch steps of each batched form (default: --steps; warm-up: --warmup)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the per-sample error check of the batched forms")
+    ap.add_argument("--profile-form", choices=["fp64", "mixed", "mixed_bem32"], default=None,
+                    help="time only this batched form (--warmup, --steps), nothing else; for kernel profiles")
+    ap.add_argument("--check-fields", type=int, default=200,
+                    help="seeded smooth fields on which the mixed form's error against FP64 is checked")
     ap.add_argument("--kernel-only", type=int, default=0, metavar="REPS",
                     help="only launch the dense boundary product REPS times, then a 1 GiB device copy REPS times "
                          "(the calibration kernel for hardware byte counters), and exit")
@@ -286,6 +319,16 @@ def main():
     if args.kernel_only:
         kernel_only(ctx, prob, args.kernel_only)
         return
+    if args.profile_form:
+        # one batched form only, no error check, no plain form, no CPU
+        # baseline: the command the kernel profiles are taken from
+        from tetmag_amd import _native as N
+        prec = {"fp64": N.TM_PREC_FP64, "mixed": N.TM_PREC_MIXED, "mixed_bem32": N.TM_PREC_MIXED_BEM32}
+        fields = P.smooth_random_fields(prob.mesh, args.batch, seed=2026)
+        leg = batched_leg(prob, ctx, dtau, prec[args.profile_form], args.batch, args.steps, args.warmup, fields,
+                          False)
+        print(json.dumps({"profile_form": leg}), flush=True)
+        return
 
     def step():
         ctx.eval_rhs(None)
@@ -297,16 +340,15 @@ def main():
     ctx.enable_timers(True)
     ctx.timers(reset=True)
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(ctx.stream)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    evs[0].record(ctx.stream)
     w0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         step()
-    e1.record(ctx.stream)
+        evs[k + 1].record(ctx.stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    ms = e0.elapsed_time(e1) / args.steps
+    ms = evs[0].elapsed_time(evs[-1]) / args.steps
     ph = ctx.timers()
     cnt = max(ph["count"], 1)
     it_n = ph["iters_neumann"] / cnt
@@ -322,6 +364,7 @@ def main():
     eval_bytes = algorithmic_bytes(n, nnz, b, it_n, it_d)
     torque = ctx.last_stats.max_torque
     plain = {"evals_per_s": 1e3 / ms, "ms_per_eval": ms, "steps": args.steps,
+             "ms_per_eval_quantiles": step_quantiles(evs),
              "cg_iters_per_eval": [round(it_n, 2), round(it_d, 2)],
              "phase_ms": {k: ph[k] / cnt for k in ("div_ms", "neumann_ms", "bem_ms", "dirichlet_ms", "fields_ms")},
              "eval_algorithmic_bytes": eval_bytes, "eval_achieved_GBs": eval_bytes / (ms * 1e-3) / 1e9,
@@ -329,11 +372,14 @@ def main():
     legs = {}
     if not args.no_batched and not args.small:
         from tetmag_amd import _native as N
-        fields = P.smooth_random_fields(prob.mesh, args.batch, seed=2026)
+        # 200 seeded smooth fields (8 Fourier modes, 20-200 nm): the first
+        # args.batch are advanced in the timed steps; the mixed form's error
+        # is checked on all of them, the other forms' on the first batch
+        fields = P.smooth_random_fields(prob.mesh, max(args.batch, args.check_fields), seed=2026)
         for prec in (N.TM_PREC_FP64, N.TM_PREC_MIXED, N.TM_PREC_MIXED_BEM32):
+            chk = False if args.no_check else (args.check_fields if prec == N.TM_PREC_MIXED else args.batch)
             leg = batched_leg(prob, ctx, dtau, prec, args.batch,
-                              args.steps if args.batch_steps is None else args.batch_steps, args.warmup, fields,
-                              not args.no_check)
+                              args.steps if args.batch_steps is None else args.batch_steps, args.warmup, fields, chk)
             legs[leg["form"]] = leg
     # headline: the fastest batched form whose field error against FP64 meets
     # the 1e-6 bar (the FP64 batched form always does), else the plain form
@@ -392,7 +438,7 @@ def main():
             ach = flops / (leg["phase_ms"]["bem_ms"] * 1e-3) / 1e12
             gt = leg.get("gemm_tuning", {})
             kern = ("k_b_gemm64, FP64 MFMA, tile shape kGemmShapes[%d] of tm_batch.hip, K split %d"
-                    % (-1 - gt["solution"], gt.get("ksplit", 1))) if gt.get("solution", 0) < 0 else "rocBLAS DGEMM"
+                    % (gt["solution"], gt.get("ksplit", 1))) if gt.get("own_kernel") else "rocBLAS DGEMM"
             result["roofline"] = {"bound": "mfma", "kernel": f"boundary GEMM B(b x b) U^T(b x {args.batch}): {kern}",
                                   "achieved": ach, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
                                   "frac": ach / FP64_MFMA_PEAK_TFS,
@@ -400,7 +446,7 @@ def main():
                                   "algorithmic_flops": flops, "b_bytes_read": 8.0 * b * b}
     if not args.no_cpu_baseline:
         try:
-            result["cpu_baseline"] = cpu_baseline(prob, ctx, dtau, args.cpu_evals, args.tol)
+            result["cpu_baseline"] = cpu_baseline(prob, dtau, args.cpu_evals, args.tol)
         except Exception as e:  # report, do not hide the GPU number
             log(f"[cpu] baseline failed: {e!r}")
             result["cpu_baseline"] = None

@@ -14,12 +14,17 @@
 //   * Sparse : compressed sparse matrix with rows(), cols(), nonZeros(),
 //              outerIndexPtr(), innerIndexPtr(), valuePtr() and a static
 //              IsRowMajor (Eigen::SparseMatrix<double, RowMajor> fits);
-//   * DevVec : device vector with data() and size() (the facade's own
-//              tmg::DeviceVector here, thrust::device_vector in the
-//              reference).
+//   * Vector : host column vector with Vector(n), size(), data()
+//              (Eigen::VectorXd fits; tmg::DenseVector standalone);
+//   * the device vector is tmg::DeviceVector, with the value semantics of
+//     the reference's thrust::device_vector<double> (dev_vec / devVecD).
 //
-// In tetmag:   using EffFieldGPU = tmg::EffFieldGPU_T<Eigen::MatrixXd, SpMat>;
-// Standalone:  using EffFieldGPU = tmg::EffFieldGPU_T<tmg::DenseMatrix, tmg::CsrMatrix>;
+// In tetmag:   using EffFieldGPU  = tmg::EffFieldGPU_T<Eigen::MatrixXd, SpMat>;
+//              using GpuPotential = tmg::GpuPotential_T<Eigen::MatrixXd, SpMat, Eigen::VectorXd>;
+//              using SpMatCUDA    = tmg::SpMatCUDA_T<SpMat, Eigen::VectorXd>;
+//              class GPUSolver : public tmg::GPUSolver_T<SolverFactory, SpMat, Eigen::VectorXd> {};
+//              using dev_vec = tmg::DeviceVector;
+// Standalone:  using EffFieldGPU = tmg::EffFieldGPU_T<tmg::DenseMatrix, tmg::CsrMatrix>;  (etc.)
 //
 // Errors: the reference prints and continues or exits; the facades throw
 // tmg::Error carrying the TM_ERR_* status, except where the reference returns
@@ -31,6 +36,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -76,7 +82,28 @@ struct CsrMatrix {  // compressed row storage
     const double* valuePtr() const { return val.data(); }
 };
 
-// Owning device vector (the thrust::device_vector<double> of the reference).
+// Column vector on the host (the Eigen::VectorXd of the reference when Eigen
+// is not available): Vector(n), size(), data(), resize(n), operator[].
+struct DenseVector {
+    std::vector<double> v;
+    DenseVector() = default;
+    explicit DenseVector(int n) : v((size_t)n, 0.0) {}
+    int size() const { return (int)v.size(); }
+    void resize(int n) { v.assign((size_t)n, 0.0); }
+    double* data() { return v.data(); }
+    const double* data() const { return v.data(); }
+    double& operator[](int i) { return v[(size_t)i]; }
+    double operator[](int i) const { return v[(size_t)i]; }
+    double* begin() { return v.data(); }
+    double* end() { return v.data() + v.size(); }
+    const double* begin() const { return v.data(); }
+    const double* end() const { return v.data() + v.size(); }
+};
+
+// Owning device vector with the value semantics of the reference's
+// thrust::device_vector<double> (deep copies on copy, moves are cheap), so
+// that expressions such as `dev_vec r = *gpucalc->ClassicLLG_dev(H, a)`
+// (LLGWrapper.cu:61, 81, 87) compile unchanged.
 class DeviceVector {
     double* p_ = nullptr;
     size_t n_ = 0;
@@ -84,8 +111,28 @@ class DeviceVector {
    public:
     DeviceVector() = default;
     explicit DeviceVector(size_t n) { resize(n); }
-    DeviceVector(const DeviceVector&) = delete;
-    DeviceVector& operator=(const DeviceVector&) = delete;
+    explicit DeviceVector(const std::vector<double>& h) {
+        resize(h.size());
+        if (n_ && hipMemcpy(p_, h.data(), n_ * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            throw Error(TM_ERR_HIP, "hipMemcpy");
+    }
+    DeviceVector(const DeviceVector& o) { *this = o; }
+    DeviceVector& operator=(const DeviceVector& o) {
+        if (this == &o) return *this;
+        resize(o.n_);
+        if (n_ && hipMemcpy(p_, o.p_, n_ * sizeof(double), hipMemcpyDeviceToDevice) != hipSuccess)
+            throw Error(TM_ERR_HIP, "hipMemcpy");
+        return *this;
+    }
+    DeviceVector(DeviceVector&& o) noexcept : p_(o.p_), n_(o.n_) {
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
+    DeviceVector& operator=(DeviceVector&& o) noexcept {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+        return *this;
+    }
     ~DeviceVector() {
         if (p_) (void)hipFree(p_);
     }
@@ -183,47 +230,113 @@ class SetDevice {
 };
 
 // ---- SpMatCUDA (SpMatCUDA.h:49-79) ---------------------------------------
-template <class Sparse>
+// Vector: the host column vector type (Eigen::VectorXd in tetmag).
+template <class Sparse, class Vector = DenseVector>
 class SpMatCUDA_T {
     tm_spmat* h_ = nullptr;
     int rows_ = 0, cols_ = 0;
-    std::vector<double> x_;
+    DeviceVector x_, y_;
 
    public:
-    const double alpha = 1.0, beta = 0.0;
+    const double alpha = 1.0;  // SAXPY parameters, as the reference exposes them
+    const double beta = 0.0;
+    double* dY = nullptr;  // output (device), valid after mvp()
+    double* dX = nullptr;  // input (device), valid after setX()
+    SpMatCUDA_T() = default;
     explicit SpMatCUDA_T(const Sparse& A) {
         detail::CsrView<Sparse> c(A);
         rows_ = c.rows;
         cols_ = c.cols;
         check(tm_spmat_create(rows_, cols_, c.rp.data(), c.ci.data(), c.val.data(), &h_), "tm_spmat_create");
+        x_.resize((size_t)cols_);
+        y_.resize((size_t)rows_);
+        dX = x_.data();
+        dY = y_.data();
     }
     SpMatCUDA_T(const SpMatCUDA_T&) = delete;
     SpMatCUDA_T& operator=(const SpMatCUDA_T&) = delete;
     virtual ~SpMatCUDA_T() { tm_spmat_destroy(h_); }
-    void setX(const std::vector<double>& x) { x_ = x; }
-    // y = A x on host data (mvp + mvpResEig)
-    std::vector<double> mvp(const std::vector<double>& x) {
-        std::vector<double> y((size_t)rows_);
-        check(tm_spmat_mvp(h_, x.data(), TM_HOST, y.data(), TM_HOST), "tm_spmat_mvp");
+    // setX (SpMatCUDA.cu:150-165): the input of the next mvp()
+    void setX(const Vector& x) { upload(x.data(), (size_t)x.size()); }
+    void setX(const std::vector<double>& x) { upload(x.data(), x.size()); }
+    void setX(const DeviceVector& x) {
+        if ((int)x.size() != cols_) throw Error(TM_ERR_BAD_ARG, "SpMatCUDA::setX");
+        x_ = x;
+        dX = x_.data();
+    }
+    // mvp() (SpMatCUDA.cu:131-141): y = A x on the device
+    void mvp() { check(tm_spmat_mvp(h_, x_.data(), TM_DEVICE, y_.data(), TM_DEVICE), "tm_spmat_mvp"); }
+    // devVecD mvp(const devVecD&) (SpMatCUDA.cu:143-148): returns A x by value
+    DeviceVector mvp(const DeviceVector& x) {
+        setX(x);
+        mvp();
+        return mvpResDev();
+    }
+    // mvpResEig / mvpResDev (SpMatCUDA.cu:160-169): the result of the last mvp()
+    Vector mvpResEig() {
+        Vector y(rows_);
+        check(hipDeviceSynchronize() == hipSuccess ? TM_OK : TM_ERR_HIP, "hipDeviceSynchronize");
+        if (rows_ && hipMemcpy(y.data(), y_.data(), (size_t)rows_ * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            throw Error(TM_ERR_HIP, "hipMemcpy");
         return y;
     }
-    std::vector<double> mvpResEig() { return mvp(x_); }
-    // y = A x on device data (mvp(devVecD) + mvpResDev)
-    void mvp(const DeviceVector& x, DeviceVector& y) {
-        y.resize((size_t)rows_);
-        check(tm_spmat_mvp(h_, x.data(), TM_DEVICE, y.data(), TM_DEVICE), "tm_spmat_mvp");
+    DeviceVector mvpResDev() {
+        check(hipDeviceSynchronize() == hipSuccess ? TM_OK : TM_ERR_HIP, "hipDeviceSynchronize");
+        return y_;
+    }
+
+   private:
+    void upload(const double* x, size_t n) {
+        if ((int)n != cols_) throw Error(TM_ERR_BAD_ARG, "SpMatCUDA::setX");
+        if (n && hipMemcpy(x_.data(), x, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            throw Error(TM_ERR_HIP, "hipMemcpy");
     }
 };
 
-// ---- GPUSolver : SolverFactory (GPUSolver.h:65-87, SolverFactory.h:42-65) --
-template <class Sparse>
-class GPUSolver_T {
+// ---- SolverFactory (SolverFactory.h:42-65) --------------------------------
+// The base of the solver family with the reference's protected state; tetmag
+// keeps its own SolverFactory (whose CPU solvers are not part of this
+// library) and derives GPUSolver_T from it instead.  makeSolver("gpu")
+// returns the GPU solver; the CPU solver types are the host's own.
+template <class Sparse, class Vector = DenseVector>
+class SolverFactory_T {
+   protected:
+    double cgTol = 1e-7;  // "CG tolerance" default, ProgramSpecs.cpp:70
+    Sparse A;
+    Vector b;
+    Vector x;
+    std::string type;
+    Vector previous;
+    std::string preconditionerType;
+
+   public:
+    static std::shared_ptr<SolverFactory_T> makeSolver(std::string type);
+    template <class V>
+    void setLoadVector(const V& v) {
+        b = Vector((int)v.size());
+        std::copy(v.data(), v.data() + v.size(), b.data());
+    }
+    virtual void solve() = 0;
+    virtual bool compute() = 0;
+    virtual void report() = 0;
+    virtual bool wasSuccessful() = 0;
+    virtual ~SolverFactory_T() {}
+    Vector result() { return x; }
+    void setMatrix(const Sparse& M) { A = M; }
+    void definePreconditioner(const std::string p) { preconditionerType = p; }
+    void setCGTolerance(const double t) { cgTol = t; }
+};
+
+// ---- GPUSolver : SolverFactory (GPUSolver.h:65-87) ------------------------
+// Base: SolverFactory_T<Sparse, Vector> or the host's own SolverFactory (it
+// must provide the protected members A, b, x, cgTol, preconditionerType of
+// SolverFactory.h:44-53).  compute() builds the solver once from A; solve()
+// solves A x = b warm-started from the previous solution (GPUSolver keeps
+// x_d between calls, GPUSolver.cu:82-86) to cgTol.
+template <class Base, class Sparse, class Vector = DenseVector>
+class GPUSolver_T : public Base {
     tm_solver* h_ = nullptr;
-    std::unique_ptr<detail::CsrView<Sparse>> A_;
-    std::vector<double> b_, x_;
-    double cgTol_ = 1e-7;  // ProgramSpecs.cpp:70 default
-    std::string preconditioner_ = "amg";
-    int iters_ = 0, status_ = TM_OK;
+    int n_ = 0, iters_ = 0, status_ = TM_OK;
     double error_ = NAN;
 
    public:
@@ -232,41 +345,62 @@ class GPUSolver_T {
     GPUSolver_T(const GPUSolver_T&) = delete;
     GPUSolver_T& operator=(const GPUSolver_T&) = delete;
     virtual ~GPUSolver_T() { tm_solver_destroy(h_); }
-    void setMatrix(const Sparse& A) { A_.reset(new detail::CsrView<Sparse>(A)); }
-    template <class Vec>
-    void setLoadVector(const Vec& b) {
-        b_.assign(b.data(), b.data() + b.size());
-    }
-    void setCGTolerance(double tol) { cgTol_ = tol; }
-    void definePreconditioner(const std::string& p) { preconditioner_ = p; }
-    // GPUSolver::compute (GPUSolver.cu:88-91): false when the matrix cannot be
-    // handled (setup throws in the reference, caught at GPUSolver.cu:74-76).
-    bool compute() {
-        if (!A_) return false;
-        const int pre = (preconditioner_ == "jacobi" || preconditioner_ == "diagonal") ? TM_PRECOND_JACOBI : TM_PRECOND_AMG;
+    // GPUSolver::compute / setup (GPUSolver.cu:57-79, 88-91): false when the
+    // matrix cannot be handled (setup throws in the reference and is caught).
+    bool compute() override {
+        detail::CsrView<Sparse> c(this->A);
+        const std::string& p = this->preconditionerType;
+        const int pre = (p == "jacobi" || p == "diagonal") ? TM_PRECOND_JACOBI : TM_PRECOND_AMG;
         tm_solver_destroy(h_);
         h_ = nullptr;
-        status_ = tm_solver_create(A_->rows, A_->rp.data(), A_->ci.data(), A_->val.data(), pre, &h_);
-        x_.assign((size_t)A_->rows, 0.0);
+        n_ = c.rows;
+        status_ = tm_solver_create(c.rows, c.rp.data(), c.ci.data(), c.val.data(), pre, &h_);
+        this->x = Vector(n_);
+        std::fill(this->x.data(), this->x.data() + n_, 0.0);
         return status_ == TM_OK;
     }
-    // warm-started from the previous solution, as GPUSolver keeps x_d
-    void solve() {
-        status_ = tm_solver_solve(h_, b_.data(), TM_HOST, x_.data(), TM_HOST, cgTol_, maxit, 2, &iters_, &error_);
-        if (status_ != TM_OK && status_ != TM_ERR_MAXITER && status_ != TM_ERR_NONFINITE) check(status_, "tm_solver_solve");
+    void solve() override {
+        if (!h_) throw Error(TM_ERR_NOT_READY, "GPUSolver::solve before compute");
+        if ((int)this->b.size() != n_) throw Error(TM_ERR_BAD_ARG, "GPUSolver::solve: load vector size");
+        if ((int)this->x.size() != n_) this->x = Vector(n_);
+        status_ = tm_solver_solve(h_, this->b.data(), TM_HOST, this->x.data(), TM_HOST, this->cgTol, maxit, 2, &iters_,
+                                  &error_);
+        if (status_ != TM_OK && status_ != TM_ERR_MAXITER && status_ != TM_ERR_NONFINITE)
+            check(status_, "tm_solver_solve");
     }
-    const std::vector<double>& result() const { return x_; }
-    bool wasSuccessful() const { return std::isfinite(error_) && status_ == TM_OK; }
+    // GPUSolver::report (GPUSolver.cu:52-54) prints the solver state
+    void report() override {
+        int nl = 0, sizes[16] = {0};
+        if (h_) tm_solver_amg_info(h_, &nl, sizes, 16);
+        std::string lv;
+        for (int l = 0; l < nl && l < 16; ++l) lv += (l ? " " : "") + std::to_string(sizes[l]);
+        std::printf("GPUSolver: PCG (%s), unknowns %d, AMG levels [%s], iterations %d, error %.3e\n",
+                    this->preconditionerType.empty() ? "amg" : this->preconditionerType.c_str(), n_, lv.c_str(),
+                    iters_, error_);
+    }
+    // GPUSolver::wasSuccessful (GPUSolver.cu:93-95): the residual is finite
+    bool wasSuccessful() override { return std::isfinite(error_) && status_ == TM_OK; }
     int iterations() const { return iters_; }
     double error() const { return error_; }
     int status() const { return status_; }
 };
+
+template <class Sparse, class Vector>
+std::shared_ptr<SolverFactory_T<Sparse, Vector>> SolverFactory_T<Sparse, Vector>::makeSolver(std::string t) {
+    // SolverFactory.cpp:124-134 ("gpu"); the other types are CPU solvers of the host
+    if (t == "gpu")
+        return std::shared_ptr<SolverFactory_T>(std::make_shared<GPUSolver_T<SolverFactory_T, Sparse, Vector>>());
+    throw Error(TM_ERR_UNSUPPORTED, "SolverFactory::makeSolver(" + t + ")");
+}
 
 // ---- EffFieldGPU (EffFieldGPU.h:54-127) ----------------------------------
 template <class Dense, class Sparse>
 class EffFieldGPU_T {
     int nx_;
     tm_ctx* h_ = nullptr;
+    // dxdt of the reference (EffFieldGPU.h:70): the *_dev results alias it
+    // and are overwritten by the next call, as in EffFieldGPU.cu:336-339
+    std::shared_ptr<DeviceVector> dxdt_;
     Dense field(int (*fn)(tm_ctx*, double*, int), const char* what) {
         Dense M(nx_, 3);
         check(fn(h_, M.data(), TM_HOST), what);
@@ -281,9 +415,14 @@ class EffFieldGPU_T {
         check(tm_llg_rhs(h_, detail::soa(H), TM_HOST, alpha, variant, out.data(), TM_HOST), "tm_llg_rhs");
         return out;
     }
+    std::shared_ptr<DeviceVector> llg_dev(const Dense& H, double alpha, int variant) {
+        check(tm_llg_rhs(h_, detail::soa(H), TM_HOST, alpha, variant, dxdt_->data(), TM_DEVICE), "tm_llg_rhs");
+        check(tm_ctx_synchronize(h_), "tm_ctx_synchronize");
+        return dxdt_;
+    }
 
    public:
-    explicit EffFieldGPU_T(int nx) : nx_(nx) { init(nx); }
+    explicit EffFieldGPU_T(const int nx) : nx_(nx) { init(nx); }
     EffFieldGPU_T(const EffFieldGPU_T&) = delete;
     EffFieldGPU_T& operator=(const EffFieldGPU_T&) = delete;
     ~EffFieldGPU_T() { tm_ctx_destroy(h_); }
@@ -291,12 +430,17 @@ class EffFieldGPU_T {
         if (h_) return;
         nx_ = nx;
         check(tm_ctx_create(nx, &h_), "tm_ctx_create");
+        dxdt_ = std::make_shared<DeviceVector>((size_t)3 * nx);
     }
     tm_ctx* handle() { return h_; }
 
-    std::vector<double> ClassicLLG_hst(const Dense& Heff, double alpha) { return llg(Heff, alpha, TM_LLG_CLASSIC); }
-    std::vector<double> LLG_noPrec_hst(const Dense& Heff, double alpha) { return llg(Heff, alpha, TM_LLG_NOPREC); }
-    std::vector<double> STT_term_LLG_hst(const Dense& U, double alpha, double beta) {
+    std::vector<double> ClassicLLG_hst(const Dense& Heff, const double alpha) {
+        return llg(Heff, alpha, TM_LLG_CLASSIC);
+    }
+    std::vector<double> LLG_noPrec_hst(const Dense& Heff, const double alpha) {
+        return llg(Heff, alpha, TM_LLG_NOPREC);
+    }
+    std::vector<double> STT_term_LLG_hst(const Dense& U, const double alpha, const double beta) {
         std::vector<double> out((size_t)3 * nx_);
         check(tm_stt_llg(h_, detail::soa(U), TM_HOST, alpha, beta, out.data(), TM_HOST), "tm_stt_llg");
         return out;
@@ -329,16 +473,18 @@ class EffFieldGPU_T {
     }
     Dense UTermSTT_GPU() { return field(tm_stt_uterm, "tm_stt_uterm"); }
 
-    // ClassicLLG_dev / LLG_noPrec_dev: result in the caller's device vector
-    void ClassicLLG_dev(const Dense& Heff, double alpha, DeviceVector& out) {
-        out.resize((size_t)3 * nx_);
-        check(tm_llg_rhs(h_, detail::soa(Heff), TM_HOST, alpha, TM_LLG_CLASSIC, out.data(), TM_DEVICE), "tm_llg_rhs");
-        check(tm_ctx_synchronize(h_), "tm_ctx_synchronize");
+    // ClassicLLG_dev / LLG_noPrec_dev / STT_term_LLG_dev (EffFieldGPU.cu:316-362,
+    // 434-459): the object's dxdt buffer, overwritten by the next call
+    std::shared_ptr<DeviceVector> ClassicLLG_dev(const Dense& Heff, const double alpha) {
+        return llg_dev(Heff, alpha, TM_LLG_CLASSIC);
     }
-    void LLG_noPrec_dev(const Dense& Heff, double alpha, DeviceVector& out) {
-        out.resize((size_t)3 * nx_);
-        check(tm_llg_rhs(h_, detail::soa(Heff), TM_HOST, alpha, TM_LLG_NOPREC, out.data(), TM_DEVICE), "tm_llg_rhs");
+    std::shared_ptr<DeviceVector> LLG_noPrec_dev(const Dense& Heff, const double alpha) {
+        return llg_dev(Heff, alpha, TM_LLG_NOPREC);
+    }
+    std::shared_ptr<DeviceVector> STT_term_LLG_dev(const Dense& U, const double alpha, const double beta) {
+        check(tm_stt_llg(h_, detail::soa(U), TM_HOST, alpha, beta, dxdt_->data(), TM_DEVICE), "tm_stt_llg");
         check(tm_ctx_synchronize(h_), "tm_ctx_synchronize");
+        return dxdt_;
     }
 
     double MaxTorque(const Dense& Heff) {
@@ -365,7 +511,7 @@ class EffFieldGPU_T {
 };
 
 // ---- GpuPotential (GpuPotential.h:50-75) ---------------------------------
-template <class Dense, class Sparse>
+template <class Dense, class Sparse, class Vector = DenseVector>
 class GpuPotential_T {
     tm_ctx* h_ = nullptr;
     int nx_ = 0;
@@ -391,8 +537,8 @@ class GpuPotential_T {
         check(tm_ctx_set_js(h_, Js.data()), "tm_ctx_set_js");
     }
     // sum_d tGrad_d (Js m_d), not divided by mu0 (GpuPotential.cu:86-106)
-    std::vector<double> calcDivM(const Dense& mag) {
-        std::vector<double> b((size_t)nx_);
+    Vector calcDivM(const Dense& mag) {
+        Vector b(nx_);
         check(tm_div_m(h_, detail::soa(mag), TM_HOST, b.data(), TM_HOST), "tm_div_m");
         return b;
     }
@@ -404,13 +550,14 @@ class GpuPotential_T {
     }
 };
 
-// ---- LLGWrapper (LLGWrapper.h:57-63) --------------------------------------
-// The reference's device right-hand side for CVODE (LLGWrapper.cu:104-107,
-// 213-228) copies u into the field object, evaluates the fields, sums them on
-// the host and returns the LLG right-hand side in a device vector.  Here one
-// call evaluates the whole right-hand side on the device of the context that
-// owns the operators (the EffFieldGPU's), including demag when the context
-// has its boundary, BEM matrix and demag solvers set.
+// ---- LLGWrapper (LLGWrapper.h:57-63) and the CVODE device right-hand side --
+// The reference's device right-hand side for CVODE (TheLLG::rhs_d,
+// TheLLG.h:109, LLGWrapper.cu:213-228) copies u into the field object,
+// evaluates the fields, sums them on the host and returns the LLG right-hand
+// side in a device vector.  Here one call evaluates the whole right-hand side
+// on the device of the context that owns the operators (the EffFieldGPU's),
+// including demag when the context has its boundary, BEM matrix and demag
+// solvers set.
 class LLGWrapper {
     int nx3 = 0;
     tm_ctx* ctx_ = nullptr;
@@ -426,9 +573,8 @@ class LLGWrapper {
         alpha_ = alpha;
         variant_ = removePrecession ? TM_LLG_NOPREC : TM_LLG_CLASSIC;
     }
-    // rhs_d: u, udot are 3n device pointers (SoA x|y|z).  Synchronous like
-    // the reference (udot is complete on return, whatever stream the caller
-    // reads it from).
+    // u, udot: 3n device pointers (SoA x|y|z).  Synchronous like the
+    // reference (udot is complete on return, whatever stream reads it).
     void rhs_d(const double* u, double* udot, tm_eval_stats* stats = nullptr) {
         if (!ctx_) throw Error(TM_ERR_NOT_READY, "LLGWrapper::rhs_d before bind");
         check(tm_eval_rhs(ctx_, u, TM_DEVICE, alpha_, variant_, udot, TM_DEVICE, nullptr, TM_DEVICE, stats),
@@ -441,6 +587,30 @@ class LLGWrapper {
         rhs_d(u.data(), udot.data());
     }
 };
+
+// user_data of the CVODE callback (TheLLG::alloc_user_data, TheLLG.cpp:657-663)
+struct UserData {
+    LLGWrapper* llg;
+    int nx;
+};
+
+// TheLLG::rhs_d (TheLLG.h:109; LLGWrapper.cu:213-228) with its CVODE
+// signature int(realtype t, N_Vector u, N_Vector u_dot, void* user_data).
+// NVector is SUNDIALS' N_Vector and DevPtr the accessor of its device array
+// (N_VGetDeviceArrayPointer_Hip for the HIP N_Vector).  Returns 0, or -1 (an
+// unrecoverable error for CVODE) when the evaluation fails; never throws.
+template <class NVector, double* (*DevPtr)(NVector)>
+int rhs_d(double t, NVector u, NVector u_dot, void* user_data) {
+    (void)t;
+    UserData* d = static_cast<UserData*>(user_data);
+    if (!d || !d->llg) return -1;
+    try {
+        d->llg->rhs_d(DevPtr(u), DevPtr(u_dot));
+    } catch (const Error&) {
+        return -1;
+    }
+    return 0;
+}
 
 }  // namespace tmg
 

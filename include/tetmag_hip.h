@@ -116,15 +116,15 @@ int tm_solver_destroy(tm_solver* s);
 int tm_solver_solve(tm_solver* s, const double* b, int bwhere, double* x, int xwhere, double tol, int maxit,
                     int warm, int* iters, double* relres);
 
-/* Preconditioner hierarchy of an AMG solver: *nlevels receives the number of
- * levels including the dense coarsest one, sizes[0..min(cap,nlevels)-1] their
- * unknowns (0 levels for a Jacobi solver). */
 /* Solves A^T x = b for the solver's matrix A (symmetric apart from identity
  * rows): CG on the free block, then the identity rows' values from the
  * coupling.  Warm start: a separate solution vector from tm_solver_solve.
  * The adjoint of GPUSolver::solve (no counterpart in the reference). */
 int tm_solver_solve_transposed(tm_solver* s, const double* b, int bwhere, double* x, int xwhere, double tol,
                                int maxit, int warm, int* iters, double* relres);
+/* Preconditioner hierarchy of an AMG solver: *nlevels receives the number of
+ * levels including the dense coarsest one, sizes[0..min(cap,nlevels)-1] their
+ * unknowns (0 levels for a Jacobi solver). */
 int tm_solver_amg_info(tm_solver* s, int* nlevels, int* sizes, int cap);
 
 /* ---- field context: EffFieldGPU + GpuPotential + DemagField state ------- */
@@ -329,6 +329,37 @@ typedef struct tm_rk4_stats {
 int tm_ctx_run_rk4(tm_ctx* c, double dtau, int nsteps, double alpha, int variant, int use_graph,
                    tm_rk4_stats* stats);
 
+typedef struct tm_ode_stats {
+    int accepted, rejected;  /* steps */
+    int evaluations;         /* right-hand side evaluations (6 per step + 1) */
+    int sync_steps;          /* steps run with host-polled solves */
+    int graph_steps;         /* steps replayed from the captured graph */
+    int redone_steps;        /* replays in which a solve missed its budget, redone */
+    int captures;
+    int budget_neumann, budget_dirichlet;
+    int max_iters_neumann, max_iters_dirichlet;
+    double tau;              /* time reached (tau units, from 0 at the call) */
+    double h_next;           /* proposed next step */
+    double last_err;         /* weighted RMS error estimate of the last step */
+    double ms;               /* device time of the call */
+} tm_ode_stats;
+/* Adaptive integration of the context magnetisation over tau_span (tau
+ * units) with the Dormand-Prince 5(4) pair, the error measured in the
+ * weighted RMS norm of CVODE with weights 1/(rtol |m_i| + atol) -- the
+ * integration TheLLG::gpuODE / integrateSUNDIALS run with CVODE's Adams
+ * method and rtol = atol = 1e-6 (LLGWrapper.cu:139-211, TheLLG.cpp:578-654).
+ * The right-hand side is that of tm_eval_rhs (demag recomputed at every
+ * stage unless frozen), parameters as set at the call.  *h_io: first step on
+ * entry (<= 0: chosen from the largest torque), proposed next step on exit
+ * (pass it to the next window).  At most max_steps steps (accepted plus
+ * rejected); TM_ERR_MAXITER if tau_span is not reached.  use_graph = 1
+ * captures one step in a HIP graph (fixed CG budgets that stop early on the
+ * device; a step whose solve missed its tolerance is redone with host-polled
+ * solves).  normalize = 1 renormalises m at the end, as TheSimulation does
+ * after each integration window (TheSimulation.cpp:293, 404-412). */
+int tm_ctx_integrate(tm_ctx* c, double tau_span, double rtol, double atol, double* h_io, int max_steps, double alpha,
+                     int variant, int use_graph, int normalize, tm_ode_stats* stats);
+
 /* Explicit Euler advance of the context magnetisation with renormalisation:
  * m <- (m + dtau dm/dtau) / |m + dtau dm/dtau|  (the renormalisation of
  * EffFieldGPU::NormalizeMag, EffFieldGPU.cu:625-663, fused with the update). */
@@ -376,7 +407,10 @@ typedef struct tm_batch_stats {
 /* Needs a context with operators, boundary matrix and AMG demag solvers. */
 int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out);
 int tm_batch_destroy(tm_batch* b);
-/* Per-sample uniform applied field (A/m), [3][S]; default = the context's. */
+/* Per-sample uniform applied field (A/m), [3][S]; default = the context's
+ * uniform field.  A per-node field set on the context (tm_ctx_set_zeeman with
+ * H3n, the carrier of the pulse / sweep / local fields of TheLLG.cpp:320-332)
+ * is added to every sample, read at each evaluation. */
 int tm_batch_set_zeeman(tm_batch* b, const double* h3s);
 int tm_batch_set_mag(tm_batch* b, const double* m, int where);
 int tm_batch_get_mag(tm_batch* b, double* m, int where);
@@ -388,19 +422,33 @@ int tm_batch_get_potentials(tm_batch* b, double* u1, double* u2, int where);
 /* m <- normalize(m + dtau dm/dtau) for every sample. */
 int tm_batch_step_euler(tm_batch* b, const double* dmdt, int where, double dtau);
 /* nsteps classical RK4 steps of every sample's magnetisation (renormalised
- * after each step), each stage a full batched evaluation; stats of the last
- * evaluation in `last` (may be null).  The hysteresis driver of the
- * reference steps one field value at a time through CVODE
- * (Hysteresis.cpp:43-169, TheLLG.cpp:578-711). */
-int tm_batch_run_rk4(tm_batch* b, double dtau, int nsteps, double alpha, int variant, tm_batch_stats* last);
+ * after each step), each stage a full batched evaluation.  The hysteresis
+ * driver of the reference steps one field value at a time through CVODE
+ * (Hysteresis.cpp:43-169, TheLLG.cpp:578-711).  use_graph = 1: after a first
+ * host-polled step has set the solver budgets (refinement steps and inner
+ * iterations per step, per solve), one whole step -- four evaluations with
+ * fixed-budget solves whose kernels return at once past convergence -- is
+ * captured in a HIP graph and replayed with one host synchronisation per
+ * step; a step in which a sample missed the tolerance is restored and redone
+ * with host-polled solves, and the budgets grow.  The captured steps run the
+ * boundary product on the fastest of the library's own FP64 GEMM kernels that
+ * agreed with rocBLAS at tm_batch_create (S = 64, FP64 boundary product);
+ * other batches run every step host-polled.
+ * `last` (may be null): statistics of the final evaluation when it was
+ * host-polled (only nsamples is set after a replayed final step); `rk` (may
+ * be null): step counts, budgets (summed inner iterations per solve) and the
+ * device time. */
+int tm_batch_run_rk4(tm_batch* b, double dtau, int nsteps, double alpha, int variant, int use_graph,
+                     tm_batch_stats* last, tm_rk4_stats* rk);
 /* The boundary GEMM chosen by timing every candidate for this shape at
- * tm_batch_create: a rocBLAS solution index (0 = library default) or, when
- * negative, -1 - k for tile shape k of the library's own FP64 MFMA kernel
- * (only chosen when it agrees with rocBLAS to 1e-12), run with `ksplit`
- * partial products over K; the number of candidates timed; the timings of
- * the library default and of the choice. */
-int tm_batch_gemm_info(tm_batch* b, int* solution, int* candidates, double* ms_default, double* ms_tuned,
-                       int* ksplit);
+ * tm_batch_create.  *own_kernel = 0: a rocBLAS GEMM, *solution its solution
+ * index (0 = library default; rocBLAS indices may be negative).
+ * *own_kernel = 1: the library's own FP64 MFMA kernel (only chosen when it
+ * agrees with rocBLAS to 1e-12), *solution its tile shape, run with `ksplit`
+ * partial products over K.  Also the number of candidates timed and the
+ * timings of the library default and of the choice.  Any output may be NULL. */
+int tm_batch_gemm_info(tm_batch* b, int* solution, int* own_kernel, int* candidates, double* ms_default,
+                       double* ms_tuned, int* ksplit);
 /* y[j][s] = sum_k B[j][k] x[k][s] for every sample s: the batch's boundary
  * product (the GEMM tm_batch_eval_rhs runs, in the batch's precision) on
  * caller values; x, y: b x S, boundary-node major, the sample index fastest.

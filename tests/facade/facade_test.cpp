@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
+#include <string>
 #include <vector>
 
 #include "tetmag_gpu.hpp"
@@ -30,14 +32,50 @@ void* ref_solver_create(int n, const int* rp, const int* col, const double* val)
 int ref_solver_solve(void* h, const double* b, double* x, double tol, int maxit, int restart, double* relres);
 void ref_solver_destroy(void* h);
 void ref_spmv(int n, const int* rp, const int* col, const double* val, const double* x, double* y);
+void ref_stt_uterm(int n, const int* rp, const int* col, const double* gX, const double* gY, const double* gZ,
+                   const double* ex, const double* ey, const double* ez, const double* m, double* U);
+void ref_stt_llg(int n, double alpha, double beta, const double* m, const double* U, double* out);
 }
 
+// The aliases a tetmag build would make (with Eigen::MatrixXd, SpMat,
+// Eigen::VectorXd in place of the standalone types).
 using Dense = tmg::DenseMatrix;
 using Sparse = tmg::CsrMatrix;
+using VectorXd = tmg::DenseVector;
+using dev_vec = tmg::DeviceVector;
 using EffFieldGPU = tmg::EffFieldGPU_T<Dense, Sparse>;
-using GpuPotential = tmg::GpuPotential_T<Dense, Sparse>;
-using GPUSolver = tmg::GPUSolver_T<Sparse>;
-using SpMatCUDA = tmg::SpMatCUDA_T<Sparse>;
+using GpuPotential = tmg::GpuPotential_T<Dense, Sparse, VectorXd>;
+using SolverFactory = tmg::SolverFactory_T<Sparse, VectorXd>;
+using SpMatCUDA = tmg::SpMatCUDA_T<Sparse, VectorXd>;
+
+// A host-side SolverFactory of its own (as tetmag keeps SolverFactory.h with
+// its CPU solvers): GPUSolver derives from it, as GPUSolver.h:65 does.
+class HostSolverFactory {
+   protected:
+    double cgTol = 1e-7;
+    Sparse A;
+    VectorXd b, x;
+    std::string preconditionerType;
+
+   public:
+    virtual ~HostSolverFactory() {}
+    virtual void solve() = 0;
+    virtual bool compute() = 0;
+    virtual void report() = 0;
+    virtual bool wasSuccessful() = 0;
+    void setMatrix(const Sparse& M) { A = M; }
+    void setLoadVector(const VectorXd& v) { b = v; }
+    void setCGTolerance(const double t) { cgTol = t; }
+    void definePreconditioner(const std::string p) { preconditionerType = p; }
+    VectorXd result() { return x; }
+};
+class GPUSolver : public tmg::GPUSolver_T<HostSolverFactory, Sparse, VectorXd> {};
+
+// A stand-in for SUNDIALS' N_Vector and its device-array accessor.
+struct FakeNVector {
+    double* dev;
+};
+static double* fake_device_pointer(FakeNVector* v) { return v->dev; }
 
 static int failures = 0;
 
@@ -123,11 +161,22 @@ int main() {
     expect("LLG_noPrec_hst", rel(dm.data(), ref.data(), ref.size()), 1e-14);
     const double mt = gpucalc.MaxTorque(H), mtr = ref_max_torque(n, m.data(), H.data());
     expect("MaxTorque", std::fabs(mt - mtr) / mtr, 1e-14);
-    tmg::DeviceVector dd;
-    gpucalc.ClassicLLG_dev(H, 0.5, dd);
+    // LLGWrapper.cu:61, 81, 87 dereference the returned pointer into a dev_vec
+    dev_vec dd = *gpucalc.ClassicLLG_dev(H, 0.5);
     std::vector<double> dh = dd.host();
     ref_llg_rhs(n, 0.5, 1, m.data(), H.data(), ref.data());
     expect("ClassicLLG_dev", rel(dh.data(), ref.data(), ref.size()), 1e-14);
+    {
+        std::shared_ptr<dev_vec> p1 = gpucalc.ClassicLLG_dev(H, 0.5);
+        std::shared_ptr<dev_vec> p2 = gpucalc.LLG_noPrec_dev(H, 0.5);
+        if (p1.get() != p2.get()) {  // one dxdt buffer per object, as EffFieldGPU.h:70
+            std::printf("the *_dev results do not alias one buffer\n");
+            ++failures;
+        }
+        std::vector<double> h2 = p2->host();
+        ref_llg_rhs(n, 0.5, 0, m.data(), H.data(), ref.data());
+        expect("LLG_noPrec_dev", rel(h2.data(), ref.data(), ref.size()), 1e-14);
+    }
 
     // LLGWrapper::rhs_d: whole right-hand side on the device (no Js set, so
     // invJs = 1 and Heff = Hx + Hk; no demag on this context).
@@ -142,12 +191,22 @@ int main() {
         for (size_t q = 0; q < Hs.size(); ++q) Hs[q] = Hx.v[q] + Hk.v[q];
         ref_llg_rhs(n, 0.5, 1, m.data(), Hs.data(), ref.data());
         expect("LLGWrapper::rhs_d", rel(got.data(), ref.data(), ref.size()), 1e-13);
+        // the CVODE callback of TheLLG::rhs_d (TheLLG.h:109)
+        int (*cb)(double, FakeNVector*, FakeNVector*, void*) = tmg::rhs_d<FakeNVector*, fake_device_pointer>;
+        tmg::UserData ud{&w, n};
+        dev_vec udot2(u.size());
+        FakeNVector nu{u.data()}, nud{udot2.data()};
+        if (cb(0.0, &nu, &nud, &ud) != 0) ++failures;
+        got = udot2.host();
+        expect("rhs_d (CVODE callback)", rel(got.data(), ref.data(), ref.size()), 1e-13);
+        if (cb(0.0, &nu, &nud, nullptr) != -1) ++failures;  // bad user data: error status, no throw
     }
 
     // GpuPotential (DemagField::initializeSolvers, DemagField.cpp:243-250)
     GpuPotential gp;
     gp.setGradientMatrices(sparse(tX, 1), sparse(tY, 1), sparse(tZ, 1), sparse(gX, 1), sparse(gY, 1), sparse(gZ, 1), Js);
-    std::vector<double> b = gp.calcDivM(magM), bref(n);
+    VectorXd b = gp.calcDivM(magM);  // DemagField.cpp:128
+    std::vector<double> bref(n);
     ref_div_m(n, rp.data(), col.data(), tX.data(), tY.data(), tZ.data(), Js.data(), m.data(), bref.data());
     for (auto& v : bref) v *= 1.256637061435917295376e-6;  // the oracle divides by mu0 like rhsPoisson
     expect("GpuPotential::calcDivM", rel(b.data(), bref.data(), b.size()), 1e-13);
@@ -169,32 +228,81 @@ int main() {
     KN.ci.resize(nn);
     KN.val.resize(nn);
     for (auto& v : b) v /= 1.256637061435917295376e-6;
+    std::vector<double> xo(n, 0.0);
+    {
+        void* h = ref_solver_create(n, KN.rp.data(), KN.ci.data(), KN.val.data());
+        double rr = 0;
+        ref_solver_solve(h, b.data(), xo.data(), 1e-14, 20000, 60, &rr);
+        ref_solver_destroy(h);
+    }
     for (const char* pre : {"amg", "jacobi"}) {
-        GPUSolver s;
-        s.setMatrix(KN);
-        s.setCGTolerance(1e-13);
-        s.definePreconditioner(pre);
-        if (!s.compute()) {
+        // DemagField::initializeSolvers (DemagField.cpp:252-266) and solvePoisson (:233-240)
+        std::shared_ptr<SolverFactory> s = SolverFactory::makeSolver("gpu");
+        s->setMatrix(KN);
+        s->setCGTolerance(1e-13);
+        s->definePreconditioner(pre);
+        if (!s->compute()) {
             std::printf("GPUSolver::compute failed\n");
             return 1;
         }
+        s->setLoadVector(b);
+        s->solve();
+        VectorXd u1 = s->result();
+        s->report();
+        expect(pre[0] == 'a' ? "makeSolver(gpu) amg" : "makeSolver(gpu) jacobi", rel(u1.data(), xo.data(), xo.size()),
+               1e-10);
+        if (!s->wasSuccessful()) ++failures;
+    }
+    {
+        GPUSolver s;  // derived from the host's own SolverFactory
+        s.setMatrix(KN);
+        s.setCGTolerance(1e-13);
+        s.definePreconditioner("amg");
+        if (!s.compute()) return 1;
         s.setLoadVector(b);
         s.solve();
-        std::vector<double> x(n, 0.0);
-        void* h = ref_solver_create(n, KN.rp.data(), KN.ci.data(), KN.val.data());
-        double rr = 0;
-        ref_solver_solve(h, b.data(), x.data(), 1e-14, 20000, 60, &rr);
-        ref_solver_destroy(h);
-        std::printf("GPUSolver(%s) iterations %d\n", pre, s.iterations());
-        expect(pre[0] == 'a' ? "GPUSolver(amg)" : "GPUSolver(jacobi)", rel(s.result().data(), x.data(), x.size()),
-               1e-10);
-        if (!s.wasSuccessful()) ++failures;
+        VectorXd u1 = s.result();
+        expect("GPUSolver : host factory", rel(u1.data(), xo.data(), xo.size()), 1e-10);
+        s.solve();  // warm start from the previous solution: already converged
+        if (s.iterations() != 0 || !s.wasSuccessful()) ++failures;
     }
-    // SpMatCUDA
+    // SpMatCUDA (SpMatCUDA.h:65-78)
     SpMatCUDA sm(sparse(K, 1.0));
-    std::vector<double> y = sm.mvp(b), yr(n);
+    std::vector<double> yr(n);
     ref_spmv(n, rp.data(), col.data(), K.data(), b.data(), yr.data());
-    expect("SpMatCUDA::mvp", rel(y.data(), yr.data(), y.size()), 1e-14);
+    sm.setX(b);
+    sm.mvp();
+    VectorXd y = sm.mvpResEig();
+    expect("SpMatCUDA setX/mvp/mvpResEig", rel(y.data(), yr.data(), yr.size()), 1e-14);
+    dev_vec bd(std::vector<double>(b.data(), b.data() + n));
+    dev_vec yd = sm.mvp(bd);
+    std::vector<double> yh = yd.host();
+    expect("SpMatCUDA mvp(devVecD)", rel(yh.data(), yr.data(), yr.size()), 1e-14);
+    sm.setX(std::vector<double>(b.data(), b.data() + n));
+    sm.mvp();
+    yh = sm.mvpResDev().host();
+    expect("SpMatCUDA mvpResDev", rel(yh.data(), yr.data(), yr.size()), 1e-14);
+
+    // Zhang-Li STT (TheLLG.cpp:402-416 -> UTermSTT_GPU, TheLLG.cpp:207 -> STT_term_LLG_hst)
+    {
+        std::vector<double> ex(n), ey(n, 0.0), ez(n);
+        for (int i = 0; i < n; ++i) {
+            ex[i] = 1e-3 * (1.0 + 0.1 * std::sin(0.3 * i));
+            ez[i] = -4e-4;
+        }
+        gpucalc.setSTTDataOnDevice(sparse(gX, 1), sparse(gY, 1), sparse(gZ, 1), ex, ey, ez);
+        Dense U = gpucalc.UTermSTT_GPU();
+        std::vector<double> Ur((size_t)3 * n);
+        ref_stt_uterm(n, rp.data(), col.data(), gX.data(), gY.data(), gZ.data(), ex.data(), ey.data(), ez.data(),
+                      m.data(), Ur.data());
+        expect("UTermSTT_GPU", rel(U.data(), Ur.data(), Ur.size()), 1e-13);
+        std::vector<double> st = gpucalc.STT_term_LLG_hst(U, 0.5, 0.04);
+        ref_stt_llg(n, 0.5, 0.04, m.data(), U.data(), ref.data());
+        expect("STT_term_LLG_hst", rel(st.data(), ref.data(), ref.size()), 1e-14);
+        dev_vec sd = *gpucalc.STT_term_LLG_dev(U, 0.5, 0.04);
+        std::vector<double> sh = sd.host();
+        expect("STT_term_LLG_dev", rel(sh.data(), ref.data(), ref.size()), 1e-14);
+    }
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
     return failures ? 1 : 0;
 }

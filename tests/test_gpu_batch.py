@@ -118,10 +118,10 @@ def test_batch_rk4_matches_single_samples(base):
     bc = BatchContext(ctx, S, N.TM_PREC_FP64)
     bc.set_zeeman(Hs)
     bc.set_mag(to_batch(ms[1:1 + S]))
-    st = bc.run_rk4(dtau, 4)
+    st, rk = bc.run_rk4(dtau, 4)
     mb = bc.get_mag()
     bc.close()
-    assert st["max_relres_neumann"] <= 1e-12
+    assert st["max_relres_neumann"] <= 1e-12 and rk["sync_steps"] == 4
     for s in range(S):
         fc = FieldContext(prob, tol=1e-12)
         fc.set_hext(Hs[:, s])  # the per-sample field replaces the context's uniform one
@@ -129,6 +129,43 @@ def test_batch_rk4_matches_single_samples(base):
         fc.run_rk4(dtau, 4, use_graph=False)
         assert rel_l2(from_batch(mb, s), fc.get_mag()) < 1e-9
         fc.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_rk4_graph_matches_eager(base, prec):
+    """One batched RK4 step captured in a HIP graph (fixed solver budgets,
+    kernels gated past convergence) and replayed: every replay is checked
+    against the eager host-polled steps of an identical batch, with the
+    program's own eager work between the replays."""
+    from tetmag_amd import _native as N, problem as P
+    from tetmag_amd.batch import BatchContext, to_batch
+    prob, ctx, ms = base
+    S = 64
+    pid = N.TM_PREC_FP64 if prec == "fp64" else N.TM_PREC_MIXED
+    fields = P.smooth_random_fields(prob.mesh, S, seed=31, lam_min=10.0, lam_max=60.0)
+    dtau = 20e-3 * prob.time_scale()
+    Hs = np.stack([np.linspace(-3e4, 3e4, S), np.zeros(S), np.full(S, 1e4)])
+    ga, gb = BatchContext(ctx, S, pid), BatchContext(ctx, S, pid)
+    for b in (ga, gb):
+        b.set_zeeman(Hs)
+        b.set_mag(to_batch(fields))
+    info = ga.gemm_info()
+    print(prec, info, flush=True)
+    total = {"graph_steps": 0, "captures": 0, "redone_steps": 0}
+    for rep in range(3):  # replays separated by eager work and synchronisations
+        _, ra = ga.run_rk4(dtau, 3, use_graph=True)
+        _, rb = gb.run_rk4(dtau, 3, use_graph=False)
+        for k in total:
+            total[k] += ra[k]
+        print(rep, ra, flush=True)
+        assert rb["graph_steps"] == 0
+        ctx.synchronize()
+        d = rel_l2(ga.get_mag(), gb.get_mag())
+        assert d < (1e-11 if prec == "fp64" else 1e-9), (rep, d)
+        ga.eval_rhs_host(ga.get_mag())  # eager work between replays
+    assert total["graph_steps"] >= 4 and total["captures"] >= 1
+    ga.close()
+    gb.close()
 
 
 @pytest.mark.parametrize("prec,S", [("fp64", 64), ("mixed", 64), ("fp64", 3), ("bem32", 64)])

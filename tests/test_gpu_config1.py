@@ -1,0 +1,141 @@
+"""Parity at full size: the 282,897-node jittered Kuhn cuboid (128 x 128 x 16
+cells at 3 nm, 40,962 boundary nodes, dense B of 13.42 GB) that the
+benchmark runs.
+
+  (i)   256 rows of the GPU's boundary-element matrix (corner, edge and face
+        nodes across the whole surface, so near and far triangles) against
+        the oracle's rows (BEMprocessing.cpp:375-411), row-relative <= 1e-12;
+        B 1 = -1 row-wise to the precision of the reference's acos form;
+  (ii)  the plain FP64 evaluation (div -> Neumann -> B u1 -> Dirichlet ->
+        fields -> LLG, DemagField.cpp:163-201, TheLLG.cpp:238-254) against the
+        FP64 oracle, which builds its own operators and its own B and solves
+        with SA-AMG GMRES; both sides solved to 1e-13; fields <= 1e-10
+        relative L2; m . dm/dt = 0;
+  (iii) four samples of a 64-sample mixed-precision batch (FP32 inner CG,
+        FP64 refinement to 1e-10) against the oracle: Heff <= 1e-6 relative
+        L2 (the mixed form's bar), demag energy reported.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_TOL = 1e-13
+GPU_TOL = 1e-13
+
+
+def log(msg):
+    print(f"[config1 {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def big(oracle_mod, gpu_available):
+    from tetmag_amd import problem as P
+    from tetmag_amd.context import FieldContext
+    log("building the 128x128x16 cuboid problem")
+    prob = P.cuboid_128x128x16()
+    assert (prob.n, prob.bnd.bnx) == (282897, 40962)
+    orc = oracle_mod.Oracle(prob.mesh.xyz, prob.mesh.fel)
+    log(f"oracle set up (threads {oracle_mod.num_threads()}); GPU context")
+    ctx = FieldContext(prob, tol=GPU_TOL)
+    log("GPU context ready")
+    yield prob, orc, ctx
+    ctx.close()
+
+
+def _sample_rows(prob, seed=0):
+    x = prob.mesh.xyz[prob.bnd.bnodes]
+    lo, hi = x.min(axis=0), x.max(axis=0)
+    nb = ((np.abs(x - lo) < 1e-9) | (np.abs(x - hi) < 1e-9)).sum(axis=1)
+    rng = np.random.default_rng(seed)
+    corners = np.nonzero(nb == 3)[0]
+    edges = rng.choice(np.nonzero(nb == 2)[0], 48, replace=False)
+    faces = rng.choice(np.nonzero(nb == 1)[0], 256 - 48 - corners.size, replace=False)
+    return np.sort(np.concatenate([corners, edges, faces])).astype(np.int32)
+
+
+def test_bem_rows_against_oracle(big):
+    import torch
+    prob, orc, ctx = big
+    rows = _sample_rows(prob)
+    assert rows.size == 256
+    t0 = time.time()
+    Ro = orc.bem_rows(rows)
+    log(f"oracle: 256 rows in {time.time() - t0:.1f}s")
+    T = ctx.bem_matrix_device()  # (b, b) view of a device copy, T[i, j] = B[i, j]
+    Rg = T[torch.as_tensor(rows, dtype=torch.long, device=T.device)].cpu().numpy()
+    del T
+    torch.cuda.empty_cache()
+    err = np.linalg.norm(Rg - Ro, axis=1) / np.linalg.norm(Ro, axis=1)
+    log(f"row-relative L2 error max {err.max():.2e} median {np.median(err):.2e}")
+    assert err.max() <= 1e-12
+    # B 1 = -1 row-wise: within the acos-form precision on the GPU, identically on both sides
+    g = ctx.bem_apply(np.ones(prob.bnd.bnx))
+    dev = np.abs(g + 1.0)
+    log(f"|B 1 + 1| max {dev.max():.2e}")
+    assert dev.max() < 1e-3
+    assert np.max(np.abs(g[rows] - Ro.sum(axis=1))) < 1e-12
+
+
+def test_plain_fp64_against_oracle(big):
+    prob, orc, ctx = big
+    n = prob.n
+    m = prob.m0
+    t0 = time.time()
+    Bo = orc.B  # the oracle's own matrix (reference form), 13.42 GB
+    log(f"oracle B assembled in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ref = orc.eval_rhs(m, Js=prob.Js, A=prob.A, Ku=prob.Ku, axis=prob.kaxis, D=prob.D, Hext=prob.hext,
+                       alpha=prob.alpha, tol=ORACLE_TOL, warm=False)
+    log(f"oracle evaluation {time.time() - t0:.1f}s, GMRES iterations {orc.last_iters}")
+    del Bo
+    dm, H, st = ctx.eval_rhs_host(m)
+    u1, u2 = ctx.potentials()
+    log(f"GPU CG iterations ({st['iters_neumann']}, {st['iters_dirichlet']}) relres "
+        f"({st['relres_neumann']:.1e}, {st['relres_dirichlet']:.1e})")
+    errs = {"Heff": rel_l2(H, ref["Heff"]), "dmdt": rel_l2(dm, ref["dmdt"]), "Hdem": None,
+            "u1": rel_l2(u1, ref["u1"]), "u2": rel_l2(u2, ref["u2"])}
+    Hd, _ = ctx.demag_field(m)
+    errs["Hdem"] = rel_l2(Hd, ref["Hdem"])
+    log("relative L2 errors vs oracle: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert st["relres_neumann"] <= GPU_TOL and st["relres_dirichlet"] <= GPU_TOL
+    for k in ("Heff", "dmdt", "Hdem"):
+        assert errs[k] <= 1e-10, k
+    assert abs(st["max_torque"] - ref["max_torque"]) <= 1e-10 * ref["max_torque"]
+    dots = dm[:n] * m[:n] + dm[n:2 * n] * m[n:2 * n] + dm[2 * n:] * m[2 * n:]
+    assert np.max(np.abs(dots)) <= 1e-12 * np.max(np.abs(dm))
+
+
+def test_mixed_batch_samples_against_oracle(big):
+    from tetmag_amd import _native as N, problem as P
+    from tetmag_amd.batch import BatchContext, to_batch, from_batch
+    prob, orc, ctx = big
+    n = prob.n
+    fields = P.smooth_random_fields(prob.mesh, 64, seed=2026)
+    ctx.set_solver_params(1e-10, 100000)
+    try:
+        bc = BatchContext(ctx, 64, N.TM_PREC_MIXED)
+        dm, H, st = bc.eval_rhs_host(to_batch(fields))
+        u1, u2 = bc.potentials()
+        bc.close()
+    finally:
+        ctx.set_solver_params(GPU_TOL, 100000)
+    log(f"mixed batch: outer ({st['outer_neumann']}, {st['outer_dirichlet']}) inner "
+        f"({st['inner_neumann']}, {st['inner_dirichlet']})")
+    V = orc.nodevol
+    for s in (0, 21, 42, 63):
+        ref = orc.eval_rhs(fields[s], Js=prob.Js, A=prob.A, Ku=prob.Ku, axis=prob.kaxis, D=prob.D,
+                           Hext=prob.hext, alpha=prob.alpha, tol=ORACLE_TOL, warm=False)
+        eH = rel_l2(from_batch(H, s), ref["Heff"])
+        eD = rel_l2(from_batch(dm, s), ref["dmdt"])
+        # demag energy density -1/2 sum Js V Hdem . m / sum V (DemagField.cpp:97-99) from the batch potentials
+        Hd = orc.grad_field(u1[:, s] + u2[:, s])
+        e_b = orc.energies(fields[s], {"Hexc": ref["Hexc"], "Hdem": Hd}, Js=prob.Js)["demag"] / V.sum()
+        e_o = orc.energies(fields[s], {"Hexc": ref["Hexc"], "Hdem": ref["Hdem"]}, Js=prob.Js)["demag"] / V.sum()
+        log(f"sample {s}: Heff {eH:.2e} dm/dt {eD:.2e} demag energy rel {abs(e_b - e_o) / abs(e_o):.2e}")
+        assert eH <= 1e-6 and eD <= 1e-6
+        assert abs(e_b - e_o) <= 1e-6 * abs(e_o)

@@ -108,6 +108,17 @@ class RK4Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class OdeStats(C.Structure):
+    _fields_ = [("accepted", C.c_int), ("rejected", C.c_int), ("evaluations", C.c_int), ("sync_steps", C.c_int),
+                ("graph_steps", C.c_int), ("redone_steps", C.c_int), ("captures", C.c_int),
+                ("budget_neumann", C.c_int), ("budget_dirichlet", C.c_int), ("max_iters_neumann", C.c_int),
+                ("max_iters_dirichlet", C.c_int), ("tau", C.c_double), ("h_next", C.c_double),
+                ("last_err", C.c_double), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("nsamples", C.c_int),
@@ -201,6 +212,7 @@ _SIGNATURES = {
     "tm_bem_apply_transposed": [_P, _P, _I, _P, _I],
     "tm_solver_solve_transposed": [_P, _P, _I, _P, _I, _D, _I, _I, _P, _P],
     "tm_ctx_run_rk4": [_P, _D, _I, _D, _I, _I, _P],
+    "tm_ctx_integrate": [_P, _D, _D, _D, _P, _I, _D, _I, _I, _I, _P],
     "tm_batch_create": [_P, _I, _I, _PP],
     "tm_batch_destroy": [_P],
     "tm_batch_set_zeeman": [_P, _P],
@@ -209,8 +221,8 @@ _SIGNATURES = {
     "tm_batch_eval_rhs": [_P, _P, _I, _D, _I, _P, _I, _P, _I, _P],
     "tm_batch_get_potentials": [_P, _P, _P, _I],
     "tm_batch_step_euler": [_P, _P, _I, _D],
-    "tm_batch_run_rk4": [_P, _D, _I, _D, _I, _P],
-    "tm_batch_gemm_info": [_P, _P, _P, _P, _P, _P],
+    "tm_batch_run_rk4": [_P, _D, _I, _D, _I, _I, _P, _P],
+    "tm_batch_gemm_info": [_P, _P, _P, _P, _P, _P, _P],
     "tm_batch_bem_apply": [_P, _P, _I, _P, _I],
     "tm_batch_enable_timers": [_P, _I],
     "tm_batch_get_timers": [_P, _P, _I],

@@ -78,21 +78,27 @@ class BatchContext:
         d = self.dmdt if dmdt is None else dmdt
         N.check(N.lib().tm_batch_step_euler(self.h, N.ptr(d), N.TM_DEVICE, float(dtau)), "tm_batch_step_euler")
 
-    def run_rk4(self, dtau: float, nsteps: int, alpha=None, variant=N.TM_LLG_CLASSIC) -> dict:
-        """nsteps RK4 steps of every sample on the device (tm_batch_run_rk4)."""
+    def run_rk4(self, dtau: float, nsteps: int, alpha=None, variant=N.TM_LLG_CLASSIC, use_graph: bool = False):
+        """nsteps RK4 steps of every sample on the device (tm_batch_run_rk4);
+        use_graph replays one captured step with fixed solver budgets.
+        Returns (statistics of the last evaluation, step statistics)."""
         alpha = self.ctx.prob.alpha if alpha is None else alpha
         st = N.BatchStats()
-        N.check(N.lib().tm_batch_run_rk4(self.h, float(dtau), int(nsteps), float(alpha), int(variant), C.byref(st)),
-                "tm_batch_run_rk4")
-        return st.as_dict()
+        rk = N.RK4Stats()
+        N.check(N.lib().tm_batch_run_rk4(self.h, float(dtau), int(nsteps), float(alpha), int(variant),
+                                         int(use_graph), C.byref(st), C.byref(rk)), "tm_batch_run_rk4")
+        return st.as_dict(), rk.as_dict()
 
     def gemm_info(self) -> dict:
-        sol, cand, ks = C.c_int(), C.c_int(), C.c_int()
+        """The tuned boundary GEMM: own_kernel True means the library's FP64
+        MFMA kernel with tile shape `solution`, else rocBLAS solution index
+        `solution` (0 = library default; rocBLAS indices may be negative)."""
+        sol, own, cand, ks = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         d, b = C.c_double(), C.c_double()
-        N.check(N.lib().tm_batch_gemm_info(self.h, C.byref(sol), C.byref(cand), C.byref(d), C.byref(b),
-                                           C.byref(ks)), "tm_batch_gemm_info")
-        return {"solution": sol.value, "candidates": cand.value, "ms_default": d.value, "ms_tuned": b.value,
-                "ksplit": ks.value}
+        N.check(N.lib().tm_batch_gemm_info(self.h, C.byref(sol), C.byref(own), C.byref(cand), C.byref(d),
+                                           C.byref(b), C.byref(ks)), "tm_batch_gemm_info")
+        return {"solution": sol.value, "own_kernel": bool(own.value), "candidates": cand.value,
+                "ms_default": d.value, "ms_tuned": b.value, "ksplit": ks.value}
 
     def bem_apply(self, x):
         """Boundary product of every sample, y[j][s] = sum_k B[j][k] x[k][s]

@@ -206,6 +206,21 @@ class FieldContext:
                                       C.byref(st)), "tm_ctx_run_rk4")
         return st.as_dict()
 
+    def integrate(self, tau_span: float, rtol: float = 1e-6, atol: float = 1e-6, h0: float = 0.0,
+                  max_steps: int = 1000000, use_graph: bool = True, normalize: bool = True, alpha=None,
+                  variant=N.TM_LLG_CLASSIC) -> dict:
+        """Adaptive Dormand-Prince 5(4) integration of the context magnetisation
+        over tau_span (tm_ctx_integrate; the reference runs CVODE with
+        rtol = atol = 1e-6).  h0 <= 0 picks the first step; the returned
+        statistics carry h_next for the next window."""
+        alpha = self.prob.alpha if alpha is None else alpha
+        st = N.OdeStats()
+        h = C.c_double(h0)
+        N.check(self.L.tm_ctx_integrate(self.h, float(tau_span), float(rtol), float(atol), C.byref(h), int(max_steps),
+                                        float(alpha), int(variant), int(use_graph), int(normalize), C.byref(st)),
+                "tm_ctx_integrate")
+        return st.as_dict()
+
     def enable_timers(self, on: bool = True):
         N.check(self.L.tm_ctx_enable_timers(self.h, int(on)), "tm_ctx_enable_timers")
 
@@ -243,10 +258,13 @@ class FieldContext:
         return st.as_dict()
 
     # -- material / field parameters (host arrays) --------------------------------
-    def set_hext(self, h3):
-        """Uniform applied field (A/m)."""
+    def set_hext(self, h3, field3n=None):
+        """Applied field (A/m): a uniform (3,) vector plus, optionally, a
+        per-node (3n,) field added to it (TheLLG.cpp:320-332 carries the
+        pulse, sweep and local fields this way)."""
         hx, hy, hz = (float(v) for v in h3)
-        N.check(self.L.tm_ctx_set_zeeman(self.h, None, hx, hy, hz), "tm_ctx_set_zeeman")
+        fp = N.ptr(N.f64(field3n)) if field3n is not None else None
+        N.check(self.L.tm_ctx_set_zeeman(self.h, fp, hx, hy, hz), "tm_ctx_set_zeeman")
 
     def set_ku(self, Ku):
         """Per-node uniaxial constant (J/m^3) along the problem's axes."""

@@ -881,6 +881,7 @@ struct BEvalArgs {
     const double* invjs; const double* ku; const double* kaxis;
     const double* dmi_fac; const double* nvs;
     const double* hext;   // [3][S] per-sample uniform applied field
+    const double* hfield; // 3n per-node applied field shared by every sample (or null)
     double alpha;
     int variant;
     double* dmdt; double* heff;
@@ -964,6 +965,11 @@ __global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
         hx = hx * ij + a.hext[lane] - dx;
         hy = hy * ij + a.hext[S + lane] - dy;
         hz = hz * ij + a.hext[2 * S + lane] - dz;
+        if (a.hfield) {
+            hx += a.hfield[row];
+            hy += a.hfield[n + row];
+            hz += a.hfield[2 * n + row];
+        }
         if (a.heff) {
             a.heff[e] = hx;
             a.heff[NS + e] = hy;
@@ -1216,6 +1222,20 @@ __global__ __launch_bounds__(kBlock) void k_b_euler(size_t NS, double* __restric
 
 __global__ void k_b_zero_small(int cnt, double* p) {
     if ((int)threadIdx.x < cnt) p[threadIdx.x] = 0.0;
+}
+
+// *fail += number of samples whose outer residual missed the tolerance
+__global__ void k_b_count_undone(int S, const OState* os, unsigned* fail) {
+    const int s = threadIdx.x;
+    if (s < S && !os->done[s]) atomicAdd(fail, 1u);
+}
+
+__global__ void k_b_reset_fail(unsigned* fail) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) fail[0] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_copy(size_t tot, const double* __restrict__ a, double* __restrict__ b) {
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) b[e] = a[e];
 }
 
 // RK4 helpers over [3][n][S]: out = base + c k;  m = normalise(base + dt/6 (k1 + 2k2 + 2k3 + k4))
@@ -1742,10 +1762,64 @@ struct BatchSolver {
         }
         if (outer_out) *outer_out = outer;
         if (inner_out) *inner_out = inner_total;
+        last_outer = outer;
         TM_HIP(hipGetLastError());
         bool all = true;
         for (int k = 0; k < S; ++k) all = all && h_ost->done[k];
         return all ? TM_OK : TM_ERR_MAXITER;
+    }
+
+    int last_outer = 0;  // refinement steps of the last host-polled solve
+
+    // The same solve with no host synchronisation (capturable in a HIP graph):
+    // exactly `nouter` refinement steps of ninner[o] inner iterations each.
+    // Every solver kernel is gated on the per-sample flags, so refinement
+    // steps and iterations past convergence cost only their launches, and
+    // iterations a sample did not take carry alpha = beta = 0 into the
+    // combination.  A final residual check adds the number of samples that
+    // missed the tolerance to *fail on the device.
+    int enqueue(const double* b, double* x, double tol, int nouter, const int* ninner, hipStream_t s,
+                unsigned* fail) {
+        if (cg->precond != TM_PRECOND_AMG || cg->amg.levels.empty()) return TM_ERR_UNSUPPORTED;
+        if (nouter < 0 || nouter > kMaxOuter || !fail) return TM_ERR_BAD_ARG;
+        const size_t NS = (size_t)n * S;
+        double* pbb = part.p;
+        double* prr = part.p + (size_t)S * G;
+        double* prz = part.p + (size_t)2 * S * G;
+        double* pzq = part.p + (size_t)3 * S * G;
+        const double itol = sizeof(T) == 4 ? 1e-5 : 0.0;
+        hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
+        for (int o = 0;; ++o) {
+            hipLaunchKernelGGL((k_b_resid<T>), dim3(G), dim3(kBlock), 0, s, sl0.args(8, wA64.p), S, b, (const double*)x,
+                               rT.p, pbb, prr);
+            hipLaunchKernelGGL(k_b_outer_red, dim3(S), dim3(kBlock), 0, s, G, S, pbb, prr, tol, itol, ost.p, bst.p);
+            if (o == nouter) break;
+            TM_CHECK(precond(rT.p, ring.p, prz, s));
+            hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, -1, bst.p, bhist.p);
+            const int K = std::max(1, std::min(kRing, ninner[o]));
+            for (int k = 0; k < K; ++k) {
+                T* zk = ring.p + (size_t)k * NS;
+                cg_spmv(zk, k, pzq, s);
+                hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, pzq, bst.p,
+                                   ahist.p + (size_t)k * kMaxS);
+                hipLaunchKernelGGL((k_b_cg_update<T>), dim3(G), dim3(kBlock), 0, s, n, S, (const T*)q.p, rT.p,
+                                   (const BState*)bst.p, prr);
+                TM_CHECK(precond(rT.p, zk + NS, prz, s));
+                hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, k, bst.p,
+                                   bhist.p + (size_t)(k + 1) * kMaxS);
+            }
+            if (S == kMaxS && K <= kCombineK) {
+                hipLaunchKernelGGL(k_b_coef, dim3(1), dim3(kMaxS), 0, s, S, K, (const double*)ahist.p,
+                                   (const double*)bhist.p, chist.p);
+                hipLaunchKernelGGL((k_b_combine4<T>), dim3(elem_grid(NS / 4)), dim3(kBlock), 0, s, NS, K,
+                                   (const T*)ring.p, (const double*)chist.p, x);
+            } else
+                hipLaunchKernelGGL((k_b_combine<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S, K,
+                                   (const T*)ring.p, (const double*)ahist.p, (const double*)bhist.p, x);
+        }
+        hipLaunchKernelGGL(k_b_count_undone, dim3(1), dim3(kMaxS), 0, s, S, (const OState*)ost.p, fail);
+        TM_HIP(hipGetLastError());
+        return TM_OK;
     }
 };
 
@@ -1772,10 +1846,21 @@ struct tm_batch {
     DevBuf<float> Pf;                  // its split-K partial products (nchunk x b x S)
     int kchunk = 512, nchunk = 0;
     DevBuf<double> rk_k[4], rk_base;   // RK4 stages and the step's start state
+    // graph-replayed RK4 steps: samples whose fixed-budget solve missed the
+    // tolerance (device counter), and the budgets learnt from host-polled
+    // solves: refinement steps and inner iterations per step of the
+    // Neumann (0) and Dirichlet (1) solves
+    DevBuf<unsigned> rk_fail;
+    bool bud_set = false;
+    int bud_outer[2] = {0, 0};
+    int bud_inner[2][8] = {};
     int gemm_sol = 0;                  // tuned rocBLAS solution index (0: library default)
     bool gemm_own = false;             // the MFMA kernel k_b_gemm64 won the tuning
     int gemm_shape = 0, gemm_ks = 1;   // its tile shape and K split
     int gemm_own_candidates = 0;       // (shape, K split) pairs timed
+    bool own_ok = false;               // an own-kernel candidate agreed with rocBLAS (used in captured graphs)
+    float own_ms = 0.0f;
+    int own_shape = 0, own_ks = 1;
     DevBuf<double> Pd;                 // its split-K partials (ks x b x S)
     double gemm_own_err = 0.0;         // its largest relative difference to rocBLAS at tuning
     int gemm_candidates = 0;
@@ -1854,7 +1939,7 @@ int gemm_run(tm_batch* bt, int sol) {
     void* G = f32 ? (void*)bt->Gf.p : (void*)bt->Gm.p;
     const rocblas_status r =
         rocblas_gemm_ex(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, al, A, t, v.bld, U, t,
-                        S, be, G, t, b, G, t, b, t, sol > 0 ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard,
+                        S, be, G, t, b, G, t, b, t, sol != 0 ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard,
                         sol, 0);
     return r == rocblas_status_success ? TM_OK : TM_ERR_HIP;
 }
@@ -1910,6 +1995,11 @@ int tune_gemm(tm_batch* bt) {
     void* G = f32 ? (void*)bt->Gf.p : (void*)bt->Gm.p;
     TM_HIP(hipMemsetAsync(U, 0, (size_t)b * S * (f32 ? sizeof(float) : sizeof(double)), bt->stream));
     if (f32) return TM_OK;  // split-K FP32 product: library defaults
+    // every candidate (rocBLAS and own) is timed on the same non-zero operand:
+    // matrix cores multiplying zeros draw less power and clock higher, which
+    // would favour whichever candidates were timed on a zero U
+    hipLaunchKernelGGL(k_b_fill_pattern, dim3(elem_grid((size_t)b * S)), dim3(kBlock), 0, bt->stream, (size_t)b * S,
+                       bt->U.p);
     rocblas_int nsol = 0;
     if (rocblas_gemm_ex_get_solutions(bt->blas, rocblas_operation_none, rocblas_operation_transpose, b, S, b, al, A,
                                       t, v.bld, U, t, S, be, G, t, b, G, t, b, t, rocblas_gemm_algo_solution_index, 0,
@@ -2001,6 +2091,12 @@ int tune_gemm(tm_batch* bt) {
                         best_shape = bt->gemm_shape;
                         best_ks = bt->gemm_ks;
                     }
+                    if (!bt->own_ok || ms < bt->own_ms) {  // the fastest verified own kernel, for graphs
+                        bt->own_ok = true;
+                        bt->own_ms = ms;
+                        bt->own_shape = bt->gemm_shape;
+                        bt->own_ks = bt->gemm_ks;
+                    }
                 }
             }
             bt->gemm_own_candidates = (int)cand.size();
@@ -2051,6 +2147,73 @@ int boundary_product(tm_batch* bt, const int* bn, const double* u, double* out) 
     }
     TM_HIP(hipGetLastError());
     return TM_OK;
+}
+
+// The fused field + LLG kernel of every (node, sample) on the batch
+// magnetisation and the potentials in usum (TheLLG.cpp:238-254,
+// EffFieldGPU.cu:316-362); with torque, the per-sample maximum torque too.
+int launch_fields(tm_batch* bt, double alpha, int variant, double* dmdt, double* heff, bool torque) {
+    const CtxView& v = bt->cv;
+    const int n = bt->n, S = bt->S;
+    hipStream_t s = bt->stream;
+    BEvalArgs a = {};
+    a.P = bt->eo.args(8, bt->exch ? bt->xw.p : nullptr);
+    a.gx = bt->gx.p;
+    a.gy = bt->gy.p;
+    a.gz = bt->gz.p;
+    if (bt->dmi) {
+        a.cx = bt->cx.p;
+        a.cy = bt->cy.p;
+        a.cz = bt->cz.p;
+        a.dmi_fac = v.dmi_fac;
+        a.nvs = v.nvs;
+    }
+    a.S = S;
+    a.n = n;
+    a.m = bt->m.p;
+    a.u = bt->usum.p;
+    a.invjs = v.invjs;
+    a.ku = v.has_ku ? v.ku : nullptr;
+    a.kaxis = v.has_ku ? v.kaxis : nullptr;
+    a.hext = bt->hext.p;
+    a.hfield = ctx_hext_field(bt->ctx);  // read per call: tm_ctx_set_zeeman may change it after creation
+    a.alpha = alpha;
+    a.variant = variant;
+    a.dmdt = dmdt;
+    a.heff = heff;
+    a.torque = torque ? bt->torque.p : nullptr;
+    if (torque) hipLaunchKernelGGL(k_b_zero_small, dim3(1), dim3(kMaxS), 0, s, S, bt->torque.p);
+    if (bt->dmi) hipLaunchKernelGGL(k_b_eval<true>, dim3(row_grid(n)), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_b_eval<false>, dim3(row_grid(n)), dim3(kBlock), 0, s, a);
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+// One evaluation of the batch magnetisation into dmdt with fixed-budget
+// solves (bt->bud_*): no host synchronisation, so it can be captured in a
+// HIP graph.  Samples whose solve missed the tolerance are counted into
+// bt->rk_fail on the device.
+int eval_enqueue(tm_batch* bt, double alpha, int variant, double* dmdt) {
+    const int n = bt->n, S = bt->S;
+    const size_t NS = (size_t)n * S;
+    hipStream_t s = bt->stream;
+    const CtxView& v = bt->cv;
+    hipLaunchKernelGGL(k_b_div, dim3(row_grid(n)), dim3(kBlock), 0, s, bt->eo.args(8, bt->tx.p),
+                       (const double*)bt->ty.p, (const double*)bt->tz.p, (const int*)bt->eci.p, S, v.js,
+                       (const double*)bt->m.p, 1.0 / kMu0, bt->rhsN.p);
+    TM_HIP(hipGetLastError());
+    if (bt->prec == TM_PREC_FP64)
+        TM_CHECK(bt->sN64.enqueue(bt->rhsN.p, bt->u1.p, v.tol, bt->bud_outer[0], bt->bud_inner[0], s, bt->rk_fail.p));
+    else
+        TM_CHECK(bt->sN32.enqueue(bt->rhsN.p, bt->u1.p, v.tol, bt->bud_outer[0], bt->bud_inner[0], s, bt->rk_fail.p));
+    TM_CHECK(boundary_product(bt, v.bnodes, bt->u1.p, bt->rhsD.p));
+    if (bt->prec == TM_PREC_FP64)
+        TM_CHECK(bt->sD64.enqueue(bt->rhsD.p, bt->u2.p, v.tol, bt->bud_outer[1], bt->bud_inner[1], s, bt->rk_fail.p));
+    else
+        TM_CHECK(bt->sD32.enqueue(bt->rhsD.p, bt->u2.p, v.tol, bt->bud_outer[1], bt->bud_inner[1], s, bt->rk_fail.p));
+    hipLaunchKernelGGL(k_b_sum2, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const double*)bt->u1.p,
+                       (const double*)bt->u2.p, bt->usum.p);
+    return launch_fields(bt, alpha, variant, dmdt, nullptr, false);
 }
 
 }  // namespace
@@ -2253,35 +2416,9 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     hipLaunchKernelGGL(k_b_sum2, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const double*)bt->u1.p,
                        (const double*)bt->u2.p, bt->usum.p);
     // fused fields + LLG
-    BEvalArgs a = {};
-    a.P = bt->eo.args(8, bt->exch ? bt->xw.p : nullptr);
-    a.gx = bt->gx.p;
-    a.gy = bt->gy.p;
-    a.gz = bt->gz.p;
-    if (bt->dmi) {
-        a.cx = bt->cx.p;
-        a.cy = bt->cy.p;
-        a.cz = bt->cz.p;
-        a.dmi_fac = v.dmi_fac;
-        a.nvs = v.nvs;
-    }
-    a.S = S;
-    a.n = n;
-    a.m = bt->m.p;
-    a.u = bt->usum.p;
-    a.invjs = v.invjs;
-    a.ku = v.has_ku ? v.ku : nullptr;
-    a.kaxis = v.has_ku ? v.kaxis : nullptr;
-    a.hext = bt->hext.p;
-    a.alpha = alpha;
-    a.variant = variant;
-    a.dmdt = dwhere == TM_DEVICE ? dmdt : bt->dmdt.p;
-    a.heff = heff ? (hwhere == TM_DEVICE ? heff : bt->heff.p) : nullptr;
-    a.torque = stats ? bt->torque.p : nullptr;
-    if (stats) hipLaunchKernelGGL(k_b_zero_small, dim3(1), dim3(kMaxS), 0, s, S, bt->torque.p);
-    if (bt->dmi) hipLaunchKernelGGL(k_b_eval<true>, dim3(row_grid(n)), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_b_eval<false>, dim3(row_grid(n)), dim3(kBlock), 0, s, a);
-    TM_HIP(hipGetLastError());
+    double* dout = dwhere == TM_DEVICE ? dmdt : bt->dmdt.p;
+    double* hout = heff ? (hwhere == TM_DEVICE ? heff : bt->heff.p) : nullptr;
+    TM_CHECK(launch_fields(bt, alpha, variant, dout, hout, stats != nullptr));
     if (bt->timers) {
         TM_HIP(hipEventRecord(bt->ev[5], s));
         TM_HIP(hipEventSynchronize(bt->ev[5]));
@@ -2296,8 +2433,8 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
         bt->tsum.iters_dirichlet += iD;
         bt->tsum.count += 1;
     }
-    if (dwhere == TM_HOST) TM_CHECK(copy_out(dmdt, TM_HOST, a.dmdt, 3 * NS, s));
-    if (heff && hwhere == TM_HOST) TM_CHECK(copy_out(heff, TM_HOST, a.heff, 3 * NS, s));
+    if (dwhere == TM_HOST) TM_CHECK(copy_out(dmdt, TM_HOST, dout, 3 * NS, s));
+    if (heff && hwhere == TM_HOST) TM_CHECK(copy_out(heff, TM_HOST, hout, 3 * NS, s));
     if (stats) {
         stats->outer_neumann = oN;
         stats->outer_dirichlet = oD;
@@ -2312,41 +2449,158 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     return stN != TM_OK ? stN : stD;
 }
 
-int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int variant, tm_batch_stats* last) {
-    if (!bt || nsteps < 0 || !(dtau > 0.0)) return TM_ERR_BAD_ARG;
+int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int variant, int use_graph,
+                     tm_batch_stats* last, tm_rk4_stats* rk) {
+    if (!bt || nsteps < 0 || !(dtau > 0.0) || (variant != TM_LLG_CLASSIC && variant != TM_LLG_NOPREC))
+        return TM_ERR_BAD_ARG;
     const size_t NS = (size_t)bt->n * bt->S, N3 = 3 * NS;
     for (auto& k : bt->rk_k) TM_CHECK(k.alloc(N3));
     TM_CHECK(bt->rk_base.alloc(N3));
+    TM_CHECK(bt->rk_fail.alloc(1));
     hipStream_t s = bt->stream;
     const dim3 g(elem_grid(N3));
     const double cst[4] = {0.0, 0.5 * dtau, 0.5 * dtau, dtau};
-    int status = TM_OK;
-    for (int step = 0; step < nsteps; ++step) {
-        TM_HIP(hipMemcpyAsync(bt->rk_base.p, bt->m.p, N3 * sizeof(double), hipMemcpyDeviceToDevice, s));
-        for (int st = 0; st < 4; ++st) {
-            // the batch magnetisation holds the stage state
-            if (st > 0)
-                hipLaunchKernelGGL(k_b_axpy, g, dim3(kBlock), 0, s, N3, (const double*)bt->rk_base.p,
-                                   (const double*)bt->rk_k[st - 1].p, cst[st], bt->m.p);
-            const bool fin = step == nsteps - 1 && st == 3;
-            const int r = tm_batch_eval_rhs(bt, nullptr, TM_DEVICE, alpha, variant, bt->rk_k[st].p, TM_DEVICE,
-                                            nullptr, TM_DEVICE, fin ? last : nullptr);
-            if (r != TM_OK && r != TM_ERR_MAXITER) return r;
-            if (r == TM_ERR_MAXITER) status = r;
+    // graphs need kernel-only boundary products (the own MFMA GEMM); the
+    // rocBLAS paths run eagerly
+    const bool graph_ok = use_graph && bt->prec != TM_PREC_MIXED_BEM32 && bt->own_ok;
+    tm_rk4_stats R = {};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TM_HIP(hipEventCreate(&e0));
+    TM_HIP(hipEventCreate(&e1));
+    TM_HIP(hipEventRecord(e0, s));
+    auto learn = [bt, &R]() {  // budgets from the last host-polled solves
+        const int on = bt->prec == TM_PREC_FP64 ? bt->sN64.last_outer : bt->sN32.last_outer;
+        const int od = bt->prec == TM_PREC_FP64 ? bt->sD64.last_outer : bt->sD32.last_outer;
+        const int* inN = bt->prec == TM_PREC_FP64 ? bt->sN64.last_inner : bt->sN32.last_inner;
+        const int* inD = bt->prec == TM_PREC_FP64 ? bt->sD64.last_inner : bt->sD32.last_inner;
+        const int o[2] = {on, od};
+        const int* in[2] = {inN, inD};
+        for (int w = 0; w < 2; ++w) {
+            bt->bud_outer[w] = std::min(8, std::max(bt->bud_outer[w], o[w]));
+            for (int k = 0; k < o[w] && k < 8; ++k) bt->bud_inner[w][k] = std::max(bt->bud_inner[w][k], in[w][k] + 2);
+            int tot = 0;
+            for (int k = 0; k < bt->bud_outer[w]; ++k) tot += bt->bud_inner[w][k];
+            if (w == 0) R.budget_neumann = tot;
+            else R.budget_dirichlet = tot;
         }
-        hipLaunchKernelGGL(k_b_rk_final, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS,
-                           (const double*)bt->rk_base.p, (const double*)bt->rk_k[0].p, (const double*)bt->rk_k[1].p,
-                           (const double*)bt->rk_k[2].p, (const double*)bt->rk_k[3].p, dtau, bt->m.p);
+        bt->bud_set = true;
+    };
+    auto polled_step = [&](bool final_step) -> int {
+        int st = TM_OK;
+        hipLaunchKernelGGL(k_b_copy, g, dim3(kBlock), 0, s, N3, (const double*)bt->m.p, bt->rk_base.p);
+        for (int k = 0; k < 4; ++k) {
+            if (k > 0)
+                hipLaunchKernelGGL(k_b_axpy, g, dim3(kBlock), 0, s, N3, (const double*)bt->rk_base.p,
+                                   (const double*)bt->rk_k[k - 1].p, cst[k], bt->m.p);
+            tm_batch_stats es = {};
+            const bool fin = final_step && k == 3;
+            const int r = tm_batch_eval_rhs(bt, nullptr, TM_DEVICE, alpha, variant, bt->rk_k[k].p, TM_DEVICE, nullptr,
+                                            TM_DEVICE, &es);
+            if (r != TM_OK && r != TM_ERR_MAXITER) return r;
+            if (r == TM_ERR_MAXITER) st = r;
+            R.max_iters_neumann = std::max(R.max_iters_neumann, es.inner_neumann);
+            R.max_iters_dirichlet = std::max(R.max_iters_dirichlet, es.inner_dirichlet);
+            if (fin && last) *last = es;
+            learn();
+        }
+        hipLaunchKernelGGL(k_b_rk_final, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const double*)bt->rk_base.p,
+                           (const double*)bt->rk_k[0].p, (const double*)bt->rk_k[1].p, (const double*)bt->rk_k[2].p,
+                           (const double*)bt->rk_k[3].p, dtau, bt->m.p);
         TM_HIP(hipGetLastError());
+        R.sync_steps += 1;
+        return st;
+    };
+    int status = TM_OK;
+    hipGraphExec_t exec = nullptr;
+    if (last) *last = tm_batch_stats{};
+    for (int step = 0; step < nsteps; ++step) {
+        const bool fin = step == nsteps - 1;
+        if (graph_ok && bt->bud_set) {
+            if (!exec) {
+                hipGraph_t gr = nullptr;
+                // the captured boundary products use the fastest verified own kernel
+                const bool k_own = bt->gemm_own;
+                const int k_shape = bt->gemm_shape, k_ks = bt->gemm_ks;
+                bt->gemm_own = true;
+                bt->gemm_shape = bt->own_shape;
+                bt->gemm_ks = bt->own_ks;
+                TM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int r = TM_OK;
+                hipLaunchKernelGGL(k_b_reset_fail, dim3(1), dim3(64), 0, s, bt->rk_fail.p);
+                hipLaunchKernelGGL(k_b_copy, g, dim3(kBlock), 0, s, N3, (const double*)bt->m.p, bt->rk_base.p);
+                for (int k = 0; k < 4 && r == TM_OK; ++k) {
+                    if (k > 0)
+                        hipLaunchKernelGGL(k_b_axpy, g, dim3(kBlock), 0, s, N3, (const double*)bt->rk_base.p,
+                                           (const double*)bt->rk_k[k - 1].p, cst[k], bt->m.p);
+                    r = eval_enqueue(bt, alpha, variant, bt->rk_k[k].p);
+                }
+                hipLaunchKernelGGL(k_b_rk_final, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS,
+                                   (const double*)bt->rk_base.p, (const double*)bt->rk_k[0].p,
+                                   (const double*)bt->rk_k[1].p, (const double*)bt->rk_k[2].p,
+                                   (const double*)bt->rk_k[3].p, dtau, bt->m.p);
+                const hipError_t ec = hipStreamEndCapture(s, &gr);
+                bt->gemm_own = k_own;
+                bt->gemm_shape = k_shape;
+                bt->gemm_ks = k_ks;
+                if (r != TM_OK || ec != hipSuccess) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    status = r != TM_OK ? r : TM_ERR_HIP;
+                    break;
+                }
+                const hipError_t ei = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (ei != hipSuccess) {
+                    exec = nullptr;
+                    status = TM_ERR_HIP;
+                    break;
+                }
+                R.captures += 1;
+            }
+            unsigned fail = 0;
+            if (hipGraphLaunch(exec, s) != hipSuccess ||
+                hipMemcpyAsync(&fail, bt->rk_fail.p, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                status = TM_ERR_HIP;
+                break;
+            }
+            if (fail == 0) {
+                R.graph_steps += 1;
+                if (fin && last) last->nsamples = bt->S;
+                continue;
+            }
+            // a sample's solve missed its tolerance within the budgets: restore
+            // the step's start state and redo it with host-polled solves
+            hipLaunchKernelGGL(k_b_copy, g, dim3(kBlock), 0, s, N3, (const double*)bt->rk_base.p, bt->m.p);
+            R.redone_steps += 1;
+            for (int w = 0; w < 2; ++w)
+                for (int k = 0; k < bt->bud_outer[w]; ++k) bt->bud_inner[w][k] += 2;
+            (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+        const int r = polled_step(fin);
+        if (r != TM_OK && r != TM_ERR_MAXITER) {
+            status = r;
+            break;
+        }
+        if (r == TM_ERR_MAXITER) status = r;
     }
-    TM_HIP(hipStreamSynchronize(s));
+    if (exec) (void)hipGraphExecDestroy(exec);
+    TM_HIP(hipEventRecord(e1, s));
+    TM_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    R.ms = ms;
+    if (rk) *rk = R;
     return status;
 }
 
-int tm_batch_gemm_info(tm_batch* bt, int* solution, int* candidates, double* ms_default, double* ms_tuned,
-                       int* ksplit) {
+int tm_batch_gemm_info(tm_batch* bt, int* solution, int* own_kernel, int* candidates, double* ms_default,
+                       double* ms_tuned, int* ksplit) {
     if (!bt) return TM_ERR_BAD_ARG;
-    if (solution) *solution = bt->gemm_own ? -1 - bt->gemm_shape : bt->gemm_sol;
+    if (solution) *solution = bt->gemm_own ? bt->gemm_shape : bt->gemm_sol;
+    if (own_kernel) *own_kernel = bt->gemm_own ? 1 : 0;
     if (candidates) *candidates = bt->gemm_candidates + bt->gemm_own_candidates;
     if (ksplit) *ksplit = bt->gemm_own ? bt->gemm_ks : 1;
     if (ms_default) *ms_default = bt->gemm_ms_default;

@@ -70,6 +70,12 @@ struct tm_ctx {
     DevBuf<unsigned> rk_fail;
     DevBuf<int> rk_imax;
     int rk_budget[2] = {0, 0};
+    // adaptive Dormand-Prince 5(4) (tm_integrate.hip): the seven stage
+    // derivatives (k[0] = f(m), carried from the last stage of an accepted
+    // step: first same as last), the stage argument, the error-norm partial
+    // sums and the device step state
+    DevBuf<double> ode_k[7], ode_y, ode_part;
+    DevBuf<unsigned char> ode_state;
     // phase timers
     bool timers = false;
     hipEvent_t ev[6] = {};

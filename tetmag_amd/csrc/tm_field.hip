@@ -1113,7 +1113,15 @@ int demag_potential_dev(tm_ctx* c, const double* md, tm_eval_stats* stats) {
     int itN = 0, itD = 0;
     double rN = 0, rD = 0;
     if (c->timers) TM_HIP(hipEventRecord(c->ev[1], c->stream));
+    // A non-finite solve leaves NaN/Inf in the warm-start vectors; they are
+    // cleared so that the next call with a finite m starts from zero.
+    auto clear_potentials = [c]() {
+        (void)hipMemsetAsync(c->u1.p, 0, sizeof(double) * c->n, c->stream);
+        (void)hipMemsetAsync(c->u2.p, 0, sizeof(double) * c->n, c->stream);
+        c->have_u = false;
+    };
     int st = c->cgN.solve(c->rhsN.p, c->u1.p, c->tol, c->maxit, c->stream, &itN, &rN);
+    if (st == TM_ERR_NONFINITE) clear_potentials();
     if (st != TM_OK && st != TM_ERR_MAXITER) return st;
     int stN = st;
     if (c->timers) TM_HIP(hipEventRecord(c->ev[2], c->stream));
@@ -1125,6 +1133,7 @@ int demag_potential_dev(tm_ctx* c, const double* md, tm_eval_stats* stats) {
     if (c->timers) TM_HIP(hipEventRecord(c->ev[4], c->stream));
     c->tsum.iters_neumann += itN;
     c->tsum.iters_dirichlet += itD;
+    if (st == TM_ERR_NONFINITE) clear_potentials();
     if (st != TM_OK && st != TM_ERR_MAXITER) return st;
     c->have_u = true;
     if (stats) {
@@ -1500,4 +1509,6 @@ int ctx_view(tm_ctx* c, CtxView* v) {
     for (int d = 0; d < 3; ++d) v->hext[d] = c->hext_u[d];
     return TM_OK;
 }
+
+const double* ctx_hext_field(tm_ctx* c) { return c && c->hext_field ? c->hext.p : nullptr; }
 }  // namespace tmh

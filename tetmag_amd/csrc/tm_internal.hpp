@@ -334,6 +334,9 @@ struct CtxView {
     double hext[3];
 };
 int ctx_view(tm_ctx* c, CtxView* v);
+// The context's per-node applied field (3n, A/m) set by tm_ctx_set_zeeman,
+// or null when only a uniform field is set.
+const double* ctx_hext_field(tm_ctx* c);
 
 }  // namespace tmh
 
