@@ -217,6 +217,20 @@ int tm_ctx_assemble_bem(tm_ctx* c, int nbel, const double* bxyz, const int* face
 int tm_ctx_get_bem_dense(tm_ctx* c, double* B, int where);
 /* DemagField::denseMVP (DemagField.cpp:145-147): g = B u on boundary vectors. */
 int tm_bem_apply(tm_ctx* c, const double* ub, int uwhere, double* gb, int gwhere);
+/* No counterpart in the reference: tm_bem_apply for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  y = B x for a dense bnx x bnx
+ * column-major B (leading dimension ld >= bnx, ld even, B 16-byte aligned),
+ * x and y device pointers, enqueued on `stream` (a hipStream_t; NULL = the
+ * default stream), no context needed.  Nothing is allocated: the partial
+ * products of the column splits live in a static device array of 32 MiB
+ * (4,194,304 doubles) that every GPU the library runs on holds until the
+ * process ends; sizes whose partials do not fit (bnx above about 92,000)
+ * return TM_ERR_BAD_ARG before any device work.  Same kernels and results
+ * as tm_bem_apply.  Calls that overlap on one GPU share that array and race.
+ * The array's address on a GPU is looked up when tm_set_device selects it,
+ * else on the first call there (call tm_set_device before a capture). */
+int tm_bem_apply_noalloc(int bnx, int ld, const double* B, const double* x, double* y, void* stream);
 
 /* DemagField::initializeSolvers (DemagField.cpp:243-282): the Neumann matrix
  * (stiffness with row nx/2 replaced by the identity) and the Dirichlet matrix

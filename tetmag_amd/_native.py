@@ -180,6 +180,7 @@ _SIGNATURES = {
     "tm_ctx_set_bem_dense": [_P, _P, _I, _I],
     "tm_ctx_assemble_bem": [_P, _I, _P, _P, _P, _P, _P, _I],
     "tm_ctx_get_bem_dense": [_P, _P, _I],
+    "tm_bem_apply_noalloc": [_I, _I, _P, _P, _P, _P],
     "tm_bem_apply": [_P, _P, _I, _P, _I],
     "tm_ctx_set_demag_solvers": [_P, _P, _P, _P, _P, _P, _P, _I],
     "tm_ctx_set_solver_params": [_P, _D, _I],

@@ -6,6 +6,8 @@
 // H2Lib).  Here both run on the device: B (bnx x bnx, FP64, column-major,
 // leading dimension padded to a multiple of 64) stays resident in HBM.
 #include <cmath>
+#include <cstdint>
+#include <mutex>
 
 #include "tm_internal.hpp"
 
@@ -328,3 +330,55 @@ int bem_gemvT(int b, int ld, const double* B, const double* src, const int* gidx
 }
 
 }  // namespace tmh
+
+// ---------------------------------------------------------------------------
+// Boundary product that allocates nothing (tm_bem_apply_noalloc): the column
+// splits' partial products live in a file-scope device array instead of a
+// context buffer.  In host code `g_noalloc_scratch` is only a host-side
+// stand-in for the symbol; its device address (one copy per GPU) is looked up
+// once per device with hipGetSymbolAddress and only that address is used.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoallocScratchDoubles = (size_t)4 << 20;  // 32 MiB per GPU
+__device__ __align__(256) double g_noalloc_scratch[kNoallocScratchDoubles];
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// Device address of the scratch array on the current device (looked up on the
+// first call per device).
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    TM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoallocMaxDevices) return TM_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_noalloc_scratch));
+        if (e != hipSuccess || !p) return TM_ERR_HIP;
+        g_noalloc_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return TM_OK;
+}
+}  // namespace
+
+namespace tmh {
+int noalloc_scratch_init() {
+    double* p = nullptr;
+    return noalloc_scratch(&p);
+}
+}  // namespace tmh
+
+extern "C" {
+
+int tm_bem_apply_noalloc(int bnx, int ld, const double* B, const double* x, double* y, void* stream) {
+    if (bnx <= 0 || ld < bnx || ld % 2 != 0 || !B || !x || !y) return TM_ERR_BAD_ARG;
+    if (reinterpret_cast<uintptr_t>(B) % 16 != 0) return TM_ERR_BAD_ARG;  // 16-byte loads of column pairs
+    if ((size_t)tmh::bem_gemv_splits(bnx) * (size_t)ld > kNoallocScratchDoubles) return TM_ERR_BAD_ARG;
+    double* part = nullptr;
+    TM_CHECK(noalloc_scratch(&part));
+    return tmh::bem_gemv(bnx, ld, B, x, nullptr, part, y, nullptr, nullptr, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

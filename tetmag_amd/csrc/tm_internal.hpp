@@ -337,6 +337,9 @@ int ctx_view(tm_ctx* c, CtxView* v);
 // The context's per-node applied field (3n, A/m) set by tm_ctx_set_zeeman,
 // or null when only a uniform field is set.
 const double* ctx_hext_field(tm_ctx* c);
+// Looks up, once per device, the device address of the no-allocation
+// boundary product's scratch array on the current device (tm_bem.hip).
+int noalloc_scratch_init();
 
 }  // namespace tmh
 

@@ -150,7 +150,8 @@ int tm_set_device(int dev, int* chosen) {
     if (dev < 0 || dev > c - 1) dev = 0;  // SetDevice.cu:54-57
     TM_HIP(hipSetDevice(dev));
     if (chosen) *chosen = dev;
-    return TM_OK;
+    // set-up of tm_bem_apply_noalloc on this device (its scratch address)
+    return tmh::noalloc_scratch_init();
 }
 
 static bool csr_ok(int rows, int cols, const int* rowptr, const int* colidx) {
