@@ -186,6 +186,14 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
                 "cg_iters_per_batch": [ph["iters_neumann"] / cnt, ph["iters_dirichlet"] / cnt]})
     log(f"[{name}] {steps} batch steps of {nsamples}: {ms:.3f} ms/batch = {out['evals_per_s']:.1f} evals/s; "
         f"phases {out['phase_ms']} iters {out['cg_iters_per_batch']}")
+    # each hot kernel of the evaluation launched alone, outside the timed
+    # region: mean duration and fraction of HBM peak at its algorithmic bytes
+    kt = bc.time_kernels(20)
+    out["kernels"] = [{**k, "gbs": k["bytes"] / (k["ms"] * 1e-3) / 1e9,
+                       "frac": k["bytes"] / (k["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS} for k in kt]
+    for k in out["kernels"]:
+        log(f"[{name}] kernel {k['kernel']}: {k['ms']:.4f} ms, {k['bytes'] / 1e6:.1f} MB, "
+            f"{k['gbs']:.0f} GB/s = {k['frac']:.3f} of HBM peak")
     bc.close()
     return out
 

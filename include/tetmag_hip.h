@@ -418,7 +418,14 @@ typedef struct tm_batch_stats {
     double max_torque[TM_BATCH_MAX];     /* per sample, A/m */
 } tm_batch_stats;
 
-/* Needs a context with operators, boundary matrix and AMG demag solvers. */
+/* Needs a context with operators, boundary matrix and AMG demag solvers.  The
+ * batch takes the context's operators, Js, solver hierarchies, solver
+ * tolerance and uniform applied field as they are at creation (create a new
+ * batch after changing them; per-sample fields: tm_batch_set_zeeman); the
+ * context's per-node applied field is read at each evaluation.  Batched
+ * vectors are addressed with 32-bit byte offsets: 3 n S 8 bytes must stay
+ * below 2 GiB (n up to 1.39 million nodes at S = 64), else
+ * TM_ERR_UNSUPPORTED. */
 int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out);
 int tm_batch_destroy(tm_batch* b);
 /* Per-sample uniform applied field (A/m), [3][S]; default = the context's
@@ -468,6 +475,22 @@ int tm_batch_gemm_info(tm_batch* b, int* solution, int* own_kernel, int* candida
  * caller values; x, y: b x S, boundary-node major, the sample index fastest.
  * The batched form of DemagField::denseMVP (DemagField.cpp:145-147). */
 int tm_batch_bem_apply(tm_batch* b, const double* x, int xwhere, double* y, int ywhere);
+/* Kernel timing of a batch (no reference counterpart): the hot kernels of an
+ * evaluation -- the divergence, the fused field + LLG kernel, and of the
+ * Neumann solve the level-0 kernels of one inner CG iteration (SpMM, the
+ * pre-smoothing restriction, the prolongation, the post-smoother) and the
+ * FP64 outer residual -- each launched `reps` times alone on the batch's
+ * stream (all samples active) and timed with HIP events.  For each: its
+ * name, the mean ms per launch and its algorithmic bytes per launch (slot
+ * offsets and weights read once, every vector read or written once, the
+ * gathered rows counted once).  Leaves the batch's solver state undefined
+ * until the next evaluation; *count receives the number written (<= cap). */
+typedef struct tm_batch_kernel_time {
+    char name[48];
+    double ms;
+    double bytes;
+} tm_batch_kernel_time;
+int tm_batch_time_kernels(tm_batch* b, int reps, tm_batch_kernel_time* out, int cap, int* count);
 int tm_batch_enable_timers(tm_batch* b, int on);
 int tm_batch_get_timers(tm_batch* b, tm_phase_times* out, int reset);
 

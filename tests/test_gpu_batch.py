@@ -216,5 +216,15 @@ def test_wide_batch_medium_mesh():
         d1, h1, _ = ctx.eval_rhs_host(ms[s])
         assert rel_l2(from_batch(H, s), h1) < 1e-9
         assert rel_l2(from_batch(dm, s), d1) < 1e-9
+    # the kernel timing hook: every hot kernel timed, and the next evaluation
+    # is unaffected by the state the timing runs leave behind
+    kt = bc.time_kernels(3)
+    names = [k["kernel"] for k in kt]
+    print("kernel times", kt)
+    assert len(kt) >= 6 and names[0] == "k_b_div" and names[1].startswith("k_b_eval")
+    assert all(k["ms"] > 0 and k["bytes"] > 0 for k in kt)
+    dm2, H2, st2 = bc.eval_rhs_host(to_batch(ms))
+    assert st2["max_relres_neumann"] <= 1e-12
+    assert rel_l2(H2, H) < 1e-10
     bc.close()
     ctx.close()

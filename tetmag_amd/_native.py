@@ -119,6 +119,10 @@ class OdeStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("nsamples", C.c_int),
@@ -223,6 +227,7 @@ _SIGNATURES = {
     "tm_batch_get_potentials": [_P, _P, _P, _I],
     "tm_batch_step_euler": [_P, _P, _I, _D],
     "tm_batch_run_rk4": [_P, _D, _I, _D, _I, _I, _P, _P],
+    "tm_batch_time_kernels": [_P, _I, _P, _I, _P],
     "tm_batch_gemm_info": [_P, _P, _P, _P, _P, _P, _P],
     "tm_batch_bem_apply": [_P, _P, _I, _P, _I],
     "tm_batch_enable_timers": [_P, _I],

@@ -89,6 +89,14 @@ class BatchContext:
                                          int(use_graph), C.byref(st), C.byref(rk)), "tm_batch_run_rk4")
         return st.as_dict(), rk.as_dict()
 
+    def time_kernels(self, reps: int = 20) -> list:
+        """Mean ms and algorithmic bytes per launch of the batch's hot kernels,
+        each launched alone (tm_batch_time_kernels)."""
+        arr = (N.KernelTime * 16)()
+        cnt = C.c_int()
+        N.check(N.lib().tm_batch_time_kernels(self.h, int(reps), arr, 16, C.byref(cnt)), "tm_batch_time_kernels")
+        return [{"kernel": arr[k].name.decode(), "ms": arr[k].ms, "bytes": arr[k].bytes} for k in range(cnt.value)]
+
     def gemm_info(self) -> dict:
         """The tuned boundary GEMM: own_kernel True means the library's FP64
         MFMA kernel with tile shape `solution`, else rocBLAS solution index

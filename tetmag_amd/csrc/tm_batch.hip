@@ -831,20 +831,19 @@ __device__ inline double buf_ld8(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
-// b = sum_d tGrad_d (Js m_d) scale for every sample (DemagField.cpp:124-136)
+// b = sum_d tGrad_d (Js m_d) / mu0 for every sample (DemagField.cpp:124-136);
+// the slot weights hold tGrad_d(i, j) Js_j / mu0 (set at batch creation)
 __global__ __launch_bounds__(kBlock) void k_b_div(SlotArgs P, const double* __restrict__ wty,
-                                                  const double* __restrict__ wtz, const int* __restrict__ cidx,
-                                                  int S, const double* __restrict__ js, const double* __restrict__ m,
-                                                  double scale, double* __restrict__ out) {
+                                                  const double* __restrict__ wtz, int S,
+                                                  const double* __restrict__ m, double* __restrict__ out) {
     const int n = P.n;
     const int lane = threadIdx.x & (kWave - 1);
     const bool on = lane < S;
     const int lc = min(lane, S - 1);
     const int c8 = n * S * 8;  // bytes of one component
     const __amdgpu_buffer_rsrc_t mr = gather_rsrc(m, 3 * c8);
-    const cint_p sp = (cint_p)P.sp, of = (cint_p)P.off, ci = (cint_p)cidx;
+    const cint_p sp = (cint_p)P.sp, of = (cint_p)P.off;
     const cw_p<double> wx = (cw_p<double>)P.w, wy = (cw_p<double>)wty, wz = (cw_p<double>)wtz;
-    const cw_p<double> jc = (cw_p<double>)js;
     const int vo = lc * 8;
     B_ROW_LOOP_XCD(n) {
         double sx = 0, sy = 0, sz = 0;
@@ -862,13 +861,12 @@ __global__ __launch_bounds__(kBlock) void k_b_div(SlotArgs P, const double* __re
             }
 #pragma unroll
             for (int k = 0; k < kHalf; ++k) {
-                const double j = jc[ci[q + k]];
-                sx += wx[q + k] * (mx[k] * j);
-                sy += wy[q + k] * (my[k] * j);
-                sz += wz[q + k] * (mz[k] * j);
+                sx += wx[q + k] * mx[k];
+                sy += wy[q + k] * my[k];
+                sz += wz[q + k] * mz[k];
             }
         }
-        if (on) out[(size_t)row * S + lane] = (sx + sy + sz) * scale;
+        if (on) out[(size_t)row * S + lane] = sx + sy + sz;
     }
 }
 
@@ -1228,6 +1226,17 @@ __global__ void k_b_zero_small(int cnt, double* p) {
 __global__ void k_b_count_undone(int S, const OState* os, unsigned* fail) {
     const int s = threadIdx.x;
     if (s < S && !os->done[s]) atomicAdd(fail, 1u);
+}
+
+// every sample active with finite CG scalars (kernel timing only)
+__global__ void k_b_activate(int S, BState* bs) {
+    const int s = threadIdx.x;
+    if (s < S) {
+        bs->active[s] = 1;
+        bs->gam[s] = 1.0;
+        bs->alpha[s] = 0.0;
+        bs->beta[s] = 0.0;
+    }
 }
 
 __global__ void k_b_reset_fail(unsigned* fail) {
@@ -1771,6 +1780,68 @@ struct BatchSolver {
 
     int last_outer = 0;  // refinement steps of the last host-polled solve
 
+    // Times the level-0 kernels of one inner iteration and of the outer
+    // residual, each launched `reps` times alone (all samples active), and
+    // appends (name, ms per launch, algorithmic bytes per launch): slot
+    // offsets (4 B) and weights once, every vector read or written once.
+    template <class Timer>
+    int time_kernels(hipStream_t s, Timer&& timeit, std::vector<tm_batch_kernel_time>& out) {
+        if (cg->amg.levels.empty()) return TM_ERR_UNSUPPORTED;
+        const int e = (int)sizeof(T);
+        const double nS = (double)n * S;
+        double* pzq = part.p + (size_t)3 * S * G;
+        double* prz = part.p + (size_t)2 * S * G;
+        auto add = [&out](const char* nm, double ms, double bytes) {
+            tm_batch_kernel_time k = {};
+            std::snprintf(k.name, sizeof(k.name), "%s", nm);
+            k.ms = ms;
+            k.bytes = bytes;
+            out.push_back(k);
+        };
+        auto slots = [](const SlotOp& o) { return (double)o.h_sp[o.n]; };
+        hipLaunchKernelGGL(k_b_activate, dim3(1), dim3(kMaxS), 0, s, S, bst.p);
+        const SlotOp& A0 = wide() ? sl0w : sl0;
+        add(wide() ? "k_b_cg_spmv_w" : "k_b_cg_spmv", timeit([&] { cg_spmv(ring.p, 0, pzq, s); }),
+            slots(A0) * (4 + e) + (wide() ? (double)n * e : 0.0) + 3.0 * nS * e);
+        const LvOps& O = lo[0];
+        const int n1 = cg->amg.levels[0].R.n;
+        add(O.RA.long_rows() ? "k_b_spmm_w<long> (R(I-AwD^-1), level 0)" : "k_b_spmm (R(I-AwD^-1), level 0)",
+            timeit([&] { spmm(O.RA.long_rows(), dim3(row_grid(n1)), O.RA.args(e, O.wRA.p), rT.p, lv[0].b.p, s); }),
+            slots(O.RA) * (4 + e) + nS * e + (double)n1 * S * e);
+        const auto& V0 = cg->amg.levels[0];
+        add(wide() ? "k_b_prolong_w (level 0)" : "k_b_prolong (level 0)", timeit([&] {
+                if (wide())
+                    prolong(O.P.long_rows(), O.P.chunk, dim3(row_grid(V0.P.n)), O.P.args(e, O.wP.p), dinv.p, rT.p,
+                            V0.omega, lv[0].b.p, lv[0].x.p, s);
+                else
+                    hipLaunchKernelGGL((k_b_prolong<T>), dim3(row_grid(V0.P.n)), dim3(kBlock), 0, s,
+                                       gated(O.P.args(e, O.wP.p)), S, dinv.p, rT.p, V0.omega, lv[0].b.p, lv[0].x.p);
+            }),
+            slots(O.P) * (4 + e) + 8.0 * n + 2.0 * nS * e + (double)n1 * S * e);
+        if constexpr (kWideT) {
+            if (wide() && post1) {
+                add("k_b_post1_w (level 0)", timeit([&] {
+                        hipLaunchKernelGGL(k_b_post1_w, dim3(G), dim3(kBlock), 0, s, gated(sl0.args(e, wIDA0.p)),
+                                           dinv.p, (const float*)rT.p, V0.omega, (const float*)lv[0].x.p,
+                                           (float*)ring.p, prz);
+                    }),
+                    slots(sl0) * (4 + e) + 8.0 * n + 3.0 * nS * e);
+            }
+        }
+        double* pbb = part.p;
+        double* prr = part.p + (size_t)S * G;
+        add(sizeof(T) == 4 ? "k_b_resid<float>" : "k_b_resid<double>", timeit([&] {
+                hipLaunchKernelGGL((k_b_resid<T>), dim3(G), dim3(kBlock), 0, s, sl0.args(8, wA64.p), S,
+                                   (const double*)lv0_b64(), (const double*)lv0_x64(), rT.p, pbb, prr);
+            }),
+            slots(sl0) * (4 + 8) + 2.0 * nS * 8 + nS * e);
+        return TM_OK;
+    }
+    // FP64 [n][S] operands of the outer residual for kernel timing (the
+    // solver's own FP64 vectors are the caller's; ring storage is large enough)
+    const double* lv0_b64() const { return (const double*)ring.p; }
+    const double* lv0_x64() const { return (const double*)(ring.p + (size_t)2 * n * S * 8 / sizeof(T)); }
+
     // The same solve with no host synchronisation (capturable in a HIP graph):
     // exactly `nouter` refinement steps of ninner[o] inner iterations each.
     // Every solver kernel is gated on the per-sample flags, so refinement
@@ -1838,7 +1909,6 @@ struct tm_batch {
     rocblas_handle blas = nullptr;
     CtxView cv{};
     SlotOp eo;                         // union pattern of the per-node operators, in row slots
-    DevBuf<int> eci;                   // its slot columns
     DevBuf<double> xw, gx, gy, gz, tx, ty, tz, cx, cy, cz;  // slot weights: exchange, grad, tGrad, curl
     bool exch = false, dmi = false;
     DevBuf<double> m, dmdt, heff, u1, u2, usum, rhsN, rhsD, U, Gm, hext, torque;
@@ -1889,10 +1959,11 @@ int csr_on_pattern(const SellPattern& P, const std::vector<int>& rp, const std::
     return TM_OK;
 }
 
-int upload_op(tm_batch* bt, int op, DevBuf<double>& dst) {
+// Slot weights of one operator; with colscale, entry (i, j) is scaled by w0 colscale[j].
+int upload_op(tm_batch* bt, int op, DevBuf<double>& dst, const double* colscale = nullptr, double w0 = 1.0) {
     std::vector<double> v;
     TM_CHECK(csr_on_pattern(*bt->cv.pat, *bt->cv.hop_rp[op], *bt->cv.hop_col[op], *bt->cv.hop_val[op], v));
-    return bt->eo.values(v, dst, bt->stream);
+    return bt->eo.values(v, dst, bt->stream, bt->cv.pat->h_csrcol.data(), colscale, w0);
 }
 
 }  // namespace
@@ -2199,8 +2270,7 @@ int eval_enqueue(tm_batch* bt, double alpha, int variant, double* dmdt) {
     hipStream_t s = bt->stream;
     const CtxView& v = bt->cv;
     hipLaunchKernelGGL(k_b_div, dim3(row_grid(n)), dim3(kBlock), 0, s, bt->eo.args(8, bt->tx.p),
-                       (const double*)bt->ty.p, (const double*)bt->tz.p, (const int*)bt->eci.p, S, v.js,
-                       (const double*)bt->m.p, 1.0 / kMu0, bt->rhsN.p);
+                       (const double*)bt->ty.p, (const double*)bt->tz.p, S, (const double*)bt->m.p, bt->rhsN.p);
     TM_HIP(hipGetLastError());
     if (bt->prec == TM_PREC_FP64)
         TM_CHECK(bt->sN64.enqueue(bt->rhsN.p, bt->u1.p, v.tol, bt->bud_outer[0], bt->bud_inner[0], s, bt->rk_fail.p));
@@ -2258,23 +2328,21 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     if ((int64_t)3 * P.n * nsamples * 8 > (int64_t)INT32_MAX) return fail(TM_ERR_UNSUPPORTED);
     if ((st = bt->eo.build(P.n, P.n, P.h_rowptr.data(), P.h_csrcol.data(), nsamples, bt->stream)) != TM_OK)
         return fail(st);
-    {
-        std::vector<int> hc((size_t)bt->eo.h_sp[P.n], 0);
-        for (int i = 0; i < P.n; ++i)
-            for (int q = P.h_rowptr[i]; q < P.h_rowptr[i + 1]; ++q)
-                hc[(size_t)bt->eo.h_sp[i] + (q - P.h_rowptr[i])] = P.h_csrcol[q];
-        if ((st = bt->eci.upload(hc.data(), hc.size(), bt->stream)) != TM_OK) return fail(st);
-        if (hipStreamSynchronize(bt->stream) != hipSuccess) return fail(TM_ERR_HIP);
-    }
     bt->exch = v.hop_set[TM_OP_EXCHANGE];
     int ops_ok = TM_OK;
     if (bt->exch) ops_ok |= upload_op(bt, TM_OP_EXCHANGE, bt->xw);
     ops_ok |= upload_op(bt, TM_OP_GRAD_X, bt->gx);
     ops_ok |= upload_op(bt, TM_OP_GRAD_Y, bt->gy);
     ops_ok |= upload_op(bt, TM_OP_GRAD_Z, bt->gz);
-    ops_ok |= upload_op(bt, TM_OP_TGRAD_X, bt->tx);
-    ops_ok |= upload_op(bt, TM_OP_TGRAD_Y, bt->ty);
-    ops_ok |= upload_op(bt, TM_OP_TGRAD_Z, bt->tz);
+    {  // the divergence weights carry Js of their column and 1/mu0 (the Js of the context now)
+        std::vector<double> jsh((size_t)P.n);
+        if (hipMemcpyAsync(jsh.data(), v.js, sizeof(double) * P.n, hipMemcpyDeviceToHost, bt->stream) != hipSuccess ||
+            hipStreamSynchronize(bt->stream) != hipSuccess)
+            return fail(TM_ERR_HIP);
+        ops_ok |= upload_op(bt, TM_OP_TGRAD_X, bt->tx, jsh.data(), 1.0 / kMu0);
+        ops_ok |= upload_op(bt, TM_OP_TGRAD_Y, bt->ty, jsh.data(), 1.0 / kMu0);
+        ops_ok |= upload_op(bt, TM_OP_TGRAD_Z, bt->tz, jsh.data(), 1.0 / kMu0);
+    }
     bt->dmi = v.has_dmi && v.hop_set[TM_OP_CURL_X] && v.hop_set[TM_OP_CURL_Y] && v.hop_set[TM_OP_CURL_Z];
     if (bt->dmi) {
         ops_ok |= upload_op(bt, TM_OP_CURL_X, bt->cx);
@@ -2394,8 +2462,7 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[0], s));
     // Poisson right-hand side (DemagField.cpp:124-136), all samples
     hipLaunchKernelGGL(k_b_div, dim3(row_grid(n)), dim3(kBlock), 0, s, bt->eo.args(8, bt->tx.p),
-                       (const double*)bt->ty.p, (const double*)bt->tz.p, (const int*)bt->eci.p, S, v.js,
-                       (const double*)bt->m.p, 1.0 / kMu0, bt->rhsN.p);
+                       (const double*)bt->ty.p, (const double*)bt->tz.p, S, (const double*)bt->m.p, bt->rhsN.p);
     TM_HIP(hipGetLastError());
     if (bt->timers) TM_HIP(hipEventRecord(bt->ev[1], s));
     int oN = 0, iN = 0, oD = 0, iD = 0;
@@ -2606,6 +2673,54 @@ int tm_batch_gemm_info(tm_batch* bt, int* solution, int* own_kernel, int* candid
     if (ms_default) *ms_default = bt->gemm_ms_default;
     if (ms_tuned) *ms_tuned = bt->gemm_ms_best;
     return TM_OK;
+}
+
+int tm_batch_time_kernels(tm_batch* bt, int reps, tm_batch_kernel_time* out, int cap, int* count) {
+    if (!bt || reps < 1 || !out || cap < 1 || !count) return TM_ERR_BAD_ARG;
+    hipStream_t s = bt->stream;
+    const int n = bt->n, S = bt->S;
+    const double nS = (double)n * S;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TM_HIP(hipEventCreate(&e0));
+    TM_HIP(hipEventCreate(&e1));
+    int status = TM_OK;
+    auto timeit = [&](auto&& launch) -> double {
+        launch();  // warm-up
+        if (hipEventRecord(e0, s) != hipSuccess) status = TM_ERR_HIP;
+        for (int r = 0; r < reps; ++r) launch();
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) status = TM_ERR_HIP;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return (double)ms / reps;
+    };
+    std::vector<tm_batch_kernel_time> v;
+    auto add = [&v](const char* nm, double ms, double bytes) {
+        tm_batch_kernel_time k = {};
+        std::snprintf(k.name, sizeof(k.name), "%s", nm);
+        k.ms = ms;
+        k.bytes = bytes;
+        v.push_back(k);
+    };
+    const double eslots = (double)bt->eo.h_sp[n];
+    add("k_b_div", timeit([&] {
+            hipLaunchKernelGGL(k_b_div, dim3(row_grid(n)), dim3(kBlock), 0, s, bt->eo.args(8, bt->tx.p),
+                               (const double*)bt->ty.p, (const double*)bt->tz.p, S, (const double*)bt->m.p,
+                               bt->rhsN.p);
+        }),
+        eslots * (4 + 3 * 8) + 3 * nS * 8 + nS * 8);
+    const int nw = (bt->exch ? 1 : 0) + 3 + (bt->dmi ? 3 : 0);
+    const double per_node = 8.0 * n * (1 + (bt->cv.has_ku ? 4 : 0) + (bt->dmi ? 4 : 0));
+    add(bt->dmi ? "k_b_eval<DMI>" : "k_b_eval", timeit([&] {
+            (void)launch_fields(bt, 0.5, TM_LLG_CLASSIC, bt->dmdt.p, nullptr, false);
+        }),
+        eslots * (4 + 8.0 * nw) + 3 * nS * 8 + nS * 8 + 3 * nS * 8 + per_node);
+    int st = bt->prec == TM_PREC_FP64 ? bt->sN64.time_kernels(s, timeit, v) : bt->sN32.time_kernels(s, timeit, v);
+    if (status == TM_OK) status = st;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *count = 0;
+    for (size_t k = 0; k < v.size() && (int)k < cap; ++k) out[(*count)++] = v[k];
+    return status;
 }
 
 int tm_batch_bem_apply(tm_batch* bt, const double* x, int xwhere, double* y, int ywhere) {
