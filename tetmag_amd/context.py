@@ -90,6 +90,9 @@ class FieldContext:
         self.stream.synchronize()
         N.check(L.tm_ctx_set_mag(self.h, N.ptr(N.f64(prob.m0)), N.TM_HOST), "tm_ctx_set_mag")
         self.last_stats = N.EvalStats()
+        # parameters last applied from torch tensors (autograd), by key; the
+        # setters below drop their key
+        self._param_cache = {}
 
     # -- setup -------------------------------------------------------------------
     def _set_op(self, op, rp, col, vals):
@@ -262,12 +265,14 @@ class FieldContext:
         """Applied field (A/m): a uniform (3,) vector plus, optionally, a
         per-node (3n,) field added to it (TheLLG.cpp:320-332 carries the
         pulse, sweep and local fields this way)."""
+        self._param_cache.pop("hext", None)
         hx, hy, hz = (float(v) for v in h3)
         fp = N.ptr(N.f64(field3n)) if field3n is not None else None
         N.check(self.L.tm_ctx_set_zeeman(self.h, fp, hx, hy, hz), "tm_ctx_set_zeeman")
 
     def set_ku(self, Ku):
         """Per-node uniaxial constant (J/m^3) along the problem's axes."""
+        self._param_cache.pop("ku", None)
         if not self.has_uniaxial:
             raise RuntimeError("context built without uniaxial anisotropy (pass uniaxial=True)")
         Kd = N.f64(Ku)
@@ -275,6 +280,7 @@ class FieldContext:
 
     def set_d(self, D):
         """Per-node bulk DMI constant (J/m^2 divided by the mesh scale)."""
+        self._param_cache.pop("d", None)
         if not self.has_dmi:
             raise RuntimeError("context built without DMI (pass dmi=True)")
         ops, bnd = self.prob.ops, self.prob.bnd
@@ -283,6 +289,7 @@ class FieldContext:
 
     def set_di(self, Di):
         """Per-node interfacial DMI constant (J/m^2 divided by the mesh scale)."""
+        self._param_cache.pop("di", None)
         if not self.has_idmi:
             raise RuntimeError("context built without interfacial DMI (pass idmi=True)")
         N.check(self.L.tm_ctx_set_dmi_interfacial(self.h, N.ptr(N.f64(Di))), "tm_ctx_set_dmi_interfacial")
