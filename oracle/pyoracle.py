@@ -375,11 +375,12 @@ class Oracle:
     def transposed_solvers(self):
         if getattr(self, "_tsolvers", None) is None:
             import scipy.sparse as sp
+            pc = self.solvers()[0].precond  # the forward solvers' preconditioner
             out = []
             for rp, col, val in (self.neumann_matrix(), self.dirichlet_matrix()):
                 T = sp.csr_matrix((val, col, rp), shape=(self.n, self.n)).T.tocsr()
                 T.sort_indices()
-                out.append(Solver(i32(T.indptr), i32(T.indices), f64(T.data)))
+                out.append(Solver(i32(T.indptr), i32(T.indices), f64(T.data), precond=pc))
             self._tsolvers = tuple(out)
         return self._tsolvers
 

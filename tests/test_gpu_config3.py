@@ -1,0 +1,101 @@
+"""The hysteresis-batch workload at its stated size: a disk of diameter
+200 nm and thickness 10 nm, layered prism mesh at 2.5 nm edge (25,785 nodes,
+120,720 tetrahedra, 11,070 boundary nodes; rows up to 19 entries),
+64 magnetisation states under B_ext = mu0 H_k (cos t_i, sin t_i, 0) with
+mu0 H_k from -80 to +80 mT and t_i seeded in [0, 5 deg] (the sweep of
+Hysteresis.cpp:43-169 run as one batch).
+
+  * every one of the 64 samples of the mixed-precision batch (FP32 inner CG,
+    FP64 refinement, autotuned boundary GEMM) against the single-sample FP64
+    path at 1e-9, two samples against the FP64 oracle at 1e-9;
+  * fixed-step RK4 at dt = 50 fs: the graph-captured batched steps against
+    eager host-polled steps of an identical batch, and two samples against
+    the single-sample RK4 of the field context.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg3(oracle_mod, gpu_available):
+    from tetmag_amd import problem as P
+    from tetmag_amd.context import FieldContext
+    prob, H, m0 = P.disk_hysteresis()
+    print(f"\n[case] config 3: {prob.summary()}", flush=True)
+    ctx = FieldContext(prob, tol=1e-10)
+    yield prob, H, m0, ctx
+    ctx.close()
+
+
+def test_hysteresis_batch_samples(cfg3, oracle_mod):
+    from tetmag_amd import _native as N
+    from tetmag_amd.batch import BatchContext, from_batch
+    prob, H, m0, ctx = cfg3
+    S = H.shape[1]
+    assert S == 64
+    bc = BatchContext(ctx, S, N.TM_PREC_MIXED)
+    print("gemm", bc.gemm_info(), flush=True)
+    bc.set_zeeman(H)
+    dm, Hb, st = bc.eval_rhs_host(np.ascontiguousarray(m0))
+    bc.close()
+    print("solves", st["outer_neumann"], st["inner_neumann"], st["outer_dirichlet"], st["inner_dirichlet"], flush=True)
+    errs = []
+    ctx.set_solver_params(1e-12, 100000)
+    try:
+        for s in range(S):
+            ctx.set_hext(H[:, s])
+            d1, h1, _ = ctx.eval_rhs_host(np.ascontiguousarray(m0[:, :, s].reshape(-1)))
+            errs.append(rel_l2(from_batch(Hb, s), h1))
+            assert rel_l2(from_batch(dm, s), d1) < 1e-8, s
+    finally:
+        ctx.set_solver_params(1e-10, 100000)
+        ctx.set_hext(prob.hext)
+    print(f"Heff vs FP64 over 64 samples: max {max(errs):.2e} median {np.median(errs):.2e}", flush=True)
+    assert max(errs) < 1e-8
+    orc = oracle_mod.Oracle(prob.mesh.xyz, prob.mesh.fel)
+    for s in (0, 47):
+        ms = np.ascontiguousarray(m0[:, :, s].reshape(-1))
+        ref = orc.eval_rhs(ms, Js=prob.Js, A=prob.A, Ku=prob.Ku, axis=prob.kaxis, D=prob.D, Hext=H[:, s],
+                           alpha=prob.alpha, tol=1e-12)
+        e = rel_l2(from_batch(Hb, s), ref["Heff"])
+        print(f"sample {s} vs oracle: Heff {e:.2e}", flush=True)
+        assert e < 1e-8
+
+
+def test_hysteresis_rk4_graph(cfg3):
+    from tetmag_amd import _native as N
+    from tetmag_amd.batch import BatchContext, from_batch
+    from tetmag_amd.context import FieldContext
+    prob, H, m0, ctx = cfg3
+    S = H.shape[1]
+    dtau = 50e-3 * prob.time_scale()  # 50 fs
+    ga, gb = BatchContext(ctx, S, N.TM_PREC_MIXED), BatchContext(ctx, S, N.TM_PREC_MIXED)
+    for b in (ga, gb):
+        b.set_zeeman(H)
+        b.set_mag(np.ascontiguousarray(m0))
+    _, ra = ga.run_rk4(dtau, 6, use_graph=True)
+    _, rb = gb.run_rk4(dtau, 6, use_graph=False)
+    print("graph", ra, "\neager", rb, flush=True)
+    # the first step is host-polled (it sets the solver budgets); a step
+    # whose solves miss their budget is redone host-polled
+    assert ra["graph_steps"] >= 3 and rb["graph_steps"] == 0
+    assert ra["graph_steps"] + ra["sync_steps"] == 6
+    ma, mb = ga.get_mag(), gb.get_mag()
+    assert rel_l2(ma, mb) < 1e-9
+    ga.close()
+    gb.close()
+    for s in (3, 60):
+        fc = FieldContext(prob, tol=1e-12)
+        try:
+            fc.set_hext(H[:, s])
+            fc.set_mag(np.ascontiguousarray(m0[:, :, s].reshape(-1)))
+            fc.run_rk4(dtau, 6, use_graph=False)
+            e = rel_l2(from_batch(ma, s), fc.get_mag())
+            print(f"sample {s}: batched graph RK4 vs single-sample RK4 {e:.2e}", flush=True)
+            assert e < 1e-8
+        finally:
+            fc.close()

@@ -33,6 +33,45 @@ def test_disk_delaunay_mesh():
     assert np.array_equal(mesh.xyz, mesh2.xyz) and np.array_equal(mesh.fel, mesh2.fel)  # seeded
 
 
+def _lam_max_h2(mesh, h):
+    """Largest eigenvalue of V^-1/2 K V^-1/2 (the exchange operator's
+    stiffest mode, which bounds explicit time steps) times h^2."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import eigsh
+    o = M.fem_operators(mesh)
+    n = mesh.n
+    K = sp.csr_matrix((o.K, o.col, o.rowptr), shape=(n, n))
+    d = sp.diags(1.0 / np.sqrt(o.nodevol))
+    return eigsh(d @ K @ d, k=1, which="LA", return_eigenvectors=False)[0] * h * h
+
+
+def test_disk_prism_mesh():
+    h = 2.0
+    mesh = M.disk_prism(40.0, 4.0, h, 0.15, 3)
+    x, f = mesh.xyz, mesh.fel
+    vol = np.einsum("ij,ij->i", x[f[:, 1]] - x[f[:, 0]], np.cross(x[f[:, 2]] - x[f[:, 0]], x[f[:, 3]] - x[f[:, 0]])) / 6
+    assert np.all(vol > 0.02 * h ** 3)  # no slivers
+    exact = np.pi * 20.0 ** 2 * 4.0
+    assert abs(vol.sum() - exact) < 0.01 * exact  # polygonal rim
+    assert sorted(set(np.round(x[:, 2], 12))) == [0.0, 2.0, 4.0]
+    # conforming: every face is shared by two tetrahedra or lies on the surface
+    from collections import Counter
+    faces = Counter()
+    for t in f:
+        for q in ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)):
+            faces[tuple(sorted(int(t[i]) for i in q))] += 1
+    assert max(faces.values()) == 2
+    bnd = M.boundary(mesh)
+    assert bnd.nbel == sum(1 for v in faces.values() if v == 1)
+    assert _closed_surface(bnd)
+    mesh2 = M.disk_prism(40.0, 4.0, h, 0.15, 3)
+    assert np.array_equal(mesh.xyz, mesh2.xyz) and np.array_equal(mesh.fel, mesh2.fel)  # seeded
+    # the stiffest exchange mode stays at the lattice value (the 3D Delaunay
+    # disk of the same points has slivers and a far stiffer one)
+    assert _lam_max_h2(mesh, h) < 60.0
+    assert _lam_max_h2(M.disk_delaunay(40.0, 4.0, h, 0.15, 3), h) > 2 * _lam_max_h2(mesh, h)
+
+
 def test_hysteresis_sweep_batch():
     prob, H, m0 = P.disk_hysteresis(h=6.0, nsamples=16)
     Bk = np.linalg.norm(H, axis=0) * MU0 * np.sign(H[0])

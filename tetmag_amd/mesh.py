@@ -138,6 +138,58 @@ def disk_delaunay(diameter: float, thickness: float, h: float, jitter: float = 0
     return Mesh(np.ascontiguousarray(P[used]), np.ascontiguousarray(remap[T].astype(np.int32)), scale)
 
 
+def disk_prism(diameter: float, thickness: float, h: float, jitter: float = 0.15, seed: int = 0,
+               scale: float = 1e-9) -> Mesh:
+    """Layered tetrahedral mesh of a disk (cylinder of the given diameter and
+    thickness, mesh units), the usual mesh of a thin film: a 2D Delaunay
+    triangulation of the cross-section (a square lattice of spacing h,
+    jittered by +-jitter*h, plus a ring of points on the rim) extruded into
+    round(thickness / h) layers of prisms, each prism split into three
+    tetrahedra.  A 2D Delaunay triangulation maximises the smallest angle, so
+    the mesh has no slivers (a 3D Delaunay of a jittered lattice has many,
+    whose stiffness entries make the exchange operator, and so the explicit
+    time step, hopeless).  The split of a prism over the triangle a < b < c
+    (node numbers) into (a,b,c,c'), (a,b,b',c'), (a,a',b',c') takes each
+    side quad's diagonal from its lowest-numbered node, so neighbouring
+    prisms share faces.  Nodes are numbered column by column (z fastest),
+    columns in raster order of the cross-section: a node's neighbours lie
+    within a few rows of the raster."""
+    from scipy.spatial import Delaunay
+    R = 0.5 * diameter
+    nz = max(1, int(round(thickness / h)))
+    zs = np.linspace(0.0, thickness, nz + 1)
+    g = np.arange(-R, R + 1e-9 * h, h)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    X, Y = X.reshape(-1), Y.reshape(-1)
+    inside = X * X + Y * Y < (R - 0.5 * h) ** 2
+    X, Y = X[inside], Y[inside]
+    u = uniform(seed, 2 * X.size)
+    X = X + jitter * h * (2 * u[:X.size] - 1)
+    Y = Y + jitter * h * (2 * u[X.size:] - 1)
+    nr = max(8, int(np.ceil(2 * np.pi * R / h)))
+    ang = 2 * np.pi * np.arange(nr) / nr
+    P2 = np.concatenate([np.stack([X, Y], axis=1), np.stack([R * np.cos(ang), R * np.sin(ang)], axis=1)])
+    P2 = P2[np.lexsort((P2[:, 0], np.round(P2[:, 1] / h).astype(np.int64)))]  # raster order
+    T2 = Delaunay(P2).simplices.astype(np.int64)
+    e1, e2 = P2[T2[:, 1]] - P2[T2[:, 0]], P2[T2[:, 2]] - P2[T2[:, 0]]
+    T2 = T2[np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]) > 1e-9 * h * h]
+    T2 = np.sort(T2, axis=1)
+    n2, L = P2.shape[0], nz + 1
+    xyz = np.empty((n2 * L, 3))
+    xyz[:, 0] = np.repeat(P2[:, 0], L)
+    xyz[:, 1] = np.repeat(P2[:, 1], L)
+    xyz[:, 2] = np.tile(zs, n2)
+    tets = []
+    for k in range(nz):
+        a, b, c = T2[:, 0] * L + k, T2[:, 1] * L + k, T2[:, 2] * L + k
+        tets += [np.stack([a, b, c, c + 1], 1), np.stack([a, b, b + 1, c + 1], 1), np.stack([a, a + 1, b + 1, c + 1], 1)]
+    T = np.concatenate(tets)
+    p = xyz[T]
+    vol = np.einsum("ij,ij->i", p[:, 1] - p[:, 0], np.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]))
+    T[vol < 0] = T[vol < 0][:, [0, 2, 1, 3]]
+    return Mesh(np.ascontiguousarray(xyz), np.ascontiguousarray(T.astype(np.int32)), scale)
+
+
 def fem_operators(mesh: Mesh) -> Operators:
     L = N.lib()
     n, nt = mesh.n, mesh.ntet

@@ -222,12 +222,12 @@ def neel_skyrmion(mesh: M.Mesh, radius: float, chirality: float = 1.0) -> np.nda
 
 
 def disk_hysteresis(seed: int = 2027, h: float = 2.5, nsamples: int = 64):
-    """Disk 200 nm x 10 nm, seeded jittered Delaunay at target edge h; the 20x20x4
+    """Disk 200 nm x 10 nm, layered prism mesh at edge h (mesh.disk_prism); the 20x20x4
     cuboid's material; `nsamples` states, each under B_ext = mu0 H_k (cos t_i, sin t_i, 0)
     with mu0 H_k spaced from -80 to +80 mT and t_i seeded in [0, 5 deg],
     starting from a slightly perturbed saturated state along x.
     Returns (problem, fields (3, S) in A/m, m0 (3, n, S))."""
-    mesh = M.disk_delaunay(200.0, 10.0, h, 0.15, seed)
+    mesh = M.disk_prism(200.0, 10.0, h, 0.15, seed)
     prob = mesh_problem(f"disk-200x10-h{h}", mesh, Material(), "random", seed)
     u = M.uniform(seed + 11, nsamples)
     th = np.deg2rad(5.0) * u
@@ -243,10 +243,10 @@ def disk_hysteresis(seed: int = 2027, h: float = 2.5, nsamples: int = 64):
 
 
 def disk_skyrmion(seed: int = 2028, h: float = 0.6) -> Problem:
-    """Disk 120 nm x 1 nm (seeded jittered Delaunay, edge h) with interfacial
+    """Disk 120 nm x 1 nm (layered prism mesh at edge h, mesh.disk_prism) with interfacial
     DMI 3e-3 J/m^2, Ku 8e5 J/m^3 along z, Ms 1.1e6 A/m, A 1.6e-11 J/m,
     alpha 0.3; Neel skyrmion of radius 15 nm."""
-    mesh = M.disk_delaunay(120.0, 1.0, h, 0.15, seed)
+    mesh = M.disk_prism(120.0, 1.0, h, 0.15, seed)
     mat = Material(Ms=1.1e6, A=1.6e-11, Ku=8e5, ku_axis=(0.0, 0.0, 1.0), alpha=0.3, Di=3e-3)
     return mesh_problem(f"disk-120x1-h{h}", mesh, mat, neel_skyrmion(mesh, 15.0), seed)
 
