@@ -253,12 +253,18 @@ __device__ inline f4 shfl_xor4(f4 v, int m) {
 }
 
 // Row walk as slot_walk (XCD-contiguous eighths).  BLK = false: one row per
-// wavefront per step.  BLK = true (operators of long rows: the coarse
-// levels' Galerkin products, the restrictions): one row per block, its
-// 16-slot chunks dealt round-robin to the block's 4 wavefronts, the
-// wavefronts' sums added in LDS.  ld(row, j) loads the epilogue's operands
-// for samples 4j .. 4j+3; epi(row, j, acc, ld) runs on lanes 0-15 (of
-// wavefront 0 with BLK).
+// lane group, four consecutive rows per wavefront step; lane j of a group
+// holds samples 4j .. 4j+3 of its row and walks all of the row's slots
+// itself (per chunk of CH slots: the offsets and weights as CH/4 16-byte
+// loads, then all CH gathers before any is used), so a wavefront has four
+// rows' gathers in flight and no cross-group sum.  BLK = true (operators of
+// long rows: the coarse levels' Galerkin products, the restrictions): one
+// row per block, its 16-slot chunks dealt round-robin to the block's 4
+// wavefronts (lane group g: slots q+4g .. q+4g+3 of chunk q), the groups'
+// and wavefronts' sums added across lanes and in LDS.  ld(row, j) loads the
+// epilogue's operands for samples 4j .. 4j+3; epi(row, j, acc, ld) runs on
+// every lane of a group whose row exists (BLK = false), on lanes 0-15 of
+// wavefront 0 (BLK = true).
 template <bool BLK, int CH = kSlot, typename LDF, typename EPIF>
 __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPIF&& epi) {
     static_assert(CH == 16 || CH == 8, "chunks of 16 or 8 slots");
@@ -268,7 +274,6 @@ __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPI
     const __amdgpu_buffer_rsrc_t xr = gather_rsrc(x, M.xbytes);
     const i4* __restrict__ of = (const i4*)M.off;
     const f4* __restrict__ w = (const f4*)M.w;
-    const cint_p sp = (cint_p)M.sp;
     const int vo = j * 16;
     int r0 = 0, r1 = n, lb = blockIdx.x, gx = gridDim.x;
     if (gridDim.x % kXcd == 0) {
@@ -278,8 +283,46 @@ __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPI
         r0 = (int)((int64_t)n * xi / kXcd);
         r1 = (int)((int64_t)n * (xi + 1) / kXcd);
     }
-    auto chunk = [&](int q, f4 acc) {
-        if constexpr (CH == 16) {  // lane group g: slots q+4g .. q+4g+3
+    if constexpr (!BLK) {
+        const int* __restrict__ sp = M.sp;
+        const int stride = gx * kRowsPerBlock * 4;
+        for (int base = __builtin_amdgcn_readfirstlane(r0 + (lb * kRowsPerBlock + wv) * 4); base < r1;
+             base += stride) {
+            const int row = base + g;
+            const bool has = row < r1;
+            const int rr = has ? row : base;
+            const int a0 = sp[rr], a1 = has ? sp[rr + 1] : a0;
+            const auto lv = ld(rr, j);
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int q = a0; q < a1; q += CH) {
+                i4 o[CH / 4];
+                f4 ww[CH / 4];
+#pragma unroll
+                for (int k = 0; k < CH / 4; ++k) {
+                    o[k] = of[(q >> 2) + k];
+                    ww[k] = w[(q >> 2) + k];
+                }
+                f4 xv[CH];
+#pragma unroll
+                for (int k = 0; k < CH / 4; ++k) {
+                    xv[4 * k] = ld4_if(xr, vo + o[k].x, ww[k].x);
+                    xv[4 * k + 1] = ld4_if(xr, vo + o[k].y, ww[k].y);
+                    xv[4 * k + 2] = ld4_if(xr, vo + o[k].z, ww[k].z);
+                    xv[4 * k + 3] = ld4_if(xr, vo + o[k].w, ww[k].w);
+                }
+#pragma unroll
+                for (int k = 0; k < CH / 4; ++k) {
+                    acc += ww[k].x * xv[4 * k];
+                    acc += ww[k].y * xv[4 * k + 1];
+                    acc += ww[k].z * xv[4 * k + 2];
+                    acc += ww[k].w * xv[4 * k + 3];
+                }
+            }
+            if (has) epi(row, j, acc, lv);
+        }
+    } else {
+        static_assert(!BLK || CH == 16, "long rows in chunks of 16");
+        auto chunk = [&](int q, f4 acc) {  // lane group g: slots q+4g .. q+4g+3
             const i4 o = of[(q >> 2) + g];
             const f4 ww = w[(q >> 2) + g];
             const f4 x0 = ld4_if(xr, vo + o.x, ww.x), x1 = ld4_if(xr, vo + o.y, ww.y);
@@ -288,27 +331,9 @@ __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPI
             acc += ww.y * x1;
             acc += ww.z * x2;
             acc += ww.w * x3;
-        } else {  // lane group g: slots q+2g, q+2g+1
-            const i2 o = ((const i2*)M.off)[(q >> 1) + g];
-            const f2 ww = ((const f2*)M.w)[(q >> 1) + g];
-            const f4 x0 = ld4_if(xr, vo + o.x, ww.x), x1 = ld4_if(xr, vo + o.y, ww.y);
-            acc += ww.x * x0;
-            acc += ww.y * x1;
-        }
-        return acc;
-    };
-    if constexpr (!BLK) {
-        const int stride = gx * kRowsPerBlock;
-        for (int row = __builtin_amdgcn_readfirstlane(r0 + lb * kRowsPerBlock + wv); row < r1; row += stride) {
-            const int a0 = sp[row], a1 = sp[row + 1];
-            const auto lv = ld(row, j);
-            f4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int q = a0; q < a1; q += CH) acc = chunk(q, acc);
-            acc += shfl_xor4(acc, 16);
-            acc += shfl_xor4(acc, 32);
-            if (g == 0) epi(row, j, acc, lv);
-        }
-    } else {
+            return acc;
+        };
+        const cint_p sp = (cint_p)M.sp;
         __shared__ f4 red[kRowsPerBlock][16];
         for (int row = r0 + lb; row < r1; row += gx) {  // block-uniform
             const int a0 = sp[row], a1 = sp[row + 1];
@@ -330,10 +355,17 @@ __device__ inline void wide_walk(const SlotArgs& M, const void* x, LDF&& ld, EPI
     }
 }
 
-// part[s*G + block] for samples 4j + c held by lanes j < 16 (v[c])
-__device__ inline void b_partial4(double* part, const double (&v)[4]) {
+// part[s*G + block] for samples 4j + c, v[c] held by the lanes j, j + 16,
+// j + 32, j + 48 (summed across the four lane groups first)
+__device__ inline void b_partial4(double* part, const double (&vin)[4]) {
     __shared__ double sp[kRowsPerBlock][kWave];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    double v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        v[c] = vin[c] + __shfl_xor(vin[c], 16);
+        v[c] += __shfl_xor(v[c], 32);
+    }
     if (lane < 16)
 #pragma unroll
         for (int c = 0; c < 4; ++c) sp[wv][4 * lane + c] = v[c];
