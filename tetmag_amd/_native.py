@@ -193,6 +193,7 @@ _SIGNATURES = {
     "tm_demag_last_potentials": [_P, _P, _P, _I],
     "tm_eval_rhs": [_P, _P, _I, _D, _I, _P, _I, _P, _I, _P],
     "tm_mesh_kuhn_cuboid_size": [_I, _I, _I, _P, _P],
+    "tm_mesh_locality_order": [_I, _P, _P],
     "tm_mesh_kuhn_cuboid": [_I, _I, _I, _D, _D, _U64, _P, _P],
     "tm_fem_pattern": [_I, _I, _P, _P],
     "tm_fem_assemble": [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],

@@ -158,6 +158,44 @@ int tm_mesh_kuhn_cuboid(int nx, int ny, int nz, double h, double jitter, uint64_
     return TM_OK;
 }
 
+// Node numbering for gather locality: nodes sorted along a Z-order (Morton)
+// curve through a cubic grid of 1024 cells over the largest extent of the
+// bounding box, the axis of smallest extent varying fastest.  Nodes close in
+// space get close numbers, so the rows a kernel walks in sequence gather
+// nearby rows that are still in cache.
+int tm_mesh_locality_order(int n, const double* xyz, int* perm) {
+    if (n < 0 || (n > 0 && (!xyz || !perm))) return TM_ERR_BAD_ARG;
+    if (n == 0) return TM_OK;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = xyz[d];
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            const double v = xyz[3 * (size_t)i + d];
+            if (!std::isfinite(v)) return TM_ERR_BAD_ARG;
+            lo[d] = std::min(lo[d], v);
+            hi[d] = std::max(hi[d], v);
+        }
+    double ext = 0.0;
+    for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d]);
+    const double cell = ext > 0.0 ? ext / 1024.0 : 1.0;
+    int axis[3] = {0, 1, 2};  // bit order: smallest extent first (fastest)
+    std::sort(axis, axis + 3, [&](int a, int b) { return hi[a] - lo[a] < hi[b] - lo[b]; });
+    std::vector<std::pair<uint64_t, int>> key((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        uint64_t q[3];
+        for (int d = 0; d < 3; ++d)
+            q[d] = (uint64_t)std::min(1023.0, std::floor((xyz[3 * (size_t)i + d] - lo[d]) / cell));
+        uint64_t code = 0;
+        for (int bit = 0; bit < 10; ++bit)
+            for (int k = 0; k < 3; ++k) code |= ((q[axis[k]] >> bit) & 1u) << (3 * bit + k);
+        key[(size_t)i] = {code, i};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint64_t, int>& a, const std::pair<uint64_t, int>& b) { return a.first < b.first; });
+    for (int k = 0; k < n; ++k) perm[k] = key[(size_t)k].second;
+    return TM_OK;
+}
+
 int tm_fem_pattern(int n, int ntet, const int* fel, int* rowptr) {
     if (n <= 0 || ntet <= 0 || !fel || !rowptr) return TM_ERR_BAD_ARG;
     std::vector<int> ptr, idx;

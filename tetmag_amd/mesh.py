@@ -85,6 +85,24 @@ def kuhn_cuboid(nx: int, ny: int, nz: int, h: float, jitter: float = 0.0, seed: 
     return Mesh(xyz, fel, scale)
 
 
+def locality_order(mesh: Mesh) -> np.ndarray:
+    """perm[k] = the old number of new node k along a Z-order curve of the
+    coordinates (tm_mesh_locality_order): spatially close nodes get close
+    numbers, which keeps the batched gather kernels' rows in cache."""
+    perm = np.empty(mesh.n, np.int32)
+    N.check(N.lib().tm_mesh_locality_order(mesh.n, N.ptr(N.f64(mesh.xyz)), N.ptr(perm)), "tm_mesh_locality_order")
+    return perm
+
+
+def renumber(mesh: Mesh, perm: np.ndarray) -> Mesh:
+    """The same mesh with node k = old node perm[k]."""
+    perm = np.asarray(perm, np.int64)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.shape[0])
+    return Mesh(np.ascontiguousarray(mesh.xyz[perm]), np.ascontiguousarray(inv[mesh.fel].astype(np.int32)),
+                mesh.scale)
+
+
 def disk_delaunay(diameter: float, thickness: float, h: float, jitter: float = 0.15, seed: int = 0,
                   scale: float = 1e-9) -> Mesh:
     """Seeded jittered Delaunay tetrahedralisation of a disk (cylinder of the

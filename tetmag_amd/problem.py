@@ -159,8 +159,13 @@ def mesh_problem(name: str, mesh: M.Mesh, mat: Material | None = None, init="ran
 
 
 def cuboid_problem(name: str, nx: int, ny: int, nz: int, h: float, jitter: float = 0.0, seed: int = 0,
-                   mat: Material | None = None, init: str = "random") -> Problem:
-    return mesh_problem(name, M.kuhn_cuboid(nx, ny, nz, h, jitter, seed), mat, init, seed)
+                   mat: Material | None = None, init: str = "random", locality: bool = False) -> Problem:
+    """Kuhn cuboid; its nodes numbered row by row (x fastest), or along a
+    Z-order curve (mesh.locality_order) with locality=True."""
+    mesh = M.kuhn_cuboid(nx, ny, nz, h, jitter, seed)
+    if locality:
+        mesh = M.renumber(mesh, M.locality_order(mesh))
+    return mesh_problem(name, mesh, mat, init, seed)
 
 
 def cuboid_20x20x4(seed: int = 2024) -> Problem:
@@ -169,12 +174,13 @@ def cuboid_20x20x4(seed: int = 2024) -> Problem:
     return cuboid_problem("cuboid-20x20x4-h5", 20, 20, 4, 5.0, 0.0, seed, Material(), "random")
 
 
-def cuboid_128x128x16(seed: int = 2025) -> Problem:
+def cuboid_128x128x16(seed: int = 2025, locality: bool = False) -> Problem:
     """Kuhn cuboid 128x128x16 cells at 3 nm with +-0.1h interior jitter:
     282,897 nodes, 1,572,864 tets, 40,962 boundary nodes; the 20x20x4 cuboid's material plus
-    Ku = 5e4 J/m^3 along a seeded unit axis; vortex initial m."""
+    Ku = 5e4 J/m^3 along a seeded unit axis; vortex initial m.  locality:
+    nodes numbered along a Z-order curve instead of row by row."""
     mat = Material(Ku=5e4, ku_axis=_unit_axis(seed + 7))
-    return cuboid_problem("cuboid-128x128x16-h3", 128, 128, 16, 3.0, 0.1, seed, mat, "vortex")
+    return cuboid_problem("cuboid-128x128x16-h3", 128, 128, 16, 3.0, 0.1, seed, mat, "vortex", locality)
 
 
 def smooth_random_fields(mesh: M.Mesh, count: int, seed: int, modes: int = 8, lam_min: float = 20.0,
