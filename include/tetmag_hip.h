@@ -476,10 +476,11 @@ int tm_batch_gemm_info(tm_batch* b, int* solution, int* own_kernel, int* candida
  * The batched form of DemagField::denseMVP (DemagField.cpp:145-147). */
 int tm_batch_bem_apply(tm_batch* b, const double* x, int xwhere, double* y, int ywhere);
 /* Kernel timing of a batch (no reference counterpart): the hot kernels of an
- * evaluation -- the divergence, the fused field + LLG kernel, and of the
- * Neumann solve the level-0 kernels of one inner CG iteration (SpMM, the
- * pre-smoothing restriction, the prolongation, the post-smoother) and the
- * FP64 outer residual -- each launched `reps` times alone on the batch's
+ * evaluation -- the divergence, the fused field + LLG kernel, the tuned
+ * boundary GEMM (FP64 forms), and of the Neumann solve the level-0 kernels
+ * of one inner CG iteration (SpMM, update, the pre-smoothing restriction,
+ * the prolongation, the post-smoother) and the FP64 outer residual -- each
+ * launched `reps` times alone on the batch's
  * stream (all samples active) and timed with HIP events.  For each: its
  * name, the mean ms per launch and its algorithmic bytes per launch (slot
  * offsets and weights read once, every vector read or written once, the

@@ -2911,6 +2911,16 @@ int tm_batch_time_kernels(tm_batch* bt, int reps, tm_batch_kernel_time* out, int
             (void)launch_fields(bt, 0.5, TM_LLG_CLASSIC, bt->dmdt.p, nullptr, false);
         }),
         eslots * (4 + 8.0 * nw) + 3 * nS * 8 + nS * 8 + 3 * nS * 8 + per_node);
+    // the boundary GEMM as the batch runs it (the tuned kernel, without the
+    // gather and scatter); bytes: B once plus U and the product
+    if (bt->prec != TM_PREC_MIXED_BEM32) {
+        const double bb = (double)bt->cv.bnx;
+        add(bt->gemm_own ? "boundary GEMM (k_b_gemm64, tuned)" : "boundary GEMM (rocBLAS, tuned)", timeit([&] {
+                if (bt->gemm_own) (void)gemm_own_run(bt, nullptr);
+                else (void)gemm_run(bt, bt->gemm_sol);
+            }),
+            8.0 * bb * bb + 2.0 * 8.0 * bb * S);
+    }
     int st = bt->prec == TM_PREC_FP64 ? bt->sN64.time_kernels(s, timeit, v) : bt->sN32.time_kernels(s, timeit, v);
     if (status == TM_OK) status = st;
     (void)hipEventDestroy(e0);
