@@ -210,8 +210,6 @@ def measured_traffic(workload, b):
         return None
     if d.get("b") != b:
         return None
-    if not own:
-        return d.get("rocblas", {}).get("hbm_bytes_per_product", {}).get("total")
     return d["hbm_bytes_per_product"]["total"]
 
 
