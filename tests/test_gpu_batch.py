@@ -16,7 +16,10 @@ def base(gpu_available):
     prob = P.small_problem(nx=24, ny=20, nz=6, h=3.0, jitter=0.1, seed=41, Ku=5e4, D=2e-3,
                            B_ext=(0.01, 0.0, -0.02))
     ms = [prob.m0] + P.smooth_random_fields(prob.mesh, 4, seed=3, lam_min=10.0, lam_max=60.0)
-    ctx = FieldContext(prob, tol=1e-12)
+    # solves to 1e-13: two independently converged solutions differ by up to
+    # cond(A) times the tolerance, so batch and single-sample results are
+    # compared at 1e-8
+    ctx = FieldContext(prob, tol=1e-13)
     yield prob, ctx, ms
     ctx.close()
 
@@ -29,7 +32,7 @@ def test_batch_matches_single_sample(base, prec):
     bc = BatchContext(ctx, len(ms), N.TM_PREC_FP64 if prec == "fp64" else N.TM_PREC_MIXED)
     dm, H, st = bc.eval_rhs_host(to_batch(ms))
     print(prec, st)
-    tol = 1e-9 if prec == "fp64" else 1e-6  # mixed: FP32 inner CG and FP32 boundary product
+    tol = 1e-8 if prec == "fp64" else 1e-6  # mixed: FP32 inner CG and FP32 boundary product
     for s, m in enumerate(ms):
         d1, h1, st1 = ctx.eval_rhs_host(m)
         assert rel_l2(from_batch(H, s), h1) < tol
@@ -123,11 +126,11 @@ def test_batch_rk4_matches_single_samples(base):
     bc.close()
     assert st["max_relres_neumann"] <= 1e-12 and rk["sync_steps"] == 4
     for s in range(S):
-        fc = FieldContext(prob, tol=1e-12)
+        fc = FieldContext(prob, tol=1e-13)
         fc.set_hext(Hs[:, s])  # the per-sample field replaces the context's uniform one
         fc.set_mag(ms[1 + s])
         fc.run_rk4(dtau, 4, use_graph=False)
-        assert rel_l2(from_batch(mb, s), fc.get_mag()) < 1e-9
+        assert rel_l2(from_batch(mb, s), fc.get_mag()) < 1e-8
         fc.close()
 
 
@@ -168,6 +171,37 @@ def test_batch_rk4_graph_matches_eager(base, prec):
     gb.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_trajectory_with_kept_solutions(base, prec):
+    """Euler steps of a batch and of the single-sample path, both starting
+    their solves from kept earlier solutions: the states agree, and the
+    solves after the first few need fewer iterations than the first."""
+    from tetmag_amd import _native as N
+    from tetmag_amd.batch import BatchContext, to_batch, from_batch
+    prob, ctx, ms = base
+    S = len(ms)
+    dtau = 20e-3 * prob.time_scale()
+    bc = BatchContext(ctx, S, N.TM_PREC_FP64 if prec == "fp64" else N.TM_PREC_MIXED)
+    bc.set_mag(to_batch(ms))
+    iters = []
+    for _ in range(8):
+        bc.eval_rhs(None, stats=True)
+        st = bc.last_stats.as_dict()
+        iters.append(st["inner_neumann"] + st["inner_dirichlet"])
+        assert st["max_relres_neumann"] <= 1e-12 and st["max_relres_dirichlet"] <= 1e-12
+        bc.step_euler(dtau)
+    mb = bc.get_mag()
+    bc.close()
+    print(prec, "inner iterations per evaluation", iters)
+    assert max(iters[5:]) < iters[0]
+    for s in (0, S - 1):
+        ctx.set_mag(ms[s])
+        for _ in range(8):
+            ctx.eval_rhs(None)
+            ctx.step_euler(dtau)
+        assert rel_l2(from_batch(mb, s), ctx.get_mag()) < 1e-8
+
+
 @pytest.mark.parametrize("prec,S", [("fp64", 64), ("mixed", 64), ("fp64", 3), ("bem32", 64)])
 def test_batch_bem_apply(base, prec, S):
     """tm_batch_bem_apply against the dense FP64 product B X on the host: the
@@ -206,7 +240,7 @@ def test_wide_batch_medium_mesh():
     from tetmag_amd.context import FieldContext
     prob = P.small_problem(nx=48, ny=40, nz=10, h=3.0, jitter=0.1, seed=52, Ku=5e4, D=2e-3,
                            B_ext=(0.0, 0.02, 0.0))
-    ctx = FieldContext(prob, tol=1e-12)
+    ctx = FieldContext(prob, tol=1e-13)
     print("AMG levels", ctx.amg_levels(0), ctx.amg_levels(1))
     ms = P.smooth_random_fields(prob.mesh, 64, seed=8, lam_min=10.0, lam_max=80.0)
     bc = BatchContext(ctx, 64, N.TM_PREC_MIXED)
@@ -214,8 +248,8 @@ def test_wide_batch_medium_mesh():
     assert st["max_relres_neumann"] <= 1e-12 and st["max_relres_dirichlet"] <= 1e-12
     for s in (0, 21, 42, 63):
         d1, h1, _ = ctx.eval_rhs_host(ms[s])
-        assert rel_l2(from_batch(H, s), h1) < 1e-9
-        assert rel_l2(from_batch(dm, s), d1) < 1e-9
+        assert rel_l2(from_batch(H, s), h1) < 1e-8
+        assert rel_l2(from_batch(dm, s), d1) < 1e-8
     # the kernel timing hook: every hot kernel timed, and the next evaluation
     # is unaffected by the state the timing runs leave behind
     kt = bc.time_kernels(3)

@@ -71,6 +71,35 @@ def test_amg_warm_start_and_fewer_iterations(mats):
     assert it["amg"] * 4 < it["jacobi"]
 
 
+def test_successive_right_hand_sides_reuse_earlier_solutions(mats):
+    """b_k = b0 + 0.1 k b1: once two solutions are kept they span every later
+    right-hand side, so the later solves start at the answer; every result
+    still meets the tolerance on its own residual."""
+    from tetmag_amd.gpu import GPUSolver
+    o, m = mats
+    rp, col, val = o.neumann_matrix()
+    A = _csr(rp, col, val, o.n)
+    rng = np.random.default_rng(5)
+    b0, b1 = rng.normal(size=o.n), rng.normal(size=o.n)
+    b0[o.n // 2] = b1[o.n // 2] = 0.0
+    s = GPUSolver()
+    s.setMatrix(A)
+    s.setCGTolerance(1e-10)
+    s.definePreconditioner("amg")
+    assert s.compute()
+    its = []
+    for k in range(6):
+        b = b0 + 0.1 * k * b1
+        s.setLoadVector(b)
+        s.solve()
+        assert s.wasSuccessful()
+        x = s.result()
+        assert np.linalg.norm(b - A @ x) <= 1.01e-10 * np.linalg.norm(b)
+        its.append(s.iters)
+    print("iterations", its)
+    assert its[0] > 5 and max(its[3:]) <= 1
+
+
 def test_nonsymmetric_matrix_is_rejected(gpu_available):
     from tetmag_amd.gpu import GPUSolver
     A = sp.csr_matrix(np.array([[4.0, 1.0, 0.0], [0.5, 4.0, 1.0], [0.0, 1.0, 4.0]]))

@@ -59,6 +59,168 @@ __device__ inline void block_partials(double* part, int stride, double v0, doubl
     }
 }
 
+// ---- initial guesses from earlier solves ---------------------------------
+// As the batched solver: x0 = sum_i c_i x_i over the kept pairs, c minimising
+// ||b - sum_i c_i b_i|| (P. F. Fischer, Comput. Methods Appl. Mech. Engrg.
+// 163 (1998) 193, in the residual norm).  Slots are ordered newest first.
+__device__ inline int cg_win_slot(const CgWindow* w, int i) {
+    return (w->head - 1 - i + 2 * kSolveWindow) % kSolveWindow;
+}
+
+// per-block partials: wpart[i * G + block] = b_slot(i) . b (i < cnt), i = kSolveWindow: b . b
+__global__ __launch_bounds__(kBlock) void k_cg_win_dots(int n, const CgWindow* w, const double* __restrict__ Bw,
+                                                        const double* __restrict__ b, double* wpart) {
+    const int cnt = w->cnt;
+    double v[kSolveWindow + 1] = {};
+    const double* src[kSolveWindow];
+#pragma unroll
+    for (int i = 0; i < kSolveWindow; ++i) src[i] = Bw + (size_t)cg_win_slot(w, i) * n;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+        const double bi = b[e];
+        v[kSolveWindow] += bi * bi;
+#pragma unroll
+        for (int i = 0; i < kSolveWindow; ++i)
+            if (i < cnt) v[i] += src[i][e] * bi;
+    }
+    __shared__ double sp[kSolveWindow + 1][kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int i = 0; i <= kSolveWindow; ++i) {
+        const double t = wave_sum(v[i]);
+        if (lane == 0) sp[i][wid] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x <= kSolveWindow) {
+        double a = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) a += sp[threadIdx.x][k];
+        wpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
+    }
+}
+
+// one block: the dots, then c from the Gram system (Cholesky dropping
+// directions whose pivot falls below 1e-14 of the largest diagonal)
+__global__ __launch_bounds__(kBlock) void k_cg_win_solve(int G, CgWindow* w, const double* wpart) {
+    __shared__ double d[kSolveWindow + 1];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    for (int i = wid; i <= kSolveWindow; i += kBlock / kWave) {  // wavefront wid: dots wid, wid + 4, ...
+        double t = 0.0;
+        for (int g = lane; g < G; g += kWave) t += wpart[(size_t)i * G + g];
+        t = wave_sum(t);
+        if (lane == 0) d[i] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int cnt = w->cnt;
+    bool finite = std::isfinite(d[kSolveWindow]);
+    for (int i = 0; i < cnt; ++i) finite = finite && std::isfinite(d[i]);
+    for (int i = 0; i < cnt; ++i) w->pend[i] = d[i];
+    w->pend[cnt] = d[kSolveWindow];
+    double c[kSolveWindow] = {};
+    if (finite && cnt > 0) {
+        double L[kSolveWindow][kSolveWindow] = {};
+        bool keep[kSolveWindow] = {};
+        double gmax = 0.0;
+        for (int i = 0; i < cnt; ++i) gmax = fmax(gmax, w->gram[cg_win_slot(w, i)][cg_win_slot(w, i)]);
+        for (int i = 0; i < cnt; ++i) {
+            const int si = cg_win_slot(w, i);
+            for (int j = 0; j <= i; ++j) {
+                if (j < i && !keep[j]) continue;
+                double a = w->gram[si][cg_win_slot(w, j)];
+                for (int k = 0; k < j; ++k)
+                    if (keep[k]) a -= L[i][k] * L[j][k];
+                if (j < i) {
+                    L[i][j] = a / L[j][j];
+                } else {
+                    keep[i] = a > 1e-14 * gmax && std::isfinite(a);
+                    L[i][i] = keep[i] ? sqrt(a) : 1.0;
+                }
+            }
+        }
+        double y[kSolveWindow] = {};
+        for (int i = 0; i < cnt; ++i) {
+            if (!keep[i]) continue;
+            double a = d[i];
+            for (int k = 0; k < i; ++k)
+                if (keep[k]) a -= L[i][k] * y[k];
+            y[i] = a / L[i][i];
+        }
+        for (int i = cnt - 1; i >= 0; --i) {
+            if (!keep[i]) continue;
+            double a = y[i];
+            for (int k = i + 1; k < cnt; ++k)
+                if (keep[k]) a -= L[k][i] * c[k];
+            c[i] = a / L[i][i];
+        }
+        // keep the projection only if its residual estimate ||b - sum c_i b_i||^2
+        // beats the newest solution alone (the plain warm start)
+        double est = d[kSolveWindow], est0 = d[kSolveWindow] - 2.0 * d[0] + w->gram[cg_win_slot(w, 0)][cg_win_slot(w, 0)];
+        bool cfin = true;
+        for (int i = 0; i < cnt; ++i) {
+            cfin = cfin && std::isfinite(c[i]);
+            est -= 2.0 * c[i] * d[i];
+            for (int j = 0; j < cnt; ++j) est += c[i] * c[j] * w->gram[cg_win_slot(w, i)][cg_win_slot(w, j)];
+        }
+        if (!cfin || !(est <= est0)) {
+            for (int i = 0; i < cnt; ++i) c[i] = 0.0;
+            c[0] = 1.0;
+        }
+    }
+    for (int i = 0; i < kSolveWindow; ++i) w->c[i] = finite ? c[i] : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_win_combine(int n, const CgWindow* w, const double* __restrict__ Xw,
+                                                           double* __restrict__ x) {
+    const int cnt = w->cnt;
+    if (cnt == 0) return;
+    double c[kSolveWindow];
+    const double* src[kSolveWindow];
+#pragma unroll
+    for (int i = 0; i < kSolveWindow; ++i) {
+        c[i] = w->c[i];
+        src[i] = Xw + (size_t)cg_win_slot(w, i) * n;
+    }
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < kSolveWindow; ++i)
+            if (i < cnt) acc += c[i] * src[i][e];
+        x[e] = acc;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_win_store(int n, const CgWindow* w, const double* __restrict__ b,
+                                                         const double* __restrict__ x, double* __restrict__ Bw,
+                                                         double* __restrict__ Xw) {
+    const size_t off = (size_t)w->head * n;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+        Bw[off + e] = b[e];
+        Xw[off + e] = x[e];
+    }
+}
+
+// the stored pair's Gram row from the pending dots; a failed or non-finite
+// solve empties the window
+__global__ void k_cg_win_advance(CgWindow* w, const CgState* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int cnt = w->cnt, h = w->head;
+    bool ok = st->done == 1 && std::isfinite(w->pend[cnt]);
+    for (int i = 0; i < cnt; ++i) ok = ok && std::isfinite(w->pend[i]);
+    if (!ok) {
+        w->cnt = 0;
+        w->head = 0;
+        return;
+    }
+    for (int i = 0; i < cnt; ++i) {
+        const int j = cg_win_slot(w, i);
+        if (j == h) continue;
+        w->gram[h][j] = w->pend[i];
+        w->gram[j][h] = w->pend[i];
+    }
+    w->gram[h][h] = w->pend[cnt];
+    w->head = (h + 1) % kSolveWindow;
+    w->cnt = min(cnt + 1, kSolveWindow);
+}
+
 // part layout: [0, G) bb | [G, 2G) rz | [2G, 3G) rr | [3G, 4G) pq
 __global__ __launch_bounds__(kBlock) void k_cg_init_fixed(int n, const unsigned char* __restrict__ fixed,
                                                           const double* __restrict__ b, double* __restrict__ x,
@@ -341,6 +503,11 @@ int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* 
     TM_CHECK(p1.alloc(n));
     grid = xcd_grid(std::min(kMaxCgBlocks, (pat.nslices + kBlock / kWave - 1) / (kBlock / kWave)));
     TM_CHECK(part.alloc((size_t)4 * grid));
+    TM_CHECK(winX.alloc((size_t)kSolveWindow * n));
+    TM_CHECK(winB.alloc((size_t)kSolveWindow * n));
+    TM_CHECK(wpart.alloc((size_t)(kSolveWindow + 1) * grid));
+    TM_CHECK(win.alloc(1));
+    TM_HIP(hipMemsetAsync(win.p, 0, sizeof(CgWindow), s));
     TM_CHECK(st.alloc(1));
     TM_HIP(hipMemsetAsync(st.p, 0, sizeof(CgState), s));
     if (!h_st) TM_HIP(hipHostMalloc((void**)&h_st, sizeof(CgState), hipHostMallocDefault));
@@ -350,11 +517,27 @@ int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* 
     return TM_OK;
 }
 
-int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* iters, double* relres) {
+void CgSolver::win_project(const double* b, double* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_win_dots, dim3(grid), dim3(kBlock), 0, s, n, (const CgWindow*)win.p,
+                       (const double*)winB.p, b, wpart.p);
+    hipLaunchKernelGGL(k_cg_win_solve, dim3(1), dim3(kBlock), 0, s, grid, win.p, (const double*)wpart.p);
+    hipLaunchKernelGGL(k_cg_win_combine, dim3(grid), dim3(kBlock), 0, s, n, (const CgWindow*)win.p,
+                       (const double*)winX.p, x);
+}
+
+void CgSolver::win_record(const double* b, const double* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_win_store, dim3(grid), dim3(kBlock), 0, s, n, (const CgWindow*)win.p, b, x, winB.p,
+                       winX.p);
+    hipLaunchKernelGGL(k_cg_win_advance, dim3(1), dim3(1), 0, s, win.p, (const CgState*)st.p);
+}
+
+int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* iters, double* relres,
+                    bool project) {
     if (n == 0) return TM_ERR_NOT_READY;
     if (!(tol > 0.0) || maxit < 1) return TM_ERR_BAD_ARG;
     const int G = grid;
     const SellView A = view(pat);
+    if (project) win_project(b, x, s);
     hipLaunchKernelGGL(k_cg_init_fixed, dim3(G), dim3(kBlock), 0, s, n, fixed.p, b, x, part.p);
     const bool jac = precond == TM_PRECOND_JACOBI;
     if (jac)
@@ -399,6 +582,7 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
     }
     const CgState hs = *h_st;
     if (hs.bb == 0.0) TM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    if (project) win_record(b, x, s);
     last_iters = hs.iters;
     if (iters) *iters = hs.iters;
     if (relres) *relres = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
@@ -408,12 +592,13 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
 }
 
 int CgSolver::enqueue(const double* b, double* x, double tol, int budget, hipStream_t s, unsigned* fails,
-                      int* imax) {
+                      int* imax, bool project) {
     if (n == 0) return TM_ERR_NOT_READY;
     if (!(tol > 0.0) || budget < 1 || !fails || !imax) return TM_ERR_BAD_ARG;
     const int G = grid;
     const SellView A = view(pat);
     const bool jac = precond == TM_PRECOND_JACOBI;
+    if (project) win_project(b, x, s);
     hipLaunchKernelGGL(k_cg_init_fixed, dim3(G), dim3(kBlock), 0, s, n, fixed.p, b, x, part.p);
     if (jac)
         hipLaunchKernelGGL(k_cg_init_res<true>, dim3(G), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p, part.p,
@@ -438,6 +623,7 @@ int CgSolver::enqueue(const double* b, double* x, double tol, int budget, hipStr
         std::swap(pcur, pold);
     }
     hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kBlock), 0, s, G, budget, (const double*)part.p, st.p, fails, imax);
+    if (project) win_record(b, x, s);
     TM_HIP(hipGetLastError());
     return TM_OK;
 }
@@ -510,7 +696,9 @@ int tm_solver_solve(tm_solver* S, const double* b, int bwhere, double* x, int xw
     double* xd = S->x.p;  // the solver keeps its solution between calls (GPUSolver::x_d)
     if (warm == 1) TM_CHECK(copy_in(xd, x, xwhere, n, S->stream));
     else if (warm == 0) TM_HIP(hipMemsetAsync(xd, 0, sizeof(double) * n, S->stream));
-    int st = S->cg.solve(bd, xd, tol, maxit, S->stream, iters, relres);
+    // warm == 2 (the solver's own previous solution): start from the best
+    // combination of its last solutions instead
+    int st = S->cg.solve(bd, xd, tol, maxit, S->stream, iters, relres, warm == 2);
     int st2 = copy_out(x, xwhere, xd, n, S->stream);
     if (xwhere == TM_DEVICE) TM_HIP(hipStreamSynchronize(S->stream));
     return st != TM_OK ? st : st2;

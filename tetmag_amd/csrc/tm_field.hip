@@ -1120,7 +1120,7 @@ int demag_potential_dev(tm_ctx* c, const double* md, tm_eval_stats* stats) {
         (void)hipMemsetAsync(c->u2.p, 0, sizeof(double) * c->n, c->stream);
         c->have_u = false;
     };
-    int st = c->cgN.solve(c->rhsN.p, c->u1.p, c->tol, c->maxit, c->stream, &itN, &rN);
+    int st = c->cgN.solve(c->rhsN.p, c->u1.p, c->tol, c->maxit, c->stream, &itN, &rN, true);
     if (st == TM_ERR_NONFINITE) clear_potentials();
     if (st != TM_OK && st != TM_ERR_MAXITER) return st;
     int stN = st;
@@ -1129,7 +1129,7 @@ int demag_potential_dev(tm_ctx* c, const double* md, tm_eval_stats* stats) {
     TM_CHECK(bem_gemv(c->bnx, c->bld, c->B.p, c->u1.p, c->bnodes.p, c->bpart.p, nullptr, c->bnodes.p, c->rhsD.p,
                       c->stream));
     if (c->timers) TM_HIP(hipEventRecord(c->ev[3], c->stream));
-    st = c->cgD.solve(c->rhsD.p, c->u2.p, c->tol, c->maxit, c->stream, &itD, &rD);
+    st = c->cgD.solve(c->rhsD.p, c->u2.p, c->tol, c->maxit, c->stream, &itD, &rD, true);
     if (c->timers) TM_HIP(hipEventRecord(c->ev[4], c->stream));
     c->tsum.iters_neumann += itN;
     c->tsum.iters_dirichlet += itD;
@@ -1169,10 +1169,10 @@ int eval_device(tm_ctx* c, const double* md, double* dmdt, double alpha, int var
                                c->opv[TM_OP_TGRAD_X].p, c->opv[TM_OP_TGRAD_Y].p, c->opv[TM_OP_TGRAD_Z].p, c->js.p, md,
                                1.0 / kMu0, c->rhsN.p);
             TM_HIP(hipGetLastError());
-            TM_CHECK(c->cgN.enqueue(c->rhsN.p, c->u1.p, c->tol, budgetN, c->stream, fails, imax2));
+            TM_CHECK(c->cgN.enqueue(c->rhsN.p, c->u1.p, c->tol, budgetN, c->stream, fails, imax2, true));
             TM_CHECK(bem_gemv(c->bnx, c->bld, c->B.p, c->u1.p, c->bnodes.p, c->bpart.p, nullptr, c->bnodes.p,
                               c->rhsD.p, c->stream));
-            TM_CHECK(c->cgD.enqueue(c->rhsD.p, c->u2.p, c->tol, budgetD, c->stream, fails, imax2 + 1));
+            TM_CHECK(c->cgD.enqueue(c->rhsD.p, c->u2.p, c->tol, budgetD, c->stream, fails, imax2 + 1, true));
             c->have_u = true;
         } else {
             st = demag_potential_dev(c, md, stats);

@@ -229,6 +229,16 @@ int copy_out(double* dst, int where, const double* src_dev, size_t count, hipStr
 // ---------------------------------------------------------------------------
 // Preconditioned CG on "symmetric matrix with identity rows" systems.
 // ---------------------------------------------------------------------------
+// Solved pairs kept per solver for initial guesses (successive right-hand
+// sides; see CgWindow, tm_cg.hip, and the batched solver).
+constexpr int kSolveWindow = 4;
+struct CgWindow {
+    int head, cnt;
+    double gram[kSolveWindow][kSolveWindow];
+    double pend[kSolveWindow + 1];
+    double c[kSolveWindow];
+};
+
 struct CgState {
     double gam[2];   // (r, z) of iterations k (slot k & 1)
     double bb;       // (b, b)
@@ -285,6 +295,9 @@ struct CgSolver {
     DevBuf<double> part;     // partial sums: bb | rz | rr | pq, grid entries each
     DevBuf<CgState> st;
     CgState* h_st = nullptr;  // pinned mirror for the convergence check
+    // the last kSolveWindow solved pairs (x, b) and the initial-guess state
+    DevBuf<double> winX, winB, wpart;
+    DevBuf<CgWindow> win;
     int precond = TM_PRECOND_JACOBI;
     AmgPrecond amg;
     int nfixed = 0;
@@ -298,12 +311,19 @@ struct CgSolver {
     int setup(int n, const int* rowptr, const int* colidx, const double* vals, int precond, hipStream_t s);
     // Solves A x = b for device vectors b, x (x is the warm start).  tol is
     // relative to ||b||.  Returns TM_OK, TM_ERR_MAXITER or TM_ERR_NONFINITE.
-    int solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* iters, double* relres);
+    // project: start from the combination of the kept solutions whose
+    // right-hand sides best match b (least-squares residual) instead of x,
+    // and keep the new pair.
+    int solve(const double* b, double* x, double tol, int maxit, hipStream_t s, int* iters, double* relres,
+              bool project = false);
     // Enqueues a solve of exactly `budget` iterations with no host
     // synchronisation (capturable in a HIP graph); every kernel returns at
     // once after convergence.  At the end *fails is incremented on the device
     // if the tolerance was not met, and *imax keeps the largest count.
-    int enqueue(const double* b, double* x, double tol, int budget, hipStream_t s, unsigned* fails, int* imax);
+    int enqueue(const double* b, double* x, double tol, int budget, hipStream_t s, unsigned* fails, int* imax,
+                bool project = false);
+    void win_project(const double* b, double* x, hipStream_t s);
+    void win_record(const double* b, const double* x, hipStream_t s);
     // Solves A^T y = w for the same matrix (identity rows on the fixed set F):
     // y_free = A_ff^-1 w_free (A_ff symmetric), y_F = w_F - A_{free,F}^T y_free.
     int solve_transposed(const double* w, double* y, double tol, int maxit, hipStream_t s, int* iters,
