@@ -110,9 +110,9 @@ int tm_solver_destroy(tm_solver* s);
 /* GPUSolver::solve (GPUSolver.cu:82-86): solves A x = b to
  * ||b - A x|| <= tol ||b||.  x is the warm start on entry when warm == 1;
  * warm == 2 starts from the solver's own earlier solutions (the reference
- * keeps x_d between calls): the combination of the last four solutions
+ * keeps x_d between calls): the combination of the last six solutions
  * whose right-hand sides best match b in the least-squares sense, the last
- * solution being one of the candidates; warm == 0 starts from zero.
+ * solution alone being one of the candidates; warm == 0 starts from zero.
  * Returns TM_ERR_MAXITER when maxit is reached first (the reference caps at
  * 100 GMRES iterations silently, GPUSolver.cu:47).  iters/relres may be NULL. */
 int tm_solver_solve(tm_solver* s, const double* b, int bwhere, double* x, int xwhere, double tol, int maxit,

@@ -1620,195 +1620,113 @@ void restrict_presmoothed(const HostCsr& R, int n, const int* arp, const int* ac
 }
 
 // ---- initial guesses from earlier solves (successive right-hand sides) ----
-// The last kWin solved pairs (x_i, b_i), A x_i = b_i, are kept per solver.
-// A new solve with right-hand side b starts from x0 = sum_i c_i x_i with c
-// minimising ||b - sum_i c_i b_i|| per sample (the projection methods of
-// P. F. Fischer, Comput. Methods Appl. Mech. Engrg. 163 (1998) 193, in the
-// residual norm): along a trajectory, where consecutive right-hand sides
-// differ smoothly, the initial residual falls by orders of magnitude
-// against the previous solution alone, which is one of the candidates, so
-// it is never worse in that norm.  The tolerance test is unchanged: the
-// FP64 residual of the final iterate decides.  Window state and the small
-// Gram systems live on the device (the kernels are capturable).
-constexpr int kWin = 4;
+// Per sample as the single-sample solver: x0 = x_0 + sum_j c_j dX_j over the
+// kept pairs, dB_j / dX_j backward differences at the newest pair and c the
+// least-squares fit of b - b_0 (the method is described with kSolveWindow in
+// tm_internal.hpp).  Window state and the small Gram systems live on the
+// device (the kernels are capturable).
 struct WinState {
-    int head;                     // slot the next pair goes to
-    int cnt;                      // pairs held (<= kWin)
-    double gram[kWin][kWin][kMaxS];  // b_i . b_j per sample (slot indices)
-    double pend[kWin + 1][kMaxS];    // the last dots: b_slot(i) . b (i < cnt), b . b (i = cnt)
-    double c[kWin][kMaxS];           // coefficients of the last projection (window order, newest first)
+    int head;                            // slot the next pair goes to
+    int cnt;                             // pairs held (<= kSolveWindow)
+    double a[kSolveWindow][kMaxS];       // x0 = sum_i a_i x_slot(i) per sample (newest first)
 };
-__device__ inline int win_slot(const WinState* ws, int i) { return (ws->head - 1 - i + 2 * kWin) % kWin; }
 
-// per-block partials of b_slot(i) . b (i < cnt) and b . b, lane = sample
-__global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinState* ws, const double* __restrict__ Bw,
+// per-block partials of the kWinDots dots, lane = sample.  The new b goes
+// into the head slot as it is read: when the window is full that slot holds
+// the oldest pair, whose value this thread has already loaded.
+__global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinState* ws, double* __restrict__ Bw,
                                                        const double* __restrict__ b, double* part) {
-    const int cnt = ws->cnt;
+    const int cnt = ws->cnt, head = ws->head;
     const size_t NS = (size_t)n * S;
-    double v[kWin + 1] = {};
-    const double* src[kWin];
+    double acc[kWinDots] = {};
+    const double* src[kSolveWindow];
 #pragma unroll
-    for (int i = 0; i < kWin; ++i) src[i] = Bw + (size_t)win_slot(ws, i) * NS;
+    for (int i = 0; i < kSolveWindow; ++i) src[i] = Bw + (size_t)win_slot(head, i) * NS;
+    double* dst = Bw + (size_t)head * NS;
     B_ROW_LOOP(n) {
         if (on) {
             const size_t e = (size_t)row * S + lane;
-            const double bi = b[e];
-            v[kWin] += bi * bi;
+            double v[kSolveWindow];
 #pragma unroll
-            for (int i = 0; i < kWin; ++i)
-                if (i < cnt) v[i] += src[i][e] * bi;
+            for (int i = 0; i < kSolveWindow; ++i) v[i] = i < cnt ? src[i][e] : 0.0;
+            const double bi = b[e];
+            const double r = bi - v[0];
+            win_differences(v, cnt);
+            win_accumulate(acc, v, r);
+            dst[e] = bi;
         }
     }
 #pragma unroll
-    for (int i = 0; i <= kWin; ++i) b_partial(part + (size_t)i * S * gridDim.x, S, v[i]);
+    for (int q = 0; q < kWinDots; ++q) b_partial(part + (size_t)q * S * gridDim.x, S, acc[q]);
 }
 
-// one block per sample: the dots, then c from the Gram system G c = d by a
-// Cholesky factorisation that drops directions whose pivot falls below
-// 1e-14 of the largest diagonal (nearly dependent right-hand sides).  A
-// non-finite dot empties the window.
+// one block per sample: the dots, then the coefficients (thread 0)
 __global__ __launch_bounds__(kBlock) void k_b_win_solve(int G, int S, WinState* ws, const double* part) {
     const int s = blockIdx.x;
-    const int cnt = ws->cnt;
-    double d[kWin + 1];
-    for (int i = 0; i <= kWin; ++i) {
-        const double t = b_total(part + (size_t)i * S * G, G);  // every thread calls b_total (it syncs)
-        d[i] = t;
+    double t[kWinDots];
+    for (int q = 0; q < kWinDots; ++q) {
+        const double v = b_total(part + (size_t)q * S * G, G);  // every thread calls b_total (it syncs)
+        t[q] = v;
         __syncthreads();
     }
-    if (threadIdx.x != 0 || s >= S) return;
-    bool finite = true;
-    for (int i = 0; i < cnt; ++i) finite = finite && std::isfinite(d[i]);
-    finite = finite && std::isfinite(d[kWin]);
-    for (int i = 0; i < cnt; ++i) ws->pend[i][s] = d[i];
-    ws->pend[cnt][s] = d[kWin];
-    double c[kWin] = {};
-    if (finite && cnt > 0) {
-        double L[kWin][kWin] = {};
-        bool keep[kWin] = {};
-        double gmax = 0.0;
-        for (int i = 0; i < cnt; ++i) gmax = fmax(gmax, ws->gram[win_slot(ws, i)][win_slot(ws, i)][s]);
-        for (int i = 0; i < cnt; ++i) {  // newest first
-            const int si = win_slot(ws, i);
-            for (int j = 0; j <= i; ++j) {
-                if (j < i && !keep[j]) continue;
-                double a = ws->gram[si][win_slot(ws, j)][s];
-                for (int k = 0; k < j; ++k)
-                    if (keep[k]) a -= L[i][k] * L[j][k];
-                if (j < i) {
-                    L[i][j] = a / L[j][j];
-                } else {
-                    keep[i] = a > 1e-14 * gmax && std::isfinite(a);
-                    L[i][i] = keep[i] ? sqrt(a) : 1.0;
-                }
-            }
-        }
-        double y[kWin] = {};
-        for (int i = 0; i < cnt; ++i) {
-            if (!keep[i]) continue;
-            double a = d[i];
-            for (int k = 0; k < i; ++k)
-                if (keep[k]) a -= L[i][k] * y[k];
-            y[i] = a / L[i][i];
-        }
-        for (int i = cnt - 1; i >= 0; --i) {
-            if (!keep[i]) continue;
-            double a = y[i];
-            for (int k = i + 1; k < cnt; ++k)
-                if (keep[k]) a -= L[k][i] * c[k];
-            c[i] = a / L[i][i];
-        }
-        // keep the projection only if its residual estimate ||b - sum c_i b_i||^2
-        // beats the newest solution alone (the plain warm start)
-        double est = d[kWin], est0 = d[kWin] - 2.0 * d[0] + ws->gram[win_slot(ws, 0)][win_slot(ws, 0)][s];
-        bool cfin = true;
-        for (int i = 0; i < cnt; ++i) {
-            cfin = cfin && std::isfinite(c[i]);
-            est -= 2.0 * c[i] * d[i];
-            for (int j = 0; j < cnt; ++j) est += c[i] * c[j] * ws->gram[win_slot(ws, i)][win_slot(ws, j)][s];
-        }
-        if (!cfin || !(est <= est0)) {
-            for (int i = 0; i < cnt; ++i) c[i] = 0.0;
-            c[0] = 1.0;
-        }
-    }
-    for (int i = 0; i < kWin; ++i) ws->c[i][s] = finite ? c[i] : 0.0;
+    const int cnt = ws->cnt;
+    if (threadIdx.x != 0 || s >= S || cnt == 0) return;
+    double a[kSolveWindow];
+    win_coefficients(cnt, t, a);
+    for (int i = 0; i < kSolveWindow; ++i) ws->a[i][s] = a[i];
 }
 
-// x0 = sum_i c_i x_slot(i) (nothing when the window is empty), lane = sample
+// x0 = sum_i a_i x_slot(i) (nothing when the window is empty), lane = sample
 __global__ __launch_bounds__(kBlock) void k_b_win_combine(int n, int S, const WinState* ws,
                                                           const double* __restrict__ Xw, double* __restrict__ x) {
     const int cnt = ws->cnt;
     if (cnt == 0) return;
     const size_t NS = (size_t)n * S;
-    double c[kWin];
-    const double* src[kWin];
+    double a[kSolveWindow];
+    const double* src[kSolveWindow];
     const int ln = min((int)(threadIdx.x & (kWave - 1)), S - 1);
 #pragma unroll
-    for (int i = 0; i < kWin; ++i) {
-        c[i] = i < cnt ? ws->c[i][ln] : 0.0;
-        src[i] = Xw + (size_t)win_slot(ws, i) * NS;
+    for (int i = 0; i < kSolveWindow; ++i) {
+        a[i] = i < cnt ? ws->a[i][ln] : 0.0;
+        src[i] = Xw + (size_t)win_slot(ws->head, i) * NS;
     }
     B_ROW_LOOP(n) {
         if (on) {
             const size_t e = (size_t)row * S + lane;
             double acc = 0.0;
 #pragma unroll
-            for (int i = 0; i < kWin; ++i)
-                if (i < cnt) acc += c[i] * src[i][e];
+            for (int i = 0; i < kSolveWindow; ++i)
+                if (i < cnt) acc += a[i] * src[i][e];
             x[e] = acc;
         }
     }
 }
 
-// (x, b) into the head slot
-__global__ __launch_bounds__(kBlock) void k_b_win_store(size_t NS, const WinState* ws, const double* __restrict__ b,
-                                                        const double* __restrict__ x, double* __restrict__ Bw,
+// the solution into the head slot (its b is there already)
+__global__ __launch_bounds__(kBlock) void k_b_win_store(size_t NS, const WinState* ws, const double* __restrict__ x,
                                                         double* __restrict__ Xw) {
     const size_t off = (size_t)ws->head * NS;
-    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock) {
-        Bw[off + e] = b[e];
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < NS; e += (size_t)gridDim.x * kBlock)
         Xw[off + e] = x[e];
-    }
 }
 
-// the Gram row of the stored pair from the pending dots; advance the head.
-// A non-finite dot of the last projection, or a failed solve (ok = 0 for
-// some sample), empties the window instead.
+// keep the pair and advance the head; a failed solve (done = 0 for some
+// sample) empties the window instead
 __global__ void k_b_win_advance(int S, WinState* ws, const OState* os) {
     const int s = threadIdx.x;
     __shared__ int bad;
     if (s == 0) bad = 0;
     __syncthreads();
-    const int cnt = ws->cnt;
-    const int h = ws->head;
-    if (s < S) {
-        bool fin = std::isfinite(ws->pend[cnt][s]) && os->done[s];
-        for (int i = 0; i < cnt; ++i) fin = fin && std::isfinite(ws->pend[i][s]);
-        if (!fin) atomicOr(&bad, 1);
-    }
+    if (s < S && !os->done[s]) atomicOr(&bad, 1);
     __syncthreads();
+    if (s != 0) return;
     if (bad) {
-        if (s == 0) {
-            ws->cnt = 0;
-            ws->head = 0;
-        }
+        ws->cnt = 0;
+        ws->head = 0;
         return;
     }
-    if (s < S) {
-        for (int i = 0; i < cnt; ++i) {
-            const int j = win_slot(ws, i);
-            if (j == h) continue;  // the pair being replaced
-            ws->gram[h][j][s] = ws->pend[i][s];
-            ws->gram[j][h][s] = ws->pend[i][s];
-        }
-        ws->gram[h][h][s] = ws->pend[cnt][s];
-    }
-    __syncthreads();
-    if (s == 0) {
-        ws->head = (h + 1) % kWin;
-        ws->cnt = min(cnt + 1, kWin);
-    }
+    ws->head = (ws->head + 1) % kSolveWindow;
+    ws->cnt = min(ws->cnt + 1, kSolveWindow);
 }
 
 template <typename T>
@@ -1844,7 +1762,7 @@ struct BatchSolver {
     DevBuf<double> part;          // 4 slots x S x G
     DevBuf<OState> ost;
     DevBuf<BState> bst;
-    DevBuf<double> winX, winB;        // kWin x [n][S]: solved pairs for the initial guess
+    DevBuf<double> winX, winB;        // kSolveWindow x [n][S]: solved pairs for the initial guess
     DevBuf<WinState> win;
     bool use_window = true;
     OState* h_ost = nullptr;
@@ -1937,9 +1855,9 @@ struct BatchSolver {
             TM_CHECK(lv[l].b.alloc((size_t)L.R.n * S));
         }
         TM_CHECK(xc.alloc((size_t)std::max(1, solver.amg.nc) * S));
-        TM_CHECK(part.alloc((size_t)std::max(4, kWin + 1) * S * G));
-        TM_CHECK(winX.alloc((size_t)kWin * NS));
-        TM_CHECK(winB.alloc((size_t)kWin * NS));
+        TM_CHECK(part.alloc((size_t)std::max(4, kWinDots) * S * G));
+        TM_CHECK(winX.alloc((size_t)kSolveWindow * NS));
+        TM_CHECK(winB.alloc((size_t)kSolveWindow * NS));
         TM_CHECK(win.alloc(1));
         TM_HIP(hipMemsetAsync(win.p, 0, sizeof(WinState), s));
         TM_CHECK(ost.alloc(1));
@@ -2063,19 +1981,21 @@ struct BatchSolver {
         hipLaunchKernelGGL((k_b_post<T>), g, dim3(kBlock), 0, s, a, S, di, bb, w, x, xo, part);
     }
 
-    // initial guess of the solve of A x = b from the window of solved pairs
+    // initial guess of the solve of A x = b from the window of solved pairs;
+    // b goes into the head slot
     void win_project(const double* b, double* x, hipStream_t s) {
-        hipLaunchKernelGGL(k_b_win_dots, dim3(G), dim3(kBlock), 0, s, n, S, (const WinState*)win.p,
-                           (const double*)winB.p, b, part.p);
+        hipLaunchKernelGGL(k_b_win_dots, dim3(G), dim3(kBlock), 0, s, n, S, (const WinState*)win.p, winB.p, b,
+                           part.p);
         hipLaunchKernelGGL(k_b_win_solve, dim3(S), dim3(kBlock), 0, s, G, S, win.p, (const double*)part.p);
         hipLaunchKernelGGL(k_b_win_combine, dim3(G), dim3(kBlock), 0, s, n, S, (const WinState*)win.p,
                            (const double*)winX.p, x);
     }
-    // the solved pair into the window (emptied when a sample did not converge)
-    void win_record(const double* b, const double* x, hipStream_t s) {
+    // the solution beside its b (after win_project); the window is emptied
+    // when a sample did not converge
+    void win_record(const double* x, hipStream_t s) {
         const size_t NS = (size_t)n * S;
-        hipLaunchKernelGGL(k_b_win_store, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const WinState*)win.p, b, x,
-                           winB.p, winX.p);
+        hipLaunchKernelGGL(k_b_win_store, dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, (const WinState*)win.p, x,
+                           winX.p);
         hipLaunchKernelGGL(k_b_win_advance, dim3(1), dim3(kMaxS), 0, s, S, win.p, (const OState*)ost.p);
     }
 
@@ -2172,7 +2092,7 @@ struct BatchSolver {
                 hipLaunchKernelGGL((k_b_combine<T>), dim3(elem_grid(NS)), dim3(kBlock), 0, s, NS, S, enq,
                                    (const T*)ring.p, (const double*)ahist.p, (const double*)bhist.p, x);
         }
-        if (use_window) win_record(b, x, s);
+        if (use_window) win_record(x, s);
         if (outer_out) *outer_out = outer;
         if (inner_out) *inner_out = inner_total;
         last_outer = outer;
@@ -2293,7 +2213,7 @@ struct BatchSolver {
                                    (const T*)ring.p, (const double*)ahist.p, (const double*)bhist.p, x);
         }
         hipLaunchKernelGGL(k_b_count_undone, dim3(1), dim3(kMaxS), 0, s, S, (const OState*)ost.p, fail);
-        if (use_window) win_record(b, x, s);
+        if (use_window) win_record(x, s);
         TM_HIP(hipGetLastError());
         return TM_OK;
     }

@@ -60,165 +60,98 @@ __device__ inline void block_partials(double* part, int stride, double v0, doubl
 }
 
 // ---- initial guesses from earlier solves ---------------------------------
-// As the batched solver: x0 = sum_i c_i x_i over the kept pairs, c minimising
-// ||b - sum_i c_i b_i|| (P. F. Fischer, Comput. Methods Appl. Mech. Engrg.
-// 163 (1998) 193, in the residual norm).  Slots are ordered newest first.
-__device__ inline int cg_win_slot(const CgWindow* w, int i) {
-    return (w->head - 1 - i + 2 * kSolveWindow) % kSolveWindow;
-}
+// x0 = x_0 + sum_j c_j dX_j over the kept pairs (backward differences at the
+// newest, least squares in the residual norm; the method is described with
+// kSolveWindow in tm_internal.hpp).  Slots are ordered newest first.
 
-// per-block partials: wpart[i * G + block] = b_slot(i) . b (i < cnt), i = kSolveWindow: b . b
-__global__ __launch_bounds__(kBlock) void k_cg_win_dots(int n, const CgWindow* w, const double* __restrict__ Bw,
+// per-block partials wpart[q * G + block] of the kWinDots dots.  The new b
+// goes into the head slot as it is read: when the window is full that slot
+// holds the oldest pair, whose value this thread has already loaded.
+__global__ __launch_bounds__(kBlock) void k_cg_win_dots(int n, const CgWindow* w, double* __restrict__ Bw,
                                                         const double* __restrict__ b, double* wpart) {
-    const int cnt = w->cnt;
-    double v[kSolveWindow + 1] = {};
+    const int cnt = w->cnt, head = w->head;
+    double acc[kWinDots] = {};
     const double* src[kSolveWindow];
 #pragma unroll
-    for (int i = 0; i < kSolveWindow; ++i) src[i] = Bw + (size_t)cg_win_slot(w, i) * n;
+    for (int i = 0; i < kSolveWindow; ++i) src[i] = Bw + (size_t)win_slot(head, i) * n;
+    double* dst = Bw + (size_t)head * n;
     for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
-        const double bi = b[e];
-        v[kSolveWindow] += bi * bi;
+        double v[kSolveWindow];
 #pragma unroll
-        for (int i = 0; i < kSolveWindow; ++i)
-            if (i < cnt) v[i] += src[i][e] * bi;
+        for (int i = 0; i < kSolveWindow; ++i) v[i] = i < cnt ? src[i][e] : 0.0;
+        const double bi = b[e];
+        const double r = bi - v[0];
+        win_differences(v, cnt);
+        win_accumulate(acc, v, r);
+        dst[e] = bi;
     }
-    __shared__ double sp[kSolveWindow + 1][kBlock / kWave];
+    __shared__ double sp[kWinDots][kBlock / kWave];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
-    for (int i = 0; i <= kSolveWindow; ++i) {
-        const double t = wave_sum(v[i]);
-        if (lane == 0) sp[i][wid] = t;
+    for (int q = 0; q < kWinDots; ++q) {
+        const double t = wave_sum(acc[q]);
+        if (lane == 0) sp[q][wid] = t;
     }
     __syncthreads();
-    if (threadIdx.x <= kSolveWindow) {
+    if (threadIdx.x < kWinDots) {
         double a = 0.0;
         for (int k = 0; k < kBlock / kWave; ++k) a += sp[threadIdx.x][k];
         wpart[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = a;
     }
 }
 
-// one block: the dots, then c from the Gram system (Cholesky dropping
-// directions whose pivot falls below 1e-14 of the largest diagonal)
+// one block: the dots, then the coefficients (thread 0)
 __global__ __launch_bounds__(kBlock) void k_cg_win_solve(int G, CgWindow* w, const double* wpart) {
-    __shared__ double d[kSolveWindow + 1];
+    __shared__ double t[kWinDots];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    for (int i = wid; i <= kSolveWindow; i += kBlock / kWave) {  // wavefront wid: dots wid, wid + 4, ...
-        double t = 0.0;
-        for (int g = lane; g < G; g += kWave) t += wpart[(size_t)i * G + g];
-        t = wave_sum(t);
-        if (lane == 0) d[i] = t;
+    for (int q = wid; q < kWinDots; q += kBlock / kWave) {  // wavefront wid: dots wid, wid + 4, ...
+        double v = 0.0;
+        for (int g = lane; g < G; g += kWave) v += wpart[(size_t)q * G + g];
+        v = wave_sum(v);
+        if (lane == 0) t[q] = v;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    const int cnt = w->cnt;
-    bool finite = std::isfinite(d[kSolveWindow]);
-    for (int i = 0; i < cnt; ++i) finite = finite && std::isfinite(d[i]);
-    for (int i = 0; i < cnt; ++i) w->pend[i] = d[i];
-    w->pend[cnt] = d[kSolveWindow];
-    double c[kSolveWindow] = {};
-    if (finite && cnt > 0) {
-        double L[kSolveWindow][kSolveWindow] = {};
-        bool keep[kSolveWindow] = {};
-        double gmax = 0.0;
-        for (int i = 0; i < cnt; ++i) gmax = fmax(gmax, w->gram[cg_win_slot(w, i)][cg_win_slot(w, i)]);
-        for (int i = 0; i < cnt; ++i) {
-            const int si = cg_win_slot(w, i);
-            for (int j = 0; j <= i; ++j) {
-                if (j < i && !keep[j]) continue;
-                double a = w->gram[si][cg_win_slot(w, j)];
-                for (int k = 0; k < j; ++k)
-                    if (keep[k]) a -= L[i][k] * L[j][k];
-                if (j < i) {
-                    L[i][j] = a / L[j][j];
-                } else {
-                    keep[i] = a > 1e-14 * gmax && std::isfinite(a);
-                    L[i][i] = keep[i] ? sqrt(a) : 1.0;
-                }
-            }
-        }
-        double y[kSolveWindow] = {};
-        for (int i = 0; i < cnt; ++i) {
-            if (!keep[i]) continue;
-            double a = d[i];
-            for (int k = 0; k < i; ++k)
-                if (keep[k]) a -= L[i][k] * y[k];
-            y[i] = a / L[i][i];
-        }
-        for (int i = cnt - 1; i >= 0; --i) {
-            if (!keep[i]) continue;
-            double a = y[i];
-            for (int k = i + 1; k < cnt; ++k)
-                if (keep[k]) a -= L[k][i] * c[k];
-            c[i] = a / L[i][i];
-        }
-        // keep the projection only if its residual estimate ||b - sum c_i b_i||^2
-        // beats the newest solution alone (the plain warm start)
-        double est = d[kSolveWindow], est0 = d[kSolveWindow] - 2.0 * d[0] + w->gram[cg_win_slot(w, 0)][cg_win_slot(w, 0)];
-        bool cfin = true;
-        for (int i = 0; i < cnt; ++i) {
-            cfin = cfin && std::isfinite(c[i]);
-            est -= 2.0 * c[i] * d[i];
-            for (int j = 0; j < cnt; ++j) est += c[i] * c[j] * w->gram[cg_win_slot(w, i)][cg_win_slot(w, j)];
-        }
-        if (!cfin || !(est <= est0)) {
-            for (int i = 0; i < cnt; ++i) c[i] = 0.0;
-            c[0] = 1.0;
-        }
-    }
-    for (int i = 0; i < kSolveWindow; ++i) w->c[i] = finite ? c[i] : 0.0;
+    if (threadIdx.x == 0 && w->cnt > 0) win_coefficients(w->cnt, t, w->a);
 }
 
+// x0 = sum_i a_i x_slot(i) (nothing when the window is empty)
 __global__ __launch_bounds__(kBlock) void k_cg_win_combine(int n, const CgWindow* w, const double* __restrict__ Xw,
                                                            double* __restrict__ x) {
     const int cnt = w->cnt;
     if (cnt == 0) return;
-    double c[kSolveWindow];
+    double a[kSolveWindow];
     const double* src[kSolveWindow];
 #pragma unroll
     for (int i = 0; i < kSolveWindow; ++i) {
-        c[i] = w->c[i];
-        src[i] = Xw + (size_t)cg_win_slot(w, i) * n;
+        a[i] = w->a[i];
+        src[i] = Xw + (size_t)win_slot(w->head, i) * n;
     }
     for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
         double acc = 0.0;
 #pragma unroll
         for (int i = 0; i < kSolveWindow; ++i)
-            if (i < cnt) acc += c[i] * src[i][e];
+            if (i < cnt) acc += a[i] * src[i][e];
         x[e] = acc;
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_win_store(int n, const CgWindow* w, const double* __restrict__ b,
-                                                         const double* __restrict__ x, double* __restrict__ Bw,
+// the solution into the head slot (its b is there already)
+__global__ __launch_bounds__(kBlock) void k_cg_win_store(int n, const CgWindow* w, const double* __restrict__ x,
                                                          double* __restrict__ Xw) {
     const size_t off = (size_t)w->head * n;
-    for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
-        Bw[off + e] = b[e];
-        Xw[off + e] = x[e];
-    }
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) Xw[off + e] = x[e];
 }
 
-// the stored pair's Gram row from the pending dots; a failed or non-finite
-// solve empties the window
+// keep the pair and advance the head; a failed solve empties the window
 __global__ void k_cg_win_advance(CgWindow* w, const CgState* st) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int cnt = w->cnt, h = w->head;
-    bool ok = st->done == 1 && std::isfinite(w->pend[cnt]);
-    for (int i = 0; i < cnt; ++i) ok = ok && std::isfinite(w->pend[i]);
-    if (!ok) {
+    if (st->done != 1) {
         w->cnt = 0;
         w->head = 0;
         return;
     }
-    for (int i = 0; i < cnt; ++i) {
-        const int j = cg_win_slot(w, i);
-        if (j == h) continue;
-        w->gram[h][j] = w->pend[i];
-        w->gram[j][h] = w->pend[i];
-    }
-    w->gram[h][h] = w->pend[cnt];
-    w->head = (h + 1) % kSolveWindow;
-    w->cnt = min(cnt + 1, kSolveWindow);
+    w->head = (w->head + 1) % kSolveWindow;
+    w->cnt = min(w->cnt + 1, kSolveWindow);
 }
 
 // part layout: [0, G) bb | [G, 2G) rz | [2G, 3G) rr | [3G, 4G) pq
@@ -505,7 +438,7 @@ int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* 
     TM_CHECK(part.alloc((size_t)4 * grid));
     TM_CHECK(winX.alloc((size_t)kSolveWindow * n));
     TM_CHECK(winB.alloc((size_t)kSolveWindow * n));
-    TM_CHECK(wpart.alloc((size_t)(kSolveWindow + 1) * grid));
+    TM_CHECK(wpart.alloc((size_t)kWinDots * grid));
     TM_CHECK(win.alloc(1));
     TM_HIP(hipMemsetAsync(win.p, 0, sizeof(CgWindow), s));
     TM_CHECK(st.alloc(1));
@@ -518,16 +451,15 @@ int CgSolver::setup(int n_, const int* rowptr, const int* colidx, const double* 
 }
 
 void CgSolver::win_project(const double* b, double* x, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_win_dots, dim3(grid), dim3(kBlock), 0, s, n, (const CgWindow*)win.p,
-                       (const double*)winB.p, b, wpart.p);
+    hipLaunchKernelGGL(k_cg_win_dots, dim3(grid), dim3(kBlock), 0, s, n, (const CgWindow*)win.p, winB.p, b, wpart.p);
     hipLaunchKernelGGL(k_cg_win_solve, dim3(1), dim3(kBlock), 0, s, grid, win.p, (const double*)wpart.p);
     hipLaunchKernelGGL(k_cg_win_combine, dim3(grid), dim3(kBlock), 0, s, n, (const CgWindow*)win.p,
                        (const double*)winX.p, x);
 }
 
-void CgSolver::win_record(const double* b, const double* x, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_win_store, dim3(grid), dim3(kBlock), 0, s, n, (const CgWindow*)win.p, b, x, winB.p,
-                       winX.p);
+// after win_project with the same b: the head slot holds b already
+void CgSolver::win_record(const double* x, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_win_store, dim3(grid), dim3(kBlock), 0, s, n, (const CgWindow*)win.p, x, winX.p);
     hipLaunchKernelGGL(k_cg_win_advance, dim3(1), dim3(1), 0, s, win.p, (const CgState*)st.p);
 }
 
@@ -582,7 +514,7 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
     }
     const CgState hs = *h_st;
     if (hs.bb == 0.0) TM_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
-    if (project) win_record(b, x, s);
+    if (project) win_record(x, s);
     last_iters = hs.iters;
     if (iters) *iters = hs.iters;
     if (relres) *relres = hs.bb > 0 ? std::sqrt(hs.rr / hs.bb) : 0.0;
@@ -623,7 +555,7 @@ int CgSolver::enqueue(const double* b, double* x, double tol, int budget, hipStr
         std::swap(pcur, pold);
     }
     hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(kBlock), 0, s, G, budget, (const double*)part.p, st.p, fails, imax);
-    if (project) win_record(b, x, s);
+    if (project) win_record(x, s);
     TM_HIP(hipGetLastError());
     return TM_OK;
 }

@@ -229,15 +229,127 @@ int copy_out(double* dst, int where, const double* src_dev, size_t count, hipStr
 // ---------------------------------------------------------------------------
 // Preconditioned CG on "symmetric matrix with identity rows" systems.
 // ---------------------------------------------------------------------------
-// Solved pairs kept per solver for initial guesses (successive right-hand
-// sides; see CgWindow, tm_cg.hip, and the batched solver).
-constexpr int kSolveWindow = 4;
+// Initial guesses from earlier solves (successive right-hand sides).
+// A solver keeps its last kSolveWindow solved pairs (x_i, b_i), A x_i = b_i,
+// i = 0 the newest.  A solve with right-hand side b starts from
+//   x0 = x_0 + sum_j c_j dX_j,
+// dB_j, dX_j the j-th backward differences of the kept b, x at the newest
+// pair (dB_0 = b_0, dB_1 = b_0 - b_1, dB_2 = b_0 - 2 b_1 + b_2, ...) and c
+// minimising ||(b - b_0) - sum_j c_j dB_j||: least squares over the span of
+// the kept right-hand sides (the projection of P. F. Fischer, Comput. Methods
+// Appl. Mech. Engrg. 163 (1998) 193, in the residual norm), where x_0 alone
+// (c = 0) is the plain warm start.  Along a smooth trajectory the
+// differences fall off geometrically; forming them element by element
+// before any dot product keeps their Gram matrix well conditioned after
+// diagonal scaling.  The Gram matrix of the raw b_i, whose condition number
+// is the square of the b_i's, loses every direction below ~1e-8 of |b| to
+// rounding.
+constexpr int kSolveWindow = 6;
+// dots per solve: the Gram matrix (packed upper triangle), dB_j . (b - b_0), |b - b_0|^2
+constexpr int kWinGram = kSolveWindow * (kSolveWindow + 1) / 2;
+constexpr int kWinDots = kWinGram + kSolveWindow + 1;
+
 struct CgWindow {
-    int head, cnt;
-    double gram[kSolveWindow][kSolveWindow];
-    double pend[kSolveWindow + 1];
-    double c[kSolveWindow];
+    int head, cnt;             // slot the next pair goes to; pairs held
+    double a[kSolveWindow];    // x0 = sum_i a_i x_i (newest first), from the last projection
 };
+
+// slot of the i-th newest pair in a ring of kSolveWindow slots
+__host__ __device__ inline int win_slot(int head, int i) { return (head - 1 - i + 2 * kSolveWindow) % kSolveWindow; }
+
+// in place: v[j] <- j-th backward difference of v[0..cnt) at v[0]
+__host__ __device__ inline void win_differences(double (&v)[kSolveWindow], int cnt) {
+#pragma unroll
+    for (int j = 1; j < kSolveWindow; ++j)
+#pragma unroll
+        for (int i = kSolveWindow - 1; i >= j; --i)
+            if (i < cnt) v[i] = v[i - 1] - v[i];
+}
+
+// acc += the products of one element: dB (differences, zero past cnt) and r = b - b_0
+__host__ __device__ inline void win_accumulate(double (&acc)[kWinDots], const double (&dv)[kSolveWindow], double r) {
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < kSolveWindow; ++i)
+#pragma unroll
+        for (int j = i; j < kSolveWindow; ++j) acc[q++] += dv[i] * dv[j];
+#pragma unroll
+    for (int i = 0; i < kSolveWindow; ++i) acc[kWinGram + i] += dv[i] * r;
+    acc[kWinDots - 1] += r * r;
+}
+
+__host__ __device__ inline int win_q(int i, int j) {  // packed index of G_ij, i <= j
+    return i * kSolveWindow - i * (i - 1) / 2 + (j - i);
+}
+
+// From the summed dots t[kWinDots] of cnt >= 1 kept pairs: the coefficients
+// a over the kept x (newest first).  The Gram system is scaled to a unit
+// diagonal and factorised by Cholesky, dropping a direction whose pivot
+// falls below 1e-12 (a difference nearly in the span of the lower ones, or
+// zero).  Non-finite dots, or a residual estimate above |b - b_0|^2, give
+// the plain warm start a = e_0.
+__host__ __device__ inline void win_coefficients(int cnt, const double* t, double* a) {
+    double sc[kSolveWindow] = {}, c[kSolveWindow] = {};
+    double L[kSolveWindow][kSolveWindow] = {};
+    bool keep[kSolveWindow] = {};
+    bool fin = true;
+    for (int q = 0; q < kWinDots; ++q) fin = fin && std::isfinite(t[q]);
+    if (fin) {
+        for (int i = 0; i < cnt; ++i) {
+            const double g = t[win_q(i, i)];
+            sc[i] = g > 0.0 ? 1.0 / sqrt(g) : 0.0;
+        }
+        for (int i = 0; i < cnt; ++i) {
+            for (int j = 0; j <= i; ++j) {
+                if (j < i && !keep[j]) continue;
+                double v = t[win_q(j, i)] * sc[i] * sc[j];
+                for (int k = 0; k < j; ++k)
+                    if (keep[k]) v -= L[i][k] * L[j][k];
+                if (j < i) {
+                    L[i][j] = v / L[j][j];
+                } else {
+                    keep[i] = sc[i] > 0.0 && v > 1e-12;
+                    L[i][i] = keep[i] ? sqrt(v) : 1.0;
+                }
+            }
+        }
+        double y[kSolveWindow] = {};
+        for (int i = 0; i < cnt; ++i) {
+            if (!keep[i]) continue;
+            double v = t[kWinGram + i] * sc[i];
+            for (int k = 0; k < i; ++k)
+                if (keep[k]) v -= L[i][k] * y[k];
+            y[i] = v / L[i][i];
+        }
+        for (int i = cnt - 1; i >= 0; --i) {
+            if (!keep[i]) continue;
+            double v = y[i];
+            for (int k = i + 1; k < cnt; ++k)
+                if (keep[k]) v -= L[k][i] * c[k];
+            c[i] = v / L[i][i];
+        }
+        double est = t[kWinDots - 1];
+        for (int i = 0; i < cnt; ++i) {
+            c[i] *= sc[i];
+            fin = fin && std::isfinite(c[i]);
+        }
+        for (int i = 0; i < cnt; ++i) {
+            est -= 2.0 * c[i] * t[kWinGram + i];
+            for (int j = 0; j < cnt; ++j) est += c[i] * c[j] * t[i < j ? win_q(i, j) : win_q(j, i)];
+        }
+        fin = fin && est <= t[kWinDots - 1];
+    }
+    // dX_j = sum_{i <= j} (-1)^i binom(j, i) x_i
+    for (int i = 0; i < kSolveWindow; ++i) a[i] = i == 0 ? 1.0 : 0.0;
+    if (!fin) return;
+    for (int j = 0; j < cnt; ++j) {
+        double bin = 1.0;  // binom(j, i)
+        for (int i = 0; i <= j; ++i) {
+            a[i] += ((i & 1) ? -bin : bin) * c[j];
+            bin = bin * (j - i) / (i + 1);
+        }
+    }
+}
 
 struct CgState {
     double gam[2];   // (r, z) of iterations k (slot k & 1)
@@ -323,7 +435,7 @@ struct CgSolver {
     int enqueue(const double* b, double* x, double tol, int budget, hipStream_t s, unsigned* fails, int* imax,
                 bool project = false);
     void win_project(const double* b, double* x, hipStream_t s);
-    void win_record(const double* b, const double* x, hipStream_t s);
+    void win_record(const double* x, hipStream_t s);  // after win_project with the same b
     // Solves A^T y = w for the same matrix (identity rows on the fixed set F):
     // y_free = A_ff^-1 w_free (A_ff symmetric), y_F = w_F - A_{free,F}^T y_free.
     int solve_transposed(const double* w, double* y, double tol, int maxit, hipStream_t s, int* iters,
