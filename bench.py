@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6   # MI355X dense FP64 matrix
 FP32_MFMA_PEAK_TFS = 157.3  # MI355X dense FP32 matrix
+INT8_MFMA_PEAK_TOPS = 5000.0  # MI355X dense int8 matrix
 
 
 def log(*a):
@@ -129,7 +130,8 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
     import torch
     from tetmag_amd import _native as N
     from tetmag_amd.batch import BatchContext, to_batch, from_batch
-    name = {N.TM_PREC_FP64: "fp64", N.TM_PREC_MIXED: "mixed", N.TM_PREC_MIXED_BEM32: "mixed_bem32"}[precision]
+    name = {N.TM_PREC_FP64: "fp64", N.TM_PREC_MIXED: "mixed", N.TM_PREC_MIXED_BEM32: "mixed_bem32",
+            N.TM_PREC_MIXED_BEMI8: "mixed_bemi8"}[precision]
     bc = BatchContext(ctx, nsamples, precision)
     mb = to_batch(fields[:nsamples])
     out = {"form": name, "samples": nsamples, "gemm_tuning": bc.gemm_info()}
@@ -204,7 +206,8 @@ def batched_leg(prob, ctx, dtau, precision, nsamples, steps, warmup, fields, che
     kt = bc.time_kernels(20)
     out["kernels"] = [{**k, "gbs": k["bytes"] / (k["ms"] * 1e-3) / 1e9,
                        "frac": k["bytes"] / (k["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                       "bound": "mfma" if k["kernel"].startswith("boundary GEMM") else "hbm/gather"} for k in kt]
+                       "bound": ("hbm" if "int8" in k["kernel"] else "mfma") if k["kernel"].startswith("boundary GEMM")
+                       else "hbm/gather"} for k in kt]
     for k in out["kernels"]:
         log(f"[{name}] kernel {k['kernel']}: {k['ms']:.4f} ms, {k['bytes'] / 1e6:.1f} MB, "
             f"{k['gbs']:.0f} GB/s = {k['frac']:.3f} of HBM peak")
@@ -241,6 +244,21 @@ def measured_gemm_traffic(workload, b, own=True):
         return None
     if not own:
         return d.get("rocblas", {}).get("hbm_bytes_per_product", {}).get("total")
+    return d["hbm_bytes_per_product"]["total"]
+
+
+def measured_i8_traffic(workload, b):
+    """HBM bytes per launch of the int8 digit-slice boundary GEMM from the
+    committed FETCH_SIZE / WRITE_SIZE counter runs, when they cover this
+    workload."""
+    path = os.path.join(ROOT, "profiles", f"boundary_gemm_i8_traffic_{workload.rsplit('-h', 1)[0]}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    if d.get("b") != b:
+        return None
     return d["hbm_bytes_per_product"]["total"]
 
 
@@ -314,7 +332,7 @@ def main():
                     help="timed batch steps of each batched form (default: --steps; warm-up: --warmup)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the per-sample error check of the batched forms")
-    ap.add_argument("--profile-form", choices=["fp64", "mixed", "mixed_bem32"], default=None,
+    ap.add_argument("--profile-form", choices=["fp64", "mixed", "mixed_bem32", "mixed_bemi8"], default=None,
                     help="time only this batched form (--warmup, --steps), nothing else; for kernel profiles")
     ap.add_argument("--check-fields", type=int, default=200,
                     help="seeded smooth fields on which the mixed form's error against FP64 is checked")
@@ -351,7 +369,8 @@ def main():
         # one batched form only, no error check, no plain form, no CPU
         # baseline: the command the kernel profiles are taken from
         from tetmag_amd import _native as N
-        prec = {"fp64": N.TM_PREC_FP64, "mixed": N.TM_PREC_MIXED, "mixed_bem32": N.TM_PREC_MIXED_BEM32}
+        prec = {"fp64": N.TM_PREC_FP64, "mixed": N.TM_PREC_MIXED, "mixed_bem32": N.TM_PREC_MIXED_BEM32,
+                "mixed_bemi8": N.TM_PREC_MIXED_BEMI8}
         fields = P.smooth_random_fields(prob.mesh, args.batch, seed=2026)
         leg = batched_leg(prob, ctx, dtau, prec[args.profile_form], args.batch, args.steps, args.warmup, fields,
                           False)
@@ -401,11 +420,14 @@ def main():
     if not args.no_batched and not args.small:
         from tetmag_amd import _native as N
         # 200 seeded smooth fields (8 Fourier modes, 20-200 nm): the first
-        # args.batch are advanced in the timed steps; the mixed form's error
-        # is checked on all of them, the other forms' on the first batch
+        # args.batch are advanced in the timed steps; the error of the mixed
+        # forms with an FP64-accurate or int8-digit boundary product (the
+        # headline candidates) is checked on all of them, the other forms' on
+        # the first batch
         fields = P.smooth_random_fields(prob.mesh, max(args.batch, args.check_fields), seed=2026)
-        for prec in (N.TM_PREC_FP64, N.TM_PREC_MIXED, N.TM_PREC_MIXED_BEM32):
-            chk = False if args.no_check else (args.check_fields if prec == N.TM_PREC_MIXED else args.batch)
+        for prec in (N.TM_PREC_FP64, N.TM_PREC_MIXED, N.TM_PREC_MIXED_BEM32, N.TM_PREC_MIXED_BEMI8):
+            full = prec in (N.TM_PREC_MIXED, N.TM_PREC_MIXED_BEMI8)
+            chk = False if args.no_check else (args.check_fields if full else args.batch)
             leg = batched_leg(prob, ctx, dtau, prec, args.batch,
                               args.steps if args.batch_steps is None else args.batch_steps, args.warmup, fields, chk)
             legs[leg["form"]] = leg
@@ -427,7 +449,9 @@ def main():
     names = {"fp64": (f"FP64, batched x{args.batch}", "f64"),
              "mixed": (f"FP32 inner CG + FP64 refinement, FP64 boundary product, batched x{args.batch}", "f32+f64"),
              "mixed_bem32": (f"FP32 inner CG + FP64 refinement, FP32 boundary product, batched x{args.batch}",
-                             "f32+f64")}
+                             "f32+f64"),
+             "mixed_bemi8": (f"FP32 inner CG + FP64 refinement, boundary product from int8 digits (exact int32 "
+                             f"digit products, FP64 combination), batched x{args.batch}", "f32+f64+i8")}
     for key, leg in legs.items():
         ok = key == "fp64" or (not args.no_check and leg["err_vs_fp64"]["heff_rel_l2_max"] <= 1e-6
                                and leg["err_vs_fp64_after_steps"]["heff_rel_l2_max"] <= 1e-6)
@@ -471,7 +495,25 @@ def main():
         key = [k for k, (f, _) in names.items() if f == form][0]
         leg = legs[key]
         flops = 2.0 * b * b * args.batch
-        if key == "mixed_bem32":
+        if key == "mixed_bemi8":
+            # the int8 digit-slice GEMM streams B's digits (5 bytes per entry)
+            # once per product: HBM-bound; its int8 MFMA rate beside it
+            gk = [k for k in leg.get("kernels", []) if k["kernel"].startswith("boundary GEMM")]
+            gt = leg.get("gemm_tuning", {})
+            ms_k = gk[0]["ms"] if gk else leg["phase_ms"]["bem_ms"]
+            alg = 5.0 * b * b
+            ach = alg / (ms_k * 1e-3) / 1e9
+            ops = 20 * flops  # digit products with p + q <= 5 (5 digits of B, 6 of U)
+            result["roofline"] = {"bound": "hbm",
+                                  "kernel": "k_b_gemm_i8 (int8 digit slices of B and U, 20 digit products, "
+                                            "%d row tile(s) of 16 per wavefront, K split %d)"
+                                            % (gt.get("solution", 1), gt.get("ksplit", 1)),
+                                  "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                                  "traffic": measured_i8_traffic(prob.name, b),
+                                  "algorithmic_bytes": alg, "kernel_ms": ms_k,
+                                  "int8_tops": ops / (ms_k * 1e-3) / 1e12, "int8_peak_tops": INT8_MFMA_PEAK_TOPS,
+                                  "boundary_phase_ms": leg["phase_ms"]["bem_ms"]}
+        elif key == "mixed_bem32":
             ach = flops / (leg["phase_ms"]["bem_ms"] * 1e-3) / 1e12
             result["roofline"] = {"bound": "mfma", "kernel": "split-K FP32 boundary GEMM (rocBLAS) + FP64 chunk sums",
                                   "achieved": ach, "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",

@@ -409,6 +409,12 @@ int tm_ctx_get_timers(tm_ctx* c, tm_phase_times* out, int reset);
 #define TM_PREC_MIXED_BEM32 2 /* as MIXED, and the boundary product from an FP32 copy of B: FP32
                                * products accumulated in FP32 over 512-column chunks, chunk sums
                                * added in FP64 */
+#define TM_PREC_MIXED_BEMI8 3 /* as MIXED, and the boundary product from int8 digits: every row of B
+                               * and sample of U scaled by a power of two and split into base-2^7
+                               * digits (five for B, six for U), digit products (p + q <= 5)
+                               * accumulated exactly in int32 on the int8 matrix cores, combined in
+                               * FP64 (~1e-8 relative product error, from B's rounding; B's digits
+                               * take 5 bytes per entry besides B) */
 
 typedef struct tm_batch tm_batch;
 typedef struct tm_batch_stats {

@@ -42,6 +42,7 @@ TM_PRECOND_AMG = 1
 TM_PREC_FP64 = 0
 TM_PREC_MIXED = 1
 TM_PREC_MIXED_BEM32 = 2
+TM_PREC_MIXED_BEMI8 = 3
 TM_BATCH_MAX = 64
 TM_BEM_REFERENCE = 0
 TM_BEM_ACCURATE = 1

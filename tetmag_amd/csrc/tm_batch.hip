@@ -1376,6 +1376,259 @@ __global__ __launch_bounds__(kBlock) void k_b_chunk_sum(int b, int S, int nch, c
     }
 }
 
+// ---- boundary product from int8 digit slices (TM_PREC_MIXED_BEMI8) -------
+// Every row j of B and every sample column s of U is scaled by a power of
+// two (sigma_j >= max_k |B_jk|, tau_s >= max_k |U_ks|) and written as
+// base-2^7 digits in [-64, 64], five for B and six for U:
+//   B_jk = sigma_j sum_p d_p 2^-(6+7p),   U_ks = tau_s sum_q e_q 2^-(6+7q).
+// Digit products accumulate exactly in int32 on the int8 matrix cores
+// (v_mfma_i32_16x16x64_i8); the products with p + q <= 5 are kept and
+// summed per p + q, so
+//   (B U^T)_js = sigma_j tau_s 2^-12 sum_{d <= 5} 2^-7d acc_d(j, s),
+//   acc_d = sum_{p+q=d} sum_k d_p(j, k) e_q(k, s).
+// The error is then B's own rounding to five digits (~2^-35 of the row
+// maximum per entry, a fixed perturbation of B, ~1e-8 relative on the
+// product); U's rounding and the dropped products are ~2^-42, so the
+// product varies as smoothly as U along a trajectory.  The splitting of a
+// floating-point product into exact integer products follows Ozaki et al.
+// (Numer. Algorithms 59 (2012) 95) and its integer-matrix-engine forms.
+// |acc_d| <= 5 * 64 * 64 * K < 2^31 for K <= 104832 columns per K range.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+constexpr int kI8Digits = 5;            // digits of B
+constexpr int kI8DigitsU = 6;           // digits of U
+constexpr int kI8MaxD = 5;              // products with p + q <= kI8MaxD
+constexpr int kI8Op = 64;               // i32x4 per MFMA operand (64 lanes x 16 bytes)
+constexpr int kI8MaxK = 104832;         // columns per K range without int32 overflow
+// Slice layouts (16-byte operands, lane l = row/column l % 16, k block l / 16):
+//   Bq: [row tile T (16 rows)][k chunk c (64)][digit p][lane l] i32x4 =
+//       d_p(B)[16T + l % 16][64c + 16(l / 16) + 0..15]
+//   Uq: [k chunk c][digit q (6)][column tile u (16 samples)][lane l] i32x4 =
+//       e_q(U)[64c + 16(l / 16) + 0..15][16u + l % 16]
+
+// ND digits of r, |r| <= 1, each exact (power-of-two scalings and rint)
+template <int ND>
+__device__ inline void i8_digits(double r, int (&d)[ND]) {
+#pragma unroll
+    for (int p = 0; p < ND; ++p) {
+        const double w = (double)(1LL << (6 + 7 * p));
+        const double t = rint(r * w);
+        d[p] = (int)t;
+        r -= t / w;
+    }
+}
+
+// the power of two 2^e >= m (m = mantissa 2^e, mantissa in [0.5, 1)); 1 for m = 0
+__device__ inline double i8_scale(double m) {
+    if (m == 0.0) return 1.0;
+    if (!(m > 0.0) || !std::isfinite(m)) return m;  // NaN and inf pass through (non-finite products)
+    int e;
+    (void)frexp(m, &e);
+    return ldexp(1.0, e);
+}
+
+// per-row maxima of |B| (B column-major, ld), K split over blockIdx.y
+__global__ __launch_bounds__(kBlock) void k_b_i8_rowmax(int b, int ld, const double* __restrict__ B, int kper,
+                                                        double* __restrict__ rmax) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= b) return;
+    const int k0 = blockIdx.y * kper, k1 = min(b, k0 + kper);
+    double m = 0.0;
+    for (int k = k0; k < k1; ++k) m = fmax(m, fabs(B[(size_t)k * ld + j]));
+    atomic_max_nonneg(rmax + j, m);
+}
+
+// Bq from B (once per batch): thread = (tile T, chunk c, lane l), 16 values
+__global__ __launch_bounds__(kBlock) void k_b_i8_bslice(int b, int ld, const double* __restrict__ B, int nT, int nC,
+                                                        const double* __restrict__ rmax, i32x4* __restrict__ Bq) {
+    const size_t tot = (size_t)nT * nC * 64;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) {
+        const int l = (int)(e & 63);
+        const size_t tc = e >> 6;
+        const int c = (int)(tc % nC), T = (int)(tc / nC);
+        const int row = 16 * T + (l & 15), kb = 64 * c + 16 * (l >> 4);
+        const double inv = row < b ? 1.0 / i8_scale(rmax[row]) : 0.0;
+        i32x4 w[kI8Digits];
+#pragma unroll
+        for (int p = 0; p < kI8Digits; ++p) w[p] = i32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int k = kb + j;
+            const double v = (row < b && k < b) ? B[(size_t)k * ld + row] * inv : 0.0;
+            int d[kI8Digits];
+            i8_digits(v, d);
+#pragma unroll
+            for (int p = 0; p < kI8Digits; ++p) w[p][j >> 2] |= (d[p] & 0xff) << (8 * (j & 3));
+        }
+#pragma unroll
+        for (int p = 0; p < kI8Digits; ++p) Bq[(tc * kI8Digits + p) * kI8Op + l] = w[p];
+    }
+}
+
+__global__ void k_b_i8_zero(int cnt, double* __restrict__ p) {
+    if ((int)threadIdx.x < cnt) p[threadIdx.x] = 0.0;
+}
+
+// per-sample maxima of |u| over the boundary nodes (lane = sample)
+__global__ __launch_bounds__(kBlock) void k_b_i8_umax(int b, int S, const int* __restrict__ bn,
+                                                      const double* __restrict__ u, double* __restrict__ umax) {
+    const int lane = threadIdx.x & 63;
+    double m = 0.0;
+    if (lane < S)
+        for (int k = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6); k < b; k += gridDim.x * (kBlock / 64)) {
+            const double v = u[(size_t)(bn ? bn[k] : k) * S + lane];
+            m = fmax(m, v == v ? fabs(v) : INFINITY);  // a NaN counts as inf: the product turns non-finite
+        }
+    __shared__ double sm[kBlock / 64][64];
+    sm[threadIdx.x >> 6][lane] = m;
+    __syncthreads();
+    if (threadIdx.x < 64 && (int)threadIdx.x < S) {
+        double t = sm[0][lane];
+        for (int w = 1; w < kBlock / 64; ++w) t = fmax(t, sm[w][lane]);
+        atomic_max_nonneg(umax + lane, t);
+    }
+}
+
+// Uq from the boundary values: thread = (chunk c, column tile u, lane l)
+__global__ __launch_bounds__(kBlock) void k_b_i8_uslice(int b, int S, int nC, const int* __restrict__ bn,
+                                                        const double* __restrict__ u,
+                                                        const double* __restrict__ umax, i32x4* __restrict__ Uq) {
+    const size_t tot = (size_t)nC * 4 * 64;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) {
+        const int l = (int)(e & 63), ut = (int)((e >> 6) & 3), c = (int)(e >> 8);
+        const int s = 16 * ut + (l & 15), kb = 64 * c + 16 * (l >> 4);
+        const double inv = s < S ? 1.0 / i8_scale(umax[s]) : 0.0;
+        i32x4 w[kI8DigitsU];
+#pragma unroll
+        for (int q = 0; q < kI8DigitsU; ++q) w[q] = i32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int k = kb + j;
+            const double v = (s < S && k < b) ? u[(size_t)(bn ? bn[k] : k) * S + s] * inv : 0.0;
+            int d[kI8DigitsU];
+            i8_digits(v, d);
+#pragma unroll
+            for (int q = 0; q < kI8DigitsU; ++q) w[q][j >> 2] |= (d[q] & 0xff) << (8 * (j & 3));
+        }
+#pragma unroll
+        for (int q = 0; q < kI8DigitsU; ++q) Uq[(((size_t)c * kI8DigitsU + q) * 4 + ut) * kI8Op + l] = w[q];
+    }
+}
+
+// P[ks][row][col] (row-major, 64 columns) = sum_d 2^-7d acc_d over the K
+// chunks [ks cper, (ks+1) cper) of the slices.  Block: 4 wavefronts x MT
+// row tiles of 16 rows; the chunk's U digits (24 KB) are staged in LDS and
+// shared by the four wavefronts, double-buffered: while the MFMAs of chunk
+// c run, the B digits of chunk c+1 are in flight into registers and its U
+// digits into registers, stored to the other buffer afterwards.
+// v_mfma_i32_16x16x64_i8: A lane l = row l % 16, B lane l = column l % 16,
+// both the 16 k of block l / 16; D lane l register r = C[4 (l / 16) + r][l % 16].
+template <int MT>
+__global__ __launch_bounds__(kBlock) void k_b_gemm_i8(int b, int nT, int nC, int cper,
+                                                      const i32x4* __restrict__ Bq, const i32x4* __restrict__ Uq,
+                                                      double* __restrict__ P) {
+    constexpr int UPT = kI8DigitsU * 4 * kI8Op / kBlock;  // U operands per thread per chunk (6)
+    __shared__ i32x4 us[2][kI8DigitsU * 4 * kI8Op];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int T0 = (blockIdx.x * (kBlock / 64) + wv) * MT;
+    const int cb = blockIdx.y * cper, ce = min(nC, cb + cper);
+    const int ncc = ce > cb ? ce - cb : 0;  // uniform over the block
+    i32x4 acc[kI8MaxD + 1][MT][4];
+#pragma unroll
+    for (int d = 0; d <= kI8MaxD; ++d)
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[d][t][u] = i32x4{0, 0, 0, 0};
+    bool tin[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) tin[t] = T0 + t < nT;
+    i32x4 a[MT][kI8Digits], an[MT][kI8Digits], ur[UPT];
+    auto load_a = [&](int c, i32x4 (&dst)[MT][kI8Digits]) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int p = 0; p < kI8Digits; ++p)
+                dst[t][p] = tin[t] ? Bq[(((size_t)(T0 + t) * nC + c) * kI8Digits + p) * kI8Op + lane]
+                                   : i32x4{0, 0, 0, 0};
+    };
+    auto load_u = [&](int c) {
+#pragma unroll
+        for (int i = 0; i < UPT; ++i) ur[i] = Uq[(size_t)c * (kI8DigitsU * 4 * kI8Op) + threadIdx.x + i * kBlock];
+    };
+    auto store_u = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < UPT; ++i) us[buf][threadIdx.x + i * kBlock] = ur[i];
+    };
+    auto mfmas = [&](const i32x4 (&av)[MT][kI8Digits], int buf) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int q = 0; q < kI8DigitsU; ++q) {
+                const i32x4 bq = us[buf][(q * 4 + u) * kI8Op + lane];
+#pragma unroll
+                for (int t = 0; t < MT; ++t)
+#pragma unroll
+                    for (int p = 0; p < kI8Digits && p + q <= kI8MaxD; ++p)
+                        acc[p + q][t][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[t][p], bq, acc[p + q][t][u], 0, 0, 0);
+            }
+    };
+    if (ncc > 0) {
+        load_u(cb);
+        load_a(cb, an);
+        store_u(0);
+    }
+    __syncthreads();
+    for (int c = 0; c < ncc; ++c) {
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int p = 0; p < kI8Digits; ++p) a[t][p] = an[t][p];
+        const bool more = c + 1 < ncc;
+        if (more) {
+            load_u(cb + c + 1);
+            load_a(cb + c + 1, an);
+        }
+        mfmas(a, c & 1);
+        if (more) store_u((c + 1) & 1);  // that buffer was last read in chunk c-1
+        __syncthreads();
+    }
+    double* out = P + (size_t)blockIdx.y * b * 64;
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = (T0 + t) * 16 + 4 * (lane >> 4) + r;
+                double v = 0.0;
+#pragma unroll
+                for (int d = kI8MaxD; d >= 0; --d) v = v * (1.0 / 128.0) + (double)acc[d][t][u][r];
+                if (tin[t] && row < b) out[(size_t)row * 64 + u * 16 + (lane & 15)] = v;
+            }
+}
+
+// out[bn[j]][s] = sigma_j tau_s 2^-12 sum_ks P[ks][j][s] (s < S); without bn: out[j][s]
+__global__ __launch_bounds__(kBlock) void k_b_i8_finish(int b, int S, int nks, const int* __restrict__ bn,
+                                                        const double* __restrict__ P, const double* __restrict__ rmax,
+                                                        const double* __restrict__ umax, double* __restrict__ out) {
+    const size_t tot = (size_t)b * 64;
+    for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock) {
+        const int j = (int)(e >> 6), s = (int)(e & 63);
+        if (s >= S) continue;
+        double acc = 0.0;
+        for (int c = 0; c < nks; ++c) acc += P[(size_t)c * tot + e];
+        out[(size_t)(bn ? bn[j] : j) * S + s] = acc * i8_scale(rmax[j]) * i8_scale(umax[s]) * (1.0 / 4096.0);
+    }
+}
+
+void launch_gemm_i8(int mt, dim3 g, hipStream_t s, int b, int nT, int nC, int cper, const i32x4* Bq,
+                    const i32x4* Uq, double* P) {
+    if (mt == 2)
+        hipLaunchKernelGGL(k_b_gemm_i8<2>, g, dim3(kBlock), 0, s, b, nT, nC, cper, Bq, Uq, P);
+    else
+        hipLaunchKernelGGL(k_b_gemm_i8<1>, g, dim3(kBlock), 0, s, b, nT, nC, cper, Bq, Uq, P);
+}
+
 __global__ __launch_bounds__(kBlock) void k_b_to_float(size_t tot, const double* __restrict__ a,
                                                        float* __restrict__ o) {
     for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < tot; e += (size_t)gridDim.x * kBlock)
@@ -2240,6 +2493,13 @@ struct tm_batch {
     DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product (TM_PREC_MIXED_BEM32)
     DevBuf<float> Pf;                  // its split-K partial products (nchunk x b x S)
     int kchunk = 512, nchunk = 0;
+    // int8 digit-slice boundary product (TM_PREC_MIXED_BEMI8): B's digits
+    // (Bq, 5 bytes per entry), the per-call digits of U, the row and sample
+    // maxima, the split-K partials, and the tuned tile rows / K split
+    DevBuf<int> Bq, Uq;
+    DevBuf<double> i8rmax, i8umax, i8P;
+    int i8nT = 0, i8nC = 0, i8mt = 1, i8ks = 1, i8candidates = 0;
+    float i8ms = 0.0f, i8ms_first = 0.0f;
     DevBuf<double> rk_k[4], rk_base;   // RK4 stages and the step's start state
     // graph-replayed RK4 steps: samples whose fixed-budget solve missed the
     // tolerance (device counter), and the budgets learnt from host-polled
@@ -2513,6 +2773,103 @@ int tune_gemm(tm_batch* bt) {
     return TM_OK;
 }
 
+// ---- int8 digit-slice boundary product: set-up, tuning, one product --------
+inline const i32x4* i8ops(const DevBuf<int>& q) { return reinterpret_cast<const i32x4*>(q.p); }
+
+int i8_gemm(tm_batch* bt, int mt, int ks) {
+    const int cper = (bt->i8nC + ks - 1) / ks;
+    const dim3 g((bt->i8nT + 4 * mt - 1) / (4 * mt), ks);
+    launch_gemm_i8(mt, g, bt->stream, bt->cv.bnx, bt->i8nT, bt->i8nC, cper, i8ops(bt->Bq), i8ops(bt->Uq), bt->i8P.p);
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+// U's digits from the boundary values u[bn[k]][s] (u[k][s] without bn)
+int i8_uslice(tm_batch* bt, const int* bn, const double* u) {
+    const int b = bt->cv.bnx, S = bt->S;
+    hipStream_t s = bt->stream;
+    hipLaunchKernelGGL(k_b_i8_zero, dim3(1), dim3(64), 0, s, 64, bt->i8umax.p);
+    hipLaunchKernelGGL(k_b_i8_umax, dim3(std::min(1024, (b + 3) / 4)), dim3(kBlock), 0, s, b, S, bn, u,
+                       bt->i8umax.p);
+    hipLaunchKernelGGL(k_b_i8_uslice, dim3(elem_grid((size_t)bt->i8nC * 256)), dim3(kBlock), 0, s, b, S, bt->i8nC,
+                       bn, u, (const double*)bt->i8umax.p, reinterpret_cast<i32x4*>(bt->Uq.p));
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+int i8_product(tm_batch* bt, const int* bn, const double* u, double* out) {
+    TM_CHECK(i8_uslice(bt, bn, u));
+    TM_CHECK(i8_gemm(bt, bt->i8mt, bt->i8ks));
+    hipLaunchKernelGGL(k_b_i8_finish, dim3(elem_grid((size_t)bt->cv.bnx * 64)), dim3(kBlock), 0, bt->stream,
+                       bt->cv.bnx, bt->S, bt->i8ks, bn, (const double*)bt->i8P.p, (const double*)bt->i8rmax.p,
+                       (const double*)bt->i8umax.p, out);
+    TM_HIP(hipGetLastError());
+    return TM_OK;
+}
+
+constexpr int kI8Splits[] = {1, 2, 3, 4, 6, 8};
+constexpr int kI8MaxSplit = 8;
+
+// B's digits (once per batch), then the tile rows (1 or 2 tiles of 16 per
+// wavefront) and K split by timing each candidate on the digits of a
+// non-zero U
+int i8_setup(tm_batch* bt) {
+    const CtxView& v = bt->cv;
+    const int b = v.bnx;
+    hipStream_t s = bt->stream;
+    bt->i8nT = (b + 15) / 16;
+    bt->i8nC = (b + 63) / 64;
+    const size_t nT = bt->i8nT, nC = bt->i8nC;
+    TM_CHECK(bt->Bq.alloc(nT * nC * kI8Digits * kI8Op * 4));
+    TM_CHECK(bt->Uq.alloc(nC * kI8DigitsU * 4 * kI8Op * 4));
+    TM_CHECK(bt->i8rmax.alloc(b));
+    TM_CHECK(bt->i8umax.alloc(64));
+    TM_CHECK(bt->i8P.alloc((size_t)kI8MaxSplit * b * 64));
+    TM_HIP(hipMemsetAsync(bt->i8rmax.p, 0, sizeof(double) * b, s));
+    const int kper = 1024;
+    hipLaunchKernelGGL(k_b_i8_rowmax, dim3((b + kBlock - 1) / kBlock, (b + kper - 1) / kper), dim3(kBlock), 0, s, b,
+                       v.bld, v.B, kper, bt->i8rmax.p);
+    hipLaunchKernelGGL(k_b_i8_bslice, dim3(elem_grid(nT * nC * 64)), dim3(kBlock), 0, s, b, v.bld, v.B, bt->i8nT,
+                       bt->i8nC, (const double*)bt->i8rmax.p, reinterpret_cast<i32x4*>(bt->Bq.p));
+    TM_HIP(hipGetLastError());
+    // tuning operand: a non-zero pattern as the boundary values
+    hipLaunchKernelGGL(k_b_fill_pattern, dim3(elem_grid((size_t)b * bt->S)), dim3(kBlock), 0, s, (size_t)b * bt->S,
+                       bt->rhsN.p);
+    TM_CHECK(i8_uslice(bt, nullptr, bt->rhsN.p));
+    hipEvent_t e0, e1;
+    TM_HIP(hipEventCreate(&e0));
+    TM_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int cand = 0;
+    for (int mt = 1; mt <= 2; ++mt)
+        for (int ks : kI8Splits) {
+            const int cper = (bt->i8nC + ks - 1) / ks;
+            if (ks > bt->i8nC || (size_t)cper * 64 > (size_t)kI8MaxK) continue;  // int32 bound per K range
+            ++cand;
+            if (i8_gemm(bt, mt, ks) != TM_OK) continue;
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 2; ++r) (void)i8_gemm(bt, mt, ks);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) continue;
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            ms *= 0.5f;
+            if (cand == 1) bt->i8ms_first = ms;
+            if (ms < best) {
+                best = ms;
+                bt->i8mt = mt;
+                bt->i8ks = ks;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    bt->i8candidates = cand;
+    bt->i8ms = best;
+    if (cand == 0) return TM_ERR_UNSUPPORTED;  // K too long for one int32 range at any split
+    TM_HIP(hipMemsetAsync(bt->rhsN.p, 0, sizeof(double) * (size_t)bt->n * bt->S, s));
+    return TM_OK;
+}
+
 // out[bn[j]][s] = sum_k B[j][k] u[bn[k]][s] for every sample (bn null: the
 // boundary values themselves, out[j][s] and u[k][s]), in the batch's
 // precision (DemagField.cpp:145-147 for one vector).  Column-major G (b x S)
@@ -2526,7 +2883,9 @@ int boundary_product(tm_batch* bt, const int* bn, const double* u, double* out) 
     const int b = v.bnx, S = bt->S;
     hipStream_t s = bt->stream;
     const dim3 g(elem_grid((size_t)b * S));
-    if (bt->prec == TM_PREC_MIXED_BEM32) {
+    if (bt->prec == TM_PREC_MIXED_BEMI8) {
+        TM_CHECK(i8_product(bt, bn, u, out));
+    } else if (bt->prec == TM_PREC_MIXED_BEM32) {
         hipLaunchKernelGGL(k_b_gather<float>, g, dim3(kBlock), 0, s, b, S, bn, u, bt->Uf.p);
         TM_CHECK(gemm_split_f32(bt));
         hipLaunchKernelGGL(k_b_chunk_sum, g, dim3(kBlock), 0, s, b, S, bt->nchunk, bn, (const float*)bt->Pf.p, out);
@@ -2630,7 +2989,8 @@ extern "C" {
 
 int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     if (!c || !out || nsamples < 1 || nsamples > kMaxS) return TM_ERR_BAD_ARG;
-    if (precision != TM_PREC_FP64 && precision != TM_PREC_MIXED && precision != TM_PREC_MIXED_BEM32)
+    if (precision != TM_PREC_FP64 && precision != TM_PREC_MIXED && precision != TM_PREC_MIXED_BEM32 &&
+        precision != TM_PREC_MIXED_BEMI8)
         return TM_ERR_BAD_ARG;
     tm_batch* bt = new tm_batch();
     int st = ctx_view(c, &bt->cv);
@@ -2704,7 +3064,7 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
         a |= bt->Gf.alloc((size_t)v.bnx * nsamples);
         a |= bt->Bf.alloc((size_t)v.bld * v.bnx);
         a |= bt->Pf.alloc((size_t)bt->nchunk * v.bnx * nsamples);
-    } else {
+    } else if (precision != TM_PREC_MIXED_BEMI8) {
         a |= bt->U.alloc((size_t)v.bnx * nsamples);
         a |= bt->Gm.alloc((size_t)v.bnx * nsamples);
     }
@@ -2732,7 +3092,12 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
         if ((st = bt->sD32.setup(*v.cgD, nsamples, bt->stream)) != TM_OK) return fail(st);
     }
     if (hipStreamSynchronize(bt->stream) != hipSuccess) return fail(TM_ERR_HIP);
-    if ((st = tune_gemm(bt)) != TM_OK) return fail(st);
+    if (precision == TM_PREC_MIXED_BEMI8) {
+        if ((st = i8_setup(bt)) != TM_OK) return fail(st);
+    } else if ((st = tune_gemm(bt)) != TM_OK) {
+        return fail(st);
+    }
+    if (hipStreamSynchronize(bt->stream) != hipSuccess) return fail(TM_ERR_HIP);
     *out = bt;
     return TM_OK;
 }
@@ -2866,7 +3231,8 @@ int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int va
     const double cst[4] = {0.0, 0.5 * dtau, 0.5 * dtau, dtau};
     // graphs need kernel-only boundary products (the own MFMA GEMM); the
     // rocBLAS paths run eagerly
-    const bool graph_ok = use_graph && bt->prec != TM_PREC_MIXED_BEM32 && bt->own_ok;
+    const bool graph_ok =
+        use_graph && (bt->prec == TM_PREC_MIXED_BEMI8 || (bt->prec != TM_PREC_MIXED_BEM32 && bt->own_ok));
     tm_rk4_stats R = {};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     TM_HIP(hipEventCreate(&e0));
@@ -3003,6 +3369,15 @@ int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int va
 int tm_batch_gemm_info(tm_batch* bt, int* solution, int* own_kernel, int* candidates, double* ms_default,
                        double* ms_tuned, int* ksplit) {
     if (!bt) return TM_ERR_BAD_ARG;
+    if (bt->prec == TM_PREC_MIXED_BEMI8) {  // k_b_gemm_i8: solution = row tiles per wavefront
+        if (solution) *solution = bt->i8mt;
+        if (own_kernel) *own_kernel = 1;
+        if (candidates) *candidates = bt->i8candidates;
+        if (ksplit) *ksplit = bt->i8ks;
+        if (ms_default) *ms_default = bt->i8ms_first;
+        if (ms_tuned) *ms_tuned = bt->i8ms;
+        return TM_OK;
+    }
     if (solution) *solution = bt->gemm_own ? bt->gemm_shape : bt->gemm_sol;
     if (own_kernel) *own_kernel = bt->gemm_own ? 1 : 0;
     if (candidates) *candidates = bt->gemm_candidates + bt->gemm_own_candidates;
@@ -3052,7 +3427,13 @@ int tm_batch_time_kernels(tm_batch* bt, int reps, tm_batch_kernel_time* out, int
         eslots * (4 + 8.0 * nw) + 3 * nS * 8 + nS * 8 + 3 * nS * 8 + per_node);
     // the boundary GEMM as the batch runs it (the tuned kernel, without the
     // gather and scatter); bytes: B once plus U and the product
-    if (bt->prec != TM_PREC_MIXED_BEM32) {
+    if (bt->prec == TM_PREC_MIXED_BEMI8) {
+        // B's digit slices once, U's digits and the FP64 partials
+        const double bytes = (double)bt->i8nT * bt->i8nC * kI8Digits * 1024.0 +
+                             (double)bt->i8nC * kI8DigitsU * 4 * 1024.0 + 8.0 * bt->i8ks * bt->cv.bnx * 64.0;
+        add("boundary GEMM (k_b_gemm_i8, int8 digits, tuned)", timeit([&] { (void)i8_gemm(bt, bt->i8mt, bt->i8ks); }),
+            bytes);
+    } else if (bt->prec != TM_PREC_MIXED_BEM32) {
         const double bb = (double)bt->cv.bnx;
         add(bt->gemm_own ? "boundary GEMM (k_b_gemm64, tuned)" : "boundary GEMM (rocBLAS, tuned)", timeit([&] {
                 if (bt->gemm_own) (void)gemm_own_run(bt, nullptr);
