@@ -264,8 +264,9 @@ def measured_i8_traffic(workload, b):
 
 def kernel_only(ctx, prob, reps):
     """Launch the dense boundary product (k_bem_gemv_part + k_bem_gemv_finish),
-    a plain 1 GiB device-to-device copy and the batched boundary product of 64
-    samples (gather, tuned FP64 GEMM, split-K sum) `reps` times each, timed
+    a plain 1 GiB device-to-device copy, the batched boundary product of 64
+    samples (gather, tuned FP64 GEMM, split-K sum) and the same product from
+    int8 digits `reps` times each, timed
     with HIP events on the stream they run on.  Algorithmic bytes: 8 b^2 + 16 b
     per product, 2 GiB per copy (read + write); the batched product: 2 b^2 64
     flops, 8 b^2 bytes of B."""
@@ -311,10 +312,24 @@ def kernel_only(ctx, prob, reps):
     bc.close()
     log(f"[kernel-only] batched boundary product (64 samples, {info}): {gemm:.3f} ms = "
         f"{2.0 * b * b * 64 / gemm / 1e9:.1f} TFLOP/s")
+    # the same product from int8 digits (k_b_i8_umax, k_b_i8_uslice, k_b_gemm_i8, k_b_i8_finish)
+    bi = BatchContext(ctx, 64, N.TM_PREC_MIXED_BEMI8)
+    bi.bem_apply(X)  # warm-up
+    e3.record(ctx.stream)
+    for _ in range(reps):
+        bi.bem_apply(X)
+    e4.record(ctx.stream)
+    torch.cuda.synchronize()
+    gemm8 = e3.elapsed_time(e4) / reps
+    info8 = bi.gemm_info()
+    bi.close()
+    log(f"[kernel-only] int8-digit boundary product (64 samples, {info8}): {gemm8:.3f} ms = "
+        f"{5.0 * b * b / gemm8 / 1e6:.0f} GB/s of B digits")
     print(json.dumps({"kernel_only": {"b": b, "reps": reps, "gemv_ms": gemv, "copy_ms": cp,
                                       "gemv_bytes": 8.0 * b * b + 16 * b, "copy_bytes": 2.0 * (1 << 30),
                                       "batched_product_ms": gemm, "batched_flops": 2.0 * b * b * 64,
-                                      "gemm_tuning": info}}), flush=True)
+                                      "gemm_tuning": info, "int8_product_ms": gemm8, "int8_b_digit_bytes": 5.0 * b * b,
+                                      "int8_tuning": info8}}), flush=True)
 
 
 def main():

@@ -12,8 +12,9 @@ benchmark runs.
         with SA-AMG GMRES; both sides solved to 1e-13; fields <= 1e-10
         relative L2; m . dm/dt = 0;
   (iii) four samples of a 64-sample mixed-precision batch (FP32 inner CG,
-        FP64 refinement to 1e-10) against the oracle: Heff <= 1e-6 relative
-        L2 (the mixed form's bar), demag energy reported.
+        FP64 refinement to 1e-10), with the FP64 boundary product and with the
+        product from int8 digits, against the oracle: Heff <= 1e-6 relative
+        L2 (the mixed forms' bar), demag energy reported.
 """
 import time
 
@@ -110,7 +111,11 @@ def test_plain_fp64_against_oracle(big):
     assert np.max(np.abs(dots)) <= 1e-12 * np.max(np.abs(dm))
 
 
-def test_mixed_batch_samples_against_oracle(big):
+_ORACLE_REFS = {}  # sample -> oracle evaluation, shared by the forms
+
+
+@pytest.mark.parametrize("form", ["mixed", "bemi8"])
+def test_mixed_batch_samples_against_oracle(big, form):
     from tetmag_amd import _native as N, problem as P
     from tetmag_amd.batch import BatchContext, to_batch, from_batch
     prob, orc, ctx = big
@@ -118,18 +123,20 @@ def test_mixed_batch_samples_against_oracle(big):
     fields = P.smooth_random_fields(prob.mesh, 64, seed=2026)
     ctx.set_solver_params(1e-10, 100000)
     try:
-        bc = BatchContext(ctx, 64, N.TM_PREC_MIXED)
+        bc = BatchContext(ctx, 64, N.TM_PREC_MIXED if form == "mixed" else N.TM_PREC_MIXED_BEMI8)
         dm, H, st = bc.eval_rhs_host(to_batch(fields))
         u1, u2 = bc.potentials()
         bc.close()
     finally:
         ctx.set_solver_params(GPU_TOL, 100000)
-    log(f"mixed batch: outer ({st['outer_neumann']}, {st['outer_dirichlet']}) inner "
+    log(f"{form} batch: outer ({st['outer_neumann']}, {st['outer_dirichlet']}) inner "
         f"({st['inner_neumann']}, {st['inner_dirichlet']})")
     V = orc.nodevol
     for s in (0, 21, 42, 63):
-        ref = orc.eval_rhs(fields[s], Js=prob.Js, A=prob.A, Ku=prob.Ku, axis=prob.kaxis, D=prob.D,
-                           Hext=prob.hext, alpha=prob.alpha, tol=ORACLE_TOL, warm=False)
+        if s not in _ORACLE_REFS:
+            _ORACLE_REFS[s] = orc.eval_rhs(fields[s], Js=prob.Js, A=prob.A, Ku=prob.Ku, axis=prob.kaxis, D=prob.D,
+                                           Hext=prob.hext, alpha=prob.alpha, tol=ORACLE_TOL, warm=False)
+        ref = _ORACLE_REFS[s]
         eH = rel_l2(from_batch(H, s), ref["Heff"])
         eD = rel_l2(from_batch(dm, s), ref["dmdt"])
         # demag energy density -1/2 sum Js V Hdem . m / sum V (DemagField.cpp:97-99) from the batch potentials

@@ -1896,34 +1896,79 @@ __global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinSt
 #pragma unroll
     for (int i = 0; i < kSolveWindow; ++i) src[i] = Bw + (size_t)win_slot(head, i) * NS;
     double* dst = Bw + (size_t)head * NS;
-    B_ROW_LOOP(n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int rstride = gridDim.x * kRowsPerBlock;
+    // two rows per iteration: both rows' loads in flight together
+    for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); row < n;
+         row += 2 * rstride) {
+        const bool two = row + rstride < n;  // wavefront-uniform
         if (on) {
-            const size_t e = (size_t)row * S + lane;
-            double v[kSolveWindow];
+            const size_t e1 = (size_t)row * S + lane, e2 = (size_t)(two ? row + rstride : row) * S + lane;
+            double v1[kSolveWindow], v2[kSolveWindow];
 #pragma unroll
-            for (int i = 0; i < kSolveWindow; ++i) v[i] = i < cnt ? src[i][e] : 0.0;
-            const double bi = b[e];
-            const double r = bi - v[0];
-            win_differences(v, cnt);
-            win_accumulate(acc, v, r);
-            dst[e] = bi;
+            for (int i = 0; i < kSolveWindow; ++i) {
+                v1[i] = i < cnt ? src[i][e1] : 0.0;
+                v2[i] = i < cnt ? src[i][e2] : 0.0;
+            }
+            const double b1 = b[e1], b2 = b[e2];
+            const double r1 = b1 - v1[0], r2 = b2 - v2[0];
+            win_differences(v1, cnt);
+            win_accumulate(acc, v1, r1);
+            if (two) {
+                win_differences(v2, cnt);
+                win_accumulate(acc, v2, r2);
+            }
+            dst[e1] = b1;  // after every load of the element (the head slot may be the oldest pair's)
+            if (two) dst[e2] = b2;
         }
     }
+    // the block's per-sample sums of all kWinDots dots, eight at a time
+    __shared__ double sp[8][kRowsPerBlock][kWave];
+    const int wv = threadIdx.x / kWave;
 #pragma unroll
-    for (int q = 0; q < kWinDots; ++q) b_partial(part + (size_t)q * S * gridDim.x, S, acc[q]);
+    for (int q0 = 0; q0 < kWinDots; q0 += 8) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (q0 + j < kWinDots) sp[j][wv][lane] = acc[q0 + j];
+        __syncthreads();
+        for (int i = threadIdx.x; i < 8 * kWave; i += kBlock) {
+            const int j = i / kWave, sm = i % kWave;
+            if (q0 + j < kWinDots && sm < S) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < kRowsPerBlock; ++k) t += sp[j][k][sm];
+                part[((size_t)(q0 + j) * S + sm) * gridDim.x + blockIdx.x] = t;
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // one block per sample: the dots, then the coefficients (thread 0)
 __global__ __launch_bounds__(kBlock) void k_b_win_solve(int G, int S, WinState* ws, const double* part) {
     const int s = blockIdx.x;
-    double t[kWinDots];
+    // one pass over the block partials of all dots (independent loads in flight)
+    double v[kWinDots] = {};
+    for (int g = threadIdx.x; g < G; g += kBlock)
+#pragma unroll
+        for (int q = 0; q < kWinDots; ++q) v[q] += part[((size_t)q * S + s) * G + g];
+    __shared__ double sr[kWinDots][kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+#pragma unroll
     for (int q = 0; q < kWinDots; ++q) {
-        const double v = b_total(part + (size_t)q * S * G, G);  // every thread calls b_total (it syncs)
-        t[q] = v;
-        __syncthreads();
+        const double w = wave_sum(v[q]);
+        if (lane == 0) sr[q][wv] = w;
     }
+    __syncthreads();
     const int cnt = ws->cnt;
     if (threadIdx.x != 0 || s >= S || cnt == 0) return;
+    double t[kWinDots];
+    for (int q = 0; q < kWinDots; ++q) {
+        double w = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) w += sr[q][k];
+        t[q] = w;
+    }
     double a[kSolveWindow];
     win_coefficients(cnt, t, a);
     for (int i = 0; i < kSolveWindow; ++i) ws->a[i][s] = a[i];
@@ -1943,14 +1988,23 @@ __global__ __launch_bounds__(kBlock) void k_b_win_combine(int n, int S, const Wi
         a[i] = i < cnt ? ws->a[i][ln] : 0.0;
         src[i] = Xw + (size_t)win_slot(ws->head, i) * NS;
     }
-    B_ROW_LOOP(n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool on = lane < S;
+    const int rstride = gridDim.x * kRowsPerBlock;
+    for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); row < n;
+         row += 2 * rstride) {
+        const bool two = row + rstride < n;  // wavefront-uniform
         if (on) {
-            const size_t e = (size_t)row * S + lane;
-            double acc = 0.0;
+            const size_t e1 = (size_t)row * S + lane, e2 = (size_t)(two ? row + rstride : row) * S + lane;
+            double acc1 = 0.0, acc2 = 0.0;
 #pragma unroll
             for (int i = 0; i < kSolveWindow; ++i)
-                if (i < cnt) acc += a[i] * src[i][e];
-            x[e] = acc;
+                if (i < cnt) {
+                    acc1 += a[i] * src[i][e1];
+                    acc2 += a[i] * src[i][e2];
+                }
+            x[e1] = acc1;
+            if (two) x[e2] = acc2;
         }
     }
 }

@@ -289,20 +289,31 @@ __host__ __device__ inline int win_q(int i, int j) {  // packed index of G_ij, i
 // zero).  Non-finite dots, or a residual estimate above |b - b_0|^2, give
 // the plain warm start a = e_0.
 __host__ __device__ inline void win_coefficients(int cnt, const double* t, double* a) {
-    double sc[kSolveWindow] = {}, c[kSolveWindow] = {};
-    double L[kSolveWindow][kSolveWindow] = {};
-    bool keep[kSolveWindow] = {};
+    // every loop runs to the compile-time window size with the count as a
+    // condition, so the small arrays stay in registers
+    double sc[kSolveWindow], c[kSolveWindow], y[kSolveWindow];
+    double L[kSolveWindow][kSolveWindow];
+    bool keep[kSolveWindow];
     bool fin = true;
+#pragma unroll
     for (int q = 0; q < kWinDots; ++q) fin = fin && std::isfinite(t[q]);
+#pragma unroll
+    for (int i = 0; i < kSolveWindow; ++i) {
+        const double g = t[win_q(i, i)];
+        sc[i] = (i < cnt && g > 0.0) ? 1.0 / sqrt(g) : 0.0;
+        c[i] = y[i] = 0.0;
+        keep[i] = false;
+#pragma unroll
+        for (int j = 0; j < kSolveWindow; ++j) L[i][j] = 0.0;
+    }
     if (fin) {
-        for (int i = 0; i < cnt; ++i) {
-            const double g = t[win_q(i, i)];
-            sc[i] = g > 0.0 ? 1.0 / sqrt(g) : 0.0;
-        }
-        for (int i = 0; i < cnt; ++i) {
+#pragma unroll
+        for (int i = 0; i < kSolveWindow; ++i) {
+#pragma unroll
             for (int j = 0; j <= i; ++j) {
-                if (j < i && !keep[j]) continue;
+                if (i >= cnt || (j < i && !keep[j])) continue;
                 double v = t[win_q(j, i)] * sc[i] * sc[j];
+#pragma unroll
                 for (int k = 0; k < j; ++k)
                     if (keep[k]) v -= L[i][k] * L[j][k];
                 if (j < i) {
@@ -313,37 +324,46 @@ __host__ __device__ inline void win_coefficients(int cnt, const double* t, doubl
                 }
             }
         }
-        double y[kSolveWindow] = {};
-        for (int i = 0; i < cnt; ++i) {
+#pragma unroll
+        for (int i = 0; i < kSolveWindow; ++i) {
             if (!keep[i]) continue;
             double v = t[kWinGram + i] * sc[i];
+#pragma unroll
             for (int k = 0; k < i; ++k)
                 if (keep[k]) v -= L[i][k] * y[k];
             y[i] = v / L[i][i];
         }
-        for (int i = cnt - 1; i >= 0; --i) {
+#pragma unroll
+        for (int i = kSolveWindow - 1; i >= 0; --i) {
             if (!keep[i]) continue;
             double v = y[i];
-            for (int k = i + 1; k < cnt; ++k)
+#pragma unroll
+            for (int k = i + 1; k < kSolveWindow; ++k)
                 if (keep[k]) v -= L[k][i] * c[k];
             c[i] = v / L[i][i];
         }
         double est = t[kWinDots - 1];
-        for (int i = 0; i < cnt; ++i) {
+#pragma unroll
+        for (int i = 0; i < kSolveWindow; ++i) {
             c[i] *= sc[i];
             fin = fin && std::isfinite(c[i]);
         }
-        for (int i = 0; i < cnt; ++i) {
+#pragma unroll
+        for (int i = 0; i < kSolveWindow; ++i) {
             est -= 2.0 * c[i] * t[kWinGram + i];
-            for (int j = 0; j < cnt; ++j) est += c[i] * c[j] * t[i < j ? win_q(i, j) : win_q(j, i)];
+#pragma unroll
+            for (int j = 0; j < kSolveWindow; ++j) est += c[i] * c[j] * t[i < j ? win_q(i, j) : win_q(j, i)];
         }
         fin = fin && est <= t[kWinDots - 1];
     }
     // dX_j = sum_{i <= j} (-1)^i binom(j, i) x_i
+#pragma unroll
     for (int i = 0; i < kSolveWindow; ++i) a[i] = i == 0 ? 1.0 : 0.0;
     if (!fin) return;
-    for (int j = 0; j < cnt; ++j) {
+#pragma unroll
+    for (int j = 0; j < kSolveWindow; ++j) {
         double bin = 1.0;  // binom(j, i)
+#pragma unroll
         for (int i = 0; i <= j; ++i) {
             a[i] += ((i & 1) ? -bin : bin) * c[j];
             bin = bin * (j - i) / (i + 1);
