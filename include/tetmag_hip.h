@@ -450,6 +450,15 @@ int tm_batch_eval_rhs(tm_batch* b, const double* m, int mwhere, double alpha, in
 int tm_batch_get_potentials(tm_batch* b, double* u1, double* u2, int where);
 /* m <- normalize(m + dtau dm/dtau) for every sample. */
 int tm_batch_step_euler(tm_batch* b, const double* dmdt, int where, double dtau);
+/* One explicit Euler step of the batch magnetisation: dm/dt at the current m
+ * as tm_batch_eval_rhs (into dmdt, stats as there), then m <- normalize(m +
+ * dtau dm/dt), the step written by the field kernel itself into a second
+ * buffer that becomes the batch's m (the same numbers as tm_batch_eval_rhs
+ * followed by tm_batch_step_euler, without the separate pass over m and
+ * dm/dt).  No reference counterpart: the reference's integrator (CVODE,
+ * LLGWrapper.cu:139-228) updates m itself. */
+int tm_batch_eval_euler(tm_batch* b, double alpha, int variant, double dtau, double* dmdt, int dwhere,
+                        tm_batch_stats* stats);
 /* nsteps classical RK4 steps of every sample's magnetisation (renormalised
  * after each step), each stage a full batched evaluation.  The hysteresis
  * driver of the reference steps one field value at a time through CVODE

@@ -275,3 +275,32 @@ def test_wide_batch_medium_mesh():
     assert rel_l2(H2, H) < 1e-10
     bc.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", ["mixed", "bemi8"])
+def test_fused_euler_step_matches_eval_then_step(base, prec):
+    """tm_batch_eval_euler (the Euler step written by the field kernel into a
+    second buffer) gives the numbers of tm_batch_eval_rhs followed by
+    tm_batch_step_euler, step after step."""
+    from tetmag_amd import problem as P
+    from tetmag_amd.batch import BatchContext, to_batch
+    prob, ctx, _ = base
+    S = 64
+    fields = P.smooth_random_fields(prob.mesh, S, seed=17, lam_min=10.0, lam_max=60.0)
+    dtau = 20e-3 * prob.time_scale()
+    a, b = BatchContext(ctx, S, prec_id(prec)), BatchContext(ctx, S, prec_id(prec))
+    for bc in (a, b):
+        bc.set_mag(to_batch(fields))
+    for _ in range(4):
+        a.eval_rhs(None, stats=False)
+        a.step_euler(dtau)
+        b.eval_euler(dtau, stats=False)
+    ctx.synchronize()
+    ma, mb = a.get_mag(), b.get_mag()
+    da, db = a.dmdt.cpu().numpy(), b.dmdt.cpu().numpy()
+    print(prec, "max |m diff|", np.max(np.abs(ma - mb)), "max |dm/dt diff|", np.max(np.abs(da - db)))
+    assert np.max(np.abs(ma - mb)) <= 1e-14  # |m| = 1: the same arithmetic on the same dm/dt
+    assert np.max(np.abs(da - db)) <= 1e-14 * np.max(np.abs(da))
+    assert np.all(np.isfinite(mb))
+    a.close()
+    b.close()

@@ -228,6 +228,7 @@ _SIGNATURES = {
     "tm_batch_eval_rhs": [_P, _P, _I, _D, _I, _P, _I, _P, _I, _P],
     "tm_batch_get_potentials": [_P, _P, _P, _I],
     "tm_batch_step_euler": [_P, _P, _I, _D],
+    "tm_batch_eval_euler": [_P, _D, _I, _D, _P, _I, _P],
     "tm_batch_run_rk4": [_P, _D, _I, _D, _I, _I, _P, _P],
     "tm_batch_time_kernels": [_P, _I, _P, _I, _P],
     "tm_batch_gemm_info": [_P, _P, _P, _P, _P, _P, _P],

@@ -78,6 +78,16 @@ class BatchContext:
         d = self.dmdt if dmdt is None else dmdt
         N.check(N.lib().tm_batch_step_euler(self.h, N.ptr(d), N.TM_DEVICE, float(dtau)), "tm_batch_step_euler")
 
+    def eval_euler(self, dtau: float, *, alpha=None, variant=N.TM_LLG_CLASSIC, stats=True):
+        """dm/dt at the batch magnetisation into self.dmdt, then the Euler step
+        of m, fused into the field kernel (tm_batch_eval_euler)."""
+        alpha = self.ctx.prob.alpha if alpha is None else alpha
+        st = self.last_stats if stats else None
+        N.check(N.lib().tm_batch_eval_euler(self.h, float(alpha), int(variant), float(dtau), N.ptr(self.dmdt),
+                                            N.TM_DEVICE, C.byref(st) if st is not None else None),
+                "tm_batch_eval_euler")
+        return self.dmdt
+
     def run_rk4(self, dtau: float, nsteps: int, alpha=None, variant=N.TM_LLG_CLASSIC, use_graph: bool = False):
         """nsteps RK4 steps of every sample on the device (tm_batch_run_rk4);
         use_graph replays one captured step with fixed solver budgets.

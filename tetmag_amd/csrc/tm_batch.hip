@@ -939,7 +939,26 @@ struct BEvalArgs {
     int variant;
     double* dmdt; double* heff;
     double* torque;       // [S], zeroed before
+    double* mnext;        // if set: the explicit Euler step m + dt dm/dt, renormalised (a buffer apart from m)
+    double dt;
 };
+
+// dm/dt of one (node, sample) into a.dmdt, and the Euler step into a.mnext
+// when asked for (the arithmetic of k_b_euler)
+__device__ inline void b_store_llg(const BEvalArgs& a, size_t e, size_t NS, double mx, double my, double mz,
+                                   double dx, double dy, double dz) {
+    a.dmdt[e] = dx;
+    a.dmdt[NS + e] = dy;
+    a.dmdt[2 * NS + e] = dz;
+    if (a.mnext) {
+        const double x = mx + a.dt * dx, y = my + a.dt * dy, z = mz + a.dt * dz;
+        const double l = sqrt(x * x + y * y + z * z);
+        const double f = l > 0.0 ? 1.0 / l : 1.0;
+        a.mnext[e] = x * f;
+        a.mnext[NS + e] = y * f;
+        a.mnext[2 * NS + e] = z * f;
+    }
+}
 
 // Fused Heff + LLG for every (node, sample): TheLLG.cpp:238-254 and
 // EffFieldGPU.cu:316-362, with the demag field -grad(u1 + u2).
@@ -1030,15 +1049,10 @@ __global__ __launch_bounds__(kBlock) void k_b_eval(BEvalArgs a) {
         }
         const double tx = my * hz - mz * hy, ty = mz * hx - mx * hz, tz = mx * hy - my * hx;
         const double bx = my * tz - mz * ty, by = mz * tx - mx * tz, bz = mx * ty - my * tx;
-        if (a.variant == TM_LLG_NOPREC) {
-            a.dmdt[e] = -a.alpha * bx;
-            a.dmdt[NS + e] = -a.alpha * by;
-            a.dmdt[2 * NS + e] = -a.alpha * bz;
-        } else {
-            a.dmdt[e] = -tx - a.alpha * bx;
-            a.dmdt[NS + e] = -ty - a.alpha * by;
-            a.dmdt[2 * NS + e] = -tz - a.alpha * bz;
-        }
+        if (a.variant == TM_LLG_NOPREC)
+            b_store_llg(a, e, NS, mx, my, mz, -a.alpha * bx, -a.alpha * by, -a.alpha * bz);
+        else
+            b_store_llg(a, e, NS, mx, my, mz, -tx - a.alpha * bx, -ty - a.alpha * by, -tz - a.alpha * bz);
         tmax = fmax(tmax, sqrt(tx * tx + ty * ty + tz * tz));
     }
     if (!a.torque) return;
@@ -1152,15 +1166,10 @@ __global__ __launch_bounds__(kBlock) void k_b_eval_w(BEvalArgs a) {
         }
         const double tx = my * hz - mz * hy, ty = mz * hx - mx * hz, tz = mx * hy - my * hx;
         const double bx = my * tz - mz * ty, by = mz * tx - mx * tz, bz = mx * ty - my * tx;
-        if (a.variant == TM_LLG_NOPREC) {
-            a.dmdt[e] = -a.alpha * bx;
-            a.dmdt[NS + e] = -a.alpha * by;
-            a.dmdt[2 * NS + e] = -a.alpha * bz;
-        } else {
-            a.dmdt[e] = -tx - a.alpha * bx;
-            a.dmdt[NS + e] = -ty - a.alpha * by;
-            a.dmdt[2 * NS + e] = -tz - a.alpha * bz;
-        }
+        if (a.variant == TM_LLG_NOPREC)
+            b_store_llg(a, e, NS, mx, my, mz, -a.alpha * bx, -a.alpha * by, -a.alpha * bz);
+        else
+            b_store_llg(a, e, NS, mx, my, mz, -tx - a.alpha * bx, -ty - a.alpha * by, -tz - a.alpha * bz);
         tmax = fmax(tmax, sqrt(tx * tx + ty * ty + tz * tz));
     }
     if (!a.torque) return;
@@ -2544,6 +2553,7 @@ struct tm_batch {
     DevBuf<double> xw, gx, gy, gz, tx, ty, tz, cx, cy, cz;  // slot weights: exchange, grad, tGrad, curl
     bool exch = false, dmi = false;
     DevBuf<double> m, dmdt, heff, u1, u2, usum, rhsN, rhsD, U, Gm, hext, torque;
+    DevBuf<double> m2;                 // the next magnetisation of tm_batch_eval_euler (swapped with m)
     DevBuf<float> Bf, Uf, Gf;          // FP32 boundary product (TM_PREC_MIXED_BEM32)
     DevBuf<float> Pf;                  // its split-K partial products (nchunk x b x S)
     int kchunk = 512, nchunk = 0;
@@ -2968,7 +2978,8 @@ void launch_div(tm_batch* bt) {
 // The fused field + LLG kernel of every (node, sample) on the batch
 // magnetisation and the potentials in usum (TheLLG.cpp:238-254,
 // EffFieldGPU.cu:316-362); with torque, the per-sample maximum torque too.
-int launch_fields(tm_batch* bt, double alpha, int variant, double* dmdt, double* heff, bool torque) {
+int launch_fields(tm_batch* bt, double alpha, int variant, double* dmdt, double* heff, bool torque,
+                  double* mnext = nullptr, double dt = 0.0) {
     const CtxView& v = bt->cv;
     const int n = bt->n, S = bt->S;
     hipStream_t s = bt->stream;
@@ -2998,6 +3009,8 @@ int launch_fields(tm_batch* bt, double alpha, int variant, double* dmdt, double*
     a.dmdt = dmdt;
     a.heff = heff;
     a.torque = torque ? bt->torque.p : nullptr;
+    a.mnext = mnext;
+    a.dt = dt;
     if (torque) hipLaunchKernelGGL(k_b_zero_small, dim3(1), dim3(kMaxS), 0, s, S, bt->torque.p);
     const dim3 g(row_grid(n));
     if (S == kMaxS) {
@@ -3206,8 +3219,14 @@ int tm_batch_get_timers(tm_batch* bt, tm_phase_times* out, int reset) {
     return TM_OK;
 }
 
-int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, int variant, double* dmdt, int dwhere,
-                      double* heff, int hwhere, tm_batch_stats* stats) {
+}  // extern "C"
+
+namespace {
+
+// tm_batch_eval_rhs; with euler_dt > 0 the field kernel also writes the
+// Euler step of m into m2, which then becomes the batch's m
+int batch_eval(tm_batch* bt, const double* m, int mwhere, double alpha, int variant, double* dmdt, int dwhere,
+               double* heff, int hwhere, tm_batch_stats* stats, double euler_dt) {
     if (!bt || !dmdt || (variant != TM_LLG_CLASSIC && variant != TM_LLG_NOPREC)) return TM_ERR_BAD_ARG;
     if ((dwhere != TM_HOST && dwhere != TM_DEVICE) || (heff && hwhere != TM_HOST && hwhere != TM_DEVICE))
         return TM_ERR_BAD_ARG;
@@ -3241,7 +3260,10 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
     // fused fields + LLG
     double* dout = dwhere == TM_DEVICE ? dmdt : bt->dmdt.p;
     double* hout = heff ? (hwhere == TM_DEVICE ? heff : bt->heff.p) : nullptr;
-    TM_CHECK(launch_fields(bt, alpha, variant, dout, hout, stats != nullptr));
+    if (euler_dt > 0.0) TM_CHECK(bt->m2.alloc(3 * NS));
+    TM_CHECK(launch_fields(bt, alpha, variant, dout, hout, stats != nullptr, euler_dt > 0.0 ? bt->m2.p : nullptr,
+                           euler_dt));
+    if (euler_dt > 0.0) std::swap(bt->m, bt->m2);
     if (bt->timers) {
         TM_HIP(hipEventRecord(bt->ev[5], s));
         TM_HIP(hipEventSynchronize(bt->ev[5]));
@@ -3270,6 +3292,21 @@ int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, i
         TM_HIP(hipStreamSynchronize(s));
     }
     return stN != TM_OK ? stN : stD;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tm_batch_eval_rhs(tm_batch* bt, const double* m, int mwhere, double alpha, int variant, double* dmdt, int dwhere,
+                      double* heff, int hwhere, tm_batch_stats* stats) {
+    return batch_eval(bt, m, mwhere, alpha, variant, dmdt, dwhere, heff, hwhere, stats, 0.0);
+}
+
+int tm_batch_eval_euler(tm_batch* bt, double alpha, int variant, double dtau, double* dmdt, int dwhere,
+                        tm_batch_stats* stats) {
+    if (!(dtau > 0.0)) return TM_ERR_BAD_ARG;
+    return batch_eval(bt, nullptr, TM_DEVICE, alpha, variant, dmdt, dwhere, nullptr, TM_DEVICE, stats, dtau);
 }
 
 int tm_batch_run_rk4(tm_batch* bt, double dtau, int nsteps, double alpha, int variant, int use_graph,
