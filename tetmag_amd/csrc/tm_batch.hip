@@ -2352,19 +2352,48 @@ struct BatchSolver {
         int outer = 0, inner_total = 0;
         if (use_window) win_project(b, x, s);
         hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
+        // Host checks: after a solve that took one refinement step (the steady
+        // state along a trajectory), the next one takes its first refinement
+        // step without waiting for the initial residual check and enqueues
+        // the inner iterations the last solve needed plus one without waiting
+        // either (iterations queued past a sample's convergence do nothing):
+        // one host synchronisation per solve.  A sample still short of its
+        // inner target after such a step is taken up by the next refinement
+        // step, checked as usual, and the solve after it checks as usual too.
+        bool spec_ok = spec_next;  // this solve may skip checks
+        bool short_seen = false;
+        bool pending = false;      // a speculative inner solve whose counts are read at the next check
+        int pend_step = 0;
         for (;;) {
             resid(b, x, pbb, prr, s);
             hipLaunchKernelGGL(k_b_outer_red, dim3(S), dim3(kBlock), 0, s, G, S, pbb, prr, tol, itol, ost.p, bst.p);
-            TM_HIP(hipMemcpyAsync(h_ost, ost.p, sizeof(OState), hipMemcpyDeviceToHost, s));
-            TM_HIP(hipStreamSynchronize(s));
-            bool all = true;
-            double worst = 0.0;
-            for (int k = 0; k < S; ++k) {
-                all = all && h_ost->done[k];
-                if (h_ost->bb[k] > 0) worst = std::max(worst, std::sqrt(h_ost->rr[k] / h_ost->bb[k]));
+            if (!(outer == 0 && spec_ok && last_inner[0] > 0)) {
+                TM_HIP(hipMemcpyAsync(h_ost, ost.p, sizeof(OState), hipMemcpyDeviceToHost, s));
+                TM_HIP(hipStreamSynchronize(s));
+                if (pending) {
+                    int used = 0;  // iterations the slowest sample ran
+                    bool still = false;
+                    for (int k = 0; k < S; ++k) {
+                        used = std::max(used, h_bst->iters[k]);
+                        still = still || h_bst->active[k];
+                    }
+                    last_inner[pend_step] = used;
+                    inner_total += used;
+                    pending = false;
+                    if (still) {  // the enqueued iterations fell short: check as usual from here on
+                        spec_ok = false;
+                        short_seen = true;
+                    }
+                }
+                bool all = true;
+                double worst = 0.0;
+                for (int k = 0; k < S; ++k) {
+                    all = all && h_ost->done[k];
+                    if (h_ost->bb[k] > 0) worst = std::max(worst, std::sqrt(h_ost->rr[k] / h_ost->bb[k]));
+                }
+                if (maxrel) *maxrel = worst;
+                if (all || outer >= 8 || inner_total >= maxit) break;
             }
-            if (maxrel) *maxrel = worst;
-            if (all || outer >= 8 || inner_total >= maxit) break;
             ++outer;
             // inner solve A d = rT from d = 0; x += d at the end
             TM_CHECK(precond(rT.p, ring.p, prz, s));
@@ -2375,6 +2404,7 @@ struct BatchSolver {
             // step needed last time, plus one (queued iterations past
             // convergence return at once)
             const int step = std::min(outer, kMaxOuter) - 1;
+            const bool spec = spec_ok && last_inner[step] > 0;
             int chunk = std::max(2, last_inner[step] + 1);
             for (;;) {
                 for (int c = 0; c < chunk && enq < kmax; ++c, ++enq) {
@@ -2388,17 +2418,24 @@ struct BatchSolver {
                                        bhist.p + (size_t)(enq + 1) * kMaxS);
                 }
                 TM_HIP(hipGetLastError());
+                // the per-sample counts (read after the next wait for the speculative step)
                 TM_HIP(hipMemcpyAsync(h_bst, bst.p, sizeof(BState), hipMemcpyDeviceToHost, s));
+                if (spec) break;
                 TM_HIP(hipStreamSynchronize(s));
                 bool any = false;
                 for (int k = 0; k < S; ++k) any = any || h_bst->active[k];
                 if (!any || enq >= kmax) break;
                 chunk = 2;
             }
-            int used = 0;  // iterations the slowest sample ran
-            for (int k = 0; k < S; ++k) used = std::max(used, h_bst->iters[k]);
-            last_inner[step] = used;
-            inner_total += used;
+            if (spec) {
+                pending = true;
+                pend_step = step;
+            } else {
+                int used = 0;  // iterations the slowest sample ran
+                for (int k = 0; k < S; ++k) used = std::max(used, h_bst->iters[k]);
+                last_inner[step] = used;
+                inner_total += used;
+            }
             if (S == kMaxS && enq <= kCombineK) {
                 hipLaunchKernelGGL(k_b_coef, dim3(1), dim3(kMaxS), 0, s, S, enq, (const double*)ahist.p,
                                    (const double*)bhist.p, chist.p);
@@ -2412,6 +2449,7 @@ struct BatchSolver {
         if (outer_out) *outer_out = outer;
         if (inner_out) *inner_out = inner_total;
         last_outer = outer;
+        spec_next = outer == 1 && !short_seen;
         TM_HIP(hipGetLastError());
         bool all = true;
         for (int k = 0; k < S; ++k) all = all && h_ost->done[k];
@@ -2419,6 +2457,7 @@ struct BatchSolver {
     }
 
     int last_outer = 0;  // refinement steps of the last host-polled solve
+    bool spec_next = false;  // the last solve took one refinement step: the next may skip host checks
 
     // Times the level-0 kernels of one inner iteration and of the outer
     // residual, each launched `reps` times alone (all samples active), and
