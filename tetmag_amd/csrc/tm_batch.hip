@@ -1984,10 +1984,12 @@ __global__ __launch_bounds__(kBlock) void k_b_win_solve(int G, int S, WinState* 
 }
 
 // x0 = sum_i a_i x_slot(i) (nothing when the window is empty), lane = sample
+// and the fixed rows' values x = b (also with an empty window)
 __global__ __launch_bounds__(kBlock) void k_b_win_combine(int n, int S, const WinState* ws,
-                                                          const double* __restrict__ Xw, double* __restrict__ x) {
+                                                          const double* __restrict__ Xw,
+                                                          const unsigned char* __restrict__ fixed,
+                                                          const double* __restrict__ b, double* __restrict__ x) {
     const int cnt = ws->cnt;
-    if (cnt == 0) return;
     const size_t NS = (size_t)n * S;
     double a[kSolveWindow];
     const double* src[kSolveWindow];
@@ -2004,7 +2006,8 @@ __global__ __launch_bounds__(kBlock) void k_b_win_combine(int n, int S, const Wi
          row += 2 * rstride) {
         const bool two = row + rstride < n;  // wavefront-uniform
         if (on) {
-            const size_t e1 = (size_t)row * S + lane, e2 = (size_t)(two ? row + rstride : row) * S + lane;
+            const int row2 = two ? row + rstride : row;
+            const size_t e1 = (size_t)row * S + lane, e2 = (size_t)row2 * S + lane;
             double acc1 = 0.0, acc2 = 0.0;
 #pragma unroll
             for (int i = 0; i < kSolveWindow; ++i)
@@ -2012,8 +2015,12 @@ __global__ __launch_bounds__(kBlock) void k_b_win_combine(int n, int S, const Wi
                     acc1 += a[i] * src[i][e1];
                     acc2 += a[i] * src[i][e2];
                 }
-            x[e1] = acc1;
-            if (two) x[e2] = acc2;
+            if (fixed[row]) x[e1] = b[e1];
+            else if (cnt > 0) x[e1] = acc1;
+            if (two) {
+                if (fixed[row2]) x[e2] = b[e2];
+                else if (cnt > 0) x[e2] = acc2;
+            }
         }
     }
 }
@@ -2304,7 +2311,7 @@ struct BatchSolver {
                            part.p);
         hipLaunchKernelGGL(k_b_win_solve, dim3(S), dim3(kBlock), 0, s, G, S, win.p, (const double*)part.p);
         hipLaunchKernelGGL(k_b_win_combine, dim3(G), dim3(kBlock), 0, s, n, S, (const WinState*)win.p,
-                           (const double*)winX.p, x);
+                           (const double*)winX.p, (const unsigned char*)fixed.p, b, x);
     }
     // the solution beside its b (after win_project); the window is emptied
     // when a sample did not converge
@@ -2350,8 +2357,8 @@ struct BatchSolver {
         double* pzq = part.p + (size_t)3 * S * G;
         const double itol = sizeof(T) == 4 ? 1e-5 : 0.0;  // FP32 inner target per refinement step
         int outer = 0, inner_total = 0;
-        if (use_window) win_project(b, x, s);
-        hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
+        if (use_window) win_project(b, x, s);  // (sets the fixed rows too)
+        else hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
         // Host checks: after a solve that took one refinement step (the steady
         // state along a trajectory), the next one takes its first refinement
         // step without waiting for the initial residual check and enqueues
@@ -2539,8 +2546,8 @@ struct BatchSolver {
         double* prz = part.p + (size_t)2 * S * G;
         double* pzq = part.p + (size_t)3 * S * G;
         const double itol = sizeof(T) == 4 ? 1e-5 : 0.0;
-        if (use_window) win_project(b, x, s);
-        hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
+        if (use_window) win_project(b, x, s);  // (sets the fixed rows too)
+        else hipLaunchKernelGGL(k_b_fix, dim3(elem_grid(NS)), dim3(kBlock), 0, s, n, S, fixed.p, b, x);
         for (int o = 0;; ++o) {
             resid(b, x, pbb, prr, s);
             hipLaunchKernelGGL(k_b_outer_red, dim3(S), dim3(kBlock), 0, s, G, S, pbb, prr, tol, itol, ost.p, bst.p);
