@@ -658,19 +658,31 @@ __global__ __launch_bounds__(kBlock) void k_b_cg_update(int n, int S, const T* _
     b_partial(part, S, v);
 }
 
-// after z = M r: gamma, convergence, beta
-// (bh[s]: the next iteration's slot of the beta history)
-__global__ void k_b_red_rz(int G, int S, const double* part_rz, const double* part_rr, int k, BState* bs,
-                           double* bh) {
+// after r -= alpha q: the iteration count and convergence on ||r||^2,
+// decided before the preconditioner runs, so that a batch whose samples
+// have all converged skips the V-cycle (its z would never be used: the
+// correction sums alpha_k p_k over the iterations taken)
+__global__ void k_b_red_rr(int G, int S, const double* part_rr, BState* bs) {
+    const int s = blockIdx.x;
+    const double rr = b_total(part_rr, G);
+    if (threadIdx.x == 0 && s < S && bs->active[s]) {
+        bs->iters[s] += 1;
+        bs->rr[s] = rr;
+        if (!(rr > bs->itol2[s]) || !std::isfinite(rr)) bs->active[s] = 0;
+    }
+}
+
+// after z = M r: gamma and beta (bh[s]: the next iteration's slot of the
+// beta history).  k < 0, the first call of an inner solve, also decides
+// convergence on the outer residual's ||r||^2; later iterations decide it
+// in k_b_red_rr.
+__global__ void k_b_red_rz(int G, int S, const double* part_rz, int k, BState* bs, double* bh) {
     const int s = blockIdx.x;
     const double rz = b_total(part_rz, G);
-    __syncthreads();
-    const double rr = k >= 0 ? b_total(part_rr, G) : 0.0;
     if (threadIdx.x == 0 && s < S) {
-        if (k >= 0) bs->rr[s] = rr;  // k < 0: initial call, rr comes from the outer residual
         const double r2 = bs->rr[s];
-        bool act = bs->active[s] && r2 > bs->itol2[s] && std::isfinite(rz) && std::isfinite(r2);
-        if (k >= 0 && bs->active[s]) bs->iters[s] += 1;
+        bool act = bs->active[s] && std::isfinite(rz);
+        if (k < 0) act = act && r2 > bs->itol2[s] && std::isfinite(r2);
         bs->beta[s] = (act && k >= 0 && bs->gam[s] != 0.0) ? rz / bs->gam[s] : 0.0;
         bh[s] = bs->beta[s];
         bs->gam[s] = rz;
@@ -2328,6 +2340,10 @@ struct BatchSolver {
         hipLaunchKernelGGL((k_b_cg_update<T>), dim3(G), dim3(kBlock), 0, s, n, S, (const T*)q.p, rT.p,
                            (const BState*)bst.p, prr);
     }
+    // per-sample convergence after cg_update, before the preconditioner
+    void conv_check(const double* prr, hipStream_t s) {
+        hipLaunchKernelGGL(k_b_red_rr, dim3(S), dim3(kBlock), 0, s, G, S, prr, bst.p);
+    }
 
     // r = b - A x in FP64 (b, x: [n][S]), rT = (T) r, partials (b, b), (r, r)
     void resid(const double* b, const double* x, double* pbb, double* prr, hipStream_t s) {
@@ -2405,7 +2421,7 @@ struct BatchSolver {
             ++outer;
             // inner solve A d = rT from d = 0; x += d at the end
             TM_CHECK(precond(rT.p, ring.p, prz, s));
-            hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, -1, bst.p, bhist.p);
+            hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, -1, bst.p, bhist.p);
             int enq = 0;
             const int kmax = std::min(kRing, maxit);
             // iterations enqueued before the first check: what this refinement
@@ -2421,8 +2437,9 @@ struct BatchSolver {
                     hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, pzq, bst.p,
                                        ahist.p + (size_t)enq * kMaxS);
                     cg_update(prr, s);
+                    conv_check(prr, s);
                     TM_CHECK(precond(rT.p, zk + NS, prz, s));
-                    hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, enq, bst.p,
+                    hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, enq, bst.p,
                                        bhist.p + (size_t)(enq + 1) * kMaxS);
                 }
                 TM_HIP(hipGetLastError());
@@ -2554,7 +2571,7 @@ struct BatchSolver {
             hipLaunchKernelGGL(k_b_outer_red, dim3(S), dim3(kBlock), 0, s, G, S, pbb, prr, tol, itol, ost.p, bst.p);
             if (o == nouter) break;
             TM_CHECK(precond(rT.p, ring.p, prz, s));
-            hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, -1, bst.p, bhist.p);
+            hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, -1, bst.p, bhist.p);
             const int K = std::max(1, std::min(kRing, ninner[o]));
             for (int k = 0; k < K; ++k) {
                 T* zk = ring.p + (size_t)k * NS;
@@ -2562,8 +2579,10 @@ struct BatchSolver {
                 hipLaunchKernelGGL(k_b_red_pq, dim3(S), dim3(kBlock), 0, s, G, S, pzq, bst.p,
                                    ahist.p + (size_t)k * kMaxS);
                 cg_update(prr, s);
+                conv_check(prr, s);
+                if (k + 1 == K) break;  // the last z would go unused
                 TM_CHECK(precond(rT.p, zk + NS, prz, s));
-                hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, prr, k, bst.p,
+                hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, k, bst.p,
                                    bhist.p + (size_t)(k + 1) * kMaxS);
             }
             if (S == kMaxS && K <= kCombineK) {
