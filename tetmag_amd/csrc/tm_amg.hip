@@ -255,10 +255,26 @@ bool dense_inverse(std::vector<double>& a, int n) {
     for (int s = s0_ + (int)threadIdx.x / kWave; s < s1_; s += kBlock / kWave)
 
 // x = w D^-1 b ;  r = b - A x   (x gathered on the fly as w dinv_j b_j)
-__global__ __launch_bounds__(kBlock) void k_amg_pre(const CgState* st, SellView A, const double* __restrict__ val,
+// With rr_part, every block first sums the CG update's (r, r) partials (G of
+// them) and all take the same decision: a CG that has reached its tolerance
+// stops here (block 0 records it) instead of after a V-cycle whose z it
+// would not use.
+__global__ __launch_bounds__(kBlock) void k_amg_pre(CgState* st, SellView A, const double* __restrict__ val,
                                                     const double* __restrict__ dinv, const double* __restrict__ b,
-                                                    double w, double* __restrict__ x, double* __restrict__ r) {
+                                                    double w, double* __restrict__ x, double* __restrict__ r,
+                                                    const double* rr_part, int G) {
     if (st && st->done) return;
+    if (st && rr_part) {
+        double t[1];
+        reduce_partials_all<1>(rr_part, G, G, t);
+        if (t[0] <= st->tol2bb) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                st->rr = t[0];
+                st->done = 1;
+            }
+            return;
+        }
+    }
     AMG_SLICE_LOOP(A) {
         double acc = 0.0;
         sell_row(A, s, lane, [&](int64_t e, int c) { acc += val[e] * (w * dinv[c] * b[c]); });
@@ -520,7 +536,8 @@ int AmgPrecond::setup(int n, const int* rowptr, const int* colidx, const double*
 }
 
 // z = M^-1 r (one V-cycle); writes the (r, z) partials of `grid` blocks to part.
-int AmgPrecond::apply(const double* r0, double* z, double* part, int grid, hipStream_t s, const CgState* st) {
+int AmgPrecond::apply(const double* r0, double* z, double* part, int grid, hipStream_t s, CgState* st,
+                      const double* rr_part) {
     const int L = (int)levels.size();
     if (L == 0) {
         hipLaunchKernelGGL(k_amg_dense, dim3((nc + 3) / 4), dim3(kBlock), 0, s, st, nc, Cinv.p, r0, z);
@@ -534,7 +551,7 @@ int AmgPrecond::apply(const double* r0, double* z, double* part, int grid, hipSt
         if (l == 0) {
             const SellPattern& A = *V.A_ext;
             hipLaunchKernelGGL(k_amg_pre, dim3(grid), dim3(kBlock), 0, s, st, view(A), V.Aval_ext, V.dinv_ext, b, V.omega,
-                               V.x.p, V.r.p);
+                               V.x.p, V.r.p, rr_part, grid);
         } else {
             csrv_launch<2>(st, V.A, nullptr, V.dinv.p, b, V.omega, V.x.p, V.r.p, s);
         }

@@ -479,7 +479,7 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
         hipLaunchKernelGGL(k_cg_init_res<false>, dim3(G), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p,
                            part.p, tol, st.p);
     TM_HIP(hipGetLastError());
-    if (!jac) TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p));
+    if (!jac) TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p, part.p + 2 * G));
     // First chunk sized from the previous solve (warm-started solves along a
     // trajectory need similar counts), then small chunks.
     int enq = 0;
@@ -496,7 +496,7 @@ int CgSolver::solve(const double* b, double* x, double tol, int maxit, hipStream
             } else {
                 hipLaunchKernelGGL(k_cg_update<false>, dim3(G), dim3(kBlock), 0, s, n, dinv.p, pcur, q.p, x, r.p,
                                    z.p, enq, part.p, st.p);
-                TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p));
+                TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p, part.p + 2 * G));
             }
             std::swap(pcur, pold);
         }
@@ -539,7 +539,7 @@ int CgSolver::enqueue(const double* b, double* x, double tol, int budget, hipStr
         hipLaunchKernelGGL(k_cg_init_res<false>, dim3(G), dim3(kBlock), 0, s, A, val.p, dinv.p, b, x, r.p, z.p,
                            part.p, tol, st.p);
     hipLaunchKernelGGL(k_cg_trivial, dim3(G), dim3(kBlock), 0, s, n, x, st.p);
-    if (!jac) TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p));
+    if (!jac) TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p, part.p + 2 * G));
     double* pcur = p0.p;
     double* pold = p1.p;
     for (int k = 0; k < budget; ++k) {
@@ -550,7 +550,8 @@ int CgSolver::enqueue(const double* b, double* x, double tol, int budget, hipStr
         } else {
             hipLaunchKernelGGL(k_cg_update<false>, dim3(G), dim3(kBlock), 0, s, n, dinv.p, pcur, q.p, x, r.p, z.p, k,
                                part.p, st.p);
-            TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p));
+            if (k + 1 < budget)  // the z of the last update would go unused
+                TM_CHECK(amg.apply(r.p, z.p, part.p + G, G, s, st.p, part.p + 2 * G));
         }
         std::swap(pcur, pold);
     }

@@ -409,8 +409,11 @@ struct AmgPrecond {
     int setup(int n, const int* rowptr, const int* colidx, const double* vals, const std::vector<unsigned char>& fixed,
               const SellPattern* A0, const double* A0val, const double* A0dinv, hipStream_t s);
     // z = M^-1 r (one V-cycle); every kernel returns at once when st != NULL
-    // and st->done is set (the CG it serves has stopped).
-    int apply(const double* r, double* z, double* part, int grid, hipStream_t s, const CgState* st = nullptr);
+    // and st->done is set (the CG it serves has stopped).  With rr_part (the
+    // grid partials of (r, r) from the CG update) the first kernel ends the
+    // CG when ||r||^2 <= st->tol2bb, so a converged CG skips the V-cycle.
+    int apply(const double* r, double* z, double* part, int grid, hipStream_t s, CgState* st = nullptr,
+              const double* rr_part = nullptr);
 };
 
 struct CgSolver {
