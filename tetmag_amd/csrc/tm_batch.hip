@@ -1967,29 +1967,46 @@ __global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinSt
 }
 
 // one block per sample: the dots, then the coefficients (thread 0)
-__global__ __launch_bounds__(kBlock) void k_b_win_solve(int G, int S, WinState* ws, const double* part) {
-    const int s = blockIdx.x;
-    // one pass over the block partials of all dots (independent loads in flight)
-    double v[kWinDots] = {};
-    for (int g = threadIdx.x; g < G; g += kBlock)
-#pragma unroll
-        for (int q = 0; q < kWinDots; ++q) v[q] += part[((size_t)g * kWinDots + q) * S + s];
-    __shared__ double sr[kWinDots][kBlock / kWave];
+// second stage of the dots: block (q, c) sums dot q over chunk c of the
+// block partials, lane = sample (rows of part are contiguous in s);
+// red[c][q][s]
+constexpr int kWinChunks = 8;
+__global__ __launch_bounds__(kBlock) void k_b_win_red(int G, int S, const double* __restrict__ part,
+                                                      double* __restrict__ red) {
+    constexpr int kW = kBlock / kWave;
+    const int q = blockIdx.x, c = blockIdx.y;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int sm = min(lane, S - 1);
+    double v = 0.0;
+    for (int g = c * kW + wv; g < G; g += kWinChunks * kW) v += part[((size_t)g * kWinDots + q) * S + sm];
+    __shared__ double sr[kW][kWave];
+    sr[wv][lane] = v;
+    __syncthreads();
+    if (wv == 0 && lane < S) {
+        double t = 0.0;
 #pragma unroll
-    for (int q = 0; q < kWinDots; ++q) {
-        const double w = wave_sum(v[q]);
-        if (lane == 0) sr[q][wv] = w;
+        for (int k = 0; k < kW; ++k) t += sr[k][lane];
+        red[((size_t)c * kWinDots + q) * S + lane] = t;
+    }
+}
+
+// the coefficients of sample s = blockIdx.x from the chunk sums (one wavefront)
+__global__ __launch_bounds__(kWave) void k_b_win_solve(int S, WinState* ws, const double* red) {
+    const int s = blockIdx.x;
+    __shared__ double sd[kWinDots];
+    const int q = threadIdx.x;
+    if (q < kWinDots && s < S) {
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < kWinChunks; ++c) v += red[((size_t)c * kWinDots + q) * S + s];
+        sd[q] = v;
     }
     __syncthreads();
     const int cnt = ws->cnt;
     if (threadIdx.x != 0 || s >= S || cnt == 0) return;
     double t[kWinDots];
-    for (int q = 0; q < kWinDots; ++q) {
-        double w = 0.0;
-        for (int k = 0; k < kBlock / kWave; ++k) w += sr[q][k];
-        t[q] = w;
-    }
+#pragma unroll
+    for (int k = 0; k < kWinDots; ++k) t[k] = sd[k];
     double a[kSolveWindow];
     win_coefficients(cnt, t, a);
     for (int i = 0; i < kSolveWindow; ++i) ws->a[i][s] = a[i];
@@ -2190,7 +2207,7 @@ struct BatchSolver {
             TM_CHECK(lv[l].b.alloc((size_t)L.R.n * S));
         }
         TM_CHECK(xc.alloc((size_t)std::max(1, solver.amg.nc) * S));
-        TM_CHECK(part.alloc((size_t)std::max(4, kWinDots) * S * G));
+        TM_CHECK(part.alloc((size_t)std::max(4, kWinDots) * S * G + (size_t)kWinDots * kWinChunks * S));
         TM_CHECK(winX.alloc((size_t)kSolveWindow * NS));
         TM_CHECK(winB.alloc((size_t)kSolveWindow * NS));
         TM_CHECK(win.alloc(1));
@@ -2322,7 +2339,10 @@ struct BatchSolver {
         const int gw = std::min(G, 1024);  // fewer, longer blocks: half the partials, 1.2x faster at 283k rows
         hipLaunchKernelGGL(k_b_win_dots, dim3(gw), dim3(kBlock), 0, s, n, S, (const WinState*)win.p, winB.p, b,
                            part.p);
-        hipLaunchKernelGGL(k_b_win_solve, dim3(S), dim3(kBlock), 0, s, gw, S, win.p, (const double*)part.p);
+        double* red = part.p + (size_t)kWinDots * S * gw;  // after the block partials
+        hipLaunchKernelGGL(k_b_win_red, dim3(kWinDots, kWinChunks), dim3(kBlock), 0, s, gw, S, (const double*)part.p,
+                           red);
+        hipLaunchKernelGGL(k_b_win_solve, dim3(S), dim3(kWave), 0, s, S, win.p, (const double*)red);
         hipLaunchKernelGGL(k_b_win_combine, dim3(G), dim3(kBlock), 0, s, n, S, (const WinState*)win.p,
                            (const double*)winX.p, (const unsigned char*)fixed.p, b, x);
     }
