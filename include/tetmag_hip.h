@@ -487,6 +487,13 @@ int tm_batch_run_rk4(tm_batch* b, double dtau, int nsteps, double alpha, int var
  * timings of the library default and of the choice.  Any output may be NULL. */
 int tm_batch_gemm_info(tm_batch* b, int* solution, int* own_kernel, int* candidates, double* ms_default,
                        double* ms_tuned, int* ksplit);
+/* In-pipeline timing of the boundary GEMM (no reference counterpart): on = 1
+ * starts recording a HIP event pair around the GEMM launch of each following
+ * evaluation on the batch's stream (no host wait; the first 256 are kept);
+ * on = 0 stops, waits for the last recorded event and returns the mean
+ * duration in ms of the launches recorded since the last start and their
+ * number (either output may be NULL). */
+int tm_batch_gemm_live(tm_batch* b, int on, double* ms_mean, int* count);
 /* y[j][s] = sum_k B[j][k] x[k][s] for every sample s: the batch's boundary
  * product (the GEMM tm_batch_eval_rhs runs, in the batch's precision) on
  * caller values; x, y: b x S, boundary-node major, the sample index fastest.

@@ -304,3 +304,32 @@ def test_fused_euler_step_matches_eval_then_step(base, prec):
     assert np.all(np.isfinite(mb))
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bemi8"])
+def test_gemm_live_events(base, prec):
+    """tm_batch_gemm_live records one event pair per evaluation around the
+    boundary GEMM and reports their mean; recording changes no result."""
+    from tetmag_amd import problem as P
+    from tetmag_amd.batch import BatchContext, to_batch
+    prob, ctx, _ = base
+    S = 8
+    fields = P.smooth_random_fields(prob.mesh, S, seed=29, lam_min=10.0, lam_max=60.0)
+    dtau = 20e-3 * prob.time_scale()
+    a, b = BatchContext(ctx, S, prec_id(prec)), BatchContext(ctx, S, prec_id(prec))
+    for bc in (a, b):
+        bc.set_mag(to_batch(fields))
+    a.gemm_live(True)
+    for _ in range(3):
+        a.eval_euler(dtau, stats=False)
+        b.eval_euler(dtau, stats=False)
+    ms, cnt = a.gemm_live(False)
+    print(prec, "live GEMM ms", ms, "launches", cnt)
+    assert cnt == 3 and ms > 0.0
+    a.eval_euler(dtau, stats=False)  # recording stopped: no more pairs
+    assert a.gemm_live(False)[1] == 3
+    b.eval_euler(dtau, stats=False)
+    ctx.synchronize()
+    assert np.array_equal(a.get_mag(), b.get_mag())
+    a.close()
+    b.close()

@@ -232,6 +232,7 @@ _SIGNATURES = {
     "tm_batch_run_rk4": [_P, _D, _I, _D, _I, _I, _P, _P],
     "tm_batch_time_kernels": [_P, _I, _P, _I, _P],
     "tm_batch_gemm_info": [_P, _P, _P, _P, _P, _P, _P],
+    "tm_batch_gemm_live": [_P, _I, _P, _P],
     "tm_batch_bem_apply": [_P, _P, _I, _P, _I],
     "tm_batch_enable_timers": [_P, _I],
     "tm_batch_get_timers": [_P, _P, _I],

@@ -118,6 +118,14 @@ class BatchContext:
         return {"solution": sol.value, "own_kernel": bool(own.value), "candidates": cand.value,
                 "ms_default": d.value, "ms_tuned": b.value, "ksplit": ks.value}
 
+    def gemm_live(self, on: bool):
+        """Event pairs around the boundary GEMM of each following evaluation
+        (tm_batch_gemm_live).  gemm_live(True) starts recording; gemm_live(False)
+        stops and returns (mean ms per launch, launches recorded)."""
+        ms, cnt = C.c_double(), C.c_int()
+        N.check(N.lib().tm_batch_gemm_live(self.h, int(bool(on)), C.byref(ms), C.byref(cnt)), "tm_batch_gemm_live")
+        return ms.value, cnt.value
+
     def bem_apply(self, x):
         """Boundary product of every sample, y[j][s] = sum_k B[j][k] x[k][s]
         (tm_batch_bem_apply), in the batch's precision.  x: (b, S) numpy

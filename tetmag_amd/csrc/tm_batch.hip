@@ -2675,6 +2675,17 @@ struct tm_batch {
     bool timers = false;
     hipEvent_t ev[6] = {};
     tm_phase_times tsum = {};
+    // in-pipeline event pairs around the boundary GEMM (tm_batch_gemm_live)
+    static constexpr int kLivePairs = 256;
+    hipEvent_t lev[2 * kLivePairs] = {};
+    int lev_n = 0;
+    bool lev_on = false;
+    // marks the start (0) or end (1) of a boundary GEMM launch while recording
+    void live_mark(int which) {
+        if (!lev_on || lev_n >= kLivePairs) return;
+        (void)hipEventRecord(lev[2 * lev_n + which], stream);
+        if (which == 1) ++lev_n;
+    }
 };
 
 namespace {
@@ -2949,7 +2960,9 @@ int i8_uslice(tm_batch* bt, const int* bn, const double* u) {
 
 int i8_product(tm_batch* bt, const int* bn, const double* u, double* out) {
     TM_CHECK(i8_uslice(bt, bn, u));
+    bt->live_mark(0);
     TM_CHECK(i8_gemm(bt, bt->i8mt, bt->i8ks));
+    bt->live_mark(1);
     hipLaunchKernelGGL(k_b_i8_finish, dim3(elem_grid((size_t)bt->cv.bnx * 64)), dim3(kBlock), 0, bt->stream,
                        bt->cv.bnx, bt->S, bt->i8ks, bn, (const double*)bt->i8P.p, (const double*)bt->i8rmax.p,
                        (const double*)bt->i8umax.p, out);
@@ -3056,16 +3069,22 @@ int boundary_product(tm_batch* bt, const int* bn, const double* u, double* out) 
         TM_CHECK(i8_product(bt, bn, u, out));
     } else if (bt->prec == TM_PREC_MIXED_BEM32) {
         hipLaunchKernelGGL(k_b_gather<float>, g, dim3(kBlock), 0, s, b, S, bn, u, bt->Uf.p);
+        bt->live_mark(0);
         TM_CHECK(gemm_split_f32(bt));
+        bt->live_mark(1);
         hipLaunchKernelGGL(k_b_chunk_sum, g, dim3(kBlock), 0, s, b, S, bt->nchunk, bn, (const float*)bt->Pf.p, out);
     } else {
         hipLaunchKernelGGL(k_b_gather<double>, g, dim3(kBlock), 0, s, b, S, bn, u, bt->U.p);
         if (bt->gemm_own) {
+            bt->live_mark(0);
             TM_CHECK(gemm_own_run(bt, nullptr));
+            bt->live_mark(1);
             hipLaunchKernelGGL(k_b_ksum_scatter, g, dim3(kBlock), 0, s, b, S, bt->gemm_ks, bn,
                                (const double*)bt->Pd.p, out);
         } else {
+            bt->live_mark(0);
             TM_CHECK(gemm_run(bt, bt->gemm_sol));
+            bt->live_mark(1);
             hipLaunchKernelGGL(k_b_scatter<double>, g, dim3(kBlock), 0, s, b, S, bn, (const double*)bt->Gm.p, out);
         }
     }
@@ -3280,6 +3299,8 @@ int tm_batch_destroy(tm_batch* bt) {
     if (bt->blas) rocblas_destroy_handle(bt->blas);
     for (auto& e : bt->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : bt->lev)
+        if (e) (void)hipEventDestroy(e);
     delete bt;
     return TM_OK;
 }
@@ -3314,6 +3335,28 @@ int tm_batch_enable_timers(tm_batch* bt, int on) {
     if (on && !bt->ev[0])
         for (auto& e : bt->ev) TM_HIP(hipEventCreate(&e));
     bt->timers = on != 0;
+    return TM_OK;
+}
+
+int tm_batch_gemm_live(tm_batch* bt, int on, double* ms_mean, int* count) {
+    if (!bt || (on != 0 && on != 1)) return TM_ERR_BAD_ARG;
+    if (on) {
+        if (!bt->lev[0])
+            for (auto& e : bt->lev) TM_HIP(hipEventCreate(&e));
+        bt->lev_n = 0;
+        bt->lev_on = true;
+        return TM_OK;
+    }
+    bt->lev_on = false;
+    double sum = 0.0;
+    if (bt->lev_n > 0) TM_HIP(hipEventSynchronize(bt->lev[2 * bt->lev_n - 1]));
+    for (int k = 0; k < bt->lev_n; ++k) {
+        float ms = 0.0f;
+        TM_HIP(hipEventElapsedTime(&ms, bt->lev[2 * k], bt->lev[2 * k + 1]));
+        sum += ms;
+    }
+    if (ms_mean) *ms_mean = bt->lev_n > 0 ? sum / bt->lev_n : 0.0;
+    if (count) *count = bt->lev_n;
     return TM_OK;
 }
 
