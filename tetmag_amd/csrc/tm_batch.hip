@@ -2445,11 +2445,13 @@ struct BatchSolver {
             int enq = 0;
             const int kmax = std::min(kRing, maxit);
             // iterations enqueued before the first check: what this refinement
-            // step needed last time, plus one (queued iterations past
-            // convergence return at once)
+            // step needed last time, plus one when the host checks after them
+            // (queued iterations past convergence return at once, but their
+            // launches still cost); unchecked, exactly that many: a sample
+            // that needs more is taken up by the next refinement step
             const int step = std::min(outer, kMaxOuter) - 1;
             const bool spec = spec_ok && last_inner[step] > 0;
-            int chunk = std::max(2, last_inner[step] + 1);
+            int chunk = spec ? last_inner[step] : std::max(2, last_inner[step] + 1);
             for (;;) {
                 for (int c = 0; c < chunk && enq < kmax; ++c, ++enq) {
                     T* zk = ring.p + (size_t)enq * NS;
