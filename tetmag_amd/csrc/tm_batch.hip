@@ -2460,6 +2460,9 @@ struct BatchSolver {
                                        ahist.p + (size_t)enq * kMaxS);
                     cg_update(prr, s);
                     conv_check(prr, s);
+                    // no iteration follows the last one of an unchecked chunk
+                    // (or of the ring): its z would go unused
+                    if ((spec && c + 1 == chunk) || enq + 1 == kmax) continue;
                     TM_CHECK(precond(rT.p, zk + NS, prz, s));
                     hipLaunchKernelGGL(k_b_red_rz, dim3(S), dim3(kBlock), 0, s, G, S, prz, enq, bst.p,
                                        bhist.p + (size_t)(enq + 1) * kMaxS);
