@@ -1966,7 +1966,6 @@ __global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinSt
     }
 }
 
-// one block per sample: the dots, then the coefficients (thread 0)
 // second stage of the dots: block (q, c) sums dot q over chunk c of the
 // block partials, lane = sample (rows of part are contiguous in s);
 // red[c][q][s]
