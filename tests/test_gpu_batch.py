@@ -333,3 +333,35 @@ def test_gemm_live_events(base, prec):
     assert np.array_equal(a.get_mag(), b.get_mag())
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bemi8"])
+def test_bem_apply_leaves_evaluations_unchanged(base, prec):
+    """A boundary product on host operands between two evaluations changes
+    neither: the Dirichlet right-hand side stays zero off the boundary (its
+    window dots read only the boundary rows) and the batch's buffers are
+    rewritten before they are read."""
+    from tetmag_amd import problem as P
+    from tetmag_amd.batch import BatchContext, to_batch
+    prob, ctx, _ = base
+    S = 64
+    fields = P.smooth_random_fields(prob.mesh, S, seed=31, lam_min=10.0, lam_max=60.0)
+    dtau = 20e-3 * prob.time_scale()
+    a, b = BatchContext(ctx, S, prec_id(prec)), BatchContext(ctx, S, prec_id(prec))
+    for bc in (a, b):
+        bc.set_mag(to_batch(fields))
+    X = np.random.default_rng(7).standard_normal((ctx.bem_matrix().shape[0], S))
+    for k in range(3):
+        a.eval_euler(dtau, stats=False)
+        b.eval_euler(dtau, stats=False)
+        b.bem_apply(X)
+    ctx.synchronize()
+    ma, mb = a.get_mag(), b.get_mag()
+    da, db = a.dmdt.cpu().numpy(), b.dmdt.cpu().numpy()
+    print(prec, "rel dm/dt diff", rel_l2(db, da), "max |m diff|", np.max(np.abs(ma - mb)))
+    # the two batches tune their boundary GEMMs separately (the summation
+    # order of the split K ranges may differ)
+    assert rel_l2(db, da) < 1e-10
+    assert np.max(np.abs(ma - mb)) < 1e-12
+    a.close()
+    b.close()

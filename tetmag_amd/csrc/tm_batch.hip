@@ -1908,8 +1908,11 @@ struct WinState {
 // per-block partials part[block][q][s] of the kWinDots dots, lane = sample.  The new b goes
 // into the head slot as it is read: when the window is full that slot holds
 // the oldest pair, whose value this thread has already loaded.
+// With rows (nr of them), only those rows are read: every other row of b
+// and of the kept b is zero (a Dirichlet right-hand side off the boundary).
 __global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinState* ws, double* __restrict__ Bw,
-                                                       const double* __restrict__ b, double* part) {
+                                                       const double* __restrict__ b, double* part, int nr,
+                                                       const int* __restrict__ rows) {
     const int cnt = ws->cnt, head = ws->head;
     const size_t NS = (size_t)n * S;
     double acc[kWinDots] = {};
@@ -1921,11 +1924,13 @@ __global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinSt
     const bool on = lane < S;
     const int rstride = gridDim.x * kRowsPerBlock;
     // two rows per iteration: both rows' loads in flight together
-    for (int row = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); row < n;
-         row += 2 * rstride) {
-        const bool two = row + rstride < n;  // wavefront-uniform
+    for (int k = __builtin_amdgcn_readfirstlane(blockIdx.x * kRowsPerBlock + (int)(threadIdx.x / kWave)); k < nr;
+         k += 2 * rstride) {
+        const bool two = k + rstride < nr;  // wavefront-uniform
+        const int k2 = two ? k + rstride : k;
+        const int row = rows ? rows[k] : k, row2 = rows ? rows[k2] : k2;
         if (on) {
-            const size_t e1 = (size_t)row * S + lane, e2 = (size_t)(two ? row + rstride : row) * S + lane;
+            const size_t e1 = (size_t)row * S + lane, e2 = (size_t)row2 * S + lane;
             double v1[kSolveWindow], v2[kSolveWindow];
 #pragma unroll
             for (int i = 0; i < kSolveWindow; ++i) {
@@ -2116,6 +2121,10 @@ struct BatchSolver {
     DevBuf<double> winX, winB;        // kSolveWindow x [n][S]: solved pairs for the initial guess
     DevBuf<WinState> win;
     bool use_window = true;
+    // rows outside which every right-hand side is zero (the boundary nodes
+    // of the Dirichlet problem), or null: the window's dots read only these
+    const int* rhs_rows = nullptr;
+    int rhs_nrows = 0;
     OState* h_ost = nullptr;
     BState* h_bst = nullptr;
     static constexpr int kMaxOuter = 8;
@@ -2335,9 +2344,10 @@ struct BatchSolver {
     // initial guess of the solve of A x = b from the window of solved pairs;
     // b goes into the head slot
     void win_project(const double* b, double* x, hipStream_t s) {
-        const int gw = std::min(G, 1024);  // fewer, longer blocks: half the partials, 1.2x faster at 283k rows
+        const int nr = rhs_rows ? rhs_nrows : n;
+        const int gw = std::min(row_grid(nr), 1024);  // fewer, longer blocks: half the partials, 1.2x faster at 283k rows
         hipLaunchKernelGGL(k_b_win_dots, dim3(gw), dim3(kBlock), 0, s, n, S, (const WinState*)win.p, winB.p, b,
-                           part.p);
+                           part.p, nr, rhs_rows);
         double* red = part.p + (size_t)kWinDots * S * gw;  // after the block partials
         hipLaunchKernelGGL(k_b_win_red, dim3(kWinDots, kWinChunks), dim3(kBlock), 0, s, gw, S, (const double*)part.p,
                            red);
@@ -3286,6 +3296,10 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
         if ((st = bt->sN32.setup(*v.cgN, nsamples, bt->stream)) != TM_OK) return fail(st);
         if ((st = bt->sD32.setup(*v.cgD, nsamples, bt->stream)) != TM_OK) return fail(st);
     }
+    // the Dirichlet right-hand side is zero off the boundary nodes (Laplace,
+    // no source; the boundary product scatters into those rows only)
+    bt->sD64.rhs_rows = bt->sD32.rhs_rows = v.bnodes;
+    bt->sD64.rhs_nrows = bt->sD32.rhs_nrows = v.bnx;
     if (hipStreamSynchronize(bt->stream) != hipSuccess) return fail(TM_ERR_HIP);
     if (precision == TM_PREC_MIXED_BEMI8) {
         if ((st = i8_setup(bt)) != TM_OK) return fail(st);
@@ -3702,7 +3716,9 @@ int tm_batch_bem_apply(tm_batch* bt, const double* x, int xwhere, double* y, int
         TM_CHECK(copy_in(bt->rhsN.p, x, TM_HOST, bs, bt->stream));
         xd = bt->rhsN.p;
     }
-    double* yd = ywhere == TM_HOST ? bt->rhsD.p : y;
+    // staged in usum (rewritten by every evaluation before it is read), not
+    // in rhsD: the Dirichlet right-hand side must stay zero off the boundary
+    double* yd = ywhere == TM_HOST ? bt->usum.p : y;
     TM_CHECK(boundary_product(bt, nullptr, xd, yd));
     if (ywhere == TM_HOST) TM_CHECK(copy_out(y, TM_HOST, yd, bs, bt->stream));
     return TM_OK;
