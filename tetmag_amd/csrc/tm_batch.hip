@@ -1908,11 +1908,14 @@ struct WinState {
 // per-block partials part[block][q][s] of the kWinDots dots, lane = sample.  The new b goes
 // into the head slot as it is read: when the window is full that slot holds
 // the oldest pair, whose value this thread has already loaded.
-// With rows (nr of them), only those rows are read: every other row of b
-// and of the kept b is zero (a Dirichlet right-hand side off the boundary).
+// The k-th row read is rows[k * st] (or k * st without a list), k < nr: with
+// a list, every row outside it is zero in b and in the kept b (a Dirichlet
+// right-hand side off the boundary); with st > 1 the dots are those of an
+// evenly spread subset of the rows (a sampled least-squares fit; the kept b
+// are only ever read, and written, on that subset).
 __global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinState* ws, double* __restrict__ Bw,
                                                        const double* __restrict__ b, double* part, int nr,
-                                                       const int* __restrict__ rows) {
+                                                       const int* __restrict__ rows, int st) {
     const int cnt = ws->cnt, head = ws->head;
     const size_t NS = (size_t)n * S;
     double acc[kWinDots] = {};
@@ -1928,7 +1931,7 @@ __global__ __launch_bounds__(kBlock) void k_b_win_dots(int n, int S, const WinSt
          k += 2 * rstride) {
         const bool two = k + rstride < nr;  // wavefront-uniform
         const int k2 = two ? k + rstride : k;
-        const int row = rows ? rows[k] : k, row2 = rows ? rows[k2] : k2;
+        const int row = rows ? rows[k * st] : k * st, row2 = rows ? rows[k2 * st] : k2 * st;
         if (on) {
             const size_t e1 = (size_t)row * S + lane, e2 = (size_t)row2 * S + lane;
             double v1[kSolveWindow], v2[kSolveWindow];
@@ -2125,6 +2128,7 @@ struct BatchSolver {
     // of the Dirichlet problem), or null: the window's dots read only these
     const int* rhs_rows = nullptr;
     int rhs_nrows = 0;
+    int win_stride = 1;  // the window's fit reads every win_stride-th of those rows
     OState* h_ost = nullptr;
     BState* h_bst = nullptr;
     static constexpr int kMaxOuter = 8;
@@ -2344,10 +2348,10 @@ struct BatchSolver {
     // initial guess of the solve of A x = b from the window of solved pairs;
     // b goes into the head slot
     void win_project(const double* b, double* x, hipStream_t s) {
-        const int nr = rhs_rows ? rhs_nrows : n;
+        const int nr = ((rhs_rows ? rhs_nrows : n) + win_stride - 1) / win_stride;
         const int gw = std::min(row_grid(nr), 1024);  // fewer, longer blocks: half the partials, 1.2x faster at 283k rows
         hipLaunchKernelGGL(k_b_win_dots, dim3(gw), dim3(kBlock), 0, s, n, S, (const WinState*)win.p, winB.p, b,
-                           part.p, nr, rhs_rows);
+                           part.p, nr, rhs_rows, win_stride);
         double* red = part.p + (size_t)kWinDots * S * gw;  // after the block partials
         hipLaunchKernelGGL(k_b_win_red, dim3(kWinDots, kWinChunks), dim3(kBlock), 0, s, gw, S, (const double*)part.p,
                            red);
@@ -3300,6 +3304,8 @@ int tm_batch_create(tm_ctx* c, int nsamples, int precision, tm_batch** out) {
     // no source; the boundary product scatters into those rows only)
     bt->sD64.rhs_rows = bt->sD32.rhs_rows = v.bnodes;
     bt->sD64.rhs_nrows = bt->sD32.rhs_nrows = v.bnx;
+    bt->sN64.win_stride = bt->sN32.win_stride = 8;
+    bt->sD64.win_stride = bt->sD32.win_stride = 4;
     if (hipStreamSynchronize(bt->stream) != hipSuccess) return fail(TM_ERR_HIP);
     if (precision == TM_PREC_MIXED_BEMI8) {
         if ((st = i8_setup(bt)) != TM_OK) return fail(st);
