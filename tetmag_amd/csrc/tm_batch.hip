@@ -2412,9 +2412,9 @@ struct BatchSolver {
         // Host checks: after a solve that took one refinement step (the steady
         // state along a trajectory), the next one takes its first refinement
         // step without waiting for the initial residual check and enqueues
-        // the inner iterations the last solve needed plus one without waiting
-        // either (iterations queued past a sample's convergence do nothing):
-        // one host synchronisation per solve.  A sample still short of its
+        // exactly the inner iterations the last solve needed without waiting
+        // either (and no V-cycle after the last of them, whose z would go
+        // unused): one host synchronisation per solve.  A sample still short of its
         // inner target after such a step is taken up by the next refinement
         // step, checked as usual, and the solve after it checks as usual too.
         bool spec_ok = spec_next;  // this solve may skip checks
