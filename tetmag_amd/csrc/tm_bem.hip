@@ -169,6 +169,22 @@ __global__ __launch_bounds__(kBlock) void k_bem_gemv_part(int b, int ld, const d
 #pragma unroll
     for (int k = 0; k < 4; ++k) valid[k] = (r0 + 512 * k) < ld;  // rows in [b, ld) are padding (zero)
     int c = c0;
+    // four columns per step: 16 loads of 16 bytes in flight per thread
+    for (; c + 3 < c1; c += 4) {
+        const double x4[4] = {xs[c - c0], xs[c + 1 - c0], xs[c + 2 - c0], xs[c + 3 - c0]};
+        dbl2 v[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double* cj = B + (size_t)(c + j) * ld + r0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                v[j][k] = valid[k] ? __builtin_nontemporal_load((const dbl2*)(cj + 512 * k)) : dbl2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc[k] += v[j][k] * x4[j];
+    }
     for (; c + 1 < c1; c += 2) {
         const double xa = xs[c - c0], xb = xs[c + 1 - c0];
         const double* ca = B + (size_t)c * ld + r0;
