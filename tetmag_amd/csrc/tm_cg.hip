@@ -101,17 +101,35 @@ __global__ __launch_bounds__(kBlock) void k_cg_win_dots(int n, const CgWindow* w
 }
 
 // one block: the dots, then the coefficients (thread 0)
+// (one pass over all partials with every load independent; the solve on
+// register copies, so the unrolled factorisation makes no memory accesses)
 __global__ __launch_bounds__(kBlock) void k_cg_win_solve(int G, CgWindow* w, const double* wpart) {
-    __shared__ double t[kWinDots];
+    double v[kWinDots] = {};
+    for (int g = threadIdx.x; g < G; g += kBlock)
+#pragma unroll
+        for (int q = 0; q < kWinDots; ++q) v[q] += wpart[(size_t)q * G + g];
+    __shared__ double sr[kWinDots][kBlock / kWave];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    for (int q = wid; q < kWinDots; q += kBlock / kWave) {  // wavefront wid: dots wid, wid + 4, ...
-        double v = 0.0;
-        for (int g = lane; g < G; g += kWave) v += wpart[(size_t)q * G + g];
-        v = wave_sum(v);
-        if (lane == 0) t[q] = v;
+#pragma unroll
+    for (int q = 0; q < kWinDots; ++q) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) sr[q][wid] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0 && w->cnt > 0) win_coefficients(w->cnt, t, w->a);
+    const int cnt = w->cnt;
+    if (threadIdx.x != 0 || cnt == 0) return;
+    double t[kWinDots];
+#pragma unroll
+    for (int q = 0; q < kWinDots; ++q) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < kBlock / kWave; ++k) s += sr[q][k];
+        t[q] = s;
+    }
+    double a[kSolveWindow];
+    win_coefficients(cnt, t, a);
+#pragma unroll
+    for (int i = 0; i < kSolveWindow; ++i) w->a[i] = a[i];
 }
 
 // x0 = sum_i a_i x_slot(i) (nothing when the window is empty)
